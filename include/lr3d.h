@@ -1,0 +1,156 @@
+/*
+ * lr3d — MI355X (gfx950) dense-reconstruction hot path: PatchMatch multi-view stereo and TSDF
+ * depth-map integration, behind a plain C ABI.
+ *
+ * Reference paths are relative to the reference repository root:
+ *   DS = frontend/src/app/vision/mvs/dense-stereo.ts
+ *   PM = frontend/src/app/vision/mvs/patchmatch.wgsl.ts
+ *   TV = frontend/src/app/vision/fusion/tsdf-volume.ts
+ *   SM = frontend/src/app/vision/mvs/methods/stereo-method.ts
+ *   GD = frontend/src/app/vision/gpu/gpu-device.ts
+ *
+ * Conventions
+ *   - Every call returns an int32_t status: LR3D_OK (0) or a negative LR3D_ERR_* code; the
+ *     message of the last failure on the calling thread is lr3d_last_error().
+ *   - Host memory passed in is copied before the call returns (like queue.writeBuffer /
+ *     writeTexture); read calls copy device data into caller-owned memory.
+ *   - Device memory is owned by the handle that allocated it and freed by its destroy call.
+ *   - Each context owns one HIP stream; calls on one context are serialised by a mutex.  Calls
+ *     return when their device work has finished (the reference awaits every submit).
+ *   - Poses are world -> camera (x_cam = R X + t), R row-major, computer-vision axes, exactly the
+ *     reference's `Pose` (frontend/src/app/vision/geometry/camera.ts:19-26).
+ */
+#ifndef LR3D_H
+#define LR3D_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define LR3D_OK 0
+#define LR3D_ERR_INVALID (-1) /* bad argument */
+#define LR3D_ERR_DEVICE (-2)  /* HIP runtime / device failure, or no device */
+#define LR3D_ERR_NOMEM (-3)   /* device allocation failed */
+#define LR3D_ERR_STATE (-4)   /* call not valid in the handle's current state */
+
+#define LR3D_MAX_SOURCES 8 /* DS:90 MAX_SOURCES */
+
+typedef struct lr3d_ctx lr3d_ctx;
+typedef struct lr3d_stereo lr3d_stereo;
+typedef struct lr3d_tsdf lr3d_tsdf;
+
+/* One camera of a stereo set: the reference's StereoView {intrinsics, pose} (DS:57-60) plus its
+ * own image size (the reference uses one size for all views; per-view sizes allow batches of
+ * mixed resolution). */
+typedef struct {
+    int32_t width, height;
+    double fx, fy, cx, cy;
+    double R[9];
+    double t[3];
+} lr3d_view;
+
+/* One PatchMatch solve: the reference's SolveOptions (DS:62-75) for reference view `ref`. */
+typedef struct {
+    int32_t ref;
+    int32_t n_sources;                  /* sliced to LR3D_MAX_SOURCES like DS:232 */
+    int32_t sources[LR3D_MAX_SOURCES];
+    double min_depth, max_depth;
+    const float *prior;                 /* width*height depth guess (0 = none), or NULL */
+    int32_t geometric;                  /* second pass: geometric-consistency term */
+    int32_t iterations;
+    int32_t from_coarse;                /* start from the coarse level's planes */
+    int32_t pack;                       /* store the packed depth map (reference default 1) */
+    int32_t topk;                       /* views averaged per cost; 0 = reference rule (PM:218) */
+} lr3d_solve_desc;
+
+/* One depth map to integrate: the reference's IntegrationView (TV:67-79) plus the image size. */
+typedef struct {
+    int32_t width, height;
+    double fx, fy, cx, cy;
+    double R[9];
+    double t[3];
+    double max_depth;
+    const float *prior;  /* width*height phone depth, or NULL (reference: zeros) */
+    double confidence;   /* typical scene depth for confidence weighting; 0 = off */
+} lr3d_integration_view;
+
+/* Depth-map selectors for the TSDF calls (which map of a stereo layer is integrated). */
+#define LR3D_MAP_RAW 0      /* the packed map written by solve (DS `depths[layer]`) */
+#define LR3D_MAP_FILTERED 1 /* the consistency-filtered map written by filter (DS `filtered`) */
+
+/* ---- runtime (replaces requestGpu / readBuffer / storageBuffer, GD:29-101) ---------------- */
+
+const char *lr3d_last_error(void);
+/* Open device `device` and create its stream. */
+int32_t lr3d_ctx_create(int32_t device, lr3d_ctx **out);
+int32_t lr3d_ctx_destroy(lr3d_ctx *ctx);
+/* Block until the context's stream is idle. */
+int32_t lr3d_ctx_synchronize(lr3d_ctx *ctx);
+/* Kernel timing with HIP events recorded around each launch on the context's stream.
+ * enable: 1 on, 0 off (statistics are reset when turned on). */
+int32_t lr3d_ctx_set_profiling(lr3d_ctx *ctx, int32_t enable);
+/* Launch count and total milliseconds of kernel `name` since profiling was enabled. */
+int32_t lr3d_ctx_kernel_stats(lr3d_ctx *ctx, const char *name, int64_t *launches, double *total_ms);
+
+/* ---- PatchMatch stereo (replaces DenseStereo, DS:107-417) -------------------------------- */
+
+/* new DenseStereo(device, width, height, views) (DS:126-218).  flags: reserved, pass 0. */
+int32_t lr3d_stereo_create(lr3d_ctx *ctx, const lr3d_view *views, int32_t n_views, uint32_t flags,
+                           lr3d_stereo **out);
+int32_t lr3d_stereo_destroy(lr3d_stereo *st);                    /* DS:403-416 */
+/* DenseStereo.setImage (DS:221-228): u8 gray, width x height of that layer. */
+int32_t lr3d_stereo_set_image(lr3d_stereo *st, int32_t layer, const uint8_t *gray, int32_t width,
+                              int32_t height);
+/* DenseStereo.solve (DS:231-285): init, `iterations` red-black rounds, pack.  One launch per
+ * kernel per phase, as the reference dispatches it (the "plain" path). */
+int32_t lr3d_stereo_solve(lr3d_stereo *st, const lr3d_solve_desc *desc);
+/* All solves of `descs` at once: every view's pixels share each launch, the iteration loop is
+ * replayed from a captured HIP graph.  Results equal n calls of lr3d_stereo_solve. */
+int32_t lr3d_stereo_solve_batched(lr3d_stereo *st, const lr3d_solve_desc *descs, int32_t n);
+/* DenseStereo.filter (DS:296-309): consistency check of `ref` into its filtered map. */
+int32_t lr3d_stereo_filter(lr3d_stereo *st, int32_t ref, const int32_t *checks, int32_t n_checks,
+                           int32_t needed);
+/* DenseStereo.depthMap (DS:355-358): packed u32 map of a layer (width*height words). */
+int32_t lr3d_stereo_read_depth(lr3d_stereo *st, int32_t layer, uint32_t *out);
+/* DenseStereo.filteredMap readback (DS:347-352) of the last filter of `layer`. */
+int32_t lr3d_stereo_read_filtered(lr3d_stereo *st, int32_t layer, uint32_t *out);
+/* DenseStereo.hypotheses (DS:288-290): planes (n.xyz, depth) and costs of a layer's last solve.
+ * hyp: width*height*4 floats, cost: width*height floats (either may be NULL). */
+int32_t lr3d_stereo_read_hyp(lr3d_stereo *st, int32_t layer, float *hyp, float *cost);
+/* Overwrite a layer's packed depth map (e.g. maps computed elsewhere, or restored state). */
+int32_t lr3d_stereo_write_depth(lr3d_stereo *st, int32_t layer, const uint32_t *packed);
+/* Attach a coarse level for from_coarse solves (coarse-to-fine, DS:77-82 CoarseLevel): the
+ * coarse stereo must hold one layer per layer of `st`; its planes are read, not copied. */
+int32_t lr3d_stereo_set_coarse(lr3d_stereo *st, lr3d_stereo *coarse);
+
+/* ---- TSDF integration (replaces TsdfVolume, TV:158-291) ---------------------------------- */
+
+/* new TsdfVolume(device, grid, layout, depth, W, H) (TV:171-217): brick pool of n_bricks 8^3
+ * bricks at `coords` (n_bricks*4 ints: bx, by, bz, 0, as BrickGrid.build returns them),
+ * tsdf := 1, weight := 0.  truncation_voxels: the reference uses 4 (TV:179). */
+int32_t lr3d_tsdf_create(lr3d_ctx *ctx, const double origin[3], double voxel,
+                         const int32_t dims[3], const int32_t *coords, int32_t n_bricks,
+                         double truncation_voxels, lr3d_tsdf **out);
+int32_t lr3d_tsdf_destroy(lr3d_tsdf *ts);                        /* TV:279-290 */
+/* TsdfVolume.integrate (TV:240-268) of one map of a stereo layer (LR3D_MAP_*). */
+int32_t lr3d_tsdf_integrate(lr3d_tsdf *ts, lr3d_stereo *st, int32_t layer, int32_t map,
+                            const lr3d_integration_view *view);
+/* Same, for a packed map given in host memory (view->width * view->height words). */
+int32_t lr3d_tsdf_integrate_host(lr3d_tsdf *ts, const uint32_t *packed,
+                                 const lr3d_integration_view *view);
+/* n views in one pass over the volume; per voxel the views apply in array order, so the result
+ * equals n calls of lr3d_tsdf_integrate in that order. */
+int32_t lr3d_tsdf_integrate_batched(lr3d_tsdf *ts, lr3d_stereo *st, const int32_t *layers,
+                                    int32_t map, const lr3d_integration_view *views, int32_t n);
+/* TsdfVolume.read (TV:270-277): n_bricks*512 floats each (either may be NULL). */
+int32_t lr3d_tsdf_read(lr3d_tsdf *ts, float *tsdf, float *weight);
+/* Voxel updates ((voxel, view) pairs written) of the integrate calls since the last reset. */
+int32_t lr3d_tsdf_set_counting(lr3d_tsdf *ts, int32_t enable);
+int32_t lr3d_tsdf_update_count(lr3d_tsdf *ts, int64_t *out);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,964 @@
+// C-ABI of the lr3d library: contexts, the PatchMatch stereo set and the TSDF volume.
+//
+// Host semantics follow the reference's DenseStereo (frontend/src/app/vision/mvs/dense-stereo.ts,
+// DS:<line>) and TsdfVolume (frontend/src/app/vision/fusion/tsdf-volume.ts, TV:<line>): the same
+// parameter packing (DS:360-401, TV:244-253), seed derivation (DS:379), schedule (DS:231-285) and
+// culling (TV:220-234).  The reference slices work into row bands and awaits every submit so a
+// phone GPU stays responsive; bands do not change results, so here each dispatch covers whole
+// images and a call synchronises once, at its end.
+#include "lr3d_internal.h"
+
+#include <math.h>
+#include <stdarg.h>
+#include <stdio.h>
+#include <string.h>
+
+#include <algorithm>
+#include <map>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "../../include/lr3d.h"
+
+using namespace lr3d;
+
+// ---- errors ---------------------------------------------------------------------------------------
+
+static thread_local std::string g_error;
+
+static int32_t fail(int32_t code, const char *fmt, ...) {
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof(buf), fmt, ap);
+    va_end(ap);
+    g_error = buf;
+    return code;
+}
+
+#define HIP_TRY(expr)                                                                              \
+    do {                                                                                           \
+        hipError_t e_ = (expr);                                                                    \
+        if (e_ != hipSuccess)                                                                      \
+            return fail(e_ == hipErrorOutOfMemory ? LR3D_ERR_NOMEM : LR3D_ERR_DEVICE, "%s: %s (%s:%d)", \
+                        #expr, hipGetErrorString(e_), __FILE__, __LINE__);                         \
+    } while (0)
+
+extern "C" const char *lr3d_last_error(void) { return g_error.c_str(); }
+
+// ---- device buffers ----------------------------------------------------------------------------------
+
+template <typename T>
+struct DevBuf {
+    T *ptr = nullptr;
+    size_t count = 0;
+    hipError_t alloc(size_t n) {
+        release();
+        if (n == 0) return hipSuccess;
+        hipError_t e = hipMalloc((void **)&ptr, n * sizeof(T));
+        if (e == hipSuccess) count = n; else ptr = nullptr;
+        return e;
+    }
+    // Grow-only reallocation (contents are not kept).
+    hipError_t reserve(size_t n) { return n <= count ? hipSuccess : alloc(n); }
+    void release() {
+        if (ptr) (void)hipFree(ptr);
+        ptr = nullptr;
+        count = 0;
+    }
+    ~DevBuf() { release(); }
+};
+
+// ---- context ------------------------------------------------------------------------------------------
+
+struct KernelStats {
+    int64_t launches = 0;
+    double total_ms = 0.0;
+};
+
+struct PendingEvent {
+    std::string name;
+    hipEvent_t start, stop;
+};
+
+struct lr3d_ctx {
+    int32_t device = 0;
+    hipStream_t stream = nullptr;
+    std::mutex mu;
+    bool profiling = false;
+    std::map<std::string, KernelStats> stats;
+    std::vector<hipEvent_t> event_pool;
+    std::vector<PendingEvent> pending;
+
+    hipEvent_t take_event() {
+        if (!event_pool.empty()) {
+            hipEvent_t e = event_pool.back();
+            event_pool.pop_back();
+            return e;
+        }
+        hipEvent_t e = nullptr;
+        if (hipEventCreate(&e) != hipSuccess) return nullptr;
+        return e;
+    }
+    // Fold finished launch timings into the statistics (stream must be idle).
+    void collect() {
+        for (auto &p : pending) {
+            float ms = 0.0f;
+            if (hipEventElapsedTime(&ms, p.start, p.stop) == hipSuccess) {
+                auto &s = stats[p.name];
+                s.launches++;
+                s.total_ms += ms;
+            }
+            event_pool.push_back(p.start);
+            event_pool.push_back(p.stop);
+        }
+        pending.clear();
+    }
+    ~lr3d_ctx() {
+        for (auto &p : pending) {
+            (void)hipEventDestroy(p.start);
+            (void)hipEventDestroy(p.stop);
+        }
+        for (auto e : event_pool) (void)hipEventDestroy(e);
+        if (stream) (void)hipStreamDestroy(stream);
+    }
+};
+
+// Records start/stop events around one launch when profiling is on.
+template <typename F>
+static hipError_t timed(lr3d_ctx *ctx, const char *name, F &&launch) {
+    if (!ctx->profiling) return launch();
+    hipEvent_t a = ctx->take_event(), b = ctx->take_event();
+    if (!a || !b) return hipErrorOutOfMemory;
+    hipError_t e = hipEventRecord(a, ctx->stream);
+    if (e != hipSuccess) return e;
+    e = launch();
+    if (e != hipSuccess) return e;
+    e = hipEventRecord(b, ctx->stream);
+    if (e != hipSuccess) return e;
+    ctx->pending.push_back({name, a, b});
+    return hipSuccess;
+}
+
+static int32_t finish(lr3d_ctx *ctx) {
+    HIP_TRY(hipStreamSynchronize(ctx->stream));
+    if (ctx->profiling) ctx->collect();
+    return LR3D_OK;
+}
+
+extern "C" int32_t lr3d_ctx_create(int32_t device, lr3d_ctx **out) {
+    if (!out) return fail(LR3D_ERR_INVALID, "lr3d_ctx_create: out is NULL");
+    *out = nullptr;
+    int count = 0;
+    hipError_t e = hipGetDeviceCount(&count);
+    if (e != hipSuccess || count == 0)
+        return fail(LR3D_ERR_DEVICE, "lr3d_ctx_create: no HIP device (%s)",
+                    e != hipSuccess ? hipGetErrorString(e) : "0 devices");
+    if (device < 0 || device >= count)
+        return fail(LR3D_ERR_INVALID, "lr3d_ctx_create: device %d out of range [0, %d)", device, count);
+    HIP_TRY(hipSetDevice(device));
+    lr3d_ctx *ctx = new lr3d_ctx();
+    ctx->device = device;
+    e = hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking);
+    if (e != hipSuccess) {
+        delete ctx;
+        return fail(LR3D_ERR_DEVICE, "lr3d_ctx_create: hipStreamCreate: %s", hipGetErrorString(e));
+    }
+    *out = ctx;
+    return LR3D_OK;
+}
+
+extern "C" int32_t lr3d_ctx_destroy(lr3d_ctx *ctx) {
+    if (!ctx) return LR3D_OK;
+    {
+        std::lock_guard<std::mutex> lock(ctx->mu);
+        (void)hipSetDevice(ctx->device);
+        (void)hipStreamSynchronize(ctx->stream);
+    }
+    delete ctx;
+    return LR3D_OK;
+}
+
+extern "C" int32_t lr3d_ctx_synchronize(lr3d_ctx *ctx) {
+    if (!ctx) return fail(LR3D_ERR_INVALID, "lr3d_ctx_synchronize: ctx is NULL");
+    std::lock_guard<std::mutex> lock(ctx->mu);
+    HIP_TRY(hipSetDevice(ctx->device));
+    return finish(ctx);
+}
+
+extern "C" int32_t lr3d_ctx_set_profiling(lr3d_ctx *ctx, int32_t enable) {
+    if (!ctx) return fail(LR3D_ERR_INVALID, "lr3d_ctx_set_profiling: ctx is NULL");
+    std::lock_guard<std::mutex> lock(ctx->mu);
+    HIP_TRY(hipSetDevice(ctx->device));
+    int32_t rc = finish(ctx);
+    if (rc != LR3D_OK) return rc;
+    ctx->profiling = enable != 0;
+    if (ctx->profiling) ctx->stats.clear();
+    return LR3D_OK;
+}
+
+extern "C" int32_t lr3d_ctx_kernel_stats(lr3d_ctx *ctx, const char *name, int64_t *launches,
+                                         double *total_ms) {
+    if (!ctx || !name) return fail(LR3D_ERR_INVALID, "lr3d_ctx_kernel_stats: NULL argument");
+    std::lock_guard<std::mutex> lock(ctx->mu);
+    auto it = ctx->stats.find(name);
+    if (launches) *launches = it == ctx->stats.end() ? 0 : it->second.launches;
+    if (total_ms) *total_ms = it == ctx->stats.end() ? 0.0 : it->second.total_ms;
+    return LR3D_OK;
+}
+
+// ---- stereo -----------------------------------------------------------------------------------------
+
+// A captured launch sequence of one batch layout, replayed while the layout is unchanged.
+struct GraphEntry {
+    std::vector<int64_t> key;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+};
+
+struct lr3d_stereo {
+    lr3d_ctx *ctx = nullptr;
+    std::vector<lr3d_view> views;
+    std::vector<uint32_t> off;  // pixel offset per layer
+    size_t total_px = 0;
+    std::vector<uint8_t> has_image;
+    std::vector<uint8_t> solved;
+    DevBuf<uint32_t> quad, depth, filt;
+    DevBuf<float4> hyp;
+    DevBuf<float> cost, prior;
+    DevBuf<uint8_t> staging;  // u8 upload of one layer
+    DevBuf<uint8_t> plan_buf; // descriptors + block tables of the current launch plan
+    lr3d_stereo *coarse = nullptr;
+    std::vector<GraphEntry> graphs;
+
+    ~lr3d_stereo() {
+        for (auto &g : graphs) {
+            if (g.exec) (void)hipGraphExecDestroy(g.exec);
+            if (g.graph) (void)hipGraphDestroy(g.graph);
+        }
+    }
+};
+
+static bool valid_view(const lr3d_view &v) {
+    return v.width > 0 && v.height > 0 && v.width <= 16384 && v.height <= 16384 && v.fx > 0 &&
+           v.fy > 0 && std::isfinite(v.cx) && std::isfinite(v.cy);
+}
+
+extern "C" int32_t lr3d_stereo_create(lr3d_ctx *ctx, const lr3d_view *views, int32_t n_views,
+                                      uint32_t flags, lr3d_stereo **out) {
+    if (!ctx || !views || !out) return fail(LR3D_ERR_INVALID, "lr3d_stereo_create: NULL argument");
+    *out = nullptr;
+    if (n_views <= 0) return fail(LR3D_ERR_INVALID, "lr3d_stereo_create: n_views must be > 0");
+    (void)flags;
+    size_t total = 0;
+    std::vector<uint32_t> off(n_views);
+    for (int32_t i = 0; i < n_views; i++) {
+        if (!valid_view(views[i]))
+            return fail(LR3D_ERR_INVALID, "lr3d_stereo_create: view %d has an invalid size or intrinsics", i);
+        off[i] = (uint32_t)total;
+        total += (size_t)views[i].width * (size_t)views[i].height;
+        if (total >= (1ull << 32))
+            return fail(LR3D_ERR_INVALID, "lr3d_stereo_create: more than 2^32 pixels in one set");
+    }
+    std::lock_guard<std::mutex> lock(ctx->mu);
+    HIP_TRY(hipSetDevice(ctx->device));
+    lr3d_stereo *st = new lr3d_stereo();
+    st->ctx = ctx;
+    st->views.assign(views, views + n_views);
+    st->off = off;
+    st->total_px = total;
+    st->has_image.assign(n_views, 0);
+    st->solved.assign(n_views, 0);
+    hipError_t e = st->quad.alloc(total);
+    if (e == hipSuccess) e = st->depth.alloc(total);
+    if (e == hipSuccess) e = st->filt.alloc(total);
+    if (e == hipSuccess) e = st->hyp.alloc(total);
+    if (e == hipSuccess) e = st->cost.alloc(total);
+    if (e == hipSuccess) e = hipMemsetAsync(st->depth.ptr, 0, total * 4, ctx->stream);
+    if (e == hipSuccess) e = hipMemsetAsync(st->filt.ptr, 0, total * 4, ctx->stream);
+    if (e == hipSuccess) e = hipMemsetAsync(st->hyp.ptr, 0, total * 16, ctx->stream);
+    if (e == hipSuccess) e = hipMemsetAsync(st->cost.ptr, 0, total * 4, ctx->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+    if (e != hipSuccess) {
+        delete st;
+        return fail(e == hipErrorOutOfMemory ? LR3D_ERR_NOMEM : LR3D_ERR_DEVICE,
+                    "lr3d_stereo_create: %s", hipGetErrorString(e));
+    }
+    *out = st;
+    return LR3D_OK;
+}
+
+extern "C" int32_t lr3d_stereo_destroy(lr3d_stereo *st) {
+    if (!st) return LR3D_OK;
+    lr3d_ctx *ctx = st->ctx;
+    std::lock_guard<std::mutex> lock(ctx->mu);
+    (void)hipSetDevice(ctx->device);
+    (void)hipStreamSynchronize(ctx->stream);
+    delete st;
+    return LR3D_OK;
+}
+
+static int32_t check_layer(const lr3d_stereo *st, int32_t layer, const char *what) {
+    if (layer < 0 || layer >= (int32_t)st->views.size())
+        return fail(LR3D_ERR_INVALID, "%s: layer %d out of range [0, %d)", what, layer,
+                    (int32_t)st->views.size());
+    return LR3D_OK;
+}
+
+extern "C" int32_t lr3d_stereo_set_image(lr3d_stereo *st, int32_t layer, const uint8_t *gray,
+                                         int32_t width, int32_t height) {
+    if (!st || !gray) return fail(LR3D_ERR_INVALID, "lr3d_stereo_set_image: NULL argument");
+    int32_t rc = check_layer(st, layer, "lr3d_stereo_set_image");
+    if (rc) return rc;
+    const lr3d_view &v = st->views[layer];
+    if (width != v.width || height != v.height)
+        return fail(LR3D_ERR_INVALID, "lr3d_stereo_set_image: layer %d is %dx%d, got %dx%d", layer,
+                    v.width, v.height, width, height);
+    lr3d_ctx *ctx = st->ctx;
+    std::lock_guard<std::mutex> lock(ctx->mu);
+    HIP_TRY(hipSetDevice(ctx->device));
+    const size_t px = (size_t)width * height;
+    HIP_TRY(st->staging.reserve(px));
+    HIP_TRY(hipMemcpyAsync(st->staging.ptr, gray, px, hipMemcpyHostToDevice, ctx->stream));
+    HIP_TRY(timed(ctx, "pm_build_quad", [&] {
+        return pm_build_quad(ctx->stream, st->staging.ptr, width, height, st->quad.ptr + st->off[layer]);
+    }));
+    st->has_image[layer] = 1;
+    return finish(ctx);
+}
+
+// Launch shapes of the PatchMatch kernels (threads x, y).  The per-pixel kernels use 64-wide
+// rows of one wavefront; propagate uses 8 x 8 threads = 16 x 8 pixels, the reference's tile.
+TileShape lr3d::pm_tile_for(const char *kernel, int32_t W, int32_t H, int32_t n_src) {
+    (void)W;
+    (void)H;
+    (void)n_src;
+    if (strcmp(kernel, "propagate") == 0) return {8, 8};
+    if (strcmp(kernel, "init") == 0) return {8, 8};
+    return {64, 4};
+}
+
+// DS:360-401 for one solve, in double then rounded to f32 like Float32Array.set.
+static int32_t build_desc(const lr3d_stereo *st, const lr3d_solve_desc &sd, uint32_t prior_off,
+                          bool require_images, PmDesc &d) {
+    const char *what = "lr3d_stereo_solve";
+    int32_t rc = check_layer(st, sd.ref, what);
+    if (rc) return rc;
+    if (sd.n_sources < 0) return fail(LR3D_ERR_INVALID, "%s: n_sources < 0", what);
+    const int32_t n_src = std::min(sd.n_sources, (int32_t)kMaxSources);  // DS:232
+    if (!(sd.min_depth > 0.0) || !(sd.max_depth > sd.min_depth))
+        return fail(LR3D_ERR_INVALID, "%s: depth range must satisfy 0 < min_depth < max_depth", what);
+    if (sd.iterations < 0 || sd.iterations > 64)
+        return fail(LR3D_ERR_INVALID, "%s: iterations must be in [0, 64]", what);
+    if (sd.topk < 0 || sd.topk > kMaxSources)
+        return fail(LR3D_ERR_INVALID, "%s: topk must be in [0, %d]", what, kMaxSources);
+    if (require_images && !st->has_image[sd.ref])
+        return fail(LR3D_ERR_STATE, "%s: layer %d has no image", what, sd.ref);
+    const lr3d_view &rv = st->views[sd.ref];
+    memset(&d, 0, sizeof(d));
+    d.W = rv.width;
+    d.H = rv.height;
+    d.ref = sd.ref;
+    d.n_src = n_src;
+    d.k[0] = (float)rv.fx;
+    d.k[1] = (float)rv.fy;
+    d.k[2] = (float)rv.cx;
+    d.k[3] = (float)rv.cy;
+    d.ikx = 1.0f / d.k[0];
+    d.iky = 1.0f / d.k[1];
+    d.kx2 = -d.k[2] / d.k[0];
+    d.ky2 = -d.k[3] / d.k[1];
+    d.inv_min = (float)(1.0 / sd.min_depth);
+    d.inv_max = (float)(1.0 / sd.max_depth);
+    d.geo_weight = 0.2f;  // DS:92
+    d.flags = sd.geometric ? 1u : 0u;
+    int32_t k = sd.topk > 0 ? std::min(sd.topk, n_src) : std::max(1, std::min(3, n_src - 1));  // PM:218
+    d.topk = std::max(1, k);
+    d.iterations = sd.iterations;
+    d.seed_base = (uint32_t)(sd.ref + 1) * 0x9e3779b1u + d.flags * 7u;  // DS:379
+    d.off = st->off[sd.ref];
+    d.has_prior = sd.prior != nullptr;
+    d.prior_off = prior_off;
+    d.from_coarse = sd.from_coarse && st->coarse != nullptr;
+    if (d.from_coarse) {
+        const lr3d_view &cv = st->coarse->views[sd.ref];
+        // CAM:28-36 scaleIntrinsics(k, cw / W, ch / H), in double
+        const double sx = (double)cv.width / (double)rv.width, sy = (double)cv.height / (double)rv.height;
+        d.ck[0] = (float)(rv.fx * sx);
+        d.ck[1] = (float)(rv.fy * sy);
+        d.ck[2] = (float)((rv.cx + 0.5) * sx - 0.5);
+        d.ck[3] = (float)((rv.cy + 0.5) * sy - 0.5);
+        d.coarse_W = cv.width;
+        d.coarse_H = cv.height;
+        d.coarse_off = st->coarse->off[sd.ref];
+    }
+    for (int32_t s = 0; s < n_src; s++) {
+        const int32_t layer = sd.sources[s];
+        rc = check_layer(st, layer, what);
+        if (rc) return rc;
+        if (require_images && !st->has_image[layer])
+            return fail(LR3D_ERR_STATE, "%s: source layer %d has no image", what, layer);
+        const lr3d_view &sv = st->views[layer];
+        // CAM:73-77 relativePose(ref, src) with math/mat3.ts multiply3 / apply3, in double
+        double R[9], t[3];
+        const double *a = rv.R, *b = sv.R;
+        const double aT[9] = {a[0], a[3], a[6], a[1], a[4], a[7], a[2], a[5], a[8]};
+        for (int r = 0; r < 3; r++)
+            for (int c = 0; c < 3; c++)
+                R[r * 3 + c] = b[r * 3] * aT[c] + b[r * 3 + 1] * aT[3 + c] + b[r * 3 + 2] * aT[6 + c];
+        for (int r = 0; r < 3; r++) {
+            const double rt = R[r * 3] * rv.t[0] + R[r * 3 + 1] * rv.t[1] + R[r * 3 + 2] * rv.t[2];
+            t[r] = sv.t[r] - rt;
+        }
+        PmSrc &o = d.src[s];
+        o.k[0] = (float)sv.fx;
+        o.k[1] = (float)sv.fy;
+        o.k[2] = (float)sv.cx;
+        o.k[3] = (float)sv.cy;
+        for (int i = 0; i < 9; i++) o.R[i] = (float)R[i];
+        for (int i = 0; i < 3; i++) o.t[i] = (float)t[i];
+        o.W = sv.width;
+        o.H = sv.height;
+        o.off = st->off[layer];
+        o.layer = layer;
+    }
+    return LR3D_OK;
+}
+
+// Device copy of descriptors and the three block tables (init, propagate, pack).
+struct Plans {
+    PmPlan init, prop, pack;
+};
+
+static int32_t upload_plans(lr3d_stereo *st, const std::vector<PmDesc> &descs,
+                            const std::vector<uint8_t> &pack, Plans &plans, std::vector<int64_t> &key) {
+    lr3d_ctx *ctx = st->ctx;
+    const int32_t n = (int32_t)descs.size();
+    const TileShape ti = pm_tile_for("init", 0, 0, 0), tp = pm_tile_for("propagate", 0, 0, 0),
+                    tk = pm_tile_for("pack", 0, 0, 0);
+    std::vector<uint32_t> bi(n + 1), bp(n + 1), bk(n + 1);
+    int32_t kmax = 1, geometric = 0;
+    key.clear();
+    key.push_back(n);
+    bi[0] = bp[0] = bk[0] = 0;
+    for (int32_t v = 0; v < n; v++) {
+        const PmDesc &d = descs[v];
+        const uint64_t rows_i = (d.H + ti.ty - 1) / ti.ty, cols_i = (d.W + ti.tx - 1) / ti.tx;
+        const uint64_t rows_p = (d.H + tp.ty - 1) / tp.ty, cols_p = (d.W + 2 * tp.tx - 1) / (2 * tp.tx);
+        const uint64_t rows_k = (d.H + tk.ty - 1) / tk.ty, cols_k = (d.W + tk.tx - 1) / tk.tx;
+        bi[v + 1] = bi[v] + (uint32_t)(rows_i * cols_i);
+        bp[v + 1] = bp[v] + (d.iterations > 0 ? (uint32_t)(rows_p * cols_p) : 0u);
+        bk[v + 1] = bk[v] + (pack[v] ? (uint32_t)(rows_k * cols_k) : 0u);
+        kmax = std::max(kmax, d.topk);
+        geometric |= (int32_t)(d.flags & 1u);
+        key.push_back(d.W);
+        key.push_back(d.H);
+        key.push_back(d.iterations);
+        key.push_back(pack[v]);
+    }
+    const size_t desc_bytes = sizeof(PmDesc) * n;
+    const size_t table_bytes = sizeof(uint32_t) * (n + 1);
+    const size_t total = desc_bytes + 3 * table_bytes;
+    HIP_TRY(st->plan_buf.reserve(total));
+    std::vector<uint8_t> host(total);
+    memcpy(host.data(), descs.data(), desc_bytes);
+    memcpy(host.data() + desc_bytes, bi.data(), table_bytes);
+    memcpy(host.data() + desc_bytes + table_bytes, bp.data(), table_bytes);
+    memcpy(host.data() + desc_bytes + 2 * table_bytes, bk.data(), table_bytes);
+    HIP_TRY(hipMemcpyAsync(st->plan_buf.ptr, host.data(), total, hipMemcpyHostToDevice, ctx->stream));
+    HIP_TRY(hipStreamSynchronize(ctx->stream));  // host staging goes out of scope
+    const PmDesc *dd = (const PmDesc *)st->plan_buf.ptr;
+    const uint32_t *tables = (const uint32_t *)(st->plan_buf.ptr + desc_bytes);
+    plans.init = {dd, tables, n, bi[n], ti.tx, ti.ty, kmax, geometric};
+    plans.prop = {dd, tables + (n + 1), n, bp[n], tp.tx, tp.ty, kmax, geometric};
+    plans.pack = {dd, tables + 2 * (n + 1), n, bk[n], tk.tx, tk.ty, kmax, geometric};
+    key.push_back(kmax);
+    key.push_back(geometric);
+    key.push_back((int64_t)(uintptr_t)st->plan_buf.ptr);
+    return LR3D_OK;
+}
+
+static PmPools pools_of(lr3d_stereo *st) {
+    PmPools p;
+    p.quad = st->quad.ptr;
+    p.hyp = st->hyp.ptr;
+    p.cost = st->cost.ptr;
+    p.depth = st->depth.ptr;
+    p.filt = st->filt.ptr;
+    p.prior = st->prior.ptr;
+    p.coarse_hyp = st->coarse ? st->coarse->hyp.ptr : nullptr;
+    return p;
+}
+
+// The schedule of DS:231-285 for a batch: init, iterations x {phase 0, 1}, pack.
+static hipError_t enqueue_schedule(lr3d_stereo *st, const Plans &plans, int32_t max_iterations,
+                                   bool any_pack) {
+    lr3d_ctx *ctx = st->ctx;
+    const PmPools pools = pools_of(st);
+    hipError_t e = timed(ctx, "pm_init", [&] { return pm_launch_init(ctx->stream, plans.init, pools); });
+    if (e != hipSuccess) return e;
+    for (int32_t it = 0; it < max_iterations; it++) {
+        const float step = (float)pow(0.5, (double)it);  // DS:249
+        for (uint32_t phase = 0; phase < 2; phase++) {
+            e = timed(ctx, "pm_propagate", [&] {
+                return pm_launch_propagate(ctx->stream, plans.prop, pools, (uint32_t)it, phase, step);
+            });
+            if (e != hipSuccess) return e;
+        }
+    }
+    if (any_pack) {
+        e = timed(ctx, "pm_pack", [&] { return pm_launch_pack(ctx->stream, plans.pack, pools); });
+        if (e != hipSuccess) return e;
+    }
+    return hipSuccess;
+}
+
+static int32_t stage_priors(lr3d_stereo *st, const lr3d_solve_desc *descs, int32_t n,
+                            std::vector<uint32_t> &prior_off) {
+    lr3d_ctx *ctx = st->ctx;
+    prior_off.assign(n, 0);
+    bool any = false;
+    for (int32_t i = 0; i < n; i++) any |= descs[i].prior != nullptr;
+    if (!any) return LR3D_OK;
+    if (!st->prior.ptr) HIP_TRY(st->prior.alloc(st->total_px));
+    for (int32_t i = 0; i < n; i++) {
+        if (!descs[i].prior) continue;
+        const int32_t ref = descs[i].ref;
+        const lr3d_view &v = st->views[ref];
+        prior_off[i] = st->off[ref];
+        HIP_TRY(hipMemcpyAsync(st->prior.ptr + st->off[ref], descs[i].prior,
+                               (size_t)v.width * v.height * 4, hipMemcpyHostToDevice, ctx->stream));
+    }
+    return LR3D_OK;
+}
+
+static int32_t solve_impl(lr3d_stereo *st, const lr3d_solve_desc *descs, int32_t n, bool batched) {
+    lr3d_ctx *ctx = st->ctx;
+    for (int32_t i = 0; i < n; i++) {
+        int32_t rc = check_layer(st, descs[i].ref, batched ? "lr3d_stereo_solve_batched" : "lr3d_stereo_solve");
+        if (rc) return rc;
+        for (int32_t j = 0; j < i; j++)
+            if (descs[j].ref == descs[i].ref)
+                return fail(LR3D_ERR_INVALID, "lr3d_stereo_solve_batched: layer %d appears twice", descs[i].ref);
+    }
+    std::vector<uint32_t> prior_off;
+    int32_t rc = stage_priors(st, descs, n, prior_off);
+    if (rc) return rc;
+    std::vector<PmDesc> dv;
+    dv.reserve(n);
+    int32_t max_it = 0;
+    bool any_pack = false;
+    std::vector<int32_t> refs;
+    std::vector<uint8_t> packs;
+    for (int32_t i = 0; i < n; i++) {
+        if (std::min(descs[i].n_sources, (int32_t)kMaxSources) <= 0) continue;  // DS:233: no-op
+        if (descs[i].from_coarse && !st->coarse)
+            return fail(LR3D_ERR_STATE, "lr3d_stereo_solve: from_coarse without a coarse level");
+        PmDesc d;
+        rc = build_desc(st, descs[i], prior_off[i], true, d);
+        if (rc) return rc;
+        dv.push_back(d);
+        packs.push_back(descs[i].pack ? 1 : 0);
+        max_it = std::max(max_it, d.iterations);
+        any_pack |= descs[i].pack != 0;
+        refs.push_back(descs[i].ref);
+    }
+    if (dv.empty()) return finish(ctx);
+    Plans plans;
+    std::vector<int64_t> key;
+    rc = upload_plans(st, dv, packs, plans, key);
+    if (rc) return rc;
+    hipError_t e = hipSuccess;
+    if (batched && !ctx->profiling) {
+        GraphEntry *entry = nullptr;
+        for (auto &g : st->graphs)
+            if (g.key == key) entry = &g;
+        if (!entry) {
+            GraphEntry g;
+            g.key = key;
+            HIP_TRY(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
+            e = enqueue_schedule(st, plans, max_it, any_pack);
+            hipGraph_t graph = nullptr;
+            hipError_t e2 = hipStreamEndCapture(ctx->stream, &graph);
+            if (e != hipSuccess || e2 != hipSuccess) {
+                if (graph) (void)hipGraphDestroy(graph);
+                return fail(LR3D_ERR_DEVICE, "lr3d_stereo_solve_batched: graph capture failed: %s",
+                            hipGetErrorString(e != hipSuccess ? e : e2));
+            }
+            g.graph = graph;
+            e = hipGraphInstantiate(&g.exec, graph, nullptr, nullptr, 0);
+            if (e != hipSuccess) {
+                (void)hipGraphDestroy(graph);
+                return fail(LR3D_ERR_DEVICE, "lr3d_stereo_solve_batched: hipGraphInstantiate: %s",
+                            hipGetErrorString(e));
+            }
+            if (st->graphs.size() >= 8) {
+                (void)hipGraphExecDestroy(st->graphs.front().exec);
+                (void)hipGraphDestroy(st->graphs.front().graph);
+                st->graphs.erase(st->graphs.begin());
+            }
+            st->graphs.push_back(g);
+            entry = &st->graphs.back();
+        }
+        HIP_TRY(hipGraphLaunch(entry->exec, ctx->stream));
+    } else {
+        e = enqueue_schedule(st, plans, max_it, any_pack);
+        if (e != hipSuccess)
+            return fail(LR3D_ERR_DEVICE, "lr3d_stereo_solve: launch failed: %s", hipGetErrorString(e));
+    }
+    rc = finish(ctx);
+    if (rc) return rc;
+    for (int32_t r : refs) st->solved[r] = 1;
+    return LR3D_OK;
+}
+
+extern "C" int32_t lr3d_stereo_solve(lr3d_stereo *st, const lr3d_solve_desc *desc) {
+    if (!st || !desc) return fail(LR3D_ERR_INVALID, "lr3d_stereo_solve: NULL argument");
+    std::lock_guard<std::mutex> lock(st->ctx->mu);
+    HIP_TRY(hipSetDevice(st->ctx->device));
+    return solve_impl(st, desc, 1, false);
+}
+
+extern "C" int32_t lr3d_stereo_solve_batched(lr3d_stereo *st, const lr3d_solve_desc *descs, int32_t n) {
+    if (!st || (!descs && n > 0)) return fail(LR3D_ERR_INVALID, "lr3d_stereo_solve_batched: NULL argument");
+    if (n < 0) return fail(LR3D_ERR_INVALID, "lr3d_stereo_solve_batched: n < 0");
+    std::lock_guard<std::mutex> lock(st->ctx->mu);
+    HIP_TRY(hipSetDevice(st->ctx->device));
+    return solve_impl(st, descs, n, true);
+}
+
+extern "C" int32_t lr3d_stereo_filter(lr3d_stereo *st, int32_t ref, const int32_t *checks,
+                                      int32_t n_checks, int32_t needed) {
+    if (!st || (!checks && n_checks > 0)) return fail(LR3D_ERR_INVALID, "lr3d_stereo_filter: NULL argument");
+    int32_t rc = check_layer(st, ref, "lr3d_stereo_filter");
+    if (rc) return rc;
+    if (n_checks < 0) return fail(LR3D_ERR_INVALID, "lr3d_stereo_filter: n_checks < 0");
+    lr3d_ctx *ctx = st->ctx;
+    std::lock_guard<std::mutex> lock(ctx->mu);
+    HIP_TRY(hipSetDevice(ctx->device));
+    // DS:296-309: the filter packs params with depth range 0.01..100 and step 0.
+    lr3d_solve_desc sd;
+    memset(&sd, 0, sizeof(sd));
+    sd.ref = ref;
+    sd.n_sources = std::min(n_checks, (int32_t)kMaxSources);
+    for (int32_t i = 0; i < sd.n_sources; i++) sd.sources[i] = checks[i];
+    sd.min_depth = 0.01;
+    sd.max_depth = 100.0;
+    PmDesc d;
+    // the filter reads packed maps only: images are not required
+    rc = build_desc(st, sd, 0, false, d);
+    if (rc) return rc;
+    std::vector<PmDesc> dv(1, d);
+    std::vector<uint8_t> packs(1, 1);
+    Plans plans;
+    std::vector<int64_t> key;
+    rc = upload_plans(st, dv, packs, plans, key);
+    if (rc) return rc;
+    const PmPools pools = pools_of(st);
+    HIP_TRY(timed(ctx, "pm_consistency", [&] {
+        return pm_launch_consistency(ctx->stream, plans.pack, pools, (uint32_t)std::max(needed, 0));
+    }));
+    return finish(ctx);
+}
+
+static int32_t read_layer(lr3d_stereo *st, int32_t layer, const void *src, void *dst, size_t elem,
+                          const char *what) {
+    int32_t rc = check_layer(st, layer, what);
+    if (rc) return rc;
+    lr3d_ctx *ctx = st->ctx;
+    std::lock_guard<std::mutex> lock(ctx->mu);
+    HIP_TRY(hipSetDevice(ctx->device));
+    const lr3d_view &v = st->views[layer];
+    HIP_TRY(hipMemcpyAsync(dst, (const uint8_t *)src + (size_t)st->off[layer] * elem,
+                           (size_t)v.width * v.height * elem, hipMemcpyDeviceToHost, ctx->stream));
+    return finish(ctx);
+}
+
+extern "C" int32_t lr3d_stereo_read_depth(lr3d_stereo *st, int32_t layer, uint32_t *out) {
+    if (!st || !out) return fail(LR3D_ERR_INVALID, "lr3d_stereo_read_depth: NULL argument");
+    return read_layer(st, layer, st->depth.ptr, out, 4, "lr3d_stereo_read_depth");
+}
+
+extern "C" int32_t lr3d_stereo_read_filtered(lr3d_stereo *st, int32_t layer, uint32_t *out) {
+    if (!st || !out) return fail(LR3D_ERR_INVALID, "lr3d_stereo_read_filtered: NULL argument");
+    return read_layer(st, layer, st->filt.ptr, out, 4, "lr3d_stereo_read_filtered");
+}
+
+extern "C" int32_t lr3d_stereo_read_hyp(lr3d_stereo *st, int32_t layer, float *hyp, float *cost) {
+    if (!st) return fail(LR3D_ERR_INVALID, "lr3d_stereo_read_hyp: NULL argument");
+    if (hyp) {
+        int32_t rc = read_layer(st, layer, st->hyp.ptr, hyp, 16, "lr3d_stereo_read_hyp");
+        if (rc) return rc;
+    }
+    if (cost) return read_layer(st, layer, st->cost.ptr, cost, 4, "lr3d_stereo_read_hyp");
+    return LR3D_OK;
+}
+
+extern "C" int32_t lr3d_stereo_write_depth(lr3d_stereo *st, int32_t layer, const uint32_t *packed) {
+    if (!st || !packed) return fail(LR3D_ERR_INVALID, "lr3d_stereo_write_depth: NULL argument");
+    int32_t rc = check_layer(st, layer, "lr3d_stereo_write_depth");
+    if (rc) return rc;
+    lr3d_ctx *ctx = st->ctx;
+    std::lock_guard<std::mutex> lock(ctx->mu);
+    HIP_TRY(hipSetDevice(ctx->device));
+    const lr3d_view &v = st->views[layer];
+    HIP_TRY(hipMemcpyAsync(st->depth.ptr + st->off[layer], packed, (size_t)v.width * v.height * 4,
+                           hipMemcpyHostToDevice, ctx->stream));
+    return finish(ctx);
+}
+
+extern "C" int32_t lr3d_stereo_set_coarse(lr3d_stereo *st, lr3d_stereo *coarse) {
+    if (!st) return fail(LR3D_ERR_INVALID, "lr3d_stereo_set_coarse: NULL argument");
+    if (coarse) {
+        if (coarse->ctx != st->ctx)
+            return fail(LR3D_ERR_INVALID, "lr3d_stereo_set_coarse: levels belong to different contexts");
+        if (coarse->views.size() != st->views.size())
+            return fail(LR3D_ERR_INVALID, "lr3d_stereo_set_coarse: layer counts differ (%d vs %d)",
+                        (int32_t)coarse->views.size(), (int32_t)st->views.size());
+    }
+    std::lock_guard<std::mutex> lock(st->ctx->mu);
+    st->coarse = coarse;
+    for (auto &g : st->graphs) {
+        if (g.exec) (void)hipGraphExecDestroy(g.exec);
+        if (g.graph) (void)hipGraphDestroy(g.graph);
+    }
+    st->graphs.clear();
+    return LR3D_OK;
+}
+
+// ---- TSDF ---------------------------------------------------------------------------------------------
+
+struct lr3d_tsdf {
+    lr3d_ctx *ctx = nullptr;
+    TsdfGridDev grid{};
+    int32_t dims[3] = {0, 0, 0};
+    DevBuf<int4> coords;
+    DevBuf<float> tsdf, weight;
+    DevBuf<uint32_t> vis;
+    DevBuf<TsdfView> views;
+    DevBuf<float> priors;
+    DevBuf<uint32_t> host_map;
+    DevBuf<unsigned long long> counter;
+    bool counting = false;
+};
+
+__global__ void fill_kernel(float *p, size_t n, float v) {
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = v;
+}
+
+extern "C" int32_t lr3d_tsdf_create(lr3d_ctx *ctx, const double origin[3], double voxel,
+                                    const int32_t dims[3], const int32_t *coords, int32_t n_bricks,
+                                    double truncation_voxels, lr3d_tsdf **out) {
+    if (!ctx || !origin || !dims || !coords || !out) return fail(LR3D_ERR_INVALID, "lr3d_tsdf_create: NULL argument");
+    *out = nullptr;
+    if (!(voxel > 0.0)) return fail(LR3D_ERR_INVALID, "lr3d_tsdf_create: voxel must be > 0");
+    if (!(truncation_voxels > 0.0)) return fail(LR3D_ERR_INVALID, "lr3d_tsdf_create: truncation must be > 0");
+    if (n_bricks <= 0) return fail(LR3D_ERR_INVALID, "lr3d_tsdf_create: n_bricks must be > 0");
+    if ((uint64_t)n_bricks * 512u >= (1ull << 32))
+        return fail(LR3D_ERR_INVALID, "lr3d_tsdf_create: more than 2^32 voxels");
+    for (int a = 0; a < 3; a++)
+        if (dims[a] <= 0) return fail(LR3D_ERR_INVALID, "lr3d_tsdf_create: dims must be > 0");
+    for (int32_t b = 0; b < n_bricks; b++)
+        for (int a = 0; a < 3; a++)
+            if (coords[b * 4 + a] < 0 || coords[b * 4 + a] >= dims[a])
+                return fail(LR3D_ERR_INVALID, "lr3d_tsdf_create: brick %d lies outside the grid", b);
+    std::lock_guard<std::mutex> lock(ctx->mu);
+    HIP_TRY(hipSetDevice(ctx->device));
+    lr3d_tsdf *ts = new lr3d_tsdf();
+    ts->ctx = ctx;
+    for (int a = 0; a < 3; a++) {
+        ts->dims[a] = dims[a];
+        ts->grid.origin[a] = (float)origin[a];
+        ts->grid.origin_d[a] = origin[a];
+    }
+    ts->grid.voxel = (float)voxel;
+    ts->grid.voxel_d = voxel;
+    ts->grid.mu_d = voxel * truncation_voxels;  // TV:179
+    ts->grid.mu = (float)ts->grid.mu_d;
+    ts->grid.n_bricks = n_bricks;
+    const size_t voxels = (size_t)n_bricks * 512u;
+    hipError_t e = ts->coords.alloc(n_bricks);
+    if (e == hipSuccess) e = ts->tsdf.alloc(voxels);
+    if (e == hipSuccess) e = ts->weight.alloc(voxels);
+    if (e == hipSuccess) e = ts->counter.alloc(1);
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(ts->coords.ptr, coords, (size_t)n_bricks * 16, hipMemcpyHostToDevice, ctx->stream);
+    if (e == hipSuccess) {
+        hipLaunchKernelGGL(fill_kernel, dim3((unsigned)((voxels + 255) / 256)), dim3(256), 0, ctx->stream,
+                           ts->tsdf.ptr, voxels, 1.0f);  // TV:188
+        e = hipGetLastError();
+    }
+    if (e == hipSuccess) e = hipMemsetAsync(ts->weight.ptr, 0, voxels * 4, ctx->stream);
+    if (e == hipSuccess) e = hipMemsetAsync(ts->counter.ptr, 0, 8, ctx->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+    if (e != hipSuccess) {
+        delete ts;
+        return fail(e == hipErrorOutOfMemory ? LR3D_ERR_NOMEM : LR3D_ERR_DEVICE, "lr3d_tsdf_create: %s",
+                    hipGetErrorString(e));
+    }
+    *out = ts;
+    return LR3D_OK;
+}
+
+extern "C" int32_t lr3d_tsdf_destroy(lr3d_tsdf *ts) {
+    if (!ts) return LR3D_OK;
+    lr3d_ctx *ctx = ts->ctx;
+    std::lock_guard<std::mutex> lock(ctx->mu);
+    (void)hipSetDevice(ctx->device);
+    (void)hipStreamSynchronize(ctx->stream);
+    delete ts;
+    return LR3D_OK;
+}
+
+// TV:244-253 uniform + double copies for the culling.
+static int32_t make_tsdf_view(const lr3d_integration_view &v, uint32_t map_off, uint32_t prior_off,
+                              TsdfView &o) {
+    if (v.width <= 0 || v.height <= 0 || !(v.fx > 0) || !(v.fy > 0))
+        return fail(LR3D_ERR_INVALID, "lr3d_tsdf_integrate: invalid view size or intrinsics");
+    memset(&o, 0, sizeof(o));
+    o.k[0] = (float)v.fx;
+    o.k[1] = (float)v.fy;
+    o.k[2] = (float)v.cx;
+    o.k[3] = (float)v.cy;
+    for (int i = 0; i < 9; i++) o.R[i] = (float)v.R[i];
+    for (int i = 0; i < 3; i++) o.t[i] = (float)v.t[i];
+    o.W = v.width;
+    o.H = v.height;
+    o.confidence = (float)v.confidence;
+    o.map_off = map_off;
+    o.has_prior = v.prior != nullptr;
+    o.prior_off = prior_off;
+    o.kd[0] = v.fx;
+    o.kd[1] = v.fy;
+    o.kd[2] = v.cx;
+    o.kd[3] = v.cy;
+    for (int i = 0; i < 9; i++) o.Rd[i] = v.R[i];
+    for (int i = 0; i < 3; i++) o.td[i] = v.t[i];
+    o.max_depth = v.max_depth;
+    return LR3D_OK;
+}
+
+static int32_t integrate_impl(lr3d_tsdf *ts, const uint32_t *maps, const std::vector<TsdfView> &tv,
+                              const lr3d_integration_view *views) {
+    lr3d_ctx *ctx = ts->ctx;
+    const int32_t n = (int32_t)tv.size();
+    if (n == 0) return finish(ctx);
+    const int32_t words = (n + 31) / 32;
+    // priors
+    size_t prior_total = 0;
+    for (int32_t i = 0; i < n; i++)
+        if (views[i].prior) prior_total += (size_t)views[i].width * views[i].height;
+    if (prior_total) {
+        HIP_TRY(ts->priors.reserve(prior_total));
+        for (int32_t i = 0; i < n; i++)
+            if (views[i].prior)
+                HIP_TRY(hipMemcpyAsync(ts->priors.ptr + tv[i].prior_off, views[i].prior,
+                                       (size_t)views[i].width * views[i].height * 4, hipMemcpyHostToDevice,
+                                       ctx->stream));
+    }
+    HIP_TRY(ts->views.reserve(n));
+    HIP_TRY(hipMemcpyAsync(ts->views.ptr, tv.data(), sizeof(TsdfView) * n, hipMemcpyHostToDevice, ctx->stream));
+    HIP_TRY(ts->vis.reserve((size_t)ts->grid.n_bricks * words));
+    HIP_TRY(timed(ctx, "tsdf_visibility", [&] {
+        return tsdf_launch_visibility(ctx->stream, ts->grid, ts->coords.ptr, ts->views.ptr, n, ts->vis.ptr, words);
+    }));
+    HIP_TRY(timed(ctx, "tsdf_integrate", [&] {
+        return tsdf_launch_integrate(ctx->stream, ts->grid, ts->views.ptr, n, ts->vis.ptr, words,
+                                     ts->coords.ptr, maps, ts->priors.ptr, ts->tsdf.ptr, ts->weight.ptr,
+                                     ts->counting ? ts->counter.ptr : nullptr);
+    }));
+    return finish(ctx);
+}
+
+static int32_t stereo_views(lr3d_tsdf *ts, lr3d_stereo *st, const int32_t *layers, int32_t map,
+                            const lr3d_integration_view *views, int32_t n, std::vector<TsdfView> &tv,
+                            const uint32_t **maps) {
+    if (!st) return fail(LR3D_ERR_INVALID, "lr3d_tsdf_integrate: stereo is NULL");
+    if (st->ctx != ts->ctx) return fail(LR3D_ERR_INVALID, "lr3d_tsdf_integrate: stereo and volume use different contexts");
+    if (map != LR3D_MAP_RAW && map != LR3D_MAP_FILTERED)
+        return fail(LR3D_ERR_INVALID, "lr3d_tsdf_integrate: unknown map selector %d", map);
+    *maps = map == LR3D_MAP_RAW ? st->depth.ptr : st->filt.ptr;
+    tv.resize(n);
+    uint32_t prior_off = 0;
+    for (int32_t i = 0; i < n; i++) {
+        int32_t rc = check_layer(st, layers[i], "lr3d_tsdf_integrate");
+        if (rc) return rc;
+        const lr3d_view &sv = st->views[layers[i]];
+        if (views[i].width != sv.width || views[i].height != sv.height)
+            return fail(LR3D_ERR_INVALID, "lr3d_tsdf_integrate: view %d is %dx%d but layer %d is %dx%d", i,
+                        views[i].width, views[i].height, layers[i], sv.width, sv.height);
+        rc = make_tsdf_view(views[i], st->off[layers[i]], prior_off, tv[i]);
+        if (rc) return rc;
+        if (views[i].prior) prior_off += (uint32_t)(views[i].width * views[i].height);
+    }
+    return LR3D_OK;
+}
+
+extern "C" int32_t lr3d_tsdf_integrate(lr3d_tsdf *ts, lr3d_stereo *st, int32_t layer, int32_t map,
+                                       const lr3d_integration_view *view) {
+    return lr3d_tsdf_integrate_batched(ts, st, &layer, map, view, 1);
+}
+
+extern "C" int32_t lr3d_tsdf_integrate_batched(lr3d_tsdf *ts, lr3d_stereo *st, const int32_t *layers,
+                                               int32_t map, const lr3d_integration_view *views, int32_t n) {
+    if (!ts || !layers || !views) return fail(LR3D_ERR_INVALID, "lr3d_tsdf_integrate: NULL argument");
+    if (n < 0) return fail(LR3D_ERR_INVALID, "lr3d_tsdf_integrate: n < 0");
+    std::lock_guard<std::mutex> lock(ts->ctx->mu);
+    HIP_TRY(hipSetDevice(ts->ctx->device));
+    std::vector<TsdfView> tv;
+    const uint32_t *maps = nullptr;
+    int32_t rc = stereo_views(ts, st, layers, map, views, n, tv, &maps);
+    if (rc) return rc;
+    return integrate_impl(ts, maps, tv, views);
+}
+
+extern "C" int32_t lr3d_tsdf_integrate_host(lr3d_tsdf *ts, const uint32_t *packed,
+                                            const lr3d_integration_view *view) {
+    if (!ts || !packed || !view) return fail(LR3D_ERR_INVALID, "lr3d_tsdf_integrate_host: NULL argument");
+    lr3d_ctx *ctx = ts->ctx;
+    std::lock_guard<std::mutex> lock(ctx->mu);
+    HIP_TRY(hipSetDevice(ctx->device));
+    std::vector<TsdfView> tv(1);
+    int32_t rc = make_tsdf_view(*view, 0, 0, tv[0]);
+    if (rc) return rc;
+    const size_t px = (size_t)view->width * view->height;
+    HIP_TRY(ts->host_map.reserve(px));
+    HIP_TRY(hipMemcpyAsync(ts->host_map.ptr, packed, px * 4, hipMemcpyHostToDevice, ctx->stream));
+    return integrate_impl(ts, ts->host_map.ptr, tv, view);
+}
+
+extern "C" int32_t lr3d_tsdf_read(lr3d_tsdf *ts, float *tsdf, float *weight) {
+    if (!ts) return fail(LR3D_ERR_INVALID, "lr3d_tsdf_read: NULL argument");
+    lr3d_ctx *ctx = ts->ctx;
+    std::lock_guard<std::mutex> lock(ctx->mu);
+    HIP_TRY(hipSetDevice(ctx->device));
+    const size_t bytes = (size_t)ts->grid.n_bricks * 512u * 4u;
+    if (tsdf) HIP_TRY(hipMemcpyAsync(tsdf, ts->tsdf.ptr, bytes, hipMemcpyDeviceToHost, ctx->stream));
+    if (weight) HIP_TRY(hipMemcpyAsync(weight, ts->weight.ptr, bytes, hipMemcpyDeviceToHost, ctx->stream));
+    return finish(ctx);
+}
+
+extern "C" int32_t lr3d_tsdf_set_counting(lr3d_tsdf *ts, int32_t enable) {
+    if (!ts) return fail(LR3D_ERR_INVALID, "lr3d_tsdf_set_counting: NULL argument");
+    lr3d_ctx *ctx = ts->ctx;
+    std::lock_guard<std::mutex> lock(ctx->mu);
+    HIP_TRY(hipSetDevice(ctx->device));
+    ts->counting = enable != 0;
+    HIP_TRY(hipMemsetAsync(ts->counter.ptr, 0, 8, ctx->stream));
+    return finish(ctx);
+}
+
+extern "C" int32_t lr3d_tsdf_update_count(lr3d_tsdf *ts, int64_t *out) {
+    if (!ts || !out) return fail(LR3D_ERR_INVALID, "lr3d_tsdf_update_count: NULL argument");
+    lr3d_ctx *ctx = ts->ctx;
+    std::lock_guard<std::mutex> lock(ctx->mu);
+    HIP_TRY(hipSetDevice(ctx->device));
+    unsigned long long v = 0;
+    HIP_TRY(hipMemcpyAsync(&v, ts->counter.ptr, 8, hipMemcpyDeviceToHost, ctx->stream));
+    int32_t rc = finish(ctx);
+    if (rc) return rc;
+    *out = (int64_t)v;
+    return LR3D_OK;
+}

@@ -1,0 +1,127 @@
+// Internal declarations shared by the lr3d HIP translation units.
+//
+// Device data layout (all views of a stereo set share one pool per kind, each layer at its own
+// pixel offset `off[layer]` = sum of the previous layers' width*height):
+//   quad   u32 per pixel: the 2x2 block I(x,y) | I(x+1,y)<<8 | I(x,y+1)<<16 | I(x+1,y+1)<<24,
+//          clamped to the image, so one 32-bit gather feeds one bilinear tap;
+//   hyp    float4 per pixel (normal xyz, depth), cost f32 per pixel;
+//   depth  packed u32 per pixel (f16 depth | 6+6-bit octahedral normal | 4-bit cost);
+//   filt   packed u32 per pixel (consistency-filtered depth);
+//   prior  f32 per pixel (only for solves that pass one).
+// TSDF: brick pool, voxel (x, y, z) of pool brick b at b*512 + x + 8y + 64z, tsdf and weight in
+// two f32 arrays (the reference layout, consumed unchanged by the surface-nets mesher).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include <string>
+
+namespace lr3d {
+
+constexpr int kMaxSources = 8;
+constexpr int kSamples = 36;
+
+// One source view of a reference view's solve (the WGSL `View`, PM:4-10, after DS:391-399).
+struct PmSrc {
+    float k[4];     // fx, fy, cx, cy
+    float R[9];     // relative rotation ref -> src, row-major
+    float t[3];
+    int32_t W, H;
+    uint32_t off;   // pixel offset of the source layer in every pool
+    int32_t layer;
+};
+
+// One reference view's solve (the WGSL `Params`, PM:12-26, minus the per-launch state).
+struct PmDesc {
+    int32_t W, H, ref, n_src;
+    float k[4];
+    float ikx, iky, kx2, ky2;  // 1/fx, 1/fy, -cx/fx, -cy/fy of the reference (f32 divisions)
+    float inv_min, inv_max, geo_weight;
+    uint32_t flags;            // bit 0: geometric pass
+    int32_t topk;              // effective k (1..n_src)
+    int32_t iterations;
+    uint32_t seed_base;        // (ref + 1) * 0x9e3779b1 + 7 * flags  (DS:379)
+    uint32_t off;              // pixel offset of the reference layer
+    int32_t has_prior;
+    uint32_t prior_off;
+    int32_t from_coarse;
+    int32_t coarse_W, coarse_H;
+    uint32_t coarse_off;       // pixel offset of this layer in the coarse stereo's hyp pool
+    float ck[4];               // coarse intrinsics (scaleIntrinsics, CAM:28-36)
+    int32_t tiles_x;           // tiles per row for the launch's tile shape (set per launch plan)
+    int32_t pad_;
+    PmSrc src[kMaxSources];
+};
+
+// Pools a PatchMatch launch reads and writes.
+struct PmPools {
+    const uint32_t *quad;
+    float4 *hyp;
+    float *cost;
+    uint32_t *depth;
+    uint32_t *filt;
+    const float *prior;
+    const float4 *coarse_hyp;
+};
+
+// Launch plan: view v owns logical blocks [block_start[v], block_start[v+1]).
+struct PmPlan {
+    const PmDesc *descs;        // device
+    const uint32_t *block_start;  // device, n+1 entries
+    int32_t n;
+    uint32_t blocks;            // total
+    int32_t tx, ty;             // threads per block (x, y)
+    int32_t kmax;               // largest top-k of the batch (selects the 3- or 8-slot network)
+    int32_t geometric;          // any solve of the batch is a geometric pass
+};
+
+// One view of a TSDF integration pass (the 128-byte uniform of TV:244-253 plus culling data).
+struct TsdfView {
+    float k[4];
+    float R[9];
+    float t[3];
+    int32_t W, H;
+    float confidence;
+    uint32_t map_off;   // word offset of the packed map
+    int32_t has_prior;
+    uint32_t prior_off;
+    double kd[4];       // double copies for the brick culling (TV:220-234 runs in double)
+    double Rd[9];
+    double td[3];
+    double max_depth;
+};
+
+struct TsdfGridDev {
+    float origin[3];
+    float voxel;
+    float mu;
+    int32_t n_bricks;
+    double origin_d[3];
+    double voxel_d;
+    double mu_d;
+};
+
+// Kernel launchers (stream-ordered; no synchronisation).
+hipError_t pm_build_quad(hipStream_t s, const uint8_t *gray, int32_t W, int32_t H, uint32_t *quad);
+hipError_t pm_launch_init(hipStream_t s, const PmPlan &plan, const PmPools &pools);
+hipError_t pm_launch_propagate(hipStream_t s, const PmPlan &plan, const PmPools &pools,
+                               uint32_t iteration, uint32_t phase, float step);
+hipError_t pm_launch_pack(hipStream_t s, const PmPlan &plan, const PmPools &pools);
+hipError_t pm_launch_consistency(hipStream_t s, const PmPlan &plan, const PmPools &pools,
+                                 uint32_t needed);
+hipError_t tsdf_launch_visibility(hipStream_t s, const TsdfGridDev &g, const int4 *coords,
+                                  const TsdfView *views, int32_t n_views, uint32_t *vis,
+                                  int32_t words);
+hipError_t tsdf_launch_integrate(hipStream_t s, const TsdfGridDev &g, const TsdfView *views,
+                                 int32_t n_views, const uint32_t *vis, int32_t words,
+                                 const int4 *coords, const uint32_t *maps, const float *priors,
+                                 float *tsdf, float *weight, unsigned long long *counter);
+
+// Launch shapes (x threads, y threads) picked per (kernel, image size).
+struct TileShape {
+    int32_t tx, ty;
+};
+TileShape pm_tile_for(const char *kernel, int32_t W, int32_t H, int32_t n_src);
+
+}  // namespace lr3d

@@ -1,0 +1,209 @@
+// TSDF depth-map integration kernels for gfx950.
+//
+// Semantics follow the reference's `integrate` WGSL kernel and its host culling
+// (frontend/src/app/vision/fusion/tsdf-volume.ts, cited TV:<line>) under the FP32 arithmetic
+// definition of the PatchMatch kernels (no contraction, fmaf only where written, IEEE division
+// and sqrt, ties-to-even rounding); the brick culling runs in double like the reference's
+// JavaScript.
+//
+// Layout and mapping: the pool keeps the reference's brick layout (voxel (x, y, z) of brick b at
+// b*512 + x + 8y + 64z).  One 64-lane wavefront owns one brick: lane = x + 8y, and each lane walks
+// its z-column, so every z step is one contiguous 256-byte access per array.  A pass loads a
+// brick's 8 tsdf/weight values per lane once, applies every visible view in order in registers,
+// and stores the voxels that changed: per voxel the update sequence is exactly the reference's
+// one-view-at-a-time sequence, so one pass over n views equals n single-view passes.
+#include "lr3d_internal.h"
+
+#include <math.h>
+
+namespace lr3d {
+
+namespace {
+
+constexpr float kMaxWeight = 96.0f;   // TV:7
+constexpr float kPriorWeight = 0.12f; // TV:8
+
+__device__ __forceinline__ float f16_bits_to_f32(uint32_t bits) {
+    return (float)__builtin_bit_cast(_Float16, (uint16_t)(bits & 0xffffu));
+}
+
+// PK:7-15 (same arithmetic as the PatchMatch kernels)
+__device__ __forceinline__ void unpack_normal(uint32_t packed, float &nx, float &ny, float &nz) {
+    float ex = ((float)((packed >> 16u) & 63u) / 63.0f) * 2.0f - 1.0f;
+    float ey = ((float)((packed >> 22u) & 63u) / 63.0f) * 2.0f - 1.0f;
+    nx = ex;
+    ny = ey;
+    nz = (1.0f - fabsf(ex)) - fabsf(ey);
+    if (nz < 0.0f) {
+        float sx = nx >= 0.0f ? 1.0f : -1.0f;
+        float sy = ny >= 0.0f ? 1.0f : -1.0f;
+        float ux = (1.0f - fabsf(ny)) * sx;
+        float uy = (1.0f - fabsf(nx)) * sy;
+        nx = ux;
+        ny = uy;
+    }
+    float inv = 1.0f / sqrtf(nx * nx + ny * ny + nz * nz);
+    nx = nx * inv;
+    ny = ny * inv;
+    nz = nz * inv;
+}
+
+// TV:220-234: one thread per (brick, 32-view word); bit j of the word = view 32*word + j visible.
+__global__ void __launch_bounds__(256) visibility_kernel(TsdfGridDev g, const int4 *__restrict__ coords,
+                                                         const TsdfView *__restrict__ views,
+                                                         int32_t n_views, uint32_t *__restrict__ vis,
+                                                         int32_t words) {
+    const uint32_t gid = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t brick = gid / (uint32_t)words;
+    const int32_t word = (int32_t)(gid % (uint32_t)words);
+    if (brick >= (uint32_t)g.n_bricks) return;
+    const int4 c = coords[brick];
+    const double side = g.voxel_d * 8.0;
+    const double radius = g.voxel_d * 8.0 * 0.87 + g.mu_d;
+    const double cx = g.origin_d[0] + ((double)c.x + 0.5) * side;
+    const double cy = g.origin_d[1] + ((double)c.y + 0.5) * side;
+    const double cz = g.origin_d[2] + ((double)c.z + 0.5) * side;
+    uint32_t bits = 0;
+    const int32_t v_end = min(n_views, word * 32 + 32);
+    for (int32_t v = word * 32; v < v_end; v++) {
+        const TsdfView &V = views[v];
+        const double p0 = (V.Rd[0] * cx + V.Rd[1] * cy + V.Rd[2] * cz) + V.td[0];
+        const double p1 = (V.Rd[3] * cx + V.Rd[4] * cy + V.Rd[5] * cz) + V.td[1];
+        const double p2 = (V.Rd[6] * cx + V.Rd[7] * cy + V.Rd[8] * cz) + V.td[2];
+        const double u = (V.kd[0] * p0) / p2 + V.kd[2];
+        const double w = (V.kd[1] * p1) / p2 + V.kd[3];
+        const double z = p2;
+        if (z < 0.02 - radius || z > V.max_depth + radius) continue;
+        const double margin = (V.kd[0] * radius) / fmax(0.02, z);
+        if (u < -margin || w < -margin || u > (double)V.W + margin || w > (double)V.H + margin) continue;
+        bits |= 1u << (v - word * 32);
+    }
+    vis[(size_t)brick * words + word] = bits;
+}
+
+// TV:103-154, one wavefront per brick, all visible views of the pass in order.
+template <bool COUNT>
+__global__ void __launch_bounds__(256) integrate_kernel(TsdfGridDev g, const TsdfView *__restrict__ views,
+                                                        int32_t n_views, const uint32_t *__restrict__ vis,
+                                                        int32_t words, const int4 *__restrict__ coords,
+                                                        const uint32_t *__restrict__ maps,
+                                                        const float *__restrict__ priors,
+                                                        float *__restrict__ tsdf, float *__restrict__ weight,
+                                                        unsigned long long *__restrict__ counter) {
+    const uint32_t brick = blockIdx.x * (blockDim.x / 64u) + (threadIdx.x >> 6);
+    const uint32_t lane = threadIdx.x & 63u;
+    if (brick >= (uint32_t)g.n_bricks) return;
+    const int4 c = coords[brick];
+    const int32_t vx = (int32_t)(lane & 7u), vy = (int32_t)(lane >> 3);
+    const float wx = g.origin[0] + ((float)(c.x * 8 + vx) + 0.5f) * g.voxel;
+    const float wy = g.origin[1] + ((float)(c.y * 8 + vy) + 0.5f) * g.voxel;
+    float wz[8];
+#pragma unroll
+    for (int z = 0; z < 8; z++) wz[z] = g.origin[2] + ((float)(c.z * 8 + z) + 0.5f) * g.voxel;
+    const float mu = g.mu;
+    const size_t base = (size_t)brick * 512u + lane;
+
+    float ts[8], ws[8];
+    uint32_t changed = 0;  // bit z: voxel z of this lane was updated
+    bool loaded = false;
+    uint32_t updates = 0;
+    for (int32_t word = 0; word < words; word++) {
+        uint32_t bits = vis[(size_t)brick * words + word];
+        while (bits) {
+            const int32_t v = word * 32 + __builtin_ctz(bits);
+            bits &= bits - 1u;
+            if (!loaded) {
+#pragma unroll
+                for (int z = 0; z < 8; z++) {
+                    ts[z] = tsdf[base + 64u * z];
+                    ws[z] = weight[base + 64u * z];
+                }
+                loaded = true;
+            }
+            const TsdfView &V = views[v];
+            const uint32_t *__restrict__ map = maps + V.map_off;
+            // cam = R world + t, row i = fmaf(R_i2, z, fmaf(R_i1, y, fmaf(R_i0, x, t_i)))
+            const float c0 = fmaf(V.R[1], wy, fmaf(V.R[0], wx, V.t[0]));
+            const float c1 = fmaf(V.R[4], wy, fmaf(V.R[3], wx, V.t[1]));
+            const float c2 = fmaf(V.R[7], wy, fmaf(V.R[6], wx, V.t[2]));
+#pragma unroll
+            for (int z = 0; z < 8; z++) {
+                const float cam0 = fmaf(V.R[2], wz[z], c0);
+                const float cam1 = fmaf(V.R[5], wz[z], c1);
+                const float cam2 = fmaf(V.R[8], wz[z], c2);
+                if (!(cam2 > 0.01f)) continue;
+                const int32_t px = (int32_t)rintf((V.k[0] * cam0) / cam2 + V.k[2]);
+                const int32_t py = (int32_t)rintf((V.k[1] * cam1) / cam2 + V.k[3]);
+                if (px < 0 || py < 0 || px >= V.W || py >= V.H) continue;
+                const uint32_t i = (uint32_t)py * (uint32_t)V.W + (uint32_t)px;
+                const uint32_t packed = map[i];
+                float d = f16_bits_to_f32(packed);
+                const bool from_map = d > 0.0f;
+                if (!from_map) d = V.has_prior ? priors[V.prior_off + i] : 0.0f;
+                if (!(d > 0.0f)) continue;
+                const float sdf = d - cam2;
+                if (sdf < -mu) continue;
+                // The weight only matters for an update, so it is evaluated after the skips.
+                float w;
+                if (from_map) {
+                    float nx, ny, nz;
+                    unpack_normal(packed, nx, ny, nz);
+                    const float inv = 1.0f / sqrtf(cam0 * cam0 + cam1 * cam1 + cam2 * cam2);
+                    w = fmaxf(0.2f, fabsf(nx * (cam0 * inv) + ny * (cam1 * inv) + nz * (cam2 * inv)));
+                    if (V.confidence > 0.0f) {
+                        const float cc = (float)(packed >> 28u) / 8.0f;
+                        const float r = fminf(fmaxf(V.confidence / d, 0.5f), 2.0f);
+                        w = w * ((fmaxf(0.15f, (1.0f - cc) * (1.0f - cc)) * r) * r);
+                    }
+                } else {
+                    w = kPriorWeight;
+                }
+                const float observed = fminf(1.0f, sdf / mu);
+                const float w0 = ws[z];
+                ts[z] = (ts[z] * w0 + observed * w) / (w0 + w);
+                ws[z] = fminf(w0 + w, kMaxWeight);
+                changed |= 1u << z;
+                if (COUNT) updates++;
+            }
+        }
+    }
+#pragma unroll
+    for (int z = 0; z < 8; z++) {
+        if (changed & (1u << z)) {
+            tsdf[base + 64u * z] = ts[z];
+            weight[base + 64u * z] = ws[z];
+        }
+    }
+    if (COUNT) {
+        // wavefront sum, one atomic per wavefront
+        for (int offset = 32; offset > 0; offset >>= 1) updates += __shfl_down(updates, offset, 64);
+        if (lane == 0 && updates) atomicAdd(counter, (unsigned long long)updates);
+    }
+}
+
+}  // namespace
+
+hipError_t tsdf_launch_visibility(hipStream_t s, const TsdfGridDev &g, const int4 *coords,
+                                  const TsdfView *views, int32_t n_views, uint32_t *vis,
+                                  int32_t words) {
+    const uint32_t threads = (uint32_t)g.n_bricks * (uint32_t)words;
+    dim3 block(256), grid((threads + 255u) / 256u);
+    hipLaunchKernelGGL(visibility_kernel, grid, block, 0, s, g, coords, views, n_views, vis, words);
+    return hipGetLastError();
+}
+
+hipError_t tsdf_launch_integrate(hipStream_t s, const TsdfGridDev &g, const TsdfView *views,
+                                 int32_t n_views, const uint32_t *vis, int32_t words,
+                                 const int4 *coords, const uint32_t *maps, const float *priors,
+                                 float *tsdf, float *weight, unsigned long long *counter) {
+    dim3 block(256), grid(((uint32_t)g.n_bricks + 3u) / 4u);
+    if (counter)
+        hipLaunchKernelGGL((integrate_kernel<true>), grid, block, 0, s, g, views, n_views, vis, words,
+                           coords, maps, priors, tsdf, weight, counter);
+    else
+        hipLaunchKernelGGL((integrate_kernel<false>), grid, block, 0, s, g, views, n_views, vis, words,
+                           coords, maps, priors, tsdf, weight, counter);
+    return hipGetLastError();
+}
+
+}  // namespace lr3d

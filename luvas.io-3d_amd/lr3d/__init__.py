@@ -1,0 +1,17 @@
+"""lr3d: MI355X (gfx950) PatchMatch MVS + TSDF integration behind the C ABI of include/lr3d.h.
+
+Host-side mirror of the reference's stage API (frontend/src/app/vision/mvs/dense-stereo.ts,
+frontend/src/app/vision/fusion/tsdf-volume.ts, frontend/src/app/vision/mvs/methods/):
+``HipDenseStereo``, ``HipTsdfVolume``, ``BrickGrid`` and the ``STEREO_METHODS`` registry.
+"""
+from ._native import Lr3dError, MAP_FILTERED, MAP_RAW, MAX_SOURCES
+from .camera import Intrinsics, Pose, StereoView, look_at, relative_pose, scale_intrinsics
+from .runtime import Context
+from .stereo import HipDenseStereo, SolveOptions
+from .tsdf import BRICK, BrickGrid, HipTsdfVolume, IntegrationView, dense_grid
+
+__all__ = [
+    "Lr3dError", "MAP_FILTERED", "MAP_RAW", "MAX_SOURCES", "Intrinsics", "Pose", "StereoView",
+    "look_at", "relative_pose", "scale_intrinsics", "Context", "HipDenseStereo", "SolveOptions",
+    "BRICK", "BrickGrid", "HipTsdfVolume", "IntegrationView", "dense_grid",
+]

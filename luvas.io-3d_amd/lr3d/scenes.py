@@ -1,0 +1,235 @@
+"""Seeded synthetic workloads with analytic ground truth (the scene specification of every
+benchmark and test configuration).
+
+Scenes are rendered by libscenegen (csrc/scenegen.c): u8 gray frames, ground-truth camera-z depth,
+and exact signed distances.  Seeds drive mulberry32 (frontend/src/app/vision/math/dense.ts:92-101).
+
+Scene choices that the workload descriptions leave open, fixed here:
+  * sphere scene: a sphere of radius 0.5 m (unit diameter, so that it fits the 1.024 m TSDF cubes
+    the same workloads fix) centred at the origin, resting on a 2 x 2 m ground rectangle at
+    y = -0.5; cameras look at the origin from an orbit at 0.4 m height, world +y up;
+  * the TSDF cubes are centred on the origin;
+  * room: the inside of a 4 x 2.5 x 3 m box (x, y, z), floor at y = 0, with 12 seeded
+    axis-aligned boxes standing on the floor; cameras at 1.2-1.8 m height inside the room look
+    at seeded points on the far walls.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import math
+import os
+from dataclasses import dataclass, field
+from typing import List, Optional
+
+import numpy as np
+
+from .camera import Intrinsics, Pose, StereoView, look_at
+
+LIB_PATH = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "lib",
+                        "libscenegen.so")
+
+SG_SPHERE, SG_BOX, SG_RECT_Y, SG_ROOM = 0, 1, 2, 3
+
+
+class _Prim(C.Structure):
+    _fields_ = [("kind", C.c_int32), ("a", C.c_double * 3), ("b", C.c_double * 3), ("r", C.c_double)]
+
+
+class _Scene(C.Structure):
+    _fields_ = [("n_prims", C.c_int32), ("prims", _Prim * 64), ("noise_offset", C.c_double * 3),
+                ("freq", C.c_double), ("sky", C.c_double)]
+
+
+_lib = None
+
+
+def _load():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise FileNotFoundError(f"{LIB_PATH} is missing: run `make -C luvas.io-3d_amd/csrc`")
+        lib = C.CDLL(LIB_PATH)
+        lib.sg_render.restype = C.c_int32
+        lib.sg_render.argtypes = [C.POINTER(_Scene), C.c_int32, C.c_int32, C.c_double, C.c_double,
+                                  C.c_double, C.c_double, C.POINTER(C.c_double),
+                                  C.POINTER(C.c_double), C.c_void_p, C.c_void_p]
+        lib.sg_signed_distance.restype = C.c_int32
+        lib.sg_signed_distance.argtypes = [C.POINTER(_Scene), C.c_void_p, C.c_int64, C.c_void_p]
+        _lib = lib
+    return _lib
+
+
+def mulberry32(seed: int):
+    """frontend/src/app/vision/math/dense.ts:92-101: deterministic PRNG returning [0, 1)."""
+    state = seed & 0xFFFFFFFF
+
+    def imul(a, b):
+        return (a * b) & 0xFFFFFFFF
+
+    def next_value() -> float:
+        nonlocal state
+        state = (state + 0x6D2B79F5) & 0xFFFFFFFF
+        t = state
+        t = imul(t ^ (t >> 15), t | 1)
+        t ^= (t + imul(t ^ (t >> 7), t | 61)) & 0xFFFFFFFF
+        return ((t ^ (t >> 14)) & 0xFFFFFFFF) / 4294967296.0
+
+    return next_value
+
+
+@dataclass
+class Primitive:
+    kind: int
+    a: tuple = (0.0, 0.0, 0.0)
+    b: tuple = (0.0, 0.0, 0.0)
+    r: float = 0.0
+
+
+@dataclass
+class SceneSpec:
+    prims: List[Primitive]
+    seed: int
+    freq: float = 1.0
+    sky: float = 200.0
+
+    def _struct(self) -> _Scene:
+        s = _Scene()
+        s.n_prims = len(self.prims)
+        for i, p in enumerate(self.prims):
+            s.prims[i].kind = p.kind
+            s.prims[i].a[:] = list(p.a)
+            s.prims[i].b[:] = list(p.b)
+            s.prims[i].r = p.r
+        rnd = mulberry32(self.seed)
+        s.noise_offset[:] = [rnd() * 100.0, rnd() * 100.0, rnd() * 100.0]
+        s.freq = self.freq
+        s.sky = self.sky
+        return s
+
+    def render(self, view: StereoView, gray: bool = True, depth: bool = True):
+        lib = _load()
+        s = self._struct()
+        k = view.intrinsics
+        R = (C.c_double * 9)(*np.asarray(view.pose.R, dtype=np.float64).reshape(9))
+        t = (C.c_double * 3)(*np.asarray(view.pose.t, dtype=np.float64).reshape(3))
+        g = np.empty(view.height * view.width, dtype=np.uint8) if gray else None
+        d = np.empty(view.height * view.width, dtype=np.float32) if depth else None
+        lib.sg_render(C.byref(s), view.width, view.height, k.fx, k.fy, k.cx, k.cy, R, t,
+                      C.c_void_p(g.ctypes.data) if g is not None else None,
+                      C.c_void_p(d.ctypes.data) if d is not None else None)
+        return g, d
+
+    def signed_distance(self, points: np.ndarray) -> np.ndarray:
+        lib = _load()
+        s = self._struct()
+        p = np.ascontiguousarray(points, dtype=np.float64).reshape(-1, 3)
+        out = np.empty(len(p), dtype=np.float64)
+        lib.sg_signed_distance(C.byref(s), C.c_void_p(p.ctypes.data), len(p), C.c_void_p(out.ctypes.data))
+        return out
+
+
+@dataclass
+class TsdfSpec:
+    origin: tuple
+    voxel: float
+    voxels: int           # cube side in voxels
+    truncation_voxels: float
+
+
+@dataclass
+class Workload:
+    """One benchmark / test configuration: cameras, frames, truth, PatchMatch and TSDF settings."""
+    name: str
+    scene: SceneSpec
+    views: List[StereoView]
+    sources: List[List[int]]
+    ranges: List[tuple]
+    iterations: int
+    topk: int
+    tsdf: TsdfSpec
+    images: List[np.ndarray] = field(default_factory=list)
+    truth: List[np.ndarray] = field(default_factory=list)
+
+    def render(self, layers: Optional[List[int]] = None) -> "Workload":
+        """Render frames and ground-truth depth (all layers, or the given ones)."""
+        layers = range(len(self.views)) if layers is None else layers
+        if not self.images:
+            self.images = [None] * len(self.views)
+            self.truth = [None] * len(self.views)
+        for i in layers:
+            g, d = self.scene.render(self.views[i])
+            self.images[i] = g.reshape(self.views[i].height, self.views[i].width)
+            self.truth[i] = d.reshape(self.views[i].height, self.views[i].width)
+        return self
+
+    @property
+    def megapixels(self) -> float:
+        return sum(v.width * v.height for v in self.views) / 1e6
+
+    def hypotheses_per_pixel(self) -> int:
+        """Nominal hypotheses per pixel: 1 (init) + 14 per iteration (8 propagated + 6 refined)."""
+        return 1 + 14 * self.iterations
+
+
+def sphere_scene(seed: int, freq: float = 1.0) -> SceneSpec:
+    """Sphere on a ground rectangle.  `freq` scales the noise frequencies so that the finest
+    octave stays above ~2 pixels at the workload's resolution (no aliasing across views)."""
+    return SceneSpec([Primitive(SG_SPHERE, (0.0, 0.0, 0.0), r=0.5),
+                      Primitive(SG_RECT_Y, (-1.0, -0.5, -1.0), (1.0, 0.0, 1.0))], seed=seed,
+                     freq=freq)
+
+
+def _ring_sources(n_views: int, n_sources: int) -> List[List[int]]:
+    """Nearest views on a ring: +1, -1, +2, -2, ..."""
+    out = []
+    for i in range(n_views):
+        src = []
+        step = 1
+        while len(src) < n_sources:
+            for s in (step, -step):
+                if len(src) < n_sources:
+                    src.append((i + s) % n_views)
+            step += 1
+        out.append(src)
+    return out
+
+
+def sphere_small(seed: int = 1) -> Workload:
+    """Plain method, small, CPU-runnable: 8 views 320x240 on a 1.5 m circle at 0.4 m height,
+    f = 288, depth 0.5-3 m; 4 neighbours, top-2, 4 iterations; TSDF 128^3 x 8 mm, truncation
+    3 voxels."""
+    n, W, H = 8, 320, 240
+    k = Intrinsics(288.0, 288.0, (W - 1) / 2, (H - 1) / 2)
+    views = []
+    for i in range(n):
+        a = 2 * math.pi * i / n
+        pose = look_at((1.5 * math.sin(a), 0.4, 1.5 * math.cos(a)))
+        views.append(StereoView(k, pose, W, H))
+    return Workload("sphere-8x320x240", sphere_scene(seed, freq=0.35), views, _ring_sources(n, 4),
+                    [(0.5, 3.0)] * n, iterations=4, topk=2,
+                    tsdf=TsdfSpec((-0.512, -0.512, -0.512), 0.008, 128, 3.0))
+
+
+def sphere_phone(seed: int = 2, n: int = 32) -> Workload:
+    """Plain method at phone scale: 32 views 1280x720 on a seeded jittered orbit (radius
+    1.2-1.8 m), f = 1000, depth 0.3-4 m; 8 neighbours, top-3, 8 iterations; TSDF 256^3 x 4 mm,
+    truncation 4 voxels."""
+    W, H = 1280, 720
+    k = Intrinsics(1000.0, 1000.0, (W - 1) / 2, (H - 1) / 2)
+    rnd = mulberry32(seed * 7919 + 13)
+    views = []
+    for i in range(n):
+        radius = 1.2 + 0.6 * rnd()
+        a = 2 * math.pi * (i + 0.6 * (rnd() - 0.5)) / n
+        height = 0.4 + 0.2 * (rnd() - 0.5)
+        pose = look_at((radius * math.sin(a), height, radius * math.cos(a)))
+        views.append(StereoView(k, pose, W, H))
+    return Workload(f"sphere-{n}x1280x720", sphere_scene(seed), views, _ring_sources(n, 8),
+                    [(0.3, 4.0)] * n, iterations=8, topk=3,
+                    tsdf=TsdfSpec((-0.512, -0.512, -0.512), 0.004, 256, 4.0))
+
+
+WORKLOADS = {
+    "sphere-small": sphere_small,
+    "sphere-phone": sphere_phone,
+}

@@ -1,0 +1,159 @@
+"""HipDenseStereo: the reference's DenseStereo (frontend/src/app/vision/mvs/dense-stereo.ts,
+DS:107-417) on the lr3d C ABI.
+
+Same members and argument meaning: ``set_image(layer, gray)``, ``solve(ref, options)``,
+``filter(ref, sources, needed)``, ``filtered_map``, ``depth_map``, ``hypotheses``,
+``destroy``; plus ``solve_batched`` (every view in shared launches, replayed from a HIP graph).
+Per-view image sizes are allowed (mixed-resolution batches).
+"""
+from __future__ import annotations
+
+import ctypes as C
+from dataclasses import dataclass, field
+from typing import Optional, Sequence
+
+import numpy as np
+
+from . import _native as N
+from .camera import StereoView
+from .runtime import Context
+
+
+@dataclass
+class SolveOptions:
+    """DS:62-75 SolveOptions, plus the top-k override (0 = the reference rule)."""
+    sources: Sequence[int]
+    min_depth: float
+    max_depth: float
+    prior: Optional[np.ndarray] = None
+    geometric: bool = False
+    iterations: int = 5
+    from_coarse: bool = False
+    pack: bool = True
+    topk: int = 0
+
+
+def _view_struct(v: StereoView) -> N.View:
+    s = N.View()
+    s.width, s.height = int(v.width), int(v.height)
+    k = v.intrinsics
+    s.fx, s.fy, s.cx, s.cy = k.fx, k.fy, k.cx, k.cy
+    s.R[:] = [float(x) for x in np.asarray(v.pose.R, dtype=np.float64).reshape(9)]
+    s.t[:] = [float(x) for x in np.asarray(v.pose.t, dtype=np.float64).reshape(3)]
+    return s
+
+
+class HipDenseStereo:
+    """GPU PatchMatch for every keyframe (DS:102-107): one device pool per buffer kind holds every
+    view, so the geometric pass and the consistency filter read any neighbour without copies."""
+
+    def __init__(self, ctx: Context, views: Sequence[StereoView]):
+        self._lib = N.load()
+        self.ctx = ctx
+        self.views = list(views)
+        arr = (N.View * len(self.views))(*[_view_struct(v) for v in self.views])
+        handle = C.c_void_p()
+        N.check(self._lib.lr3d_stereo_create(ctx.handle, arr, len(self.views), 0, C.byref(handle)))
+        self.handle = handle
+        self._keep = []
+
+    # DS:126-133 exposes width/height of the (single-size) reference set.
+    @property
+    def width(self) -> int:
+        return self.views[0].width
+
+    @property
+    def height(self) -> int:
+        return self.views[0].height
+
+    def size(self, layer: int) -> tuple[int, int]:
+        return self.views[layer].width, self.views[layer].height
+
+    def set_image(self, layer: int, gray: np.ndarray) -> None:
+        """DS:221-228"""
+        w, h = self.size(layer)
+        g = np.ascontiguousarray(gray, dtype=np.uint8).reshape(h, w)
+        N.check(self._lib.lr3d_stereo_set_image(self.handle, layer, N.ptr(g), w, h))
+
+    def _desc(self, ref: int, o: SolveOptions):
+        d = N.SolveDesc()
+        d.ref = ref
+        sources = list(o.sources)[: N.MAX_SOURCES]  # DS:232
+        d.n_sources = len(sources)
+        for i, s in enumerate(sources):
+            d.sources[i] = int(s)
+        d.min_depth, d.max_depth = float(o.min_depth), float(o.max_depth)
+        prior = None
+        if o.prior is not None:
+            w, h = self.size(ref)
+            prior = np.ascontiguousarray(o.prior, dtype=np.float32).reshape(h * w)
+        d.prior = N.float_ptr(prior)
+        d.geometric = 1 if o.geometric else 0
+        d.iterations = int(o.iterations)
+        d.from_coarse = 1 if o.from_coarse else 0
+        d.pack = 1 if o.pack else 0
+        d.topk = int(o.topk)
+        return d, prior
+
+    def solve(self, ref: int, options: SolveOptions) -> None:
+        """DS:231-285: one launch per kernel per red-black phase."""
+        if len(options.sources) == 0:
+            return  # DS:233
+        d, keep = self._desc(ref, options)
+        N.check(self._lib.lr3d_stereo_solve(self.handle, C.byref(d)))
+
+    def solve_batched(self, solves: Sequence[tuple[int, SolveOptions]]) -> None:
+        """All (ref, options) pairs in shared launches; equals the solves one by one."""
+        descs, keep = [], []
+        for ref, o in solves:
+            d, k = self._desc(ref, o)
+            descs.append(d)
+            keep.append(k)
+        arr = (N.SolveDesc * len(descs))(*descs)
+        N.check(self._lib.lr3d_stereo_solve_batched(self.handle, arr, len(descs)))
+
+    def filter(self, ref: int, sources: Sequence[int], needed: int) -> None:
+        """DS:296-309: consistency check of `ref` into its filtered map (stays on the GPU)."""
+        checked = list(sources)[: N.MAX_SOURCES]
+        arr = (C.c_int32 * max(1, len(checked)))(*checked)
+        N.check(self._lib.lr3d_stereo_filter(self.handle, ref, arr, len(checked), int(needed)))
+
+    def filtered_map(self, ref: int, sources: Sequence[int], needed: int) -> np.ndarray:
+        """DS:347-352"""
+        self.filter(ref, sources, needed)
+        return self.read_filtered(ref)
+
+    def read_filtered(self, ref: int) -> np.ndarray:
+        w, h = self.size(ref)
+        out = np.empty(w * h, dtype=np.uint32)
+        N.check(self._lib.lr3d_stereo_read_filtered(self.handle, ref, N.ptr(out)))
+        return out
+
+    def depth_map(self, ref: int) -> np.ndarray:
+        """DS:355-358: packed u32 map of a layer."""
+        w, h = self.size(ref)
+        out = np.empty(w * h, dtype=np.uint32)
+        N.check(self._lib.lr3d_stereo_read_depth(self.handle, ref, N.ptr(out)))
+        return out
+
+    def write_depth(self, ref: int, packed: np.ndarray) -> None:
+        w, h = self.size(ref)
+        p = np.ascontiguousarray(packed, dtype=np.uint32).reshape(w * h)
+        N.check(self._lib.lr3d_stereo_write_depth(self.handle, ref, N.ptr(p)))
+
+    def hypotheses(self, ref: int) -> tuple[np.ndarray, np.ndarray]:
+        """DS:288-290: planes (n.xyz, d) and costs of a layer's last solve."""
+        w, h = self.size(ref)
+        hyp = np.empty((w * h, 4), dtype=np.float32)
+        cost = np.empty(w * h, dtype=np.float32)
+        N.check(self._lib.lr3d_stereo_read_hyp(self.handle, ref, N.ptr(hyp), N.ptr(cost)))
+        return hyp, cost
+
+    def set_coarse(self, coarse: Optional["HipDenseStereo"]) -> None:
+        N.check(self._lib.lr3d_stereo_set_coarse(self.handle, coarse.handle if coarse else None))
+
+    def destroy(self) -> None:
+        """DS:403-416"""
+        if self.handle:
+            self._lib.lr3d_stereo_destroy(self.handle)
+            self.handle = None

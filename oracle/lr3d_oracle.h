@@ -1,0 +1,175 @@
+/*
+ * CPU oracle for the dense-reconstruction hot path (PatchMatch MVS + TSDF integration).
+ *
+ * TEST INFRASTRUCTURE ONLY.  This is a plain-C restatement of the reference's WGSL kernels and
+ * their host drivers, used as the parity checker for the HIP library and as the timed CPU
+ * baseline.  Nothing in the product library links, loads or calls it.
+ *
+ * Each function cites the reference file:line it restates (paths relative to the reference
+ * repository root):
+ *   PM = frontend/src/app/vision/mvs/patchmatch.wgsl.ts
+ *   PK = frontend/src/app/vision/mvs/depth-packing.wgsl.ts
+ *   DS = frontend/src/app/vision/mvs/dense-stereo.ts
+ *   TV = frontend/src/app/vision/fusion/tsdf-volume.ts
+ *   CAM = frontend/src/app/vision/geometry/camera.ts
+ *
+ * Arithmetic definition.  WGSL leaves the precision of division, sqrt, exp/sin/cos, texture
+ * filtering and FMA contraction to the implementation, so the reference itself has no single
+ * bit pattern.  The oracle fixes one (and the HIP kernels follow the same one):
+ *   - FP32, no contraction (-ffp-contract=off); fmaf() only where written out;
+ *   - division and sqrt are IEEE correctly rounded;
+ *   - exp, sin, cos are evaluated in double and rounded to float;
+ *   - images are sampled in u8 units (0..255) by an FP32 bilinear filter with clamp-to-edge;
+ *     NCC is invariant to that scale, and the two scale-dependent constants are rescaled
+ *     (sigma_colour 0.06 -> 15.3, variance floor 1e-5 -> 1e-5*255^2);
+ *   - f32 -> f16 rounds to nearest even.
+ * Pinning of each function (closed forms, brute force, invariants) lives in tests/.
+ */
+#ifndef LR3D_ORACLE_H
+#define LR3D_ORACLE_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define ORC_MAX_SOURCES 8
+
+/* One camera: intrinsics + world->camera pose (x_cam = R X + t), CV axes.  CAM:12-26 */
+typedef struct {
+    int32_t width, height;
+    double fx, fy, cx, cy;
+    double R[9]; /* row-major */
+    double t[3];
+} orc_view;
+
+/* One source view in a PatchMatch dispatch: the `View` struct (PM:4-10) after the host packed
+ * it (DS:391-399): intrinsics, relative rotation ref->src (rows), translation, layer. */
+typedef struct {
+    float k[4];
+    float R[9];
+    float t[3];
+    int32_t layer;
+    int32_t width, height; /* source image size (equal to the reference size in the reference) */
+} orc_src;
+
+/* The `Params` uniform (PM:12-26) as DenseStereo.params() fills it (DS:360-401). */
+typedef struct {
+    int32_t width, height, ref;
+    float k[4];
+    float inv_min, inv_max, step, geo_weight; /* range */
+    uint32_t iteration, phase, seed, flags;   /* state */
+    uint32_t needed;                          /* band.z (consistency filter) */
+    int32_t n_src;
+    int32_t topk; /* 0 = reference rule max(1, min(3, n-1)) (PM:218) */
+    orc_src src[ORC_MAX_SOURCES];
+} orc_params;
+
+/* Images: one u8 plane per layer, each with its own size. */
+typedef struct {
+    int32_t n_layers;
+    const uint8_t *const *pixels;
+    const int32_t *widths;
+    const int32_t *heights;
+} orc_images;
+
+/* ---- RNG (PM:56-65, PM:251-253, DS:378-380) ---- */
+uint32_t orc_pcg(uint32_t v);
+uint32_t orc_host_seed(int32_t ref, uint32_t iteration, uint32_t phase, uint32_t flags);
+uint32_t orc_pixel_rng(uint32_t seed, uint32_t iteration, uint32_t phase, uint32_t index);
+float orc_random(uint32_t *rng);
+
+/* ---- packing (PK:3-33) ---- */
+uint32_t orc_pack_sample(float depth, const float normal[3], float cost);
+float orc_unpack_depth(uint32_t packed);
+void orc_unpack_normal(uint32_t packed, float out[3]);
+uint32_t orc_unpack_cost(uint32_t packed);
+uint16_t orc_f32_to_f16(float v);
+float orc_f16_to_f32(uint16_t h);
+
+/* ---- host parameter packing (DS:360-401, CAM:73-77) ---- */
+void orc_make_params(const orc_view *views, int32_t n_views, int32_t ref, const int32_t *sources,
+                     int32_t n_src, double min_depth, double max_depth, uint32_t iteration,
+                     uint32_t phase, double step, uint32_t flags, uint32_t needed, int32_t topk,
+                     orc_params *out);
+
+/* ---- cost pieces, exposed for the pinning tests ---- */
+/* Bilateral window of pixel (x, y) (PM:90-110): values[36], weights[36], stats[3] = sum, mean, var */
+void orc_window(const orc_params *p, const orc_images *img, int32_t x, int32_t y, float *values,
+                float *weights, float *stats);
+/* Plane homography of source s (PM:116-128), row-major 3x3 */
+void orc_homography(const orc_params *p, int32_t s, const float n[3], float rho, float H[9]);
+/* Bilateral NCC cost of one source view (PM:130-159) */
+float orc_view_cost(const orc_params *p, const orc_images *img, int32_t s, const float H[9],
+                    int32_t x, int32_t y, const float *values, const float *weights,
+                    const float *stats);
+/* Aggregated cost of a plane hypothesis at pixel (x, y) (PM:192-233).  `depths` (all layers'
+ * packed maps, layer-major) is read only when flags & 1. */
+float orc_hypothesis_cost(const orc_params *p, const orc_images *img, const uint32_t *depths,
+                          int32_t x, int32_t y, const float *values, const float *weights,
+                          const float *stats, const float n[3], float d);
+
+/* ---- kernels (whole-image dispatches; bands do not change results) ---- */
+/* PM:255-289.  hyp: W*H*4 floats (n.xyz, d); cost: W*H.  prior may be NULL (= zeros). */
+void orc_init(const orc_params *p, const orc_images *img, const float *prior,
+              const uint32_t *depths, float *hyp, float *cost);
+/* PM:349-423, one red-black phase (p->phase). */
+void orc_propagate(const orc_params *p, const orc_images *img, const uint32_t *depths, float *hyp,
+                   float *cost);
+/* PM:425-433 */
+void orc_pack(const orc_params *p, const float *hyp, const float *cost, uint32_t *out);
+/* PM:436-469 */
+void orc_consistency(const orc_params *p, const uint32_t *depths, uint32_t *filtered);
+/* PM:296-333: coarse-to-fine start from the coarse level's planes (coarse_k, coarse W x H). */
+void orc_init_coarse(const orc_params *p, const orc_images *img, const float coarse_k[4],
+                     int32_t coarse_w, int32_t coarse_h, const float *coarse_hyp, float *hyp,
+                     float *cost);
+
+/* DenseStereo.solve (DS:231-285): init, `iterations` x {phase 0, 1}, pack into depths[ref].
+ * geometric: flags bit 0.  Returns 0, or -1 when there are no sources (a no-op, DS:233). */
+int32_t orc_solve(const orc_view *views, int32_t n_views, const orc_images *img, int32_t ref,
+                  const int32_t *sources, int32_t n_src, double min_depth, double max_depth,
+                  const float *prior, int32_t geometric, int32_t iterations, int32_t topk,
+                  int32_t pack, uint32_t *depths, float *hyp, float *cost);
+
+/* DenseStereo.filter (DS:296-309) */
+void orc_filter(const orc_view *views, int32_t n_views, int32_t ref, const int32_t *checks,
+                int32_t n_checks, int32_t needed, const uint32_t *depths, uint32_t *filtered);
+
+/* ---- TSDF (TV:13-65, TV:103-154, TV:171-268) ---- */
+typedef struct {
+    double origin[3];
+    double voxel;
+    int32_t dims[3];       /* grid size in bricks */
+    int32_t n_bricks;
+    const int32_t *coords; /* n_bricks * 4: (bx, by, bz, 0) */
+    double truncation;     /* metres (reference: 4 * voxel, TV:179) */
+} orc_tsdf_grid;
+
+typedef struct {
+    int32_t width, height;
+    double fx, fy, cx, cy;
+    double R[9];
+    double t[3];
+    double max_depth;
+    double confidence;
+} orc_integration_view;
+
+/* BrickGrid.build over all cells (dense grid) (TV:54-64); coords must hold cells*4 ints */
+int32_t orc_dense_coords(const int32_t dims[3], int32_t *coords);
+/* TsdfVolume.visibleBricks (TV:220-234): flags[b] = 1 if brick b is visible */
+int32_t orc_visible_bricks(const orc_tsdf_grid *g, const orc_integration_view *v, uint8_t *flags);
+/* TsdfVolume.integrate (TV:240-268) + the WGSL `integrate` (TV:103-154) for one view.
+ * depth: packed map W*H; prior may be NULL.  Returns the number of voxel updates. */
+int64_t orc_tsdf_integrate(const orc_tsdf_grid *g, const orc_integration_view *v,
+                           const uint32_t *depth, const float *prior, float *tsdf, float *weight);
+
+/* Thread count used by the OpenMP loops (0 = runtime default). */
+void orc_set_threads(int32_t n);
+int32_t orc_get_threads(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,334 @@
+"""ctypes front-end of the CPU oracle (liblr3d_oracle.so).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and bench.py's CPU-baseline
+leg, as the parity checker / reported CPU baseline.  The product library never uses it.
+
+Every function restates the reference file:line named in oracle/lr3d_oracle.h.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from typing import Optional, Sequence
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "liblr3d_oracle.so")
+MAX_SOURCES = 8
+
+
+class OrcView(C.Structure):
+    _fields_ = [("width", C.c_int32), ("height", C.c_int32),
+                ("fx", C.c_double), ("fy", C.c_double), ("cx", C.c_double), ("cy", C.c_double),
+                ("R", C.c_double * 9), ("t", C.c_double * 3)]
+
+
+class OrcSrc(C.Structure):
+    _fields_ = [("k", C.c_float * 4), ("R", C.c_float * 9), ("t", C.c_float * 3),
+                ("layer", C.c_int32), ("width", C.c_int32), ("height", C.c_int32)]
+
+
+class OrcParams(C.Structure):
+    _fields_ = [("width", C.c_int32), ("height", C.c_int32), ("ref", C.c_int32),
+                ("k", C.c_float * 4),
+                ("inv_min", C.c_float), ("inv_max", C.c_float), ("step", C.c_float),
+                ("geo_weight", C.c_float),
+                ("iteration", C.c_uint32), ("phase", C.c_uint32), ("seed", C.c_uint32),
+                ("flags", C.c_uint32), ("needed", C.c_uint32), ("n_src", C.c_int32),
+                ("topk", C.c_int32), ("src", OrcSrc * MAX_SOURCES)]
+
+
+class OrcImages(C.Structure):
+    _fields_ = [("n_layers", C.c_int32), ("pixels", C.POINTER(C.c_void_p)),
+                ("widths", C.POINTER(C.c_int32)), ("heights", C.POINTER(C.c_int32))]
+
+
+class OrcTsdfGrid(C.Structure):
+    _fields_ = [("origin", C.c_double * 3), ("voxel", C.c_double), ("dims", C.c_int32 * 3),
+                ("n_bricks", C.c_int32), ("coords", C.POINTER(C.c_int32)), ("truncation", C.c_double)]
+
+
+class OrcIntegrationView(C.Structure):
+    _fields_ = [("width", C.c_int32), ("height", C.c_int32),
+                ("fx", C.c_double), ("fy", C.c_double), ("cx", C.c_double), ("cy", C.c_double),
+                ("R", C.c_double * 9), ("t", C.c_double * 3), ("max_depth", C.c_double),
+                ("confidence", C.c_double)]
+
+
+_lib = None
+
+
+def load():
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise FileNotFoundError(f"{LIB_PATH} missing: run `make -C oracle`")
+    lib = C.CDLL(LIB_PATH)
+    V, P = C.c_void_p, C.POINTER
+    u32, i32, f32, dbl = C.c_uint32, C.c_int32, C.c_float, C.c_double
+    sig = {
+        "orc_pcg": (u32, [u32]),
+        "orc_host_seed": (u32, [i32, u32, u32, u32]),
+        "orc_pixel_rng": (u32, [u32, u32, u32, u32]),
+        "orc_random": (f32, [P(u32)]),
+        "orc_pack_sample": (u32, [f32, P(f32), f32]),
+        "orc_unpack_depth": (f32, [u32]),
+        "orc_unpack_normal": (None, [u32, P(f32)]),
+        "orc_unpack_cost": (u32, [u32]),
+        "orc_f32_to_f16": (C.c_uint16, [f32]),
+        "orc_f16_to_f32": (f32, [C.c_uint16]),
+        "orc_make_params": (None, [P(OrcView), i32, i32, P(i32), i32, dbl, dbl, u32, u32, dbl, u32,
+                                   u32, i32, P(OrcParams)]),
+        "orc_window": (None, [P(OrcParams), P(OrcImages), i32, i32, V, V, V]),
+        "orc_homography": (None, [P(OrcParams), i32, P(f32), f32, V]),
+        "orc_view_cost": (f32, [P(OrcParams), P(OrcImages), i32, V, i32, i32, V, V, V]),
+        "orc_hypothesis_cost": (f32, [P(OrcParams), P(OrcImages), V, i32, i32, V, V, V, P(f32), f32]),
+        "orc_init": (None, [P(OrcParams), P(OrcImages), V, V, V, V]),
+        "orc_propagate": (None, [P(OrcParams), P(OrcImages), V, V, V]),
+        "orc_pack": (None, [P(OrcParams), V, V, V]),
+        "orc_consistency": (None, [P(OrcParams), V, V]),
+        "orc_init_coarse": (None, [P(OrcParams), P(OrcImages), P(f32), i32, i32, V, V, V]),
+        "orc_solve": (i32, [P(OrcView), i32, P(OrcImages), i32, P(i32), i32, dbl, dbl, V, i32, i32,
+                            i32, i32, V, V, V]),
+        "orc_filter": (None, [P(OrcView), i32, i32, P(i32), i32, i32, V, V]),
+        "orc_dense_coords": (i32, [P(i32), V]),
+        "orc_visible_bricks": (i32, [P(OrcTsdfGrid), P(OrcIntegrationView), V]),
+        "orc_tsdf_integrate": (C.c_int64, [P(OrcTsdfGrid), P(OrcIntegrationView), V, V, V, V]),
+        "orc_set_threads": (None, [i32]),
+        "orc_get_threads": (i32, []),
+    }
+    for name, (res, args) in sig.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    _lib = lib
+    return lib
+
+
+def _p(a):
+    return C.c_void_p(a.ctypes.data) if a is not None else None
+
+
+def set_threads(n: int) -> None:
+    load().orc_set_threads(int(n))
+
+
+def get_threads() -> int:
+    return load().orc_get_threads()
+
+
+# ---- plain helpers -----------------------------------------------------------------------------
+
+def pcg(v: int) -> int:
+    return load().orc_pcg(v & 0xFFFFFFFF)
+
+
+def host_seed(ref: int, iteration: int, phase: int, flags: int) -> int:
+    return load().orc_host_seed(ref, iteration, phase, flags)
+
+
+def pixel_rng(seed: int, iteration: int, phase: int, index: int) -> int:
+    return load().orc_pixel_rng(seed, iteration, phase, index)
+
+
+def random_stream(state: int, n: int) -> list:
+    s = C.c_uint32(state)
+    return [load().orc_random(C.byref(s)) for _ in range(n)]
+
+
+def f32_to_f16(v: float) -> int:
+    return load().orc_f32_to_f16(v)
+
+
+def f16_to_f32(h: int) -> float:
+    return load().orc_f16_to_f32(h)
+
+
+def pack_sample(depth: float, normal, cost: float) -> int:
+    n = (C.c_float * 3)(*normal)
+    return load().orc_pack_sample(depth, n, cost)
+
+
+def unpack_depth(p: int) -> float:
+    return load().orc_unpack_depth(p)
+
+
+def unpack_normal(p: int):
+    out = (C.c_float * 3)()
+    load().orc_unpack_normal(p, out)
+    return np.array(list(out), dtype=np.float32)
+
+
+def unpack_cost(p: int) -> int:
+    return load().orc_unpack_cost(p)
+
+
+# ---- stereo ------------------------------------------------------------------------------------
+
+def view_struct(width, height, fx, fy, cx, cy, R, t) -> OrcView:
+    v = OrcView()
+    v.width, v.height = width, height
+    v.fx, v.fy, v.cx, v.cy = fx, fy, cx, cy
+    v.R[:] = [float(x) for x in np.asarray(R, dtype=np.float64).reshape(9)]
+    v.t[:] = [float(x) for x in np.asarray(t, dtype=np.float64).reshape(3)]
+    return v
+
+
+def views_array(views) -> "C.Array":
+    """From objects with .width .height .intrinsics(.fx .fy .cx .cy) .pose(.R .t)."""
+    structs = [view_struct(v.width, v.height, v.intrinsics.fx, v.intrinsics.fy, v.intrinsics.cx,
+                           v.intrinsics.cy, v.pose.R, v.pose.t) for v in views]
+    return (OrcView * len(structs))(*structs)
+
+
+class Images:
+    """Keeps the u8 planes alive and exposes the orc_images view of them."""
+
+    def __init__(self, images: Sequence[np.ndarray]):
+        self.arrays = [np.ascontiguousarray(im, dtype=np.uint8) for im in images]
+        n = len(self.arrays)
+        self._ptrs = (C.c_void_p * n)(*[a.ctypes.data for a in self.arrays])
+        self._w = (C.c_int32 * n)(*[a.shape[1] for a in self.arrays])
+        self._h = (C.c_int32 * n)(*[a.shape[0] for a in self.arrays])
+        self.struct = OrcImages(n, self._ptrs, self._w, self._h)
+
+
+def make_params(views, ref, sources, min_depth, max_depth, iteration=0, phase=0, step=1.0,
+                flags=0, needed=0, topk=0) -> OrcParams:
+    arr = views_array(views)
+    src = (C.c_int32 * max(1, len(sources)))(*sources)
+    p = OrcParams()
+    load().orc_make_params(arr, len(views), ref, src, len(sources), min_depth, max_depth, iteration,
+                           phase, step, flags, needed, topk, C.byref(p))
+    return p
+
+
+def window(p: OrcParams, images: Images, x: int, y: int):
+    values = np.empty(36, dtype=np.float32)
+    weights = np.empty(36, dtype=np.float32)
+    stats = np.empty(3, dtype=np.float32)
+    load().orc_window(C.byref(p), C.byref(images.struct), x, y, _p(values), _p(weights), _p(stats))
+    return values, weights, stats
+
+
+def homography(p: OrcParams, s: int, n, rho: float) -> np.ndarray:
+    H = np.empty(9, dtype=np.float32)
+    nn = (C.c_float * 3)(*n)
+    load().orc_homography(C.byref(p), s, nn, rho, _p(H))
+    return H
+
+
+def view_cost(p, images, s, H, x, y, win) -> float:
+    values, weights, stats = win
+    H = np.ascontiguousarray(H, dtype=np.float32)
+    return load().orc_view_cost(C.byref(p), C.byref(images.struct), s, _p(H), x, y, _p(values),
+                                _p(weights), _p(stats))
+
+
+def hypothesis_cost(p, images, x, y, win, n, d, depths=None) -> float:
+    values, weights, stats = win
+    nn = (C.c_float * 3)(*n)
+    return load().orc_hypothesis_cost(C.byref(p), C.byref(images.struct), _p(depths), x, y,
+                                      _p(values), _p(weights), _p(stats), nn, d)
+
+
+def init(p, images, prior=None, depths=None, hyp=None, cost=None):
+    n = p.width * p.height
+    hyp = np.zeros((n, 4), dtype=np.float32) if hyp is None else hyp
+    cost = np.zeros(n, dtype=np.float32) if cost is None else cost
+    load().orc_init(C.byref(p), C.byref(images.struct), _p(prior), _p(depths), _p(hyp), _p(cost))
+    return hyp, cost
+
+
+def propagate(p, images, hyp, cost, depths=None):
+    load().orc_propagate(C.byref(p), C.byref(images.struct), _p(depths), _p(hyp), _p(cost))
+    return hyp, cost
+
+
+def pack(p, hyp, cost) -> np.ndarray:
+    out = np.empty(p.width * p.height, dtype=np.uint32)
+    load().orc_pack(C.byref(p), _p(hyp), _p(cost), _p(out))
+    return out
+
+
+def solve(views, images: Images, ref: int, sources: Sequence[int], min_depth: float,
+          max_depth: float, prior: Optional[np.ndarray] = None, geometric: bool = False,
+          iterations: int = 5, topk: int = 0, pack: bool = True,
+          depths: Optional[np.ndarray] = None):
+    """DS:231-285.  depths: all layers' packed maps (layer-major, equal layer sizes)."""
+    arr = views_array(views)
+    W, H = views[ref].width, views[ref].height
+    if depths is None:
+        depths = np.zeros(len(views) * W * H, dtype=np.uint32)
+    hyp = np.zeros((W * H, 4), dtype=np.float32)
+    cost = np.zeros(W * H, dtype=np.float32)
+    src = (C.c_int32 * max(1, len(sources)))(*sources)
+    if prior is not None:
+        prior = np.ascontiguousarray(prior, dtype=np.float32).reshape(-1)
+    load().orc_solve(arr, len(views), C.byref(images.struct), ref, src, len(sources), min_depth,
+                     max_depth, _p(prior), 1 if geometric else 0, iterations, topk, 1 if pack else 0,
+                     _p(depths), _p(hyp), _p(cost))
+    return depths, hyp, cost
+
+
+def filter_map(views, ref, checks, needed, depths) -> np.ndarray:
+    arr = views_array(views)
+    W, H = views[ref].width, views[ref].height
+    out = np.zeros(W * H, dtype=np.uint32)
+    src = (C.c_int32 * max(1, len(checks)))(*checks)
+    load().orc_filter(arr, len(views), ref, src, len(checks), needed, _p(depths), _p(out))
+    return out
+
+
+def init_coarse(p, images, ck, coarse_w, coarse_h, coarse_hyp):
+    n = p.width * p.height
+    hyp = np.zeros((n, 4), dtype=np.float32)
+    cost = np.zeros(n, dtype=np.float32)
+    k = (C.c_float * 4)(*ck)
+    ch = np.ascontiguousarray(coarse_hyp, dtype=np.float32)
+    load().orc_init_coarse(C.byref(p), C.byref(images.struct), k, coarse_w, coarse_h, _p(ch),
+                           _p(hyp), _p(cost))
+    return hyp, cost
+
+
+# ---- TSDF --------------------------------------------------------------------------------------
+
+class Grid:
+    def __init__(self, origin, voxel, dims, coords, truncation):
+        self.coords = np.ascontiguousarray(coords, dtype=np.int32).reshape(-1)
+        g = OrcTsdfGrid()
+        g.origin[:] = list(origin)
+        g.voxel = voxel
+        g.dims[:] = list(dims)
+        g.n_bricks = len(self.coords) // 4
+        g.coords = self.coords.ctypes.data_as(C.POINTER(C.c_int32))
+        g.truncation = truncation
+        self.struct = g
+        self.n_bricks = g.n_bricks
+
+
+def integration_view(width, height, fx, fy, cx, cy, R, t, max_depth, confidence=0.0):
+    v = OrcIntegrationView()
+    v.width, v.height = width, height
+    v.fx, v.fy, v.cx, v.cy = fx, fy, cx, cy
+    v.R[:] = [float(x) for x in np.asarray(R, dtype=np.float64).reshape(9)]
+    v.t[:] = [float(x) for x in np.asarray(t, dtype=np.float64).reshape(3)]
+    v.max_depth = max_depth
+    v.confidence = confidence
+    return v
+
+
+def visible_bricks(grid: Grid, view) -> np.ndarray:
+    flags = np.zeros(grid.n_bricks, dtype=np.uint8)
+    load().orc_visible_bricks(C.byref(grid.struct), C.byref(view), _p(flags))
+    return flags
+
+
+def tsdf_integrate(grid: Grid, view, depth: np.ndarray, prior, tsdf, weight) -> int:
+    depth = np.ascontiguousarray(depth, dtype=np.uint32).reshape(-1)
+    if prior is not None:
+        prior = np.ascontiguousarray(prior, dtype=np.float32).reshape(-1)
+    return load().orc_tsdf_integrate(C.byref(grid.struct), C.byref(view), _p(depth), _p(prior),
+                                     _p(tsdf), _p(weight))

@@ -91,7 +91,9 @@ int32_t lr3d_ctx_synchronize(lr3d_ctx *ctx);
 /* Kernel timing with HIP events recorded around each launch on the context's stream.
  * enable: 1 on, 0 off (statistics are reset when turned on). */
 int32_t lr3d_ctx_set_profiling(lr3d_ctx *ctx, int32_t enable);
-/* Launch count and total milliseconds of kernel `name` since profiling was enabled. */
+/* Launch count and total milliseconds of kernel `name` since profiling was enabled.  Profiled
+ * calls launch eagerly (HIP cannot time event records inside graphs); the name "graph_launch"
+ * returns the number of graph replays of the context instead. */
 int32_t lr3d_ctx_kernel_stats(lr3d_ctx *ctx, const char *name, int64_t *launches, double *total_ms);
 
 /* ---- PatchMatch stereo (replaces DenseStereo, DS:107-417) -------------------------------- */

@@ -90,6 +90,7 @@ struct lr3d_ctx {
     std::map<std::string, KernelStats> stats;
     std::vector<hipEvent_t> event_pool;
     std::vector<PendingEvent> pending;
+    int64_t graph_launches = 0;
 
     hipEvent_t take_event() {
         if (!event_pool.empty()) {
@@ -109,6 +110,8 @@ struct lr3d_ctx {
                 auto &s = stats[p.name];
                 s.launches++;
                 s.total_ms += ms;
+            } else {
+                (void)hipGetLastError();
             }
             event_pool.push_back(p.start);
             event_pool.push_back(p.stop);
@@ -128,6 +131,7 @@ struct lr3d_ctx {
 // Records start/stop events around one launch when profiling is on.
 template <typename F>
 static hipError_t timed(lr3d_ctx *ctx, const char *name, F &&launch) {
+    (void)hipGetLastError();  // launchers report hipGetLastError(): start from a clean slate
     if (!ctx->profiling) return launch();
     hipEvent_t a = ctx->take_event(), b = ctx->take_event();
     if (!a || !b) return hipErrorOutOfMemory;
@@ -202,6 +206,11 @@ extern "C" int32_t lr3d_ctx_kernel_stats(lr3d_ctx *ctx, const char *name, int64_
                                          double *total_ms) {
     if (!ctx || !name) return fail(LR3D_ERR_INVALID, "lr3d_ctx_kernel_stats: NULL argument");
     std::lock_guard<std::mutex> lock(ctx->mu);
+    if (strcmp(name, "graph_launch") == 0) {  // replays of captured launch sequences (lifetime count)
+        if (launches) *launches = ctx->graph_launches;
+        if (total_ms) *total_ms = 0.0;
+        return LR3D_OK;
+    }
     auto it = ctx->stats.find(name);
     if (launches) *launches = it == ctx->stats.end() ? 0 : it->second.launches;
     if (total_ms) *total_ms = it == ctx->stats.end() ? 0.0 : it->second.total_ms;
@@ -216,6 +225,13 @@ struct GraphEntry {
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
 };
+
+static void destroy_graph(GraphEntry &g) {
+    if (g.exec) (void)hipGraphExecDestroy(g.exec);
+    if (g.graph) (void)hipGraphDestroy(g.graph);
+    g.exec = nullptr;
+    g.graph = nullptr;
+}
 
 struct lr3d_stereo {
     lr3d_ctx *ctx = nullptr;
@@ -233,10 +249,7 @@ struct lr3d_stereo {
     std::vector<GraphEntry> graphs;
 
     ~lr3d_stereo() {
-        for (auto &g : graphs) {
-            if (g.exec) (void)hipGraphExecDestroy(g.exec);
-            if (g.graph) (void)hipGraphDestroy(g.graph);
-        }
+        for (auto &g : graphs) destroy_graph(g);
     }
 };
 
@@ -570,8 +583,11 @@ static int32_t solve_impl(lr3d_stereo *st, const lr3d_solve_desc *descs, int32_t
     rc = upload_plans(st, dv, packs, plans, key);
     if (rc) return rc;
     hipError_t e = hipSuccess;
-    if (batched && !ctx->profiling) {
-        GraphEntry *entry = nullptr;
+    // HIP cannot time event records captured into a graph (hipEventElapsedTime reports an
+    // invalid handle), so profiled calls launch eagerly with an event pair around each kernel.
+    const bool use_graph = batched && !ctx->profiling;
+    GraphEntry *entry = nullptr;
+    if (use_graph) {
         for (auto &g : st->graphs)
             if (g.key == key) entry = &g;
         if (!entry) {
@@ -581,27 +597,22 @@ static int32_t solve_impl(lr3d_stereo *st, const lr3d_solve_desc *descs, int32_t
             e = enqueue_schedule(st, plans, max_it, any_pack);
             hipGraph_t graph = nullptr;
             hipError_t e2 = hipStreamEndCapture(ctx->stream, &graph);
+            g.graph = graph;
+            if (e == hipSuccess && e2 == hipSuccess) e = hipGraphInstantiate(&g.exec, graph, nullptr, nullptr, 0);
             if (e != hipSuccess || e2 != hipSuccess) {
-                if (graph) (void)hipGraphDestroy(graph);
+                destroy_graph(g);
                 return fail(LR3D_ERR_DEVICE, "lr3d_stereo_solve_batched: graph capture failed: %s",
                             hipGetErrorString(e != hipSuccess ? e : e2));
             }
-            g.graph = graph;
-            e = hipGraphInstantiate(&g.exec, graph, nullptr, nullptr, 0);
-            if (e != hipSuccess) {
-                (void)hipGraphDestroy(graph);
-                return fail(LR3D_ERR_DEVICE, "lr3d_stereo_solve_batched: hipGraphInstantiate: %s",
-                            hipGetErrorString(e));
-            }
             if (st->graphs.size() >= 8) {
-                (void)hipGraphExecDestroy(st->graphs.front().exec);
-                (void)hipGraphDestroy(st->graphs.front().graph);
+                destroy_graph(st->graphs.front());
                 st->graphs.erase(st->graphs.begin());
             }
-            st->graphs.push_back(g);
+            st->graphs.push_back(std::move(g));
             entry = &st->graphs.back();
         }
         HIP_TRY(hipGraphLaunch(entry->exec, ctx->stream));
+        ctx->graph_launches++;
     } else {
         e = enqueue_schedule(st, plans, max_it, any_pack);
         if (e != hipSuccess)
@@ -719,10 +730,7 @@ extern "C" int32_t lr3d_stereo_set_coarse(lr3d_stereo *st, lr3d_stereo *coarse) 
     }
     std::lock_guard<std::mutex> lock(st->ctx->mu);
     st->coarse = coarse;
-    for (auto &g : st->graphs) {
-        if (g.exec) (void)hipGraphExecDestroy(g.exec);
-        if (g.graph) (void)hipGraphDestroy(g.graph);
-    }
+    for (auto &g : st->graphs) destroy_graph(g);
     st->graphs.clear();
     return LR3D_OK;
 }

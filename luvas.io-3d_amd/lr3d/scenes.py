@@ -166,6 +166,13 @@ class Workload:
     def megapixels(self) -> float:
         return sum(v.width * v.height for v in self.views) / 1e6
 
+    def mesh_needed(self, ref: int, agreement: float = 0.25) -> int:
+        """Views of the consistency check that must agree before a depth is fused: the
+        reference's agreeing() (frontend/src/app/vision/pipeline/stages/stereo-plan.ts:99-101)
+        with the High profile's mesh agreement 0.25 (core/models/quality.ts:133)."""
+        count = min(len(self.sources[ref]), 8)
+        return min(count, max(2, int(math.floor(agreement * count + 0.5))))
+
     def hypotheses_per_pixel(self) -> int:
         """Nominal hypotheses per pixel: 1 (init) + 14 per iteration (8 propagated + 6 refined)."""
         return 1 + 14 * self.iterations

@@ -1,0 +1,218 @@
+"""GPU parity: the HIP path through the C ABI against the CPU oracle, on the seeded 8-view
+320x240 sphere workload.
+
+Bars (stated per test): indices, RNG streams and schedule are bit-exact by construction, so the
+planes and costs are compared bit for bit and must match on >= 99.9 % of pixels; packed depths
+must agree within 1e-3 relative on >= 99.5 % of valid pixels; TSDF values within 1e-4 of the
+truncation distance, weights exactly.
+"""
+import sys
+
+import numpy as np
+import pytest
+
+import oracle as O
+from lr3d import scenes
+from lr3d.packing import unpack_depth
+
+pytestmark = pytest.mark.gpu
+
+
+def say(msg):
+    print(msg, flush=True)
+    sys.stdout.flush()
+
+
+@pytest.fixture(scope="module")
+def work():
+    return scenes.sphere_small().render()
+
+
+@pytest.fixture(scope="module")
+def oracle_maps(work):
+    """Oracle solves of every view (packed maps, layer-major) plus view 0's planes."""
+    imgs = O.Images(work.images)
+    W, H = 320, 240
+    depths = np.zeros(len(work.views) * W * H, dtype=np.uint32)
+    hyp0 = cost0 = None
+    for ref in range(len(work.views)):
+        _, hyp, cost = O.solve(work.views, imgs, ref, work.sources[ref], *work.ranges[ref],
+                               iterations=work.iterations, topk=work.topk, depths=depths)
+        if ref == 0:
+            hyp0, cost0 = hyp.copy(), cost.copy()
+    return depths, hyp0, cost0
+
+
+def make_stereo(ctx, work):
+    from lr3d import HipDenseStereo
+    st = HipDenseStereo(ctx, work.views)
+    for i, im in enumerate(work.images):
+        st.set_image(i, im)
+    return st
+
+
+def opts(work, ref, **kw):
+    from lr3d import SolveOptions
+    lo, hi = work.ranges[ref]
+    base = dict(sources=work.sources[ref], min_depth=lo, max_depth=hi, iterations=work.iterations,
+                topk=work.topk)
+    base.update(kw)
+    return SolveOptions(**base)
+
+
+def depth_agreement(a, b):
+    """Fraction of pixels valid in either map whose depths agree within 1e-3 relative."""
+    da, db = unpack_depth(a), unpack_depth(b)
+    valid = (a != 0) | (b != 0)
+    ok = (a != 0) & (b != 0) & (np.abs(da - db) <= 1e-3 * np.maximum(np.abs(db), 1e-12))
+    return ok[valid].mean() if valid.any() else 1.0
+
+
+def test_init_bit_exact(ctx, work):
+    say("case: init (iterations=0) view 0, sphere 8x320x240")
+    st = make_stereo(ctx, work)
+    st.solve(0, opts(work, 0, iterations=0))
+    hyp, cost = st.hypotheses(0)
+    imgs = O.Images(work.images)
+    p = O.make_params(work.views, 0, work.sources[0], *work.ranges[0], topk=work.topk)
+    ohyp, ocost = O.init(p, imgs)
+    exact = np.all(hyp.view(np.uint32) == ohyp.view(np.uint32), axis=1) & \
+        (cost.view(np.uint32) == ocost.view(np.uint32))
+    say(f"init bit-exact fraction {exact.mean():.6f}")
+    st.destroy()
+    assert exact.mean() >= 0.999
+
+
+def test_solve_matches_oracle(ctx, work, oracle_maps):
+    say("case: plain solve view 0, 4 iterations, top-2")
+    depths, ohyp, ocost = oracle_maps
+    st = make_stereo(ctx, work)
+    st.solve(0, opts(work, 0))
+    hyp, cost = st.hypotheses(0)
+    packed = st.depth_map(0)
+    st.destroy()
+    exact = np.all(hyp.view(np.uint32) == ohyp.view(np.uint32), axis=1)
+    agree = depth_agreement(packed, depths[:320 * 240])
+    say(f"solve: planes bit-exact {exact.mean():.6f}, packed depth agreement {agree:.6f}")
+    assert agree >= 0.995
+    assert exact.mean() >= 0.99
+
+
+def test_batched_equals_plain(ctx, work):
+    say("case: batched (graph) vs plain solves, all 8 views")
+    a = make_stereo(ctx, work)
+    for ref in range(len(work.views)):
+        a.solve(ref, opts(work, ref))
+    b = make_stereo(ctx, work)
+    b.solve_batched([(ref, opts(work, ref)) for ref in range(len(work.views))])
+    for ref in range(len(work.views)):
+        pa, pb = a.depth_map(ref), b.depth_map(ref)
+        ha, hb = a.hypotheses(ref)[0], b.hypotheses(ref)[0]
+        assert np.array_equal(pa, pb), f"view {ref} packed maps differ"
+        assert np.array_equal(ha.view(np.uint32), hb.view(np.uint32)), f"view {ref} planes differ"
+    # replays of the captured graph, with eager work in between, stay identical
+    first = [b.depth_map(r).copy() for r in range(len(work.views))]
+    for replay in range(3):
+        say(f"graph replay {replay}")
+        a.solve(0, opts(work, 0))
+        b.solve_batched([(ref, opts(work, ref)) for ref in range(len(work.views))])
+        for ref in range(len(work.views)):
+            assert np.array_equal(b.depth_map(ref), first[ref])
+    a.destroy()
+    b.destroy()
+
+
+def test_all_views_match_oracle(ctx, work, oracle_maps):
+    say("case: batched solve of all views vs oracle")
+    depths = oracle_maps[0]
+    st = make_stereo(ctx, work)
+    st.solve_batched([(ref, opts(work, ref)) for ref in range(len(work.views))])
+    n = 320 * 240
+    for ref in range(len(work.views)):
+        agree = depth_agreement(st.depth_map(ref), depths[ref * n:(ref + 1) * n])
+        say(f"view {ref}: depth agreement {agree:.6f}")
+        assert agree >= 0.995
+    st.destroy()
+
+
+def test_filter_matches_oracle(ctx, work, oracle_maps):
+    say("case: consistency filter of view 0 on oracle maps")
+    depths = oracle_maps[0]
+    st = make_stereo(ctx, work)
+    n = 320 * 240
+    for ref in range(len(work.views)):
+        st.write_depth(ref, depths[ref * n:(ref + 1) * n])
+    checks = work.sources[0]
+    gpu = st.filtered_map(0, checks, 2)
+    ref_map = O.filter_map(work.views, 0, checks, 2, depths)
+    st.destroy()
+    say(f"filter: exact fraction {(gpu == ref_map).mean():.6f}, kept {(ref_map != 0).mean():.3f}")
+    assert np.array_equal(gpu, ref_map)
+
+
+def _tsdf_setup(work):
+    from lr3d import IntegrationView, dense_grid
+    spec = work.tsdf
+    grid, layout = dense_grid(spec.origin, spec.voxel, spec.voxels)
+    views = [IntegrationView(v.intrinsics, v.pose, work.ranges[i][1], v.width, v.height)
+             for i, v in enumerate(work.views)]
+    return grid, layout, views
+
+
+def _oracle_tsdf(work, grid, layout, depths, n_views):
+    spec = work.tsdf
+    g = O.Grid(grid.origin, grid.voxel, grid.dims, layout[0], spec.voxel * spec.truncation_voxels)
+    tsdf = np.ones(layout[2] * 512, dtype=np.float32)
+    weight = np.zeros(layout[2] * 512, dtype=np.float32)
+    n = 320 * 240
+    updates = 0
+    for i in range(n_views):
+        v = work.views[i]
+        iv = O.integration_view(v.width, v.height, v.intrinsics.fx, v.intrinsics.fy, v.intrinsics.cx,
+                                v.intrinsics.cy, v.pose.R, v.pose.t, work.ranges[i][1])
+        updates += O.tsdf_integrate(g, iv, depths[i * n:(i + 1) * n], None, tsdf, weight)
+    return tsdf, weight, updates
+
+
+def test_tsdf_matches_oracle(ctx, work, oracle_maps):
+    say("case: TSDF 128^3 x 8 mm, 8 oracle maps, sequential host-map integration")
+    from lr3d import HipTsdfVolume
+    depths = oracle_maps[0]
+    grid, layout, views = _tsdf_setup(work)
+    vol = HipTsdfVolume(ctx, grid, layout, work.tsdf.truncation_voxels)
+    vol.set_counting(True)
+    n = 320 * 240
+    for i, v in enumerate(views):
+        vol.integrate_host(depths[i * n:(i + 1) * n], v)
+    tsdf, weight = vol.read()
+    count = vol.update_count()
+    vol.destroy()
+    otsdf, oweight, oupdates = _oracle_tsdf(work, grid, layout, depths, len(views))
+    mu = work.tsdf.voxel * work.tsdf.truncation_voxels
+    say(f"tsdf: max |diff| {np.abs(tsdf - otsdf).max():.3g} (bar {1e-4 * mu:.3g}), "
+        f"updates gpu {count} oracle {oupdates}")
+    assert np.array_equal(weight, oweight)
+    assert np.abs(tsdf - otsdf).max() <= 1e-4 * mu
+    assert count == oupdates
+
+
+def test_tsdf_batched_equals_sequential(ctx, work, oracle_maps):
+    say("case: TSDF batched (8 views, one pass) vs sequential")
+    from lr3d import HipTsdfVolume
+    depths = oracle_maps[0]
+    grid, layout, views = _tsdf_setup(work)
+    st = make_stereo(ctx, work)
+    n = 320 * 240
+    for i in range(len(views)):
+        st.write_depth(i, depths[i * n:(i + 1) * n])
+    a = HipTsdfVolume(ctx, grid, layout, work.tsdf.truncation_voxels)
+    b = HipTsdfVolume(ctx, grid, layout, work.tsdf.truncation_voxels)
+    for i, v in enumerate(views):
+        a.integrate(st, i, v)
+    b.integrate_batched(st, list(range(len(views))), views)
+    ta, wa = a.read()
+    tb, wb = b.read()
+    for x in (a, b, st):
+        x.destroy()
+    assert np.array_equal(ta.view(np.uint32), tb.view(np.uint32))
+    assert np.array_equal(wa.view(np.uint32), wb.view(np.uint32))

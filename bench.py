@@ -2,8 +2,10 @@
 """Benchmark of the dense-reconstruction hot path on one MI355X.
 
 One step = batched PatchMatch of every view (init + `iterations` red-black rounds + pack, all
-views sharing each launch, replayed from a HIP graph), the consistency filter of every view, and
-one batched TSDF integration pass of every filtered map.  Inputs (u8 frames, cameras) are
+views sharing each launch), the consistency filter of every view, and one batched TSDF
+integration pass of every filtered map.  The timed region launches eagerly with a HIP event pair
+around every kernel (HIP cannot time events inside graphs); the same steps are then replayed from
+the captured HIP graph and reported under "graph".  Inputs (u8 frames, cameras) are
 resident in device memory before the timed region; nothing is read back inside it.
 
 Default workload: 32 views at 1280x720 of the seeded sphere scene, 8 neighbours, top-3,
@@ -12,7 +14,7 @@ Default workload: 32 views at 1280x720 of the seeded sphere scene, 8 neighbours,
   metric value   PatchMatch Mpx*hyp/s over the whole step (Mpx * (1 + 14*iterations) per step)
   tsdf           voxel updates per step and per second of the TSDF kernels
   roofline       the dominant kernel (pm_propagate) against the FP32 vector peak, timed with HIP
-                 events recorded inside the captured graph over the timed region
+                 events on the library's stream over the timed region
   cpu_baseline   the C oracle (same arithmetic, OpenMP) on a bounded sample of the same workload
   plain          the per-view path (one solve / integrate call per view) for the speed-up ratio
 """

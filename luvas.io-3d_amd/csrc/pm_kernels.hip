@@ -198,41 +198,75 @@ __device__ __forceinline__ void homography(const PmDesc &d, const PmSrc &s, floa
 
 // ---- bilateral NCC of one source view (PM:130-159) ---------------------------------------------
 //
-// All 36 taps are evaluated; the view is INVALID if any tap is (the reference returns at the
-// first such tap, with the same result).  Sample coordinates are clamped into the image so
-// that invalid taps still read inside the source image.
+// Tap i at offset (ox, oy) maps to q = H (X, Y, 1) with X = px + ox, Y = py + oy, evaluated as
+// q = fma(H_0, X, fma(H_1, Y, H_2)) per row, so the Y part is shared by the six taps of a window
+// row.  All 36 taps are evaluated; the view is INVALID if any tap is (the reference returns at
+// the first such tap, with the same result).  Sample indices are clamped into the image so that
+// invalid taps still read inside the source image.
+//
+// 1/q.z: v_rcp_f32 followed by one Newton step with FMA reproduces the IEEE quotient 1.0f / q.z
+// bit for bit for every |q.z| in [2^-60, 2^61) (checked exhaustively on gfx950).  Valid taps
+// have q.z > 1e-6 > 2^-60; a view whose window could reach |q.z| >= 2^60 takes the IEEE
+// division for the whole wavefront.
+__device__ __forceinline__ float rcp_exact(float z) {
+    const float r = __builtin_amdgcn_rcpf(z);
+    return fmaf(fmaf(-z, r, 1.0f), r, r);
+}
+
+template <bool FAST>
+__device__ __forceinline__ void view_taps(const uint32_t *__restrict__ img, int32_t W, int32_t H,
+                                          const Mat3 &Hm, float px, float py, const Window &win,
+                                          float &ss, float &sss, float &srs, bool &ok) {
+    const float wmax = (float)W - 1.0f, hmax = (float)H - 1.0f;
+    const uint32_t stride = (uint32_t)W;
+#pragma unroll
+    for (int row = 0; row < 6; row++) {
+        const float Y = py + (float)(row * 2 - 5);
+        const float rx = fmaf(Hm.m[1], Y, Hm.m[2]);
+        const float ry = fmaf(Hm.m[4], Y, Hm.m[5]);
+        const float rw = fmaf(Hm.m[7], Y, Hm.m[8]);
+#pragma unroll
+        for (int col = 0; col < 6; col++) {
+            const int i = row * 6 + col;
+            const float X = px + (float)(col * 2 - 5);
+            const float qx = fmaf(Hm.m[0], X, rx);
+            const float qy = fmaf(Hm.m[3], X, ry);
+            const float qz = fmaf(Hm.m[6], X, rw);
+            const float rz = FAST ? rcp_exact(qz) : 1.0f / qz;
+            const float u = qx * rz, v = qy * rz;
+            ok = ok & (qz > 1e-6f) & (u >= 0.0f) & (v >= 0.0f) & (u <= wmax) & (v <= hmax);
+            // Valid taps have 0 <= u <= W-1, so trunc(u) = floor(u) and fract(u) = u - floor(u)
+            // exactly; the clamp (NaN-safe: maxnum returns the number) keeps invalid taps inside
+            // the image.
+            const int32_t ix = (int32_t)fminf(fmaxf(u, 0.0f), wmax);
+            const int32_t iy = (int32_t)fminf(fmaxf(v, 0.0f), hmax);
+            const float fx = __builtin_amdgcn_fractf(u), fy = __builtin_amdgcn_fractf(v);
+            const uint32_t q = img[__umul24((uint32_t)iy, stride) + (uint32_t)ix];
+            const float a = (float)(q & 0xffu), b = (float)((q >> 8) & 0xffu);
+            const float c = (float)((q >> 16) & 0xffu), e = (float)(q >> 24);
+            const float top = fmaf(fx, b - a, a);
+            const float bot = fmaf(fx, e - c, c);
+            const float val = fmaf(fy, bot - top, top);
+            const float wv = win.w[i] * val;
+            ss = ss + wv;
+            sss = fmaf(wv, val, sss);
+            srs = fmaf(wv, win.v[i], srs);
+        }
+    }
+}
+
 __device__ __forceinline__ float view_cost(const PmSrc &s, const Mat3 &H, float px, float py,
                                            const Window &win, const uint32_t *__restrict__ quad) {
     const uint32_t *__restrict__ img = quad + s.off;
-    const float wmax = (float)s.W - 1.0f, hmax = (float)s.H - 1.0f;
-    const uint32_t stride = (uint32_t)s.W;
     float ss = 0.0f, sss = 0.0f, srs = 0.0f;
     bool ok = true;
-#pragma unroll
-    for (int i = 0; i < kSamples; i++) {
-        const float X = px + (float)((i % 6) * 2 - 5);
-        const float Y = py + (float)((i / 6) * 2 - 5);
-        float qx = fmaf(H.m[1], Y, fmaf(H.m[0], X, H.m[2]));
-        float qy = fmaf(H.m[4], Y, fmaf(H.m[3], X, H.m[5]));
-        float qz = fmaf(H.m[7], Y, fmaf(H.m[6], X, H.m[8]));
-        float rz = 1.0f / qz;
-        float u = qx * rz, v = qy * rz;
-        ok = ok & (qz > 1e-6f) & (u >= 0.0f) & (v >= 0.0f) & (u <= wmax) & (v <= hmax);
-        u = fminf(fmaxf(u, 0.0f), wmax);
-        v = fminf(fmaxf(v, 0.0f), hmax);
-        float x0 = floorf(u), y0 = floorf(v);
-        float fx = u - x0, fy = v - y0;
-        uint32_t q = img[__umul24((uint32_t)y0, stride) + (uint32_t)x0];
-        float a = (float)(q & 0xffu), b = (float)((q >> 8) & 0xffu);
-        float c = (float)((q >> 16) & 0xffu), e = (float)(q >> 24);
-        float top = fmaf(fx, b - a, a);
-        float bot = fmaf(fx, e - c, c);
-        float val = fmaf(fy, bot - top, top);
-        float wv = win.w[i] * val;
-        ss = ss + wv;
-        sss = fmaf(wv, val, sss);
-        srs = fmaf(wv, win.v[i], srs);
-    }
+    // |q.z| over the window is at most |H6| (|px| + 5) + |H7| (|py| + 5) + |H8|
+    const float bound = fabsf(H.m[6]) * (fabsf(px) + 5.0f) + fabsf(H.m[7]) * (fabsf(py) + 5.0f) +
+                        fabsf(H.m[8]);
+    if (__all(bound < 0x1p60f))
+        view_taps<true>(img, s.W, s.H, H, px, py, win, ss, sss, srs, ok);
+    else
+        view_taps<false>(img, s.W, s.H, H, px, py, win, ss, sss, srs, ok);
     if (!ok) return kInvalid;
     float mean = ss / win.sw;
     float var_s = fmaxf(sss / win.sw - mean * mean, 0.0f);
@@ -514,67 +548,71 @@ __global__ void __launch_bounds__(256) pm_propagate_kernel(PmPlan plan, PmPools 
     float4 best = hyp[index];
     float best_cost = cost[index];
 
-    for (int region = 0; region < 8; region++) {
-        const int count = region < 4 ? 7 : 11;
-        int32_t chosen = -1;
-        float chosen_cost = 1e9f;
-        for (int i = 0; i < count; i++) {
-            int ox, oy;
-            region_offset(region, i, ox, oy);
-            const int32_t qx = x + ox, qy = y + oy;
-            if (qx < 0 || qy < 0 || qx >= W || qy >= H) continue;
-            const int32_t qi = qy * W + qx;
-            const float qc = cost[qi];
-            if (qc < chosen_cost) {
-                chosen_cost = qc;
-                chosen = qi;
-            }
-        }
-        if (chosen < 0) continue;
-        const float4 h = hyp[chosen];
-        const float qrx = ((float)(chosen % W) - d.k[2]) / d.k[0];
-        const float qry = ((float)(chosen / W) - d.k[3]) / d.k[1];
-        const float den = dot3(h.x, h.y, h.z, rx, ry, 1.0f);
-        if (fabsf(den) < 1e-6f) continue;
-        const float cd = dot3(h.x, h.y, h.z, qrx * h.w, qry * h.w, 1.0f * h.w) / den;
-        const float c = hypothesis_cost<KMAX, GEOMETRIC>(d, pools, px, py, rx, ry, win, h.x, h.y,
-                                                         h.z, cd);
-        if (c < best_cost) {
-            best_cost = c;
-            best = make_float4(h.x, h.y, h.z, cd);
-        }
-    }
-
-    // PM:401-420: random and perturbed refinement.
-    float vx = rx, vy = ry, vz = 1.0f;
-    normalize3(vx, vy, vz);
-    float rnx, rny, rnz;
-    random_normal(rng, rx, ry, rnx, rny, rnz);
-    const float d_random = random_depth(rng, d);
-    const float d_perturbed =
-        1.0f / ((1.0f / best.w) * (1.0f + ((next_random(rng) * 2.0f - 1.0f) * 0.15f) * step));
-    const float r1 = next_random(rng), r2 = next_random(rng), r3 = next_random(rng);
-    float pnx = best.x + ((r1 * 2.0f - 1.0f) * 0.6f) * step;
-    float pny = best.y + ((r2 * 2.0f - 1.0f) * 0.6f) * step;
-    float pnz = best.z + ((r3 * 2.0f - 1.0f) * 0.6f) * step;
-    normalize3(pnx, pny, pnz);
-    if (!(dot3(pnx, pny, pnz, vx, vy, vz) < 0.0f)) {
-        pnx = best.x;
-        pny = best.y;
-        pnz = best.z;
-    }
-    const float cnx = best.x, cny = best.y, cnz = best.z, cdepth = best.w;
-#pragma unroll
-    for (int c = 0; c < 6; c++) {
-        // (n_r,d_r) (n_p,d_p) (n,d_r) (n_r,d) (n,d_p) (n_p,d)
+    // Fourteen candidates in the reference's order, through one evaluation site: the plane of
+    // the cheapest neighbour of each of the 8 regions (PM:365-399), then the 6 random /
+    // perturbed planes (PM:401-420), drawn once the regions are done.
+    float rnx = 0.0f, rny = 0.0f, rnz = 0.0f, pnx = 0.0f, pny = 0.0f, pnz = 0.0f;
+    float cnx = 0.0f, cny = 0.0f, cnz = 0.0f, cdepth = 0.0f, d_random = 0.0f, d_perturbed = 0.0f;
+#pragma unroll 1
+    for (int c = 0; c < 14; c++) {
         float tx, ty, tz, td;
-        switch (c) {
-        case 0: tx = rnx; ty = rny; tz = rnz; td = d_random; break;
-        case 1: tx = pnx; ty = pny; tz = pnz; td = d_perturbed; break;
-        case 2: tx = cnx; ty = cny; tz = cnz; td = d_random; break;
-        case 3: tx = rnx; ty = rny; tz = rnz; td = cdepth; break;
-        case 4: tx = cnx; ty = cny; tz = cnz; td = d_perturbed; break;
-        default: tx = pnx; ty = pny; tz = pnz; td = cdepth; break;
+        if (c < 8) {
+            const int region = c;
+            const int count = region < 4 ? 7 : 11;
+            int32_t chosen = -1;
+            float chosen_cost = 1e9f;
+            for (int i = 0; i < count; i++) {
+                int ox, oy;
+                region_offset(region, i, ox, oy);
+                const int32_t qx = x + ox, qy = y + oy;
+                if (qx < 0 || qy < 0 || qx >= W || qy >= H) continue;
+                const int32_t qi = qy * W + qx;
+                const float qc = cost[qi];
+                if (qc < chosen_cost) {
+                    chosen_cost = qc;
+                    chosen = qi;
+                }
+            }
+            if (chosen < 0) continue;
+            const float4 h = hyp[chosen];
+            const float qrx = ((float)(chosen % W) - d.k[2]) / d.k[0];
+            const float qry = ((float)(chosen / W) - d.k[3]) / d.k[1];
+            const float den = dot3(h.x, h.y, h.z, rx, ry, 1.0f);
+            if (fabsf(den) < 1e-6f) continue;
+            tx = h.x;
+            ty = h.y;
+            tz = h.z;
+            td = dot3(h.x, h.y, h.z, qrx * h.w, qry * h.w, 1.0f * h.w) / den;
+        } else {
+            if (c == 8) {
+                float vx = rx, vy = ry, vz = 1.0f;
+                normalize3(vx, vy, vz);
+                random_normal(rng, rx, ry, rnx, rny, rnz);
+                d_random = random_depth(rng, d);
+                d_perturbed = 1.0f / ((1.0f / best.w) *
+                                      (1.0f + ((next_random(rng) * 2.0f - 1.0f) * 0.15f) * step));
+                const float r1 = next_random(rng), r2 = next_random(rng), r3 = next_random(rng);
+                pnx = best.x + ((r1 * 2.0f - 1.0f) * 0.6f) * step;
+                pny = best.y + ((r2 * 2.0f - 1.0f) * 0.6f) * step;
+                pnz = best.z + ((r3 * 2.0f - 1.0f) * 0.6f) * step;
+                normalize3(pnx, pny, pnz);
+                if (!(dot3(pnx, pny, pnz, vx, vy, vz) < 0.0f)) {
+                    pnx = best.x;
+                    pny = best.y;
+                    pnz = best.z;
+                }
+                cnx = best.x;
+                cny = best.y;
+                cnz = best.z;
+                cdepth = best.w;
+            }
+            // (n_r,d_r) (n_p,d_p) (n,d_r) (n_r,d) (n,d_p) (n_p,d)
+            const int k = c - 8;
+            const bool nr = k == 0 || k == 3, np = k == 1 || k == 5;
+            tx = nr ? rnx : (np ? pnx : cnx);
+            ty = nr ? rny : (np ? pny : cny);
+            tz = nr ? rnz : (np ? pnz : cnz);
+            td = (k == 0 || k == 2) ? d_random : ((k == 1 || k == 4) ? d_perturbed : cdepth);
         }
         const float cand = hypothesis_cost<KMAX, GEOMETRIC>(d, pools, px, py, rx, ry, win, tx, ty,
                                                             tz, td);

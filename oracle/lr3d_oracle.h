@@ -16,7 +16,8 @@
  * Arithmetic definition.  WGSL leaves the precision of division, sqrt, exp/sin/cos, texture
  * filtering and FMA contraction to the implementation, so the reference itself has no single
  * bit pattern.  The oracle fixes one (and the HIP kernels follow the same one):
- *   - FP32, no contraction (-ffp-contract=off); fmaf() only where written out;
+ *   - FP32, no contraction (-ffp-contract=off); fmaf() only where written out (a homography
+ *     tap is q = fma(H_0, X, fma(H_1, Y, H_2)) per row);
  *   - division and sqrt are IEEE correctly rounded;
  *   - exp, sin, cos are evaluated in double and rounded to float;
  *   - images are sampled in u8 units (0..255) by an FP32 bilinear filter with clamp-to-edge;

@@ -347,9 +347,9 @@ float orc_view_cost(const orc_params *p, const orc_images *img, int32_t s, const
     float ss = 0.0f, sss = 0.0f, srs = 0.0f;
     for (int i = 0; i < SAMPLES; i++) {
         float X = (float)(x + offset_x(i)), Y = (float)(y + offset_y(i));
-        float qx = fmaf(H[1], Y, fmaf(H[0], X, H[2]));
-        float qy = fmaf(H[4], Y, fmaf(H[3], X, H[5]));
-        float qz = fmaf(H[7], Y, fmaf(H[6], X, H[8]));
+        float qx = fmaf(H[0], X, fmaf(H[1], Y, H[2]));
+        float qy = fmaf(H[3], X, fmaf(H[4], Y, H[5]));
+        float qz = fmaf(H[6], X, fmaf(H[7], Y, H[8]));
         if (!(qz > 1e-6f)) return INVALID;
         float rz = 1.0f / qz;
         float u = qx * rz, v = qy * rz;

@@ -234,12 +234,13 @@ __device__ __forceinline__ void view_taps(const uint32_t *__restrict__ img, int3
             const float qz = fmaf(Hm.m[6], X, rw);
             const float rz = FAST ? rcp_exact(qz) : 1.0f / qz;
             const float u = qx * rz, v = qy * rz;
-            ok = ok & (qz > 1e-6f) & (u >= 0.0f) & (v >= 0.0f) & (u <= wmax) & (v <= hmax);
-            // Valid taps have 0 <= u <= W-1, so trunc(u) = floor(u) and fract(u) = u - floor(u)
-            // exactly; the clamp (NaN-safe: maxnum returns the number) keeps invalid taps inside
-            // the image.
-            const int32_t ix = (int32_t)fminf(fmaxf(u, 0.0f), wmax);
-            const int32_t iy = (int32_t)fminf(fmaxf(v, 0.0f), hmax);
+            // Clamp into the image (NaN-safe: maxnum returns the number); the tap is inside
+            // exactly when the clamp leaves it unchanged (a NaN never compares equal).  Valid
+            // taps have 0 <= u <= W-1, so trunc(u) = floor(u) and fract(u) = u - floor(u).
+            const float uc = fminf(fmaxf(u, 0.0f), wmax), vc = fminf(fmaxf(v, 0.0f), hmax);
+            ok = ok & (qz > 1e-6f) & (uc == u) & (vc == v);
+            const int32_t ix = (int32_t)uc;
+            const int32_t iy = (int32_t)vc;
             const float fx = __builtin_amdgcn_fractf(u), fy = __builtin_amdgcn_fractf(v);
             const uint32_t q = img[__umul24((uint32_t)iy, stride) + (uint32_t)ix];
             const float a = (float)(q & 0xffu), b = (float)((q >> 8) & 0xffu);
@@ -255,18 +256,115 @@ __device__ __forceinline__ void view_taps(const uint32_t *__restrict__ img, int3
     }
 }
 
+// Taps of a window known to lie inside the source image (see view_cost): no per-tap checks.
+// The linear index is still clamped to the image, so no tap can read outside it.
+__device__ __forceinline__ void view_taps_inside(const uint32_t *__restrict__ img, int32_t W, int32_t H,
+                                                 const Mat3 &Hm, float px, float py,
+                                                 const Window &win, float &ss, float &sss,
+                                                 float &srs) {
+    const uint32_t stride = (uint32_t)W;
+    const uint32_t last = (uint32_t)W * (uint32_t)H - 1u;
+#pragma unroll
+    for (int row = 0; row < 6; row++) {
+        const float Y = py + (float)(row * 2 - 5);
+        const float rx = fmaf(Hm.m[1], Y, Hm.m[2]);
+        const float ry = fmaf(Hm.m[4], Y, Hm.m[5]);
+        const float rw = fmaf(Hm.m[7], Y, Hm.m[8]);
+#pragma unroll
+        for (int col = 0; col < 6; col++) {
+            const int i = row * 6 + col;
+            const float X = px + (float)(col * 2 - 5);
+            const float qx = fmaf(Hm.m[0], X, rx);
+            const float qy = fmaf(Hm.m[3], X, ry);
+            const float qz = fmaf(Hm.m[6], X, rw);
+            const float rz = rcp_exact(qz);
+            const float u = qx * rz, v = qy * rz;
+            const uint32_t ix = (uint32_t)u, iy = (uint32_t)v;
+            const float fx = __builtin_amdgcn_fractf(u), fy = __builtin_amdgcn_fractf(v);
+            const uint32_t q = img[min(__umul24(iy, stride) + ix, last)];
+            const float a = (float)(q & 0xffu), b = (float)((q >> 8) & 0xffu);
+            const float c = (float)((q >> 16) & 0xffu), e = (float)(q >> 24);
+            const float top = fmaf(fx, b - a, a);
+            const float bot = fmaf(fx, e - c, c);
+            const float val = fmaf(fy, bot - top, top);
+            const float wv = win.w[i] * val;
+            ss = ss + wv;
+            sss = fmaf(wv, val, sss);
+            srs = fmaf(wv, win.v[i], srs);
+        }
+    }
+}
+
+// Window classes of one source view, decided from the four corner taps (offsets (+-5, +-5)):
+//   kOutside  a corner tap is invalid, so the view is INVALID (the reference stops at the first
+//             invalid tap; every tap, corners included, is evaluated exactly as in the full loop);
+//   kInside   every corner lies inside the image by more than a rounding margin: q.z is affine in
+//             (X, Y) and positive on the window, so the warped window is the convex hull of the
+//             warped corners, and every tap is inside as computed;
+//   kCareful  anything else (windows touching the border, huge |q.z|): per-tap checks.
+enum { kOutside = 0, kInside = 1, kCareful = 2 };
+
+__device__ __forceinline__ int classify_window(const PmSrc &s, const Mat3 &H, float px, float py) {
+    const float wmax = (float)s.W - 1.0f, hmax = (float)s.H - 1.0f;
+    const float ax = fabsf(px) + 5.0f, ay = fabsf(py) + 5.0f;
+    // bounds on the magnitude of the terms of q.x, q.y, q.z over the window
+    const float bx = fabsf(H.m[0]) * ax + fabsf(H.m[1]) * ay + fabsf(H.m[2]);
+    const float by = fabsf(H.m[3]) * ax + fabsf(H.m[4]) * ay + fabsf(H.m[5]);
+    const float bz = fabsf(H.m[6]) * ax + fabsf(H.m[7]) * ay + fabsf(H.m[8]);
+    if (!(bz < 0x1p60f)) return kCareful;  // outside the range where rcp_exact == 1/q.z
+    bool valid = true;
+    float zmin = INFINITY, umin = INFINITY, umax = -INFINITY, vmin = INFINITY, vmax = -INFINITY;
+#pragma unroll
+    for (int corner = 0; corner < 4; corner++) {
+        const float X = px + ((corner & 1) ? 5.0f : -5.0f);
+        const float Y = py + ((corner & 2) ? 5.0f : -5.0f);
+        const float qx = fmaf(H.m[0], X, fmaf(H.m[1], Y, H.m[2]));
+        const float qy = fmaf(H.m[3], X, fmaf(H.m[4], Y, H.m[5]));
+        const float qz = fmaf(H.m[6], X, fmaf(H.m[7], Y, H.m[8]));
+        const float rz = rcp_exact(qz);
+        const float u = qx * rz, v = qy * rz;
+        valid = valid & (qz > 1e-6f) & (u >= 0.0f) & (v >= 0.0f) & (u <= wmax) & (v <= hmax);
+        zmin = fminf(zmin, qz);
+        umin = fminf(umin, u);
+        umax = fmaxf(umax, u);
+        vmin = fminf(vmin, v);
+        vmax = fmaxf(vmax, v);
+    }
+    if (!valid) return kOutside;
+    // Each computed q component is within 2^-20 of its term bound of the exact value (two FMAs
+    // round at most 2 ulp of the larger term), so exact q.z >= zmin - ez on the window.  A
+    // computed u = q.x / q.z then lies within eu of the exact ratio; require corners inside
+    // by 4 eu so that exact and computed interior taps stay inside.
+    const float ez = bz * 0x1p-20f;
+    const float zlow = zmin - 2.0f * ez;
+    if (!(zlow > 2e-6f)) return kCareful;
+    const float ilow = 1.0f / zlow;
+    const float eu = (bx * 0x1p-20f + (wmax + 1.0f) * ez) * ilow + (wmax + 1.0f) * 0x1p-20f;
+    const float ev = (by * 0x1p-20f + (hmax + 1.0f) * ez) * ilow + (hmax + 1.0f) * 0x1p-20f;
+    const bool inside = umin >= 4.0f * eu && umax <= wmax - 4.0f * eu && vmin >= 4.0f * ev &&
+                        vmax <= hmax - 4.0f * ev;
+    return inside ? kInside : kCareful;
+}
+
 __device__ __forceinline__ float view_cost(const PmSrc &s, const Mat3 &H, float px, float py,
                                            const Window &win, const uint32_t *__restrict__ quad) {
     const uint32_t *__restrict__ img = quad + s.off;
+    const int cls = classify_window(s, H, px, py);
     float ss = 0.0f, sss = 0.0f, srs = 0.0f;
-    bool ok = true;
-    // |q.z| over the window is at most |H6| (|px| + 5) + |H7| (|py| + 5) + |H8|
-    const float bound = fabsf(H.m[6]) * (fabsf(px) + 5.0f) + fabsf(H.m[7]) * (fabsf(py) + 5.0f) +
-                        fabsf(H.m[8]);
-    if (__all(bound < 0x1p60f))
-        view_taps<true>(img, s.W, s.H, H, px, py, win, ss, sss, srs, ok);
-    else
-        view_taps<false>(img, s.W, s.H, H, px, py, win, ss, sss, srs, ok);
+    bool ok = cls != kOutside;
+    if (__any(cls == kCareful)) {
+        // the checked loop is exact for every class; lanes already known outside stay invalid
+        const float bound = fabsf(H.m[6]) * (fabsf(px) + 5.0f) + fabsf(H.m[7]) * (fabsf(py) + 5.0f) +
+                            fabsf(H.m[8]);
+        bool tap_ok = true;
+        if (__all(bound < 0x1p60f))
+            view_taps<true>(img, s.W, s.H, H, px, py, win, ss, sss, srs, tap_ok);
+        else
+            view_taps<false>(img, s.W, s.H, H, px, py, win, ss, sss, srs, tap_ok);
+        ok = ok & tap_ok;
+    } else if (__any(cls == kInside)) {
+        if (cls == kInside) view_taps_inside(img, s.W, s.H, H, px, py, win, ss, sss, srs);
+    }
     if (!ok) return kInvalid;
     float mean = ss / win.sw;
     float var_s = fmaxf(sss / win.sw - mean * mean, 0.0f);
@@ -525,8 +623,12 @@ __device__ __forceinline__ void region_offset(int region, int i, int &ox, int &o
 }
 
 // PM:349-423: one red-black phase.
+#ifndef LR3D_PM_WAVES_PER_EU
+#define LR3D_PM_WAVES_PER_EU 1
+#endif
 template <int KMAX, bool GEOMETRIC>
-__global__ void __launch_bounds__(256) pm_propagate_kernel(PmPlan plan, PmPools pools,
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(LR3D_PM_WAVES_PER_EU, 8)))
+pm_propagate_kernel(PmPlan plan, PmPools pools,
                                                            uint32_t iteration, uint32_t phase,
                                                            float step) {
     PixelTask task = locate(plan, 2 * (int32_t)blockDim.x);

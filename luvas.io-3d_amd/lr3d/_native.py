@@ -10,7 +10,7 @@ import ctypes as C
 import os
 
 LIB_DIR = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "lib")
-LIB_PATH = os.path.join(LIB_DIR, "liblr3d.so")
+LIB_PATH = os.environ.get("LR3D_LIBRARY") or os.path.join(LIB_DIR, "liblr3d.so")
 
 LR3D_OK = 0
 LR3D_ERR_INVALID = -1

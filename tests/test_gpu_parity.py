@@ -216,3 +216,103 @@ def test_tsdf_batched_equals_sequential(ctx, work, oracle_maps):
         x.destroy()
     assert np.array_equal(ta.view(np.uint32), tb.view(np.uint32))
     assert np.array_equal(wa.view(np.uint32), wb.view(np.uint32))
+
+
+def test_geometric_pass_matches_oracle(ctx, work, oracle_maps):
+    say("case: geometric (ACMM) pass of view 0 over the oracle's photometric maps")
+    depths = oracle_maps[0].copy()
+    st = make_stereo(ctx, work)
+    n = 320 * 240
+    for ref in range(len(work.views)):
+        st.write_depth(ref, depths[ref * n:(ref + 1) * n])
+    st.solve(0, opts(work, 0, geometric=True, iterations=2))
+    gpu = st.depth_map(0)
+    hyp, _ = st.hypotheses(0)
+    st.destroy()
+    imgs = O.Images(work.images)
+    _, ohyp, _ = O.solve(work.views, imgs, 0, work.sources[0], *work.ranges[0], geometric=True,
+                         iterations=2, topk=work.topk, depths=depths)
+    exact = np.all(hyp.view(np.uint32) == ohyp.view(np.uint32), axis=1).mean()
+    agree = depth_agreement(gpu, depths[:n])
+    say(f"geometric: planes bit-exact {exact:.6f}, depth agreement {agree:.6f}")
+    assert agree >= 0.995 and exact >= 0.99
+
+
+def test_prior_matches_oracle(ctx, work):
+    say("case: solve with a phone-depth prior (noisy truth), view 0")
+    rng = np.random.default_rng(7)
+    truth = work.truth[0].reshape(-1)
+    prior = np.where(rng.random(truth.size) < 0.7, truth * (1 + 0.02 * rng.standard_normal(truth.size)),
+                     0).astype(np.float32)
+    st = make_stereo(ctx, work)
+    st.solve(0, opts(work, 0, prior=prior, iterations=1))
+    hyp, cost = st.hypotheses(0)
+    st.destroy()
+    imgs = O.Images(work.images)
+    _, ohyp, ocost = O.solve(work.views, imgs, 0, work.sources[0], *work.ranges[0], prior=prior,
+                             iterations=1, topk=work.topk)
+    exact = np.all(hyp.view(np.uint32) == ohyp.view(np.uint32), axis=1).mean()
+    say(f"prior: planes bit-exact {exact:.6f}")
+    assert exact >= 0.99
+
+
+def test_coarse_to_fine_matches_oracle(ctx, work):
+    say("case: coarse-to-fine (160x120 coarse level, 320x240 fine), view 0")
+    from lr3d import HipDenseStereo, StereoView, scale_intrinsics
+    cw, ch = 160, 120
+    cviews = [StereoView(scale_intrinsics(v.intrinsics, cw / v.width, ch / v.height), v.pose, cw, ch)
+              for v in work.views]
+    cimages = [work.scene.render(v, depth=False)[0].reshape(ch, cw) for v in cviews]
+    coarse = HipDenseStereo(ctx, cviews)
+    for i, im in enumerate(cimages):
+        coarse.set_image(i, im)
+    fine = make_stereo(ctx, work)
+    fine.set_coarse(coarse)
+    coarse.solve(0, opts(work, 0, iterations=2, pack=False))
+    fine.solve(0, opts(work, 0, iterations=1, from_coarse=True))
+    hyp, cost = fine.hypotheses(0)
+    chyp, _ = coarse.hypotheses(0)
+    fine.destroy()
+    coarse.destroy()
+    # oracle: coarse solve, then init_coarse + propagate + pack at the fine level
+    _, ochyp, _ = O.solve(cviews, O.Images(cimages), 0, work.sources[0], *work.ranges[0],
+                          iterations=2, topk=work.topk, pack=False)
+    assert np.array_equal(chyp.view(np.uint32), ochyp.view(np.uint32))
+    imgs = O.Images(work.images)
+    ck = scale_intrinsics(work.views[0].intrinsics, cw / 320, ch / 240)
+    p = O.make_params(work.views, 0, work.sources[0], *work.ranges[0], topk=work.topk)
+    ohyp, ocost = O.init_coarse(p, imgs, [ck.fx, ck.fy, ck.cx, ck.cy], cw, ch, ochyp)
+    for phase in (0, 1):
+        p = O.make_params(work.views, 0, work.sources[0], *work.ranges[0], iteration=0, phase=phase,
+                          step=1.0, topk=work.topk)
+        O.propagate(p, imgs, ohyp, ocost)
+    exact = np.all(hyp.view(np.uint32) == ohyp.view(np.uint32), axis=1).mean()
+    say(f"coarse-to-fine: planes bit-exact {exact:.6f}")
+    assert exact >= 0.99
+
+
+def test_mixed_resolution_batch_matches_oracle(ctx, work):
+    say("case: one batch of views at 320x240, 400x300 and 256x192")
+    from lr3d import HipDenseStereo, SolveOptions, StereoView, scale_intrinsics
+    sizes = [(320, 240), (400, 300), (256, 192)]
+    views, images = [], []
+    for i, v in enumerate(work.views[:6]):
+        w, h = sizes[i % 3]
+        k = scale_intrinsics(v.intrinsics, w / v.width, h / v.height)
+        sv = StereoView(k, v.pose, w, h)
+        views.append(sv)
+        images.append(work.scene.render(sv, depth=False)[0].reshape(h, w))
+    srcs = {0: [1, 5, 2], 1: [0, 2, 3, 5], 2: [1, 3]}
+    st = HipDenseStereo(ctx, views)
+    for i, im in enumerate(images):
+        st.set_image(i, im)
+    st.solve_batched([(r, SolveOptions(sources=s, min_depth=0.5, max_depth=3.0, iterations=2))
+                      for r, s in srcs.items()])
+    imgs = O.Images(images)
+    for r, s in srcs.items():
+        hyp, _ = st.hypotheses(r)
+        _, ohyp, _ = O.solve(views, imgs, r, s, 0.5, 3.0, iterations=2)
+        exact = np.all(hyp.view(np.uint32) == ohyp.view(np.uint32), axis=1).mean()
+        say(f"mixed: view {r} ({views[r].width}x{views[r].height}, {len(s)} sources) bit-exact {exact:.6f}")
+        assert exact >= 0.99
+    st.destroy()

@@ -1,0 +1,160 @@
+"""Pins of the oracle's schedule: the red-black colouring and region offsets (PM:335-352), the
+propagation control flow and RNG consumption (PM:349-423, replayed here in NumPy float32 with
+the cost taken from the oracle), and the reference's own known-answer check of PatchMatch
+(frontend/tools/verify/stereo-check.ts:56-72) on analytic ground truth."""
+import math
+
+import numpy as np
+import pytest
+
+import oracle as O
+from lr3d import StereoView, scale_intrinsics, scenes
+
+f32 = np.float32
+
+# The reference's neighbour tables as data (PM:335-347): the 7 "V" taps of regions 0-3 and the
+# 11 line taps of regions 4-7, before orientation.
+NEAR = [(0, -1), (-1, -2), (1, -2), (-2, -3), (2, -3), (-3, -4), (3, -4)]
+FAR = [(0, -3 - 2 * i) for i in range(11)]
+
+
+def orient(o, direction):
+    x, y = o
+    return [(x, y), (x, -y), (y, x), (-y, x)][direction]
+
+
+def region_taps(region):
+    return [orient(o, region % 4) for o in (NEAR if region < 4 else FAR)]
+
+
+def test_every_region_tap_reads_the_other_colour():
+    # race freedom of red-black propagation: a pixel only reads pixels of the opposite colour
+    for region in range(8):
+        taps = region_taps(region)
+        assert len(taps) == (7 if region < 4 else 11)
+        for dx, dy in taps:
+            assert (abs(dx) + abs(dy)) % 2 == 1
+
+
+def test_regions_are_disjoint_and_cover_four_directions():
+    seen = set()
+    for region in range(8):
+        for t in region_taps(region):
+            assert t not in seen
+            seen.add(t)
+    assert len(seen) == 4 * 7 + 4 * 11
+
+
+def _dot(a, b):
+    return f32(f32(f32(a[0] * b[0]) + f32(a[1] * b[1])) + f32(a[2] * b[2]))
+
+
+def _normalize(v):
+    inv = f32(f32(1) / np.sqrt(_dot(v, v)))
+    return np.array([v[0] * inv, v[1] * inv, v[2] * inv], dtype=f32)
+
+
+def _replay(p, imgs, hyp, cost):
+    """PM:349-423 for one phase, from the reference text."""
+    W, H = p.width, p.height
+    k = [f32(v) for v in p.k]
+    out_h, out_c = hyp.copy(), cost.copy()
+    step = f32(p.step)
+    for y in range(H):
+        for g in range((W + 1) // 2):
+            x = 2 * g + ((y + p.phase) & 1)
+            if x >= W:
+                continue
+            i = y * W + x
+            win = O.window(p, imgs, x, y)
+            ray = np.array([(f32(x) - k[2]) / k[0], (f32(y) - k[3]) / k[1], f32(1)], dtype=f32)
+            best = hyp[i].copy()
+            best_cost = f32(cost[i])
+            for region in range(8):
+                chosen, chosen_cost = -1, f32(1e9)
+                for dx, dy in region_taps(region):
+                    qx, qy = x + dx, y + dy
+                    if qx < 0 or qy < 0 or qx >= W or qy >= H:
+                        continue
+                    if cost[qy * W + qx] < chosen_cost:
+                        chosen_cost, chosen = cost[qy * W + qx], qy * W + qx
+                if chosen < 0:
+                    continue
+                h = hyp[chosen]
+                qray = np.array([(f32(chosen % W) - k[2]) / k[0], (f32(chosen // W) - k[3]) / k[1], f32(1)],
+                                dtype=f32)
+                den = _dot(h[:3], ray)
+                if abs(den) < f32(1e-6):
+                    continue
+                d = f32(_dot(h[:3], qray * h[3]) / den)
+                c = O.hypothesis_cost(p, imgs, x, y, win, h[:3], float(d))
+                if c < best_cost:
+                    best_cost, best = f32(c), np.array([h[0], h[1], h[2], d], dtype=f32)
+            r = [f32(v) for v in O.random_stream(O.pixel_rng(p.seed, p.iteration, p.phase, i), 7)]
+            view = _normalize(ray)
+            z = f32(r[0] * f32(2) - f32(1))
+            a = f32(r[1] * f32(6.2831853))
+            rr = np.sqrt(max(f32(0), f32(f32(1) - z * z)))
+            n = np.array([rr * f32(math.cos(float(a))), rr * f32(math.sin(float(a))), z], dtype=f32)
+            if _dot(n, view) > 0:
+                n = -n
+            n_random = _normalize(n - view * f32(0.5))
+            d_random = f32(f32(1) / f32(f32(p.inv_max) * f32(f32(1) - r[2]) + f32(p.inv_min) * r[2]))
+            d_pert = f32(f32(1) / f32(f32(f32(1) / best[3]) *
+                                      f32(f32(1) + f32(f32(f32(r[3] * f32(2) - f32(1)) * f32(0.15)) * step))))
+            n_pert = _normalize(np.array([best[j] + f32(f32(f32(r[4 + j] * f32(2) - f32(1)) * f32(0.6)) * step)
+                                          for j in range(3)], dtype=f32))
+            if not _dot(n_pert, view) < 0:
+                n_pert = best[:3].copy()
+            cur_n, cur_d = best[:3].copy(), best[3]
+            for nn, dd in ((n_random, d_random), (n_pert, d_pert), (cur_n, d_random), (n_random, cur_d),
+                           (cur_n, d_pert), (n_pert, cur_d)):
+                c = O.hypothesis_cost(p, imgs, x, y, win, nn, float(dd))
+                if c < best_cost:
+                    best_cost, best = f32(c), np.array([nn[0], nn[1], nn[2], dd], dtype=f32)
+            out_h[i], out_c[i] = best, best_cost
+    return out_h, out_c
+
+
+@pytest.fixture(scope="module")
+def tiny():
+    base = scenes.sphere_small()
+    views = [StereoView(scale_intrinsics(v.intrinsics, 48 / 320), v.pose, 48, 36) for v in base.views[:3]]
+    images = [base.scene.render(v, depth=False)[0].reshape(36, 48) for v in views]
+    return views, O.Images(images)
+
+
+@pytest.mark.parametrize("phase", [0, 1])
+def test_propagate_matches_a_replay_of_the_reference_control_flow(tiny, phase):
+    views, imgs = tiny
+    p0 = O.make_params(views, 0, [1, 2], 0.5, 3.0)
+    hyp, cost = O.init(p0, imgs)
+    p = O.make_params(views, 0, [1, 2], 0.5, 3.0, iteration=1, phase=phase, step=0.5)
+    want_h, want_c = _replay(p, imgs, hyp, cost)
+    got_h, got_c = O.propagate(p, imgs, hyp.copy(), cost.copy())
+    assert np.array_equal(got_h.view(np.uint32), want_h.view(np.uint32))
+    assert np.array_equal(got_c.view(np.uint32), want_c.view(np.uint32))
+    # only the active colour changed
+    ys, xs = np.divmod(np.arange(48 * 36), 48)
+    other = ((xs + ys + phase) & 1) == 1
+    assert np.array_equal(got_h[other], hyp[other])
+
+
+def test_patchmatch_recovers_depth_on_true_poses():
+    """The reference's known-answer check (stereo-check.ts:69-72): on true poses, > 60 % of
+    surface pixels valid and median relative depth error < 0.4 %."""
+    w = scenes.sphere_small().render([0, 1, 2, 6, 7])
+    imgs = O.Images([im if im is not None else np.zeros((240, 320), np.uint8) for im in w.images])
+    depths, _, _ = O.solve(w.views, imgs, 0, w.sources[0], *w.ranges[0], iterations=w.iterations,
+                           topk=w.topk)
+    packed = depths[:320 * 240]
+    truth = w.truth[0].reshape(-1)
+    covered = truth > 0
+    valid = covered & (packed != 0)
+    err = np.abs(O_unpack(packed[valid]) - truth[valid]) / truth[valid]
+    assert valid.sum() > 0.6 * covered.sum()
+    assert np.median(err) < 0.004
+
+
+def O_unpack(packed):
+    return (packed & 0xFFFF).astype(np.uint16).view(np.float16).astype(np.float32)

@@ -1,0 +1,302 @@
+'use strict';
+/*
+ * Host side of the lr3d path for Node / TypeScript (types: lr3d.d.ts).
+ *
+ * Drop-in counterparts of the reference's stage objects, over the N-API addon lr3d.node:
+ *   HipDenseStereo  ~ DenseStereo   (frontend/src/app/vision/mvs/dense-stereo.ts)
+ *   HipTsdfVolume   ~ TsdfVolume    (frontend/src/app/vision/fusion/tsdf-volume.ts)
+ *   BrickGrid       ~ BrickGrid     (same file, :13-65)
+ *   STEREO_METHODS  ~ the StereoMethod registry (frontend/src/app/vision/mvs/methods/index.ts)
+ * Same member names, argument meaning and await points; GPU errors reject the promises.
+ * Written for Node 12 (CommonJS, no optional chaining).
+ */
+const path = require('path');
+
+const native = require(path.join(__dirname, '..', 'lib', 'lr3d.node'));
+
+const MAP_RAW = 0;
+const MAP_FILTERED = 1;
+const MAX_SOURCES = 8;
+const BRICK = 8;
+
+class Context {
+    /** Opens HIP device `device` (the reference's requestGpu()). */
+    constructor(device) {
+        this.handle = native.ctxCreate(device === undefined ? 0 : device);
+    }
+    destroy() {
+        if (this.handle) native.ctxDestroy(this.handle);
+        this.handle = null;
+    }
+}
+
+function cameraOf(view, width, height) {
+    const k = view.intrinsics;
+    return {
+        width: view.width !== undefined ? view.width : width,
+        height: view.height !== undefined ? view.height : height,
+        fx: k.fx, fy: k.fy, cx: k.cx, cy: k.cy,
+        R: Array.from(view.pose.R), t: Array.from(view.pose.t),
+    };
+}
+
+class HipDenseStereo {
+    /**
+     * new DenseStereo(device, width, height, views, options) (DS:126-218).  `views` may carry
+     * their own width/height (mixed-resolution sets); options.coarse is the coarse level.
+     */
+    constructor(ctx, width, height, views, options) {
+        this.ctx = ctx;
+        this.width = width;
+        this.height = height;
+        this.views = views;
+        this.cameras = views.map((v) => cameraOf(v, width, height));
+        this.handle = native.stereoCreate(ctx.handle, this.cameras);
+        this.lastFiltered = -1;
+        const coarse = options && options.coarse ? options.coarse : null;
+        if (coarse) native.stereoSetCoarse(this.handle, coarse.hypotheses.handle);
+    }
+
+    /** DS:221-228 */
+    setImage(layer, gray) {
+        const c = this.cameras[layer];
+        native.stereoSetImage(this.handle, layer, gray, c.width, c.height);
+    }
+
+    static descOf(ref, options) {
+        return {
+            ref,
+            sources: options.sources.slice(0, MAX_SOURCES),
+            minDepth: options.minDepth,
+            maxDepth: options.maxDepth,
+            prior: options.prior || null,
+            geometric: !!options.geometric,
+            iterations: options.iterations,
+            fromCoarse: !!options.fromCoarse,
+            pack: options.pack === undefined ? true : options.pack,
+            topk: options.topk || 0,
+        };
+    }
+
+    /** DS:231-285 */
+    async solve(ref, options) {
+        if (options.sources.length === 0) return;
+        await native.stereoSolve(this.handle, HipDenseStereo.descOf(ref, options));
+    }
+
+    /** Every (ref, options) pair in shared launches; the same result as solving them in order. */
+    async solveBatched(solves) {
+        const descs = solves.filter((s) => s[1].sources.length > 0)
+            .map((s) => HipDenseStereo.descOf(s[0], s[1]));
+        if (descs.length) await native.stereoSolve(this.handle, descs);
+    }
+
+    /** The planes of the last solve (DS:288-290): this level, usable as a coarse level. */
+    get hypotheses() {
+        return this;
+    }
+
+    /** DS:296-309 */
+    async filter(ref, sources, needed) {
+        await native.stereoFilter(this.handle, ref, sources.slice(0, MAX_SOURCES), needed);
+        this.lastFiltered = ref;
+    }
+
+    /** DS:347-352 */
+    async filteredMap(ref, sources, needed) {
+        await this.filter(ref, sources, needed);
+        const c = this.cameras[ref];
+        return native.stereoReadFiltered(this.handle, ref, c.width, c.height);
+    }
+
+    /** DS:355-358 */
+    async depthMap(ref) {
+        const c = this.cameras[ref];
+        return native.stereoReadDepth(this.handle, ref, c.width, c.height);
+    }
+
+    /** Planes (n.xyz, depth) and costs of a layer's last solve. */
+    async planes(ref) {
+        const c = this.cameras[ref];
+        return native.stereoReadHyp(this.handle, ref, c.width, c.height);
+    }
+
+    destroy() {
+        if (this.handle) native.stereoDestroy(this.handle);
+        this.handle = null;
+    }
+}
+
+class BrickGrid {
+    /** TV:17-29 */
+    constructor(origin, voxel, extent) {
+        this.origin = origin;
+        this.voxel = voxel;
+        const side = voxel * BRICK;
+        this.dims = extent.map((e) => Math.max(1, Math.ceil(e / side)));
+        this.marks = new Uint8Array(this.dims[0] * this.dims[1] * this.dims[2]);
+    }
+
+    get cells() {
+        return this.marks.length;
+    }
+
+    /** TV:36-51 */
+    mark(x, y, z, radius) {
+        const side = this.voxel * BRICK;
+        const o = this.origin;
+        const d = this.dims;
+        const lo = [x, y, z].map((v, a) => Math.max(0, Math.floor((v - radius - o[a]) / side)));
+        const hi = [x, y, z].map((v, a) => Math.min(d[a] - 1, Math.floor((v + radius - o[a]) / side)));
+        for (let bz = lo[2]; bz <= hi[2]; bz++) {
+            for (let by = lo[1]; by <= hi[1]; by++) {
+                for (let bx = lo[0]; bx <= hi[0]; bx++) this.marks[(bz * d[1] + by) * d[0] + bx] = 1;
+            }
+        }
+    }
+
+    /** TV:54-64 */
+    build() {
+        const lookup = new Int32Array(this.marks.length).fill(-1);
+        const coords = [];
+        const dx = this.dims[0];
+        const dy = this.dims[1];
+        for (let i = 0; i < this.marks.length; i++) {
+            if (!this.marks[i]) continue;
+            lookup[i] = coords.length / 4;
+            coords.push(i % dx, Math.floor(i / dx) % dy, Math.floor(i / (dx * dy)), 0);
+        }
+        return { coords: Int32Array.from(coords), lookup, count: coords.length / 4 };
+    }
+}
+
+class HipTsdfVolume {
+    /**
+     * new TsdfVolume(device, grid, layout, depth, width, height) (TV:171-217); `depth` is the
+     * stereo level whose filtered map is integrated (the reference binds `stereo.filtered`).
+     * truncationVoxels: 4 in the reference (TV:179).
+     */
+    constructor(ctx, grid, layout, stereo, width, height, truncationVoxels) {
+        this.ctx = ctx;
+        this.grid = grid;
+        this.layout = layout;
+        this.stereo = stereo;
+        this.width = width;
+        this.height = height;
+        this.truncation = grid.voxel * (truncationVoxels || 4);
+        this.handle = native.tsdfCreate(ctx.handle, grid.origin, grid.voxel, grid.dims, layout.coords,
+            truncationVoxels || 4);
+    }
+
+    static viewOf(view, width, height) {
+        const c = cameraOf(view, width, height);
+        c.maxDepth = view.maxDepth;
+        c.prior = view.prior || null;
+        c.confidence = view.confidence || 0;
+        return c;
+    }
+
+    /** TV:240-268: integrates the map the stereo level filtered last. */
+    async integrate(view) {
+        const layer = this.stereo.lastFiltered;
+        if (layer < 0) throw new Error('HipTsdfVolume.integrate: no filtered map yet (call stereo.filter first)');
+        await native.tsdfIntegrate(this.handle, this.stereo.handle, [layer], MAP_FILTERED,
+            [HipTsdfVolume.viewOf(view, this.width, this.height)]);
+    }
+
+    /** Several layers' maps (MAP_RAW or MAP_FILTERED) in one pass, applied in array order. */
+    async integrateLayers(layers, views, map) {
+        await native.tsdfIntegrate(this.handle, this.stereo.handle, layers, map === undefined ? MAP_RAW : map,
+            views.map((v) => HipTsdfVolume.viewOf(v, this.width, this.height)));
+    }
+
+    /** TV:270-277 */
+    async read() {
+        return native.tsdfRead(this.handle, this.layout.count);
+    }
+
+    destroy() {
+        if (this.handle) native.tsdfDestroy(this.handle);
+        this.handle = null;
+    }
+}
+
+async function uploadImages(stereo, input) {
+    for (let layer = 0; layer < input.views.length; layer++) {
+        stereo.setImage(layer, await input.gray(layer, stereo.width, stereo.height));
+    }
+}
+
+/** StereoMethod entries (methods/index.ts:7-10): create(ctx, input, settings) -> StereoRun. */
+const STEREO_METHODS = {
+    'hip-single-scale': {
+        async create(ctx, input, settings) {
+            const stereo = new HipDenseStereo(ctx, input.width, input.height, input.views);
+            try {
+                await uploadImages(stereo, input);
+            } catch (error) {
+                stereo.destroy();
+                throw error;
+            }
+            return {
+                stereo,
+                solve: (layer) => stereo.solve(layer, {
+                    sources: input.sources[layer],
+                    minDepth: input.ranges[layer][0],
+                    maxDepth: input.ranges[layer][1],
+                    prior: input.prior(layer, input.width, input.height),
+                    geometric: false,
+                    iterations: settings.iterations,
+                }),
+                destroy: () => stereo.destroy(),
+            };
+        },
+    },
+    'hip-coarse-to-fine': {
+        async create(ctx, input, settings) {
+            const cw = Math.round(input.width / 2);
+            const ch = Math.round(input.height / 2);
+            const scale = (k) => ({
+                fx: k.fx * cw / input.width, fy: k.fy * ch / input.height,
+                cx: (k.cx + 0.5) * cw / input.width - 0.5, cy: (k.cy + 0.5) * ch / input.height - 0.5,
+            });
+            const coarse = new HipDenseStereo(ctx, cw, ch,
+                input.views.map((v) => ({ pose: v.pose, intrinsics: scale(v.intrinsics) })));
+            const stereo = new HipDenseStereo(ctx, input.width, input.height, input.views, { coarse });
+            try {
+                await uploadImages(coarse, input);
+                await uploadImages(stereo, input);
+            } catch (error) {
+                stereo.destroy();
+                coarse.destroy();
+                throw error;
+            }
+            return {
+                stereo,
+                async solve(layer) {
+                    const range = {
+                        sources: input.sources[layer],
+                        minDepth: input.ranges[layer][0],
+                        maxDepth: input.ranges[layer][1],
+                        geometric: false,
+                    };
+                    await coarse.solve(layer, Object.assign({}, range, {
+                        prior: input.prior(layer, cw, ch), iterations: settings.iterations, pack: false,
+                    }));
+                    await stereo.solve(layer, Object.assign({}, range, {
+                        prior: null, iterations: settings.fineIterations, fromCoarse: true,
+                    }));
+                },
+                destroy() {
+                    stereo.destroy();
+                    coarse.destroy();
+                },
+            };
+        },
+    },
+};
+
+module.exports = {
+    Context, HipDenseStereo, HipTsdfVolume, BrickGrid, STEREO_METHODS, MAP_RAW, MAP_FILTERED,
+    MAX_SOURCES, BRICK, native,
+};

@@ -1,0 +1,516 @@
+/*
+ * lr3d.node — thin N-API binding of the lr3d C ABI (include/lr3d.h) for the TypeScript host.
+ *
+ * It binds exactly the calls the reference's stage objects make on the GPU:
+ * DenseStereo (frontend/src/app/vision/mvs/dense-stereo.ts) and TsdfVolume
+ * (frontend/src/app/vision/fusion/tsdf-volume.ts).  Handles are N-API externals; long calls
+ * (solve, filter, integrate, reads) run as async work on the libuv pool and return Promises, the
+ * way the reference awaits queue submissions.  Inputs are copied before the call returns.
+ */
+#define NAPI_VERSION 6
+#include <node_api.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+
+#include "../../include/lr3d.h"
+
+#define NAPI_CALL(env, call)                                                                       \
+    do {                                                                                           \
+        if ((call) != napi_ok) {                                                                   \
+            napi_throw_error((env), NULL, "lr3d.node: N-API call failed: " #call);               \
+            return NULL;                                                                           \
+        }                                                                                          \
+    } while (0)
+
+static napi_value throw_status(napi_env env, int32_t status) {
+    char msg[600];
+    snprintf(msg, sizeof(msg), "lr3d error %d: %s", (int)status, lr3d_last_error());
+    napi_throw_error(env, NULL, msg);
+    return NULL;
+}
+
+static napi_value undefined(napi_env env) {
+    napi_value u;
+    napi_get_undefined(env, &u);
+    return u;
+}
+
+/* ---- argument helpers ------------------------------------------------------------------------ */
+
+static void *get_external(napi_env env, napi_value v) {
+    void *p = NULL;
+    if (napi_get_value_external(env, v, &p) != napi_ok) return NULL;
+    return p;
+}
+
+static double get_num(napi_env env, napi_value obj, const char *name, double dflt) {
+    bool has = false;
+    napi_has_named_property(env, obj, name, &has);
+    if (!has) return dflt;
+    napi_value v;
+    double d = dflt;
+    napi_get_named_property(env, obj, name, &v);
+    napi_valuetype t;
+    napi_typeof(env, v, &t);
+    if (t == napi_number) napi_get_value_double(env, v, &d);
+    else if (t == napi_boolean) {
+        bool b;
+        napi_get_value_bool(env, v, &b);
+        d = b ? 1.0 : 0.0;
+    }
+    return d;
+}
+
+static int get_doubles(napi_env env, napi_value obj, const char *name, double *out, uint32_t n) {
+    napi_value arr;
+    if (napi_get_named_property(env, obj, name, &arr) != napi_ok) return 0;
+    for (uint32_t i = 0; i < n; i++) {
+        napi_value e;
+        if (napi_get_element(env, arr, i, &e) != napi_ok) return 0;
+        if (napi_get_value_double(env, e, &out[i]) != napi_ok) return 0;
+    }
+    return 1;
+}
+
+/* Copy of a typed array's bytes (NULL when the value is null/undefined). */
+static void *copy_typed(napi_env env, napi_value v, size_t elem, size_t *count) {
+    napi_valuetype t;
+    napi_typeof(env, v, &t);
+    if (t == napi_null || t == napi_undefined) return NULL;
+    bool is_ta = false;
+    napi_is_typedarray(env, v, &is_ta);
+    if (!is_ta) return NULL;
+    napi_typedarray_type type;
+    size_t length = 0, offset = 0;
+    void *data = NULL;
+    napi_value ab;
+    napi_get_typedarray_info(env, v, &type, &length, &data, &ab, &offset);
+    void *copy = malloc(length * elem + 1);
+    if (copy && length) memcpy(copy, data, length * elem);
+    if (count) *count = length;
+    return copy;
+}
+
+static int read_view(napi_env env, napi_value o, lr3d_view *v) {
+    memset(v, 0, sizeof(*v));
+    v->width = (int32_t)get_num(env, o, "width", 0);
+    v->height = (int32_t)get_num(env, o, "height", 0);
+    v->fx = get_num(env, o, "fx", 0);
+    v->fy = get_num(env, o, "fy", 0);
+    v->cx = get_num(env, o, "cx", 0);
+    v->cy = get_num(env, o, "cy", 0);
+    return get_doubles(env, o, "R", v->R, 9) && get_doubles(env, o, "t", v->t, 3);
+}
+
+/* ---- async work -------------------------------------------------------------------------------- */
+
+enum { OP_SOLVE, OP_SOLVE_BATCHED, OP_FILTER, OP_INTEGRATE, OP_READ_DEPTH, OP_READ_FILTERED,
+       OP_READ_HYP, OP_READ_TSDF };
+
+typedef struct {
+    int op;
+    napi_deferred deferred;
+    napi_async_work work;
+    int32_t status;
+    char error[600];
+    lr3d_stereo *st;
+    lr3d_tsdf *ts;
+    lr3d_solve_desc *descs;
+    float **priors;
+    int32_t n;
+    int32_t layer, map, needed, n_checks;
+    int32_t checks[LR3D_MAX_SOURCES];
+    int32_t *layers;
+    lr3d_integration_view *iviews;
+    void *out_a, *out_b;
+    size_t bytes_a, bytes_b;
+} job_t;
+
+static void job_free(job_t *j) {
+    if (j->priors) {
+        for (int32_t i = 0; i < j->n; i++) free(j->priors[i]);
+        free(j->priors);
+    }
+    free(j->descs);
+    free(j->layers);
+    free(j->iviews);
+    free(j->out_a);
+    free(j->out_b);
+    free(j);
+}
+
+static void job_execute(napi_env env, void *data) {
+    (void)env;
+    job_t *j = (job_t *)data;
+    switch (j->op) {
+    case OP_SOLVE: j->status = lr3d_stereo_solve(j->st, &j->descs[0]); break;
+    case OP_SOLVE_BATCHED: j->status = lr3d_stereo_solve_batched(j->st, j->descs, j->n); break;
+    case OP_FILTER: j->status = lr3d_stereo_filter(j->st, j->layer, j->checks, j->n_checks, j->needed); break;
+    case OP_INTEGRATE:
+        j->status = lr3d_tsdf_integrate_batched(j->ts, j->st, j->layers, j->map, j->iviews, j->n);
+        break;
+    case OP_READ_DEPTH: j->status = lr3d_stereo_read_depth(j->st, j->layer, (uint32_t *)j->out_a); break;
+    case OP_READ_FILTERED: j->status = lr3d_stereo_read_filtered(j->st, j->layer, (uint32_t *)j->out_a); break;
+    case OP_READ_HYP: j->status = lr3d_stereo_read_hyp(j->st, j->layer, (float *)j->out_a, (float *)j->out_b); break;
+    case OP_READ_TSDF: j->status = lr3d_tsdf_read(j->ts, (float *)j->out_a, (float *)j->out_b); break;
+    }
+    if (j->status != LR3D_OK) snprintf(j->error, sizeof(j->error), "lr3d error %d: %s", (int)j->status, lr3d_last_error());
+}
+
+static napi_value typed_from(napi_env env, void *src, size_t bytes, napi_typedarray_type type, size_t elem) {
+    napi_value ab, ta;
+    void *dst = NULL;
+    napi_create_arraybuffer(env, bytes, &dst, &ab);
+    if (bytes) memcpy(dst, src, bytes);
+    napi_create_typedarray(env, type, bytes / elem, ab, 0, &ta);
+    return ta;
+}
+
+static void job_complete(napi_env env, napi_status status, void *data) {
+    job_t *j = (job_t *)data;
+    if (status != napi_ok || j->status != LR3D_OK) {
+        napi_value msg, err;
+        napi_create_string_utf8(env, j->status != LR3D_OK ? j->error : "lr3d.node: async work cancelled",
+                                NAPI_AUTO_LENGTH, &msg);
+        napi_create_error(env, NULL, msg, &err);
+        napi_reject_deferred(env, j->deferred, err);
+    } else {
+        napi_value result = undefined(env);
+        if (j->op == OP_READ_DEPTH || j->op == OP_READ_FILTERED) {
+            result = typed_from(env, j->out_a, j->bytes_a, napi_uint32_array, 4);
+        } else if (j->op == OP_READ_HYP || j->op == OP_READ_TSDF) {
+            napi_create_object(env, &result);
+            napi_set_named_property(env, result, j->op == OP_READ_HYP ? "hyp" : "tsdf",
+                                    typed_from(env, j->out_a, j->bytes_a, napi_float32_array, 4));
+            napi_set_named_property(env, result, j->op == OP_READ_HYP ? "cost" : "weight",
+                                    typed_from(env, j->out_b, j->bytes_b, napi_float32_array, 4));
+        }
+        napi_resolve_deferred(env, j->deferred, result);
+    }
+    napi_delete_async_work(env, j->work);
+    job_free(j);
+}
+
+static napi_value queue(napi_env env, job_t *j) {
+    napi_value promise, name;
+    NAPI_CALL(env, napi_create_promise(env, &j->deferred, &promise));
+    NAPI_CALL(env, napi_create_string_utf8(env, "lr3d", NAPI_AUTO_LENGTH, &name));
+    NAPI_CALL(env, napi_create_async_work(env, NULL, name, job_execute, job_complete, j, &j->work));
+    NAPI_CALL(env, napi_queue_async_work(env, j->work));
+    return promise;
+}
+
+static job_t *new_job(int op) {
+    job_t *j = (job_t *)calloc(1, sizeof(job_t));
+    if (j) j->op = op;
+    return j;
+}
+
+/* ---- bindings ------------------------------------------------------------------------------------ */
+
+#define ARGS(n)                                                                                    \
+    size_t argc = (n);                                                                             \
+    napi_value argv[(n) > 0 ? (n) : 1];                                                            \
+    NAPI_CALL(env, napi_get_cb_info(env, info, &argc, argv, NULL, NULL))
+
+static napi_value ctx_create(napi_env env, napi_callback_info info) {
+    ARGS(1);
+    int32_t device = 0;
+    if (argc > 0) napi_get_value_int32(env, argv[0], &device);
+    lr3d_ctx *ctx = NULL;
+    int32_t rc = lr3d_ctx_create(device, &ctx);
+    if (rc != LR3D_OK) return throw_status(env, rc);
+    napi_value out;
+    NAPI_CALL(env, napi_create_external(env, ctx, NULL, NULL, &out));
+    return out;
+}
+
+static napi_value ctx_destroy(napi_env env, napi_callback_info info) {
+    ARGS(1);
+    lr3d_ctx_destroy((lr3d_ctx *)get_external(env, argv[0]));
+    return undefined(env);
+}
+
+static napi_value stereo_create(napi_env env, napi_callback_info info) {
+    ARGS(2);
+    lr3d_ctx *ctx = (lr3d_ctx *)get_external(env, argv[0]);
+    uint32_t n = 0;
+    NAPI_CALL(env, napi_get_array_length(env, argv[1], &n));
+    lr3d_view *views = (lr3d_view *)calloc(n ? n : 1, sizeof(lr3d_view));
+    for (uint32_t i = 0; i < n; i++) {
+        napi_value o;
+        napi_get_element(env, argv[1], i, &o);
+        if (!read_view(env, o, &views[i])) {
+            free(views);
+            napi_throw_type_error(env, NULL, "lr3d.node: a view needs width, height, fx, fy, cx, cy, R[9], t[3]");
+            return NULL;
+        }
+    }
+    lr3d_stereo *st = NULL;
+    int32_t rc = lr3d_stereo_create(ctx, views, (int32_t)n, 0, &st);
+    free(views);
+    if (rc != LR3D_OK) return throw_status(env, rc);
+    napi_value out;
+    NAPI_CALL(env, napi_create_external(env, st, NULL, NULL, &out));
+    return out;
+}
+
+static napi_value stereo_destroy(napi_env env, napi_callback_info info) {
+    ARGS(1);
+    lr3d_stereo_destroy((lr3d_stereo *)get_external(env, argv[0]));
+    return undefined(env);
+}
+
+static napi_value stereo_set_image(napi_env env, napi_callback_info info) {
+    ARGS(5);
+    lr3d_stereo *st = (lr3d_stereo *)get_external(env, argv[0]);
+    int32_t layer, w, h;
+    napi_get_value_int32(env, argv[1], &layer);
+    napi_get_value_int32(env, argv[3], &w);
+    napi_get_value_int32(env, argv[4], &h);
+    size_t count = 0;
+    uint8_t *gray = (uint8_t *)copy_typed(env, argv[2], 1, &count);
+    if (!gray || count < (size_t)w * (size_t)h) {
+        free(gray);
+        napi_throw_type_error(env, NULL, "lr3d.node: setImage needs a Uint8Array of width*height bytes");
+        return NULL;
+    }
+    int32_t rc = lr3d_stereo_set_image(st, layer, gray, w, h);
+    free(gray);
+    if (rc != LR3D_OK) return throw_status(env, rc);
+    return undefined(env);
+}
+
+/* SolveOptions (DS:62-75) + ref -> lr3d_solve_desc; the prior is copied into *prior_out */
+static int read_solve(napi_env env, napi_value o, lr3d_solve_desc *d, float **prior_out) {
+    memset(d, 0, sizeof(*d));
+    d->ref = (int32_t)get_num(env, o, "ref", 0);
+    napi_value sources;
+    if (napi_get_named_property(env, o, "sources", &sources) != napi_ok) return 0;
+    uint32_t n = 0;
+    napi_get_array_length(env, sources, &n);
+    d->n_sources = n > LR3D_MAX_SOURCES ? LR3D_MAX_SOURCES : (int32_t)n;
+    for (int32_t i = 0; i < d->n_sources; i++) {
+        napi_value e;
+        napi_get_element(env, sources, (uint32_t)i, &e);
+        napi_get_value_int32(env, e, &d->sources[i]);
+    }
+    d->min_depth = get_num(env, o, "minDepth", 0);
+    d->max_depth = get_num(env, o, "maxDepth", 0);
+    d->geometric = (int32_t)get_num(env, o, "geometric", 0);
+    d->iterations = (int32_t)get_num(env, o, "iterations", 5);
+    d->from_coarse = (int32_t)get_num(env, o, "fromCoarse", 0);
+    d->pack = (int32_t)get_num(env, o, "pack", 1);
+    d->topk = (int32_t)get_num(env, o, "topk", 0);
+    *prior_out = NULL;
+    bool has = false;
+    napi_has_named_property(env, o, "prior", &has);
+    if (has) {
+        napi_value p;
+        napi_get_named_property(env, o, "prior", &p);
+        *prior_out = (float *)copy_typed(env, p, 4, NULL);
+    }
+    d->prior = *prior_out;
+    return 1;
+}
+
+static napi_value stereo_solve(napi_env env, napi_callback_info info) {
+    ARGS(2);
+    job_t *j = new_job(OP_SOLVE_BATCHED);
+    j->st = (lr3d_stereo *)get_external(env, argv[0]);
+    bool is_array = false;
+    napi_is_array(env, argv[1], &is_array);
+    uint32_t n = 1;
+    if (is_array) napi_get_array_length(env, argv[1], &n);
+    else j->op = OP_SOLVE;
+    j->n = (int32_t)n;
+    j->descs = (lr3d_solve_desc *)calloc(n ? n : 1, sizeof(lr3d_solve_desc));
+    j->priors = (float **)calloc(n ? n : 1, sizeof(float *));
+    for (uint32_t i = 0; i < n; i++) {
+        napi_value o = argv[1];
+        if (is_array) napi_get_element(env, argv[1], i, &o);
+        if (!read_solve(env, o, &j->descs[i], &j->priors[i])) {
+            job_free(j);
+            napi_throw_type_error(env, NULL, "lr3d.node: solve options need ref and sources");
+            return NULL;
+        }
+    }
+    return queue(env, j);
+}
+
+static napi_value stereo_filter(napi_env env, napi_callback_info info) {
+    ARGS(4);
+    job_t *j = new_job(OP_FILTER);
+    j->st = (lr3d_stereo *)get_external(env, argv[0]);
+    napi_get_value_int32(env, argv[1], &j->layer);
+    uint32_t n = 0;
+    napi_get_array_length(env, argv[2], &n);
+    j->n_checks = n > LR3D_MAX_SOURCES ? LR3D_MAX_SOURCES : (int32_t)n;
+    for (int32_t i = 0; i < j->n_checks; i++) {
+        napi_value e;
+        napi_get_element(env, argv[2], (uint32_t)i, &e);
+        napi_get_value_int32(env, e, &j->checks[i]);
+    }
+    napi_get_value_int32(env, argv[3], &j->needed);
+    return queue(env, j);
+}
+
+static napi_value stereo_read(napi_env env, napi_callback_info info, int op) {
+    ARGS(4);
+    job_t *j = new_job(op);
+    j->st = (lr3d_stereo *)get_external(env, argv[0]);
+    napi_get_value_int32(env, argv[1], &j->layer);
+    int32_t w = 0, h = 0;
+    napi_get_value_int32(env, argv[2], &w);
+    napi_get_value_int32(env, argv[3], &h);
+    size_t px = (size_t)w * (size_t)h;
+    j->bytes_a = px * (op == OP_READ_HYP ? 16 : 4);
+    j->out_a = malloc(j->bytes_a + 1);
+    if (op == OP_READ_HYP) {
+        j->bytes_b = px * 4;
+        j->out_b = malloc(j->bytes_b + 1);
+    }
+    return queue(env, j);
+}
+
+static napi_value stereo_read_depth(napi_env env, napi_callback_info info) { return stereo_read(env, info, OP_READ_DEPTH); }
+static napi_value stereo_read_filtered(napi_env env, napi_callback_info info) { return stereo_read(env, info, OP_READ_FILTERED); }
+static napi_value stereo_read_hyp(napi_env env, napi_callback_info info) { return stereo_read(env, info, OP_READ_HYP); }
+
+static napi_value stereo_set_coarse(napi_env env, napi_callback_info info) {
+    ARGS(2);
+    napi_valuetype t;
+    napi_typeof(env, argv[1], &t);
+    int32_t rc = lr3d_stereo_set_coarse((lr3d_stereo *)get_external(env, argv[0]),
+                                        t == napi_external ? (lr3d_stereo *)get_external(env, argv[1]) : NULL);
+    if (rc != LR3D_OK) return throw_status(env, rc);
+    return undefined(env);
+}
+
+static napi_value tsdf_create(napi_env env, napi_callback_info info) {
+    ARGS(6);
+    lr3d_ctx *ctx = (lr3d_ctx *)get_external(env, argv[0]);
+    double origin[3], voxel = 0, trunc = 4;
+    int32_t dims[3];
+    for (uint32_t a = 0; a < 3; a++) {
+        napi_value e;
+        napi_get_element(env, argv[1], a, &e);
+        napi_get_value_double(env, e, &origin[a]);
+        napi_get_element(env, argv[3], a, &e);
+        napi_get_value_int32(env, e, &dims[a]);
+    }
+    napi_get_value_double(env, argv[2], &voxel);
+    size_t count = 0;
+    int32_t *coords = (int32_t *)copy_typed(env, argv[4], 4, &count);
+    napi_get_value_double(env, argv[5], &trunc);
+    if (!coords) {
+        napi_throw_type_error(env, NULL, "lr3d.node: coords must be an Int32Array");
+        return NULL;
+    }
+    lr3d_tsdf *ts = NULL;
+    int32_t rc = lr3d_tsdf_create(ctx, origin, voxel, dims, coords, (int32_t)(count / 4), trunc, &ts);
+    free(coords);
+    if (rc != LR3D_OK) return throw_status(env, rc);
+    napi_value out;
+    NAPI_CALL(env, napi_create_external(env, ts, NULL, NULL, &out));
+    return out;
+}
+
+static napi_value tsdf_destroy(napi_env env, napi_callback_info info) {
+    ARGS(1);
+    lr3d_tsdf_destroy((lr3d_tsdf *)get_external(env, argv[0]));
+    return undefined(env);
+}
+
+/* IntegrationView (TV:67-79) + size -> lr3d_integration_view (prior copied) */
+static int read_iview(napi_env env, napi_value o, lr3d_integration_view *v, float **prior) {
+    memset(v, 0, sizeof(*v));
+    v->width = (int32_t)get_num(env, o, "width", 0);
+    v->height = (int32_t)get_num(env, o, "height", 0);
+    v->fx = get_num(env, o, "fx", 0);
+    v->fy = get_num(env, o, "fy", 0);
+    v->cx = get_num(env, o, "cx", 0);
+    v->cy = get_num(env, o, "cy", 0);
+    v->max_depth = get_num(env, o, "maxDepth", 0);
+    v->confidence = get_num(env, o, "confidence", 0);
+    *prior = NULL;
+    bool has = false;
+    napi_has_named_property(env, o, "prior", &has);
+    if (has) {
+        napi_value p;
+        napi_get_named_property(env, o, "prior", &p);
+        *prior = (float *)copy_typed(env, p, 4, NULL);
+    }
+    v->prior = *prior;
+    return get_doubles(env, o, "R", v->R, 9) && get_doubles(env, o, "t", v->t, 3);
+}
+
+static napi_value tsdf_integrate(napi_env env, napi_callback_info info) {
+    ARGS(5);
+    job_t *j = new_job(OP_INTEGRATE);
+    j->ts = (lr3d_tsdf *)get_external(env, argv[0]);
+    j->st = (lr3d_stereo *)get_external(env, argv[1]);
+    uint32_t n = 0;
+    napi_get_array_length(env, argv[2], &n);
+    napi_get_value_int32(env, argv[3], &j->map);
+    j->n = (int32_t)n;
+    j->layers = (int32_t *)calloc(n ? n : 1, sizeof(int32_t));
+    j->iviews = (lr3d_integration_view *)calloc(n ? n : 1, sizeof(lr3d_integration_view));
+    j->priors = (float **)calloc(n ? n : 1, sizeof(float *));
+    for (uint32_t i = 0; i < n; i++) {
+        napi_value e, o;
+        napi_get_element(env, argv[2], i, &e);
+        napi_get_value_int32(env, e, &j->layers[i]);
+        napi_get_element(env, argv[4], i, &o);
+        if (!read_iview(env, o, &j->iviews[i], &j->priors[i])) {
+            job_free(j);
+            napi_throw_type_error(env, NULL, "lr3d.node: an integration view needs intrinsics, R[9], t[3]");
+            return NULL;
+        }
+    }
+    return queue(env, j);
+}
+
+static napi_value tsdf_read(napi_env env, napi_callback_info info) {
+    ARGS(2);
+    job_t *j = new_job(OP_READ_TSDF);
+    j->ts = (lr3d_tsdf *)get_external(env, argv[0]);
+    int32_t bricks = 0;
+    napi_get_value_int32(env, argv[1], &bricks);
+    j->bytes_a = j->bytes_b = (size_t)bricks * 512u * 4u;
+    j->out_a = malloc(j->bytes_a + 1);
+    j->out_b = malloc(j->bytes_b + 1);
+    return queue(env, j);
+}
+
+static napi_value last_error(napi_env env, napi_callback_info info) {
+    (void)info;
+    napi_value s;
+    napi_create_string_utf8(env, lr3d_last_error(), NAPI_AUTO_LENGTH, &s);
+    return s;
+}
+
+static napi_value init(napi_env env, napi_value exports) {
+    const struct {
+        const char *name;
+        napi_callback fn;
+    } fns[] = {
+        {"ctxCreate", ctx_create},           {"ctxDestroy", ctx_destroy},
+        {"stereoCreate", stereo_create},     {"stereoDestroy", stereo_destroy},
+        {"stereoSetImage", stereo_set_image}, {"stereoSolve", stereo_solve},
+        {"stereoFilter", stereo_filter},     {"stereoReadDepth", stereo_read_depth},
+        {"stereoReadFiltered", stereo_read_filtered}, {"stereoReadHyp", stereo_read_hyp},
+        {"stereoSetCoarse", stereo_set_coarse}, {"tsdfCreate", tsdf_create},
+        {"tsdfDestroy", tsdf_destroy},       {"tsdfIntegrate", tsdf_integrate},
+        {"tsdfRead", tsdf_read},             {"lastError", last_error},
+    };
+    for (size_t i = 0; i < sizeof(fns) / sizeof(fns[0]); i++) {
+        napi_value f;
+        NAPI_CALL(env, napi_create_function(env, fns[i].name, NAPI_AUTO_LENGTH, fns[i].fn, NULL, &f));
+        NAPI_CALL(env, napi_set_named_property(env, exports, fns[i].name, f));
+    }
+    return exports;
+}
+
+NAPI_MODULE(NODE_GYP_MODULE_NAME, init)

@@ -1,0 +1,47 @@
+'use strict';
+// Runs the reference's stereo -> filter -> fusion call sequence (reconstructor.ts:82-152 shape)
+// through the Node host (lr3d.js over lr3d.node) on a job written by tests/test_node_host.py,
+// and writes the packed depth maps, filtered maps and TSDF volume as raw little-endian files.
+const fs = require('fs');
+const path = require('path');
+const lr3d = require(path.join(__dirname, '..', '..', 'luvas.io-3d_amd', 'node', 'lr3d.js'));
+
+async function main() {
+    const dir = process.argv[2];
+    const job = JSON.parse(fs.readFileSync(path.join(dir, 'job.json'), 'utf8'));
+    const ctx = new lr3d.Context(0);
+    const input = {
+        width: job.width,
+        height: job.height,
+        views: job.views,
+        sources: job.sources,
+        ranges: job.ranges,
+        gray: async (layer) => new Uint8Array(fs.readFileSync(path.join(dir, `gray${layer}.u8`))),
+        prior: () => null,
+    };
+    const run = await lr3d.STEREO_METHODS[job.method].create(ctx, input, { iterations: job.iterations, fineIterations: job.iterations });
+    for (let layer = 0; layer < job.views.length; layer++) await run.solve(layer);
+    const grid = new lr3d.BrickGrid(job.origin, job.voxel, job.extent);
+    grid.mark(job.centre[0], job.centre[1], job.centre[2], job.radius);
+    const layout = grid.build();
+    const volume = new lr3d.HipTsdfVolume(ctx, grid, layout, run.stereo, job.width, job.height, job.truncation);
+    for (let layer = 0; layer < job.views.length; layer++) {
+        fs.writeFileSync(path.join(dir, `depth${layer}.u32`), Buffer.from((await run.stereo.depthMap(layer)).buffer));
+        await run.stereo.filter(layer, job.sources[layer], job.needed);
+        fs.writeFileSync(path.join(dir, `filtered${layer}.u32`), Buffer.from((await run.stereo.filteredMap(layer, job.sources[layer], job.needed)).buffer));
+        await volume.integrate(Object.assign({}, job.views[layer], { maxDepth: job.ranges[layer][1], prior: null, confidence: 0 }));
+    }
+    const { tsdf, weight } = await volume.read();
+    fs.writeFileSync(path.join(dir, 'tsdf.f32'), Buffer.from(tsdf.buffer));
+    fs.writeFileSync(path.join(dir, 'weight.f32'), Buffer.from(weight.buffer));
+    fs.writeFileSync(path.join(dir, 'coords.i32'), Buffer.from(layout.coords.buffer));
+    volume.destroy();
+    run.destroy();
+    ctx.destroy();
+    console.log('node pipeline done', job.views.length, 'views', layout.count, 'bricks');
+}
+
+main().catch((e) => {
+    console.error('node pipeline failed:', e.message);
+    process.exit(1);
+});

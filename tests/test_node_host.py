@@ -1,0 +1,102 @@
+"""The Node / TypeScript host (luvas.io-3d_amd/node/lr3d.js over the N-API addon lr3d.node):
+it loads and exports the binding, fails loudly without a device, and on the GPU runs the
+reference's stereo -> filter -> fusion sequence with results identical to the Python host."""
+import json
+import os
+import shutil
+import subprocess
+
+import numpy as np
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+NODE = shutil.which("node")
+ADDON = os.path.join(ROOT, "luvas.io-3d_amd", "lib", "lr3d.node")
+
+pytestmark = pytest.mark.skipif(NODE is None, reason="node is not installed")
+
+EXPORTS = ["ctxCreate", "ctxDestroy", "stereoCreate", "stereoDestroy", "stereoSetImage", "stereoSolve",
+           "stereoFilter", "stereoReadDepth", "stereoReadFiltered", "stereoReadHyp", "stereoSetCoarse",
+           "tsdfCreate", "tsdfDestroy", "tsdfIntegrate", "tsdfRead", "lastError"]
+
+
+def node(code, timeout=60):
+    return subprocess.run([NODE, "-e", code], capture_output=True, text=True, timeout=timeout, cwd=ROOT)
+
+
+def test_addon_exports_the_binding():
+    r = node("const m=require('./luvas.io-3d_amd/node/lr3d.js');"
+             "console.log(JSON.stringify(Object.keys(m.native)));"
+             "console.log(Object.keys(m.STEREO_METHODS).join(','))")
+    assert r.returncode == 0, r.stderr
+    lines = r.stdout.strip().split("\n")
+    assert sorted(json.loads(lines[0])) == sorted(EXPORTS)
+    assert lines[1] == "hip-single-scale,hip-coarse-to-fine"
+
+
+def test_brickgrid_matches_python_host():
+    from lr3d import BrickGrid
+    r = node("const m=require('./luvas.io-3d_amd/node/lr3d.js');"
+             "const g=new m.BrickGrid([-0.1,0,0.2],0.01,[0.5,0.3,0.41]); g.mark(0.1,0.1,0.4,0.05); g.mark(0.3,0.2,0.5,0.02);"
+             "const l=g.build(); console.log(JSON.stringify({dims:g.dims,coords:Array.from(l.coords),count:l.count}))")
+    assert r.returncode == 0, r.stderr
+    js = json.loads(r.stdout)
+    g = BrickGrid((-0.1, 0, 0.2), 0.01, (0.5, 0.3, 0.41))
+    g.mark(0.1, 0.1, 0.4, 0.05)
+    g.mark(0.3, 0.2, 0.5, 0.02)
+    coords, lookup, count = g.build()
+    assert list(g.dims) == js["dims"] and count == js["count"] and list(coords) == js["coords"]
+
+
+def test_context_without_device_rejects():
+    from lr3d import Context, Lr3dError
+    try:
+        Context(0).destroy()
+        pytest.skip("a HIP device is present")
+    except Lr3dError:
+        pass
+    r = node("const m=require('./luvas.io-3d_amd/node/lr3d.js'); try { new m.Context(0) } catch (e) { console.log(e.message) }")
+    assert "no HIP device" in r.stdout
+
+
+@pytest.mark.gpu
+def test_node_pipeline_matches_python_host(tmp_path, ctx):
+    import lr3d
+    from lr3d import scenes
+    w = scenes.sphere_small().render()
+    views = [{"intrinsics": {"fx": v.intrinsics.fx, "fy": v.intrinsics.fy, "cx": v.intrinsics.cx, "cy": v.intrinsics.cy},
+              "pose": {"R": list(np.asarray(v.pose.R).reshape(9)), "t": list(np.asarray(v.pose.t))}}
+             for v in w.views]
+    job = {"method": "hip-single-scale", "width": 320, "height": 240, "views": views, "sources": w.sources,
+           "ranges": w.ranges, "iterations": 3, "needed": 2, "origin": [-0.512] * 3, "voxel": 0.008,
+           "extent": [1.024] * 3, "centre": [0.0, 0.0, 0.0], "radius": 0.6, "truncation": 4}
+    for i, im in enumerate(w.images):
+        im.tofile(tmp_path / f"gray{i}.u8")
+    (tmp_path / "job.json").write_text(json.dumps(job))
+    print("case: node host pipeline, 8 views 320x240", flush=True)
+    r = subprocess.run([NODE, os.path.join(ROOT, "tests", "node", "pipeline.js"), str(tmp_path)],
+                       capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr + r.stdout
+    # the same sequence through the Python host
+    st = lr3d.HipDenseStereo(ctx, w.views)
+    for i, im in enumerate(w.images):
+        st.set_image(i, im)
+    for i in range(8):
+        st.solve(i, lr3d.SolveOptions(sources=w.sources[i], min_depth=w.ranges[i][0], max_depth=w.ranges[i][1],
+                                      iterations=3))
+    grid = lr3d.BrickGrid((-0.512,) * 3, 0.008, (1.024,) * 3)
+    grid.mark(0.0, 0.0, 0.0, 0.6)
+    layout = grid.build()
+    vol = lr3d.HipTsdfVolume(ctx, grid, layout, 4)
+    for i in range(8):
+        assert np.array_equal(st.depth_map(i), np.fromfile(tmp_path / f"depth{i}.u32", np.uint32))
+        st.filter(i, w.sources[i], 2)
+        assert np.array_equal(st.read_filtered(i), np.fromfile(tmp_path / f"filtered{i}.u32", np.uint32))
+        v = w.views[i]
+        vol.integrate(st, i, lr3d.IntegrationView(v.intrinsics, v.pose, w.ranges[i][1], 320, 240), map=lr3d.MAP_FILTERED)
+    tsdf, weight = vol.read()
+    assert np.array_equal(np.fromfile(tmp_path / "coords.i32", np.int32), layout[0])
+    assert np.array_equal(tsdf.view(np.uint32), np.fromfile(tmp_path / "tsdf.f32", np.uint32))
+    assert np.array_equal(weight.view(np.uint32), np.fromfile(tmp_path / "weight.f32", np.uint32))
+    vol.destroy()
+    st.destroy()

@@ -127,6 +127,30 @@ int32_t lr3d_stereo_write_depth(lr3d_stereo *st, int32_t layer, const uint32_t *
  * coarse stereo must hold one layer per layer of `st`; its planes are read, not copied. */
 int32_t lr3d_stereo_set_coarse(lr3d_stereo *st, lr3d_stereo *coarse);
 
+/* Replace a layer's pose (world -> camera) for later solves, filters and adjoints (pose
+ * refinement; the reference keeps poses in its StereoView list, DS:57-60). */
+int32_t lr3d_stereo_set_pose(lr3d_stereo *st, int32_t layer, const double R[9], const double t[3]);
+/* Overwrite a layer's planes (n.xyz, depth per pixel, width*height*4 floats). */
+int32_t lr3d_stereo_write_hyp(lr3d_stereo *st, int32_t layer, const float *hyp);
+
+/* ---- Adjoint of the aggregated NCC cost (no counterpart in the reference) -------------------
+ * C_p = the aggregated cost (PM:192-233) of the current plane of pixel p of desc->ref, evaluated as
+ * the forward kernels do; L = sum_p weights[p] * C_p (weights NULL = 1).  Windows and taps are
+ * recomputed, nothing is stored by the forward pass.  Outputs (each may be NULL): cost (width*height),
+ * grad_depth = dL/dd (width*height), grad_normal = dL/dn (width*height*3), grad_pose = dL/dR (9,
+ * row-major) and dL/dt (3) of the relative pose ref -> source of each of the 8 source slots
+ * (8*12 doubles, summed over pixels in a fixed order), loss = L over pixels with a valid plane.
+ * INVALID, degenerate and clamped view costs and the top-k choice carry no gradient. */
+int32_t lr3d_ncc_backward(lr3d_stereo *st, const lr3d_solve_desc *desc, const float *weights,
+                          float *cost, float *grad_depth, float *grad_normal, double *grad_pose,
+                          double *loss);
+/* One gradient step on the planes of every desc's reference view (weights 1): depth moves by
+ * -step_depth * dL/dd (at most 0.5 %, kept in the depth range), the normal by -step_normal times
+ * the tangential part of dL/dn (kept facing the camera).  Returns, per desc, the relative-pose
+ * gradients (n*8*12 doubles, as lr3d_ncc_backward) and the loss before the step (n doubles). */
+int32_t lr3d_ncc_refine_step(lr3d_stereo *st, const lr3d_solve_desc *descs, int32_t n,
+                             float step_depth, float step_normal, double *grad_pose, double *loss);
+
 /* ---- TSDF integration (replaces TsdfVolume, TV:158-291) ---------------------------------- */
 
 /* new TsdfVolume(device, grid, layout, depth, W, H) (TV:171-217): brick pool of n_bricks 8^3

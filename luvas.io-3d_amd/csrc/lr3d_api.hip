@@ -245,6 +245,8 @@ struct lr3d_stereo {
     DevBuf<float> cost, prior;
     DevBuf<uint8_t> staging;  // u8 upload of one layer
     DevBuf<uint8_t> plan_buf; // descriptors + block tables of the current launch plan
+    DevBuf<float> adj_weights, adj_cost, adj_gd, adj_gn;  // adjoint inputs / outputs
+    DevBuf<double> adj_partials;
     lr3d_stereo *coarse = nullptr;
     std::vector<GraphEntry> graphs;
 
@@ -733,6 +735,128 @@ extern "C" int32_t lr3d_stereo_set_coarse(lr3d_stereo *st, lr3d_stereo *coarse) 
     for (auto &g : st->graphs) destroy_graph(g);
     st->graphs.clear();
     return LR3D_OK;
+}
+
+// ---- pose / plane updates and the NCC adjoint -------------------------------------------------------
+
+extern "C" int32_t lr3d_stereo_set_pose(lr3d_stereo *st, int32_t layer, const double R[9], const double t[3]) {
+    if (!st || !R || !t) return fail(LR3D_ERR_INVALID, "lr3d_stereo_set_pose: NULL argument");
+    int32_t rc = check_layer(st, layer, "lr3d_stereo_set_pose");
+    if (rc) return rc;
+    std::lock_guard<std::mutex> lock(st->ctx->mu);
+    for (int i = 0; i < 9; i++) st->views[layer].R[i] = R[i];
+    for (int i = 0; i < 3; i++) st->views[layer].t[i] = t[i];
+    return LR3D_OK;
+}
+
+extern "C" int32_t lr3d_stereo_write_hyp(lr3d_stereo *st, int32_t layer, const float *hyp) {
+    if (!st || !hyp) return fail(LR3D_ERR_INVALID, "lr3d_stereo_write_hyp: NULL argument");
+    int32_t rc = check_layer(st, layer, "lr3d_stereo_write_hyp");
+    if (rc) return rc;
+    lr3d_ctx *ctx = st->ctx;
+    std::lock_guard<std::mutex> lock(ctx->mu);
+    HIP_TRY(hipSetDevice(ctx->device));
+    const lr3d_view &v = st->views[layer];
+    HIP_TRY(hipMemcpyAsync(st->hyp.ptr + st->off[layer], hyp, (size_t)v.width * v.height * 16,
+                           hipMemcpyHostToDevice, ctx->stream));
+    return finish(ctx);
+}
+
+// Shared part of the adjoint entry points: plans over the given solves (8 x 8 tiles, one
+// wavefront per block), one launch, per-view pose gradients and losses summed from the block
+// partials in block order.
+static int32_t adjoint_impl(lr3d_stereo *st, const lr3d_solve_desc *descs, int32_t n, const float *weights,
+                            bool want_maps, float step_depth, float step_normal, int32_t apply,
+                            double *grad_pose, double *loss) {
+    lr3d_ctx *ctx = st->ctx;
+    std::vector<PmDesc> dv(n);
+    std::vector<uint8_t> packs(n, 1);
+    for (int32_t i = 0; i < n; i++) {
+        int32_t rc = build_desc(st, descs[i], 0, true, dv[i]);
+        if (rc) return rc;
+        if (dv[i].n_src <= 0) return fail(LR3D_ERR_INVALID, "lr3d_ncc: solve %d has no sources", i);
+        for (int32_t j = 0; j < i; j++)
+            if (descs[j].ref == descs[i].ref) return fail(LR3D_ERR_INVALID, "lr3d_ncc: layer %d appears twice", descs[i].ref);
+    }
+    Plans plans;
+    std::vector<int64_t> key;
+    int32_t rc = upload_plans(st, dv, packs, plans, key);
+    if (rc) return rc;
+    const PmPlan &plan = plans.init;  // 8 x 8 threads per block
+    const size_t per_block = (size_t)kMaxSources * 12 + 1;
+    HIP_TRY(st->adj_partials.reserve((size_t)plan.blocks * per_block));
+    HIP_TRY(hipMemsetAsync(st->adj_partials.ptr, 0, (size_t)plan.blocks * per_block * 8, ctx->stream));
+    const float *wdev = nullptr;
+    if (weights) {
+        HIP_TRY(st->adj_weights.reserve(st->total_px));
+        const lr3d_view &v = st->views[descs[0].ref];
+        HIP_TRY(hipMemcpyAsync(st->adj_weights.ptr + st->off[descs[0].ref], weights, (size_t)v.width * v.height * 4,
+                               hipMemcpyHostToDevice, ctx->stream));
+        wdev = st->adj_weights.ptr;
+    }
+    if (want_maps) {
+        HIP_TRY(st->adj_cost.reserve(st->total_px));
+        HIP_TRY(st->adj_gd.reserve(st->total_px));
+        HIP_TRY(st->adj_gn.reserve(st->total_px * 3));
+    }
+    const PmPools pools = pools_of(st);
+    HIP_TRY(timed(ctx, "ncc_backward", [&] {
+        return ncc_launch_backward(ctx->stream, plan, pools, wdev, want_maps ? st->adj_cost.ptr : nullptr,
+                                   want_maps ? st->adj_gd.ptr : nullptr, want_maps ? st->adj_gn.ptr : nullptr,
+                                   st->adj_partials.ptr, step_depth, step_normal, apply);
+    }));
+    std::vector<double> partials((size_t)plan.blocks * per_block);
+    HIP_TRY(hipMemcpyAsync(partials.data(), st->adj_partials.ptr, partials.size() * 8, hipMemcpyDeviceToHost,
+                           ctx->stream));
+    rc = finish(ctx);
+    if (rc) return rc;
+    std::vector<uint32_t> starts(n + 1, 0);
+    for (int32_t v = 0; v < n; v++) {
+        const PmDesc &d = dv[v];
+        starts[v + 1] = starts[v] + (uint32_t)(((d.H + plan.ty - 1) / plan.ty) * ((d.W + plan.tx - 1) / plan.tx));
+    }
+    for (int32_t v = 0; v < n; v++) {
+        double *gp = grad_pose ? grad_pose + (size_t)v * kMaxSources * 12 : nullptr;
+        if (gp) std::fill(gp, gp + kMaxSources * 12, 0.0);
+        double l = 0.0;
+        for (uint32_t b = starts[v]; b < starts[v + 1]; b++) {
+            const double *p = &partials[(size_t)b * per_block];
+            l += p[0];
+            if (gp)
+                for (int j = 0; j < kMaxSources * 12; j++) gp[j] += p[1 + j];
+        }
+        if (loss) loss[v] = l;
+    }
+    return LR3D_OK;
+}
+
+extern "C" int32_t lr3d_ncc_backward(lr3d_stereo *st, const lr3d_solve_desc *desc, const float *weights,
+                                     float *cost, float *grad_depth, float *grad_normal, double *grad_pose,
+                                     double *loss) {
+    if (!st || !desc) return fail(LR3D_ERR_INVALID, "lr3d_ncc_backward: NULL argument");
+    int32_t rc = check_layer(st, desc->ref, "lr3d_ncc_backward");
+    if (rc) return rc;
+    lr3d_ctx *ctx = st->ctx;
+    std::lock_guard<std::mutex> lock(ctx->mu);
+    HIP_TRY(hipSetDevice(ctx->device));
+    rc = adjoint_impl(st, desc, 1, weights, true, 0.0f, 0.0f, 0, grad_pose, loss);
+    if (rc) return rc;
+    const lr3d_view &v = st->views[desc->ref];
+    const size_t px = (size_t)v.width * v.height, off = st->off[desc->ref];
+    if (cost) HIP_TRY(hipMemcpyAsync(cost, st->adj_cost.ptr + off, px * 4, hipMemcpyDeviceToHost, ctx->stream));
+    if (grad_depth) HIP_TRY(hipMemcpyAsync(grad_depth, st->adj_gd.ptr + off, px * 4, hipMemcpyDeviceToHost, ctx->stream));
+    if (grad_normal)
+        HIP_TRY(hipMemcpyAsync(grad_normal, st->adj_gn.ptr + off * 3, px * 12, hipMemcpyDeviceToHost, ctx->stream));
+    return finish(ctx);
+}
+
+extern "C" int32_t lr3d_ncc_refine_step(lr3d_stereo *st, const lr3d_solve_desc *descs, int32_t n,
+                                        float step_depth, float step_normal, double *grad_pose, double *loss) {
+    if (!st || !descs || n <= 0) return fail(LR3D_ERR_INVALID, "lr3d_ncc_refine_step: NULL argument or n <= 0");
+    lr3d_ctx *ctx = st->ctx;
+    std::lock_guard<std::mutex> lock(ctx->mu);
+    HIP_TRY(hipSetDevice(ctx->device));
+    return adjoint_impl(st, descs, n, nullptr, false, step_depth, step_normal, 1, grad_pose, loss);
 }
 
 // ---- TSDF ---------------------------------------------------------------------------------------------

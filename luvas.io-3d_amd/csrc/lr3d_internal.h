@@ -110,6 +110,13 @@ hipError_t pm_launch_propagate(hipStream_t s, const PmPlan &plan, const PmPools 
 hipError_t pm_launch_pack(hipStream_t s, const PmPlan &plan, const PmPools &pools);
 hipError_t pm_launch_consistency(hipStream_t s, const PmPlan &plan, const PmPools &pools,
                                  uint32_t needed);
+// NCC adjoint: costs, dL/d(depth, normal) per pixel and per-block pose partials (1 + 8*12
+// doubles per block: loss, then dL/dR (9) and dL/dt (3) of each source); apply != 0 also takes one
+// gradient step on the planes.
+hipError_t ncc_launch_backward(hipStream_t s, const PmPlan &plan, const PmPools &pools,
+                               const float *weights, float *cost, float *gd, float *gn,
+                               double *pose_partials, float step_depth, float step_normal,
+                               int32_t apply);
 hipError_t tsdf_launch_visibility(hipStream_t s, const TsdfGridDev &g, const int4 *coords,
                                   const TsdfView *views, int32_t n_views, uint32_t *vis,
                                   int32_t words);

@@ -1,0 +1,509 @@
+// Device-side building blocks of the PatchMatch kernels (forward and adjoint): RNG, packing,
+// bilateral window, plane homography, bilateral NCC, hypothesis cost.
+//
+// Semantics follow the reference WGSL module (frontend/src/app/vision/mvs/patchmatch.wgsl.ts,
+// cited PM:<line>) under the FP32 arithmetic definition stated at the top of pm_kernels.hip.
+// Included by one translation unit per kernel family; everything is internal to it.
+#pragma once
+
+#include "lr3d_internal.h"
+
+#include <math.h>
+
+namespace lr3d {
+namespace {
+
+constexpr float kInvalid = 2.0f;        // PM:41
+constexpr float kGeometricCap = 3.0f;   // PM:42
+constexpr uint32_t kFilterCost = 7u;    // PM:43
+constexpr float kFilterPixels = 1.0f;   // PM:44
+constexpr float kFilterDepth = 0.01f;   // PM:45
+// PM:39-40 and PM:154 in u8 units: 1 / (2 (0.06 * 255)^2), 1 / (2 * 5^2), 1e-5 * 255^2.
+constexpr float kColorK = (float)(1.0 / (2.0 * (0.06 * 255.0) * (0.06 * 255.0)));
+constexpr float kSpaceK = 0.02f;
+constexpr float kDegenerateDen = (float)(1e-5 * 255.0 * 255.0);
+
+// ---- RNG (PM:56-65, PM:251-253) --------------------------------------------------------------
+
+__device__ __forceinline__ uint32_t pcg(uint32_t v) {
+    uint32_t state = v * 747796405u + 2891336453u;
+    uint32_t word = ((state >> ((state >> 28u) + 4u)) ^ state) * 277803737u;
+    return (word >> 22u) ^ word;
+}
+
+__device__ __forceinline__ float next_random(uint32_t &rng) {
+    rng = pcg(rng);
+    return (float)(rng >> 8u) / 16777216.0f;
+}
+
+// ---- small helpers -------------------------------------------------------------------------
+
+__device__ __forceinline__ float dot3(float ax, float ay, float az, float bx, float by, float bz) {
+    return ax * bx + ay * by + az * bz;
+}
+
+__device__ __forceinline__ void normalize3(float &x, float &y, float &z) {
+    float inv = 1.0f / sqrtf(dot3(x, y, z, x, y, z));
+    x = x * inv;
+    y = y * inv;
+    z = z * inv;
+}
+
+__device__ __forceinline__ float clampf(float v, float lo, float hi) { return fminf(fmaxf(v, lo), hi); }
+
+__device__ __forceinline__ uint32_t f32_to_f16_bits(float v) {
+    _Float16 h = (_Float16)v;  // round to nearest even
+    return (uint32_t)__builtin_bit_cast(uint16_t, h);
+}
+
+__device__ __forceinline__ float f16_bits_to_f32(uint32_t bits) {
+    return (float)__builtin_bit_cast(_Float16, (uint16_t)(bits & 0xffffu));
+}
+
+// PK:7-15
+__device__ __forceinline__ void unpack_normal(uint32_t packed, float &nx, float &ny, float &nz) {
+    float ex = ((float)((packed >> 16u) & 63u) / 63.0f) * 2.0f - 1.0f;
+    float ey = ((float)((packed >> 22u) & 63u) / 63.0f) * 2.0f - 1.0f;
+    nx = ex;
+    ny = ey;
+    nz = (1.0f - fabsf(ex)) - fabsf(ey);
+    if (nz < 0.0f) {
+        float sx = nx >= 0.0f ? 1.0f : -1.0f;
+        float sy = ny >= 0.0f ? 1.0f : -1.0f;
+        float ux = (1.0f - fabsf(ny)) * sx;
+        float uy = (1.0f - fabsf(nx)) * sy;
+        nx = ux;
+        ny = uy;
+    }
+    normalize3(nx, ny, nz);
+}
+
+// PK:21-33
+__device__ __forceinline__ uint32_t pack_sample(float depth, float nx, float ny, float nz, float cost) {
+    if (!(depth > 0.0f) || cost >= 1.0f) return 0u;
+    float l1 = (fabsf(nx) + fabsf(ny)) + fabsf(nz);
+    float ex = nx / l1;
+    float ey = ny / l1;
+    if (nz < 0.0f) {
+        float sx = ex >= 0.0f ? 1.0f : -1.0f;
+        float sy = ey >= 0.0f ? 1.0f : -1.0f;
+        float ux = (1.0f - fabsf(ey)) * sx;
+        float uy = (1.0f - fabsf(ex)) * sy;
+        ex = ux;
+        ey = uy;
+    }
+    uint32_t qx = (uint32_t)(clampf(ex * 0.5f + 0.5f, 0.0f, 1.0f) * 63.0f + 0.5f);
+    uint32_t qy = (uint32_t)(clampf(ey * 0.5f + 0.5f, 0.0f, 1.0f) * 63.0f + 0.5f);
+    uint32_t c = (uint32_t)(cost * 15.0f);
+    c = c > 15u ? 15u : c;
+    return f32_to_f16_bits(depth) | (qx << 16u) | (qy << 22u) | (c << 28u);
+}
+
+// ---- work mapping ---------------------------------------------------------------------------
+
+// Hardware block i runs on XCD i % 8; give each XCD a contiguous range of logical blocks.
+__device__ __forceinline__ uint32_t xcd_remap(uint32_t i, uint32_t n) {
+    constexpr uint32_t kXcds = 8;
+    uint32_t xcd = i % kXcds, local = i / kXcds;
+    uint32_t per = n / kXcds, rem = n % kXcds;
+    return xcd * per + (xcd < rem ? xcd : rem) + local;
+}
+
+// View owning logical block b (uniform: block_start is read through the scalar cache).
+__device__ __forceinline__ int32_t find_view(const uint32_t *__restrict__ block_start, int32_t n,
+                                             uint32_t b) {
+    int32_t lo = 0, hi = n;
+    while (hi - lo > 1) {
+        int32_t mid = (lo + hi) >> 1;
+        if (block_start[mid] <= b) lo = mid; else hi = mid;
+    }
+    return lo;
+}
+
+// ---- bilateral window (PM:76-110) -------------------------------------------------------------
+
+struct Window {
+    float v[kSamples];
+    float w[kSamples];
+    float sw, mean, var;
+};
+
+__device__ __forceinline__ float ref_pixel(const uint32_t *__restrict__ img, int32_t W, int32_t H,
+                                           int32_t x, int32_t y) {
+    x = min(max(x, 0), W - 1);
+    y = min(max(y, 0), H - 1);
+    return (float)(img[__umul24((uint32_t)y, (uint32_t)W) + (uint32_t)x] & 0xffu);
+}
+
+__device__ __forceinline__ void prepare_window(const uint32_t *__restrict__ img, int32_t W, int32_t H,
+                                               int32_t x, int32_t y, Window &win) {
+    float center = ref_pixel(img, W, H, x, y);
+    float sw = 0.0f, sr = 0.0f, srr = 0.0f;
+#pragma unroll
+    for (int i = 0; i < kSamples; i++) {
+        const int ox = (i % 6) * 2 - 5, oy = (i / 6) * 2 - 5;
+        float v = ref_pixel(img, W, H, x + ox, y + oy);
+        float dc = v - center;
+        const float ds = (float)(ox * ox + oy * oy);
+        float w = (float)exp((double)(-(dc * dc) * kColorK - ds * kSpaceK));
+        win.v[i] = v;
+        win.w[i] = w;
+        float wv = w * v;
+        sw = sw + w;
+        sr = sr + wv;
+        srr = srr + wv * v;
+    }
+    float mean = sr / sw;
+    win.sw = sw;
+    win.mean = mean;
+    win.var = fmaxf(srr / sw - mean * mean, 0.0f);
+}
+
+// ---- plane homography (PM:112-128) -------------------------------------------------------------
+
+struct Mat3 {
+    float m[9];
+};
+
+__device__ __forceinline__ void homography(const PmDesc &d, const PmSrc &s, float nx, float ny,
+                                           float nz, float rho, Mat3 &H) {
+    const float n[3] = {nx, ny, nz};
+    const float t[3] = {s.t[0] / rho, s.t[1] / rho, s.t[2] / rho};
+    float M[9];
+#pragma unroll
+    for (int i = 0; i < 3; i++)
+#pragma unroll
+        for (int j = 0; j < 3; j++) M[i * 3 + j] = s.R[i * 3 + j] + t[i] * n[j];
+    float A[9];
+#pragma unroll
+    for (int j = 0; j < 3; j++) {
+        A[j] = s.k[0] * M[j] + s.k[2] * M[6 + j];
+        A[3 + j] = s.k[1] * M[3 + j] + s.k[3] * M[6 + j];
+        A[6 + j] = M[6 + j];
+    }
+#pragma unroll
+    for (int i = 0; i < 3; i++) {
+        H.m[i * 3 + 0] = A[i * 3 + 0] * d.ikx;
+        H.m[i * 3 + 1] = A[i * 3 + 1] * d.iky;
+        H.m[i * 3 + 2] = (A[i * 3 + 0] * d.kx2 + A[i * 3 + 1] * d.ky2) + A[i * 3 + 2];
+    }
+}
+
+// ---- bilateral NCC of one source view (PM:130-159) ---------------------------------------------
+//
+// Tap i at offset (ox, oy) maps to q = H (X, Y, 1) with X = px + ox, Y = py + oy, evaluated as
+// q = fma(H_0, X, fma(H_1, Y, H_2)) per row, so the Y part is shared by the six taps of a window
+// row.  All 36 taps are evaluated; the view is INVALID if any tap is (the reference returns at
+// the first such tap, with the same result).  Sample indices are clamped into the image so that
+// invalid taps still read inside the source image.
+//
+// 1/q.z: v_rcp_f32 followed by one Newton step with FMA reproduces the IEEE quotient 1.0f / q.z
+// bit for bit for every |q.z| in [2^-60, 2^61) (checked exhaustively on gfx950).  Valid taps
+// have q.z > 1e-6 > 2^-60; a view whose window could reach |q.z| >= 2^60 takes the IEEE
+// division for the whole wavefront.
+__device__ __forceinline__ float rcp_exact(float z) {
+    const float r = __builtin_amdgcn_rcpf(z);
+    return fmaf(fmaf(-z, r, 1.0f), r, r);
+}
+
+template <bool FAST>
+__device__ __forceinline__ void view_taps(const uint32_t *__restrict__ img, int32_t W, int32_t H,
+                                          const Mat3 &Hm, float px, float py, const Window &win,
+                                          float &ss, float &sss, float &srs, bool &ok) {
+    const float wmax = (float)W - 1.0f, hmax = (float)H - 1.0f;
+    const uint32_t stride = (uint32_t)W;
+#pragma unroll
+    for (int row = 0; row < 6; row++) {
+        const float Y = py + (float)(row * 2 - 5);
+        const float rx = fmaf(Hm.m[1], Y, Hm.m[2]);
+        const float ry = fmaf(Hm.m[4], Y, Hm.m[5]);
+        const float rw = fmaf(Hm.m[7], Y, Hm.m[8]);
+#pragma unroll
+        for (int col = 0; col < 6; col++) {
+            const int i = row * 6 + col;
+            const float X = px + (float)(col * 2 - 5);
+            const float qx = fmaf(Hm.m[0], X, rx);
+            const float qy = fmaf(Hm.m[3], X, ry);
+            const float qz = fmaf(Hm.m[6], X, rw);
+            const float rz = FAST ? rcp_exact(qz) : 1.0f / qz;
+            const float u = qx * rz, v = qy * rz;
+            // Clamp into the image (NaN-safe: maxnum returns the number); the tap is inside
+            // exactly when the clamp leaves it unchanged (a NaN never compares equal).  Valid
+            // taps have 0 <= u <= W-1, so trunc(u) = floor(u) and fract(u) = u - floor(u).
+            const float uc = fminf(fmaxf(u, 0.0f), wmax), vc = fminf(fmaxf(v, 0.0f), hmax);
+            ok = ok & (qz > 1e-6f) & (uc == u) & (vc == v);
+            const int32_t ix = (int32_t)uc;
+            const int32_t iy = (int32_t)vc;
+            const float fx = __builtin_amdgcn_fractf(u), fy = __builtin_amdgcn_fractf(v);
+            const uint32_t q = img[__umul24((uint32_t)iy, stride) + (uint32_t)ix];
+            const float a = (float)(q & 0xffu), b = (float)((q >> 8) & 0xffu);
+            const float c = (float)((q >> 16) & 0xffu), e = (float)(q >> 24);
+            const float top = fmaf(fx, b - a, a);
+            const float bot = fmaf(fx, e - c, c);
+            const float val = fmaf(fy, bot - top, top);
+            const float wv = win.w[i] * val;
+            ss = ss + wv;
+            sss = fmaf(wv, val, sss);
+            srs = fmaf(wv, win.v[i], srs);
+        }
+    }
+}
+
+// Taps of a window known to lie inside the source image (see view_cost): no per-tap checks.
+// The linear index is still clamped to the image, so no tap can read outside it.
+__device__ __forceinline__ void view_taps_inside(const uint32_t *__restrict__ img, int32_t W, int32_t H,
+                                                 const Mat3 &Hm, float px, float py,
+                                                 const Window &win, float &ss, float &sss,
+                                                 float &srs) {
+    const uint32_t stride = (uint32_t)W;
+    const uint32_t last = (uint32_t)W * (uint32_t)H - 1u;
+#pragma unroll
+    for (int row = 0; row < 6; row++) {
+        const float Y = py + (float)(row * 2 - 5);
+        const float rx = fmaf(Hm.m[1], Y, Hm.m[2]);
+        const float ry = fmaf(Hm.m[4], Y, Hm.m[5]);
+        const float rw = fmaf(Hm.m[7], Y, Hm.m[8]);
+#pragma unroll
+        for (int col = 0; col < 6; col++) {
+            const int i = row * 6 + col;
+            const float X = px + (float)(col * 2 - 5);
+            const float qx = fmaf(Hm.m[0], X, rx);
+            const float qy = fmaf(Hm.m[3], X, ry);
+            const float qz = fmaf(Hm.m[6], X, rw);
+            const float rz = rcp_exact(qz);
+            const float u = qx * rz, v = qy * rz;
+            const uint32_t ix = (uint32_t)u, iy = (uint32_t)v;
+            const float fx = __builtin_amdgcn_fractf(u), fy = __builtin_amdgcn_fractf(v);
+            const uint32_t q = img[min(__umul24(iy, stride) + ix, last)];
+            const float a = (float)(q & 0xffu), b = (float)((q >> 8) & 0xffu);
+            const float c = (float)((q >> 16) & 0xffu), e = (float)(q >> 24);
+            const float top = fmaf(fx, b - a, a);
+            const float bot = fmaf(fx, e - c, c);
+            const float val = fmaf(fy, bot - top, top);
+            const float wv = win.w[i] * val;
+            ss = ss + wv;
+            sss = fmaf(wv, val, sss);
+            srs = fmaf(wv, win.v[i], srs);
+        }
+    }
+}
+
+// Window classes of one source view, decided from the four corner taps (offsets (+-5, +-5)):
+//   kOutside  a corner tap is invalid, so the view is INVALID (the reference stops at the first
+//             invalid tap; every tap, corners included, is evaluated exactly as in the full loop);
+//   kInside   every corner lies inside the image by more than a rounding margin: q.z is affine in
+//             (X, Y) and positive on the window, so the warped window is the convex hull of the
+//             warped corners, and every tap is inside as computed;
+//   kCareful  anything else (windows touching the border, huge |q.z|): per-tap checks.
+enum { kOutside = 0, kInside = 1, kCareful = 2 };
+
+__device__ __forceinline__ int classify_window(const PmSrc &s, const Mat3 &H, float px, float py) {
+    const float wmax = (float)s.W - 1.0f, hmax = (float)s.H - 1.0f;
+    const float ax = fabsf(px) + 5.0f, ay = fabsf(py) + 5.0f;
+    // bounds on the magnitude of the terms of q.x, q.y, q.z over the window
+    const float bx = fabsf(H.m[0]) * ax + fabsf(H.m[1]) * ay + fabsf(H.m[2]);
+    const float by = fabsf(H.m[3]) * ax + fabsf(H.m[4]) * ay + fabsf(H.m[5]);
+    const float bz = fabsf(H.m[6]) * ax + fabsf(H.m[7]) * ay + fabsf(H.m[8]);
+    if (!(bz < 0x1p60f)) return kCareful;  // outside the range where rcp_exact == 1/q.z
+    bool valid = true;
+    float zmin = INFINITY, umin = INFINITY, umax = -INFINITY, vmin = INFINITY, vmax = -INFINITY;
+#pragma unroll
+    for (int corner = 0; corner < 4; corner++) {
+        const float X = px + ((corner & 1) ? 5.0f : -5.0f);
+        const float Y = py + ((corner & 2) ? 5.0f : -5.0f);
+        const float qx = fmaf(H.m[0], X, fmaf(H.m[1], Y, H.m[2]));
+        const float qy = fmaf(H.m[3], X, fmaf(H.m[4], Y, H.m[5]));
+        const float qz = fmaf(H.m[6], X, fmaf(H.m[7], Y, H.m[8]));
+        const float rz = rcp_exact(qz);
+        const float u = qx * rz, v = qy * rz;
+        valid = valid & (qz > 1e-6f) & (u >= 0.0f) & (v >= 0.0f) & (u <= wmax) & (v <= hmax);
+        zmin = fminf(zmin, qz);
+        umin = fminf(umin, u);
+        umax = fmaxf(umax, u);
+        vmin = fminf(vmin, v);
+        vmax = fmaxf(vmax, v);
+    }
+    if (!valid) return kOutside;
+    // Each computed q component is within 2^-20 of its term bound of the exact value (two FMAs
+    // round at most 2 ulp of the larger term), so exact q.z >= zmin - ez on the window.  A
+    // computed u = q.x / q.z then lies within eu of the exact ratio; require corners inside
+    // by 4 eu so that exact and computed interior taps stay inside.
+    const float ez = bz * 0x1p-20f;
+    const float zlow = zmin - 2.0f * ez;
+    if (!(zlow > 2e-6f)) return kCareful;
+    const float ilow = 1.0f / zlow;
+    const float eu = (bx * 0x1p-20f + (wmax + 1.0f) * ez) * ilow + (wmax + 1.0f) * 0x1p-20f;
+    const float ev = (by * 0x1p-20f + (hmax + 1.0f) * ez) * ilow + (hmax + 1.0f) * 0x1p-20f;
+    const bool inside = umin >= 4.0f * eu && umax <= wmax - 4.0f * eu && vmin >= 4.0f * ev &&
+                        vmax <= hmax - 4.0f * ev;
+    return inside ? kInside : kCareful;
+}
+
+__device__ __forceinline__ float view_cost(const PmSrc &s, const Mat3 &H, float px, float py,
+                                           const Window &win, const uint32_t *__restrict__ quad) {
+    const uint32_t *__restrict__ img = quad + s.off;
+    const int cls = classify_window(s, H, px, py);
+    float ss = 0.0f, sss = 0.0f, srs = 0.0f;
+    bool ok = cls != kOutside;
+    if (__any(cls == kCareful)) {
+        // the checked loop is exact for every class; lanes already known outside stay invalid
+        const float bound = fabsf(H.m[6]) * (fabsf(px) + 5.0f) + fabsf(H.m[7]) * (fabsf(py) + 5.0f) +
+                            fabsf(H.m[8]);
+        bool tap_ok = true;
+        if (__all(bound < 0x1p60f))
+            view_taps<true>(img, s.W, s.H, H, px, py, win, ss, sss, srs, tap_ok);
+        else
+            view_taps<false>(img, s.W, s.H, H, px, py, win, ss, sss, srs, tap_ok);
+        ok = ok & tap_ok;
+    } else if (__any(cls == kInside)) {
+        if (cls == kInside) view_taps_inside(img, s.W, s.H, H, px, py, win, ss, sss, srs);
+    }
+    if (!ok) return kInvalid;
+    float mean = ss / win.sw;
+    float var_s = fmaxf(sss / win.sw - mean * mean, 0.0f);
+    float den = sqrtf(win.var * var_s);
+    if (den < kDegenerateDen) return 1.0f;
+    float cov = srs / win.sw - win.mean * mean;
+    return clampf(1.0f - cov / den, 0.0f, 2.0f);
+}
+
+// PM:161-190: forward-backward reprojection through a source view's packed depth map.
+// Returns the pixel distance in r0 and the back-projected depth in r1; false when invalid.
+__device__ __forceinline__ bool reproject(const PmDesc &d, const PmSrc &s,
+                                          const uint32_t *__restrict__ depth, float px, float py,
+                                          float x0, float x1, float x2, float &r0, float &r1) {
+    float xs0 = fmaf(s.R[2], x2, fmaf(s.R[1], x1, fmaf(s.R[0], x0, s.t[0])));
+    float xs1 = fmaf(s.R[5], x2, fmaf(s.R[4], x1, fmaf(s.R[3], x0, s.t[1])));
+    float xs2 = fmaf(s.R[8], x2, fmaf(s.R[7], x1, fmaf(s.R[6], x0, s.t[2])));
+    if (!(xs2 > 1e-6f)) return false;
+    int32_t pxs = (int32_t)rintf((s.k[0] * xs0) / xs2 + s.k[2]);
+    int32_t pys = (int32_t)rintf((s.k[1] * xs1) / xs2 + s.k[3]);
+    if (pxs < 0 || pys < 0 || pxs >= s.W || pys >= s.H) return false;
+    uint32_t packed = depth[s.off + (uint32_t)pys * (uint32_t)s.W + (uint32_t)pxs];
+    float ds = f16_bits_to_f32(packed);
+    if (!(ds > 0.0f) || (packed >> 28u) > kFilterCost) return false;
+    float b0 = (((float)pxs - s.k[2]) / s.k[0]) * ds;
+    float b1 = (((float)pys - s.k[3]) / s.k[1]) * ds;
+    float e0 = b0 - s.t[0], e1 = b1 - s.t[1], e2 = ds - s.t[2];
+    float back0 = fmaf(s.R[6], e2, fmaf(s.R[3], e1, s.R[0] * e0));
+    float back1 = fmaf(s.R[7], e2, fmaf(s.R[4], e1, s.R[1] * e0));
+    float back2 = fmaf(s.R[8], e2, fmaf(s.R[5], e1, s.R[2] * e0));
+    if (!(back2 > 1e-6f)) return false;
+    float qx = (d.k[0] * back0) / back2 + d.k[2];
+    float qy = (d.k[1] * back1) / back2 + d.k[3];
+    float dx = qx - px, dy = qy - py;
+    r0 = sqrtf(dx * dx + dy * dy);
+    r1 = back2;
+    return true;
+}
+
+// Sorted k smallest of the view costs.  The reference's partial selection sort (PM:218-231)
+// adds the k smallest costs in ascending order; keeping them sorted in registers with a
+// min/max insertion network gives the same values in the same order.
+template <int KMAX>
+struct TopK {
+    float s[KMAX];
+    __device__ __forceinline__ void reset() {
+#pragma unroll
+        for (int j = 0; j < KMAX; j++) s[j] = INFINITY;
+    }
+    __device__ __forceinline__ void insert(float c) {
+#pragma unroll
+        for (int j = KMAX - 1; j > 0; j--) s[j] = fminf(s[j], fmaxf(s[j - 1], c));
+        s[0] = fminf(s[0], c);
+    }
+    __device__ __forceinline__ float mean(int32_t k) const {
+        float total = 0.0f;
+#pragma unroll
+        for (int j = 0; j < KMAX; j++)
+            if (j < k) total = total + s[j];
+        return total / (float)k;
+    }
+};
+
+// PM:192-233
+template <int KMAX, bool GEOMETRIC>
+__device__ __forceinline__ float hypothesis_cost(const PmDesc &d, const PmPools &pools, float px,
+                                                 float py, float rx, float ry, const Window &win,
+                                                 float nx, float ny, float nz, float depth) {
+    float inv = 1.0f / depth;
+    if (!(depth > 0.0f) || inv > d.inv_min || inv < d.inv_max) return kInvalid;
+    if (!(dot3(nx, ny, nz, rx, ry, 1.0f) < -1e-3f)) return kInvalid;
+    float X0 = rx * depth, X1 = ry * depth, X2 = 1.0f * depth;
+    float rho = dot3(nx, ny, nz, X0, X1, X2);
+    const int32_t count = d.n_src;
+    if (count == 0) return kInvalid;
+    TopK<KMAX> top;
+    top.reset();
+    for (int32_t s = 0; s < count; s++) {
+        const PmSrc &src = d.src[s];
+        Mat3 H;
+        homography(d, src, nx, ny, nz, rho, H);
+        float c = view_cost(src, H, px, py, win, pools.quad);
+        if (GEOMETRIC && (d.flags & 1u) != 0u) {
+            float r0 = 0.0f, r1 = 0.0f;
+            bool valid = reproject(d, src, pools.depth, px, py, X0, X1, X2, r0, r1);
+            float e = valid ? fminf(r0, kGeometricCap) : kGeometricCap;
+            c = c + d.geo_weight * e;
+        }
+        top.insert(c);
+    }
+    return top.mean(d.topk);
+}
+
+// PM:235-245
+__device__ __forceinline__ void random_normal(uint32_t &rng, float rx, float ry, float &nx,
+                                              float &ny, float &nz) {
+    float z = next_random(rng) * 2.0f - 1.0f;
+    float a = next_random(rng) * 6.2831853f;
+    float r = sqrtf(fmaxf(0.0f, 1.0f - z * z));
+    nx = r * (float)cos((double)a);
+    ny = r * (float)sin((double)a);
+    nz = z;
+    float vx = rx, vy = ry, vz = 1.0f;
+    normalize3(vx, vy, vz);
+    if (dot3(nx, ny, nz, vx, vy, vz) > 0.0f) {
+        nx = -nx;
+        ny = -ny;
+        nz = -nz;
+    }
+    nx = nx - vx * 0.5f;
+    ny = ny - vy * 0.5f;
+    nz = nz - vz * 0.5f;
+    normalize3(nx, ny, nz);
+}
+
+// PM:247-249 with WGSL mix(a, b, t) = a * (1 - t) + b * t
+__device__ __forceinline__ float random_depth(uint32_t &rng, const PmDesc &d) {
+    float t = next_random(rng);
+    return 1.0f / (d.inv_max * (1.0f - t) + d.inv_min * t);
+}
+
+__device__ __forceinline__ uint32_t pixel_rng(const PmDesc &d, uint32_t iteration, uint32_t phase,
+                                              uint32_t index) {
+    uint32_t seed = d.seed_base + iteration * 97u + phase * 13u;  // DS:379
+    return pcg(index ^ pcg(seed + iteration * 7919u + phase * 104729u));  // PM:251-253
+}
+
+
+// Locate this thread's pixel: tile of (tx * cols_per_thread) x ty pixels.
+struct PixelTask {
+    int32_t view;
+    int32_t gx, gy;  // thread column / row inside the view
+};
+
+__device__ __forceinline__ PixelTask locate(const PmPlan &plan, int32_t tile_w) {
+    uint32_t b = xcd_remap(blockIdx.x, gridDim.x);
+    PixelTask t;
+    t.view = find_view(plan.block_start, plan.n, b);
+    const PmDesc &d = plan.descs[t.view];
+    uint32_t local = b - plan.block_start[t.view];
+    int32_t tiles_x = (d.W + tile_w - 1) / tile_w;
+    int32_t bx = (int32_t)(local % (uint32_t)tiles_x), by = (int32_t)(local / (uint32_t)tiles_x);
+    t.gx = bx * (int32_t)blockDim.x + (int32_t)threadIdx.x;
+    t.gy = by * (int32_t)blockDim.y + (int32_t)threadIdx.y;
+    return t;
+}
+
+}  // namespace
+}  // namespace lr3d

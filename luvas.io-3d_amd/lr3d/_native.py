@@ -28,7 +28,8 @@ EXPORTED = (
     "lr3d_stereo_create", "lr3d_stereo_destroy", "lr3d_stereo_set_image", "lr3d_stereo_solve",
     "lr3d_stereo_solve_batched", "lr3d_stereo_filter", "lr3d_stereo_read_depth",
     "lr3d_stereo_read_filtered", "lr3d_stereo_read_hyp", "lr3d_stereo_write_depth",
-    "lr3d_stereo_set_coarse",
+    "lr3d_stereo_set_coarse", "lr3d_stereo_set_pose", "lr3d_stereo_write_hyp",
+    "lr3d_ncc_backward", "lr3d_ncc_refine_step",
     "lr3d_tsdf_create", "lr3d_tsdf_destroy", "lr3d_tsdf_integrate", "lr3d_tsdf_integrate_host",
     "lr3d_tsdf_integrate_batched", "lr3d_tsdf_read", "lr3d_tsdf_set_counting",
     "lr3d_tsdf_update_count",
@@ -97,6 +98,12 @@ def load() -> C.CDLL:
         "lr3d_stereo_read_hyp": (i32, [P, i32, C.c_void_p, C.c_void_p]),
         "lr3d_stereo_write_depth": (i32, [P, i32, C.c_void_p]),
         "lr3d_stereo_set_coarse": (i32, [P, P]),
+        "lr3d_stereo_set_pose": (i32, [P, i32, C.POINTER(dbl), C.POINTER(dbl)]),
+        "lr3d_stereo_write_hyp": (i32, [P, i32, C.c_void_p]),
+        "lr3d_ncc_backward": (i32, [P, C.POINTER(SolveDesc), C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
+                                    C.c_void_p, C.c_void_p]),
+        "lr3d_ncc_refine_step": (i32, [P, C.POINTER(SolveDesc), i32, C.c_float, C.c_float, C.c_void_p,
+                                       C.c_void_p]),
         "lr3d_tsdf_create": (i32, [P, C.POINTER(dbl), dbl, C.POINTER(i32), C.c_void_p, i32, dbl,
                                    C.POINTER(P)]),
         "lr3d_tsdf_destroy": (i32, [P]),

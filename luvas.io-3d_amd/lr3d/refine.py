@@ -1,0 +1,80 @@
+"""Gradient refinement of PatchMatch planes and camera poses with the NCC adjoint.
+
+The reference has no refinement of this kind; this module drives lr3d_ncc_refine_step (planes,
+on the GPU) and turns the relative-pose gradients it returns into gradients of the world -> camera
+poses (6-DoF per view), which are updated on the host.
+
+Pose perturbation (left, in the camera frame): R' = exp([w]x) R, t' = exp([w]x) t + tau.
+For the relative pose of reference r and source s (R = R_s R_r^T, t = t_s - R t_r,
+frontend/src/app/vision/geometry/camera.ts:73-77), first order:
+  source:    dR = [w]x R,  dt = [w]x t + tau   ->  dL/dw = sum_k R[:,k] x G[:,k] + t x g,  dL/dtau = g
+  reference: dR = -R [w]x, dt = -R tau         ->  dL/dw = -vee(B - B^T) with B = R^T G,  dL/dtau = -R^T g
+where G = dL/dR and g = dL/dt.
+"""
+from __future__ import annotations
+
+import numpy as np
+
+from .camera import Pose, relative_pose
+
+
+def _vee_skew(B):
+    return np.array([B[2, 1] - B[1, 2], B[0, 2] - B[2, 0], B[1, 0] - B[0, 1]])
+
+
+def world_pose_gradients(views, ref: int, sources, grad_rel) -> dict:
+    """{layer: (dL/dw, dL/dtau)} from relative-pose gradients grad_rel[s] = (dL/dR (9), dL/dt (3))."""
+    out = {}
+    for s, layer in enumerate(sources):
+        G = np.asarray(grad_rel[s][:9], dtype=np.float64).reshape(3, 3)
+        g = np.asarray(grad_rel[s][9:12], dtype=np.float64)
+        rel = relative_pose(views[ref].pose, views[layer].pose)
+        R, t = rel.R, rel.t
+        w_src = sum(np.cross(R[:, k], G[:, k]) for k in range(3)) + np.cross(t, g)
+        w_ref = -_vee_skew(R.T @ G)
+        for lay, w, tau in ((layer, w_src, g), (ref, w_ref, -R.T @ g)):
+            a, b = out.get(lay, (np.zeros(3), np.zeros(3)))
+            out[lay] = (a + w, b + tau)
+    return out
+
+
+def exp_so3(w) -> np.ndarray:
+    w = np.asarray(w, dtype=np.float64)
+    theta = np.linalg.norm(w)
+    K = np.array([[0, -w[2], w[1]], [w[2], 0, -w[0]], [-w[1], w[0], 0]])
+    if theta < 1e-12:
+        return np.eye(3) + K
+    return np.eye(3) + np.sin(theta) / theta * K + (1 - np.cos(theta)) / theta ** 2 * (K @ K)
+
+
+def perturb(pose: Pose, w, tau) -> Pose:
+    """R' = exp([w]x) R, t' = exp([w]x) t + tau."""
+    E = exp_so3(w)
+    return Pose(E @ pose.R, E @ pose.t + np.asarray(tau, dtype=np.float64))
+
+
+def refine(stereo, solves, steps: int = 50, step_depth: float = 2e-4, step_normal: float = 2e-3,
+           step_rot: float = 2e-4, step_trans: float = 2e-4, fixed=(0,), log=None):
+    """`steps` joint gradient steps on the planes of every (ref, options) and on the poses of every
+    view not in `fixed` (gauge).  Pose steps are normalised per view: each moves by at most
+    step_rot radians and step_trans metres along its negative gradient.  Returns the loss history
+    (sum over views, before each step)."""
+    history = []
+    for it in range(steps):
+        grads, losses = stereo.refine_step(solves, step_depth, step_normal)
+        history.append(float(losses.sum()))
+        total = {}
+        for (ref, o), g in zip(solves, grads):
+            srcs = list(o.sources)[:8]
+            for lay, (w, tau) in world_pose_gradients(stereo.views, ref, srcs, g[:len(srcs)]).items():
+                a, b = total.get(lay, (np.zeros(3), np.zeros(3)))
+                total[lay] = (a + w, b + tau)
+        for lay, (w, tau) in total.items():
+            if lay in fixed:
+                continue
+            dw = -step_rot * w / (np.linalg.norm(w) + 1e-30)
+            dt = -step_trans * tau / (np.linalg.norm(tau) + 1e-30)
+            stereo.set_pose(lay, perturb(stereo.views[lay].pose, dw, dt))
+        if log:
+            log(it, history[-1])
+    return history

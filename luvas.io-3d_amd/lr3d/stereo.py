@@ -149,6 +149,44 @@ class HipDenseStereo:
         N.check(self._lib.lr3d_stereo_read_hyp(self.handle, ref, N.ptr(hyp), N.ptr(cost)))
         return hyp, cost
 
+    def set_pose(self, layer: int, pose) -> None:
+        """Replace a layer's world -> camera pose (pose refinement)."""
+        R = (C.c_double * 9)(*[float(v) for v in np.asarray(pose.R, dtype=np.float64).reshape(9)])
+        t = (C.c_double * 3)(*[float(v) for v in np.asarray(pose.t, dtype=np.float64).reshape(3)])
+        N.check(self._lib.lr3d_stereo_set_pose(self.handle, layer, R, t))
+        self.views[layer].pose = pose
+
+    def write_hyp(self, layer: int, hyp: np.ndarray) -> None:
+        w, h = self.size(layer)
+        a = np.ascontiguousarray(hyp, dtype=np.float32).reshape(w * h * 4)
+        N.check(self._lib.lr3d_stereo_write_hyp(self.handle, layer, N.ptr(a)))
+
+    def ncc_backward(self, ref: int, options: SolveOptions, weights: Optional[np.ndarray] = None) -> dict:
+        """Forward cost and adjoint of L = sum_p weights[p] * C_p for the current planes of `ref`."""
+        w, h = self.size(ref)
+        d, _ = self._desc(ref, options)
+        wts = None if weights is None else np.ascontiguousarray(weights, dtype=np.float32).reshape(w * h)
+        cost = np.empty(w * h, dtype=np.float32)
+        gd = np.empty(w * h, dtype=np.float32)
+        gn = np.empty((w * h, 3), dtype=np.float32)
+        gp = np.zeros((N.MAX_SOURCES, 12), dtype=np.float64)
+        loss = np.zeros(1, dtype=np.float64)
+        N.check(self._lib.lr3d_ncc_backward(self.handle, C.byref(d), N.ptr(wts) if wts is not None else None,
+                                            N.ptr(cost), N.ptr(gd), N.ptr(gn), N.ptr(gp), N.ptr(loss)))
+        return {"cost": cost, "grad_depth": gd, "grad_normal": gn, "grad_pose": gp[:d.n_sources],
+                "loss": float(loss[0])}
+
+    def refine_step(self, solves, step_depth: float, step_normal: float):
+        """One gradient step on the planes of every (ref, options); returns the relative-pose
+        gradients (n, n_src, 12) and the losses before the step (n,)."""
+        descs = [self._desc(r, o)[0] for r, o in solves]
+        arr = (N.SolveDesc * len(descs))(*descs)
+        gp = np.zeros((len(descs), N.MAX_SOURCES, 12), dtype=np.float64)
+        loss = np.zeros(len(descs), dtype=np.float64)
+        N.check(self._lib.lr3d_ncc_refine_step(self.handle, arr, len(descs), float(step_depth),
+                                               float(step_normal), N.ptr(gp), N.ptr(loss)))
+        return gp, loss
+
     def set_coarse(self, coarse: Optional["HipDenseStereo"]) -> None:
         N.check(self._lib.lr3d_stereo_set_coarse(self.handle, coarse.handle if coarse else None))
 

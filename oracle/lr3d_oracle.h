@@ -166,6 +166,37 @@ int32_t orc_visible_bricks(const orc_tsdf_grid *g, const orc_integration_view *v
 int64_t orc_tsdf_integrate(const orc_tsdf_grid *g, const orc_integration_view *v,
                            const uint32_t *depth, const float *prior, float *tsdf, float *weight);
 
+/* ---- FP64 cost and its gradient (adjoint_oracle.c) ---- */
+typedef struct {
+    double k[4];
+    double R[9];
+    double t[3];
+    int32_t layer, width, height;
+} orc64_src;
+
+typedef struct {
+    double k[4];
+    double inv_min, inv_max;
+    int32_t n_src, topk;
+    orc64_src src[ORC_MAX_SOURCES];
+} orc64_params;
+
+typedef struct {
+    double cost;
+    double d_depth;
+    double d_normal[3];
+    double d_R[ORC_MAX_SOURCES][9];
+    double d_t[ORC_MAX_SOURCES][3];
+    int32_t selected[ORC_MAX_SOURCES];
+    double view_cost[ORC_MAX_SOURCES];
+    double min_kink;
+} orc64_result;
+
+void orc64_homography(const orc64_params *p, int32_t s, const double n[3], double rho, double H[9]);
+/* Aggregated cost of plane (n, d) at reference pixel (x, y) (layer `ref`) and its gradient. */
+int32_t orc64_cost_grad(const orc64_params *p, const orc_images *img, int32_t ref, int32_t x, int32_t y,
+                        const double n[3], double d, orc64_result *out);
+
 /* Thread count used by the OpenMP loops (0 = runtime default). */
 void orc_set_threads(int32_t n);
 int32_t orc_get_threads(void);

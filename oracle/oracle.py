@@ -332,3 +332,58 @@ def tsdf_integrate(grid: Grid, view, depth: np.ndarray, prior, tsdf, weight) -> 
         prior = np.ascontiguousarray(prior, dtype=np.float32).reshape(-1)
     return load().orc_tsdf_integrate(C.byref(grid.struct), C.byref(view), _p(depth), _p(prior),
                                      _p(tsdf), _p(weight))
+
+
+# ---- FP64 cost and gradient (adjoint_oracle.c) -------------------------------------------------
+
+class Orc64Src(C.Structure):
+    _fields_ = [("k", C.c_double * 4), ("R", C.c_double * 9), ("t", C.c_double * 3),
+                ("layer", C.c_int32), ("width", C.c_int32), ("height", C.c_int32)]
+
+
+class Orc64Params(C.Structure):
+    _fields_ = [("k", C.c_double * 4), ("inv_min", C.c_double), ("inv_max", C.c_double),
+                ("n_src", C.c_int32), ("topk", C.c_int32), ("src", Orc64Src * MAX_SOURCES)]
+
+
+class Orc64Result(C.Structure):
+    _fields_ = [("cost", C.c_double), ("d_depth", C.c_double), ("d_normal", C.c_double * 3),
+                ("d_R", (C.c_double * 9) * MAX_SOURCES), ("d_t", (C.c_double * 3) * MAX_SOURCES),
+                ("selected", C.c_int32 * MAX_SOURCES), ("view_cost", C.c_double * MAX_SOURCES),
+                ("min_kink", C.c_double)]
+
+
+def params64(views, ref, sources, min_depth, max_depth, topk=0, rel=None) -> Orc64Params:
+    """FP64 parameters; `rel` optionally overrides the relative poses [(R 3x3, t 3)] per source."""
+    rv = views[ref]
+    p = Orc64Params()
+    p.k[:] = [rv.intrinsics.fx, rv.intrinsics.fy, rv.intrinsics.cx, rv.intrinsics.cy]
+    p.inv_min, p.inv_max = 1.0 / min_depth, 1.0 / max_depth
+    p.n_src, p.topk = len(sources), topk
+    Ra, ta = np.asarray(rv.pose.R, float), np.asarray(rv.pose.t, float)
+    for s, layer in enumerate(sources):
+        sv = views[layer]
+        if rel is None:
+            R = np.asarray(sv.pose.R, float) @ Ra.T
+            t = np.asarray(sv.pose.t, float) - R @ ta
+        else:
+            R, t = rel[s]
+        o = p.src[s]
+        o.k[:] = [sv.intrinsics.fx, sv.intrinsics.fy, sv.intrinsics.cx, sv.intrinsics.cy]
+        o.R[:] = [float(x) for x in np.asarray(R).reshape(9)]
+        o.t[:] = [float(x) for x in np.asarray(t).reshape(3)]
+        o.layer, o.width, o.height = layer, sv.width, sv.height
+    return p
+
+
+def cost_grad64(p: Orc64Params, images: Images, ref: int, x: int, y: int, n, d: float) -> Orc64Result:
+    lib = load()
+    if not hasattr(lib, "_cg64"):
+        lib.orc64_cost_grad.restype = C.c_int32
+        lib.orc64_cost_grad.argtypes = [C.POINTER(Orc64Params), C.POINTER(OrcImages), C.c_int32, C.c_int32,
+                                        C.c_int32, C.POINTER(C.c_double), C.c_double, C.POINTER(Orc64Result)]
+        lib._cg64 = True
+    r = Orc64Result()
+    nn = (C.c_double * 3)(*[float(v) for v in n])
+    lib.orc64_cost_grad(C.byref(p), C.byref(images.struct), ref, x, y, nn, float(d), C.byref(r))
+    return r

@@ -146,10 +146,13 @@ int32_t lr3d_ncc_backward(lr3d_stereo *st, const lr3d_solve_desc *desc, const fl
                           double *loss);
 /* One gradient step on the planes of every desc's reference view (weights 1): depth moves by
  * -step_depth * dL/dd (at most 0.5 %, kept in the depth range), the normal by -step_normal times
- * the tangential part of dL/dn (kept facing the camera).  Returns, per desc, the relative-pose
- * gradients (n*8*12 doubles, as lr3d_ncc_backward) and the loss before the step (n doubles). */
+ * the tangential part of dL/dn (kept facing the camera); a pixel keeps the step only where its
+ * aggregated cost becomes lower, and the packed depth maps of descs with pack set are rewritten.
+ * Returns, per desc, the relative-pose gradients at the planes before the step (n*8*12 doubles, as
+ * lr3d_ncc_backward) and the loss before and after the step (n doubles each; any may be NULL). */
 int32_t lr3d_ncc_refine_step(lr3d_stereo *st, const lr3d_solve_desc *descs, int32_t n,
-                             float step_depth, float step_normal, double *grad_pose, double *loss);
+                             float step_depth, float step_normal, double *grad_pose,
+                             double *loss_before, double *loss_after);
 
 /* ---- TSDF integration (replaces TsdfVolume, TV:158-291) ---------------------------------- */
 

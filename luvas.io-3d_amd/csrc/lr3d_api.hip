@@ -247,6 +247,7 @@ struct lr3d_stereo {
     DevBuf<uint8_t> plan_buf; // descriptors + block tables of the current launch plan
     DevBuf<float> adj_weights, adj_cost, adj_gd, adj_gn;  // adjoint inputs / outputs
     DevBuf<double> adj_partials;
+    DevBuf<AdjGeom> adj_geom;
     lr3d_stereo *coarse = nullptr;
     std::vector<GraphEntry> graphs;
 
@@ -355,6 +356,20 @@ TileShape lr3d::pm_tile_for(const char *kernel, int32_t W, int32_t H, int32_t n_
 }
 
 // DS:360-401 for one solve, in double then rounded to f32 like Float32Array.set.
+// CAM:73-77 relativePose(ref, src) with math/mat3.ts multiply3 / apply3, in double:
+// R = R_s R_r^T, t = t_s - R t_r.
+static void relative_pose(const lr3d_view &rv, const lr3d_view &sv, double R[9], double t[3]) {
+    const double *a = rv.R, *b = sv.R;
+    const double aT[9] = {a[0], a[3], a[6], a[1], a[4], a[7], a[2], a[5], a[8]};
+    for (int r = 0; r < 3; r++)
+        for (int c = 0; c < 3; c++)
+            R[r * 3 + c] = b[r * 3] * aT[c] + b[r * 3 + 1] * aT[3 + c] + b[r * 3 + 2] * aT[6 + c];
+    for (int r = 0; r < 3; r++) {
+        const double rt = R[r * 3] * rv.t[0] + R[r * 3 + 1] * rv.t[1] + R[r * 3 + 2] * rv.t[2];
+        t[r] = sv.t[r] - rt;
+    }
+}
+
 static int32_t build_desc(const lr3d_stereo *st, const lr3d_solve_desc &sd, uint32_t prior_off,
                           bool require_images, PmDesc &d) {
     const char *what = "lr3d_stereo_solve";
@@ -415,17 +430,8 @@ static int32_t build_desc(const lr3d_stereo *st, const lr3d_solve_desc &sd, uint
         if (require_images && !st->has_image[layer])
             return fail(LR3D_ERR_STATE, "%s: source layer %d has no image", what, layer);
         const lr3d_view &sv = st->views[layer];
-        // CAM:73-77 relativePose(ref, src) with math/mat3.ts multiply3 / apply3, in double
         double R[9], t[3];
-        const double *a = rv.R, *b = sv.R;
-        const double aT[9] = {a[0], a[3], a[6], a[1], a[4], a[7], a[2], a[5], a[8]};
-        for (int r = 0; r < 3; r++)
-            for (int c = 0; c < 3; c++)
-                R[r * 3 + c] = b[r * 3] * aT[c] + b[r * 3 + 1] * aT[3 + c] + b[r * 3 + 2] * aT[6 + c];
-        for (int r = 0; r < 3; r++) {
-            const double rt = R[r * 3] * rv.t[0] + R[r * 3 + 1] * rv.t[1] + R[r * 3 + 2] * rv.t[2];
-            t[r] = sv.t[r] - rt;
-        }
+        relative_pose(rv, sv, R, t);
         PmSrc &o = d.src[s];
         o.k[0] = (float)sv.fx;
         o.k[1] = (float)sv.fy;
@@ -767,9 +773,10 @@ extern "C" int32_t lr3d_stereo_write_hyp(lr3d_stereo *st, int32_t layer, const f
 // partials in block order.
 static int32_t adjoint_impl(lr3d_stereo *st, const lr3d_solve_desc *descs, int32_t n, const float *weights,
                             bool want_maps, float step_depth, float step_normal, int32_t apply,
-                            double *grad_pose, double *loss) {
+                            double *grad_pose, double *loss_before, double *loss_after) {
     lr3d_ctx *ctx = st->ctx;
     std::vector<PmDesc> dv(n);
+    std::vector<AdjGeom> geo(n);
     std::vector<uint8_t> packs(n, 1);
     for (int32_t i = 0; i < n; i++) {
         int32_t rc = build_desc(st, descs[i], 0, true, dv[i]);
@@ -777,15 +784,32 @@ static int32_t adjoint_impl(lr3d_stereo *st, const lr3d_solve_desc *descs, int32
         if (dv[i].n_src <= 0) return fail(LR3D_ERR_INVALID, "lr3d_ncc: solve %d has no sources", i);
         for (int32_t j = 0; j < i; j++)
             if (descs[j].ref == descs[i].ref) return fail(LR3D_ERR_INVALID, "lr3d_ncc: layer %d appears twice", descs[i].ref);
+        const lr3d_view &rv = st->views[descs[i].ref];
+        AdjGeom &g = geo[i];
+        memset(&g, 0, sizeof(g));
+        g.kr[0] = rv.fx;
+        g.kr[1] = rv.fy;
+        g.kr[2] = rv.cx;
+        g.kr[3] = rv.cy;
+        for (int32_t s = 0; s < dv[i].n_src; s++) {
+            const lr3d_view &sv = st->views[descs[i].sources[s]];
+            relative_pose(rv, sv, g.R[s], g.t[s]);
+            g.k[s][0] = sv.fx;
+            g.k[s][1] = sv.fy;
+            g.k[s][2] = sv.cx;
+            g.k[s][3] = sv.cy;
+        }
+        packs[i] = descs[i].pack ? 1 : 0;
     }
     Plans plans;
     std::vector<int64_t> key;
     int32_t rc = upload_plans(st, dv, packs, plans, key);
     if (rc) return rc;
     const PmPlan &plan = plans.init;  // 8 x 8 threads per block
-    const size_t per_block = (size_t)kMaxSources * 12 + 1;
-    HIP_TRY(st->adj_partials.reserve((size_t)plan.blocks * per_block));
-    HIP_TRY(hipMemsetAsync(st->adj_partials.ptr, 0, (size_t)plan.blocks * per_block * 8, ctx->stream));
+    HIP_TRY(st->adj_geom.reserve((size_t)n));
+    HIP_TRY(hipMemcpyAsync(st->adj_geom.ptr, geo.data(), sizeof(AdjGeom) * n, hipMemcpyHostToDevice, ctx->stream));
+    HIP_TRY(st->adj_partials.reserve((size_t)plan.blocks * kAdjPartial));
+    HIP_TRY(hipMemsetAsync(st->adj_partials.ptr, 0, (size_t)plan.blocks * kAdjPartial * 8, ctx->stream));
     const float *wdev = nullptr;
     if (weights) {
         HIP_TRY(st->adj_weights.reserve(st->total_px));
@@ -801,11 +825,16 @@ static int32_t adjoint_impl(lr3d_stereo *st, const lr3d_solve_desc *descs, int32
     }
     const PmPools pools = pools_of(st);
     HIP_TRY(timed(ctx, "ncc_backward", [&] {
-        return ncc_launch_backward(ctx->stream, plan, pools, wdev, want_maps ? st->adj_cost.ptr : nullptr,
-                                   want_maps ? st->adj_gd.ptr : nullptr, want_maps ? st->adj_gn.ptr : nullptr,
-                                   st->adj_partials.ptr, step_depth, step_normal, apply);
+        return ncc_launch_backward(ctx->stream, plan, pools, st->adj_geom.ptr, wdev,
+                                   want_maps ? st->adj_cost.ptr : nullptr, want_maps ? st->adj_gd.ptr : nullptr,
+                                   want_maps ? st->adj_gn.ptr : nullptr, st->adj_partials.ptr, step_depth,
+                                   step_normal, apply);
     }));
-    std::vector<double> partials((size_t)plan.blocks * per_block);
+    if (apply && plans.pack.blocks > 0) {
+        // refined planes -> packed depth maps of the descs that pack (DS:279-282)
+        HIP_TRY(timed(ctx, "pm_pack", [&] { return pm_launch_pack(ctx->stream, plans.pack, pools); }));
+    }
+    std::vector<double> partials((size_t)plan.blocks * kAdjPartial);
     HIP_TRY(hipMemcpyAsync(partials.data(), st->adj_partials.ptr, partials.size() * 8, hipMemcpyDeviceToHost,
                            ctx->stream));
     rc = finish(ctx);
@@ -818,14 +847,16 @@ static int32_t adjoint_impl(lr3d_stereo *st, const lr3d_solve_desc *descs, int32
     for (int32_t v = 0; v < n; v++) {
         double *gp = grad_pose ? grad_pose + (size_t)v * kMaxSources * 12 : nullptr;
         if (gp) std::fill(gp, gp + kMaxSources * 12, 0.0);
-        double l = 0.0;
+        double l0 = 0.0, l1 = 0.0;
         for (uint32_t b = starts[v]; b < starts[v + 1]; b++) {
-            const double *p = &partials[(size_t)b * per_block];
-            l += p[0];
+            const double *p = &partials[(size_t)b * kAdjPartial];
+            l0 += p[0];
+            l1 += p[1];
             if (gp)
-                for (int j = 0; j < kMaxSources * 12; j++) gp[j] += p[1 + j];
+                for (int j = 0; j < kMaxSources * 12; j++) gp[j] += p[2 + j];
         }
-        if (loss) loss[v] = l;
+        if (loss_before) loss_before[v] = l0;
+        if (loss_after) loss_after[v] = l1;
     }
     return LR3D_OK;
 }
@@ -839,7 +870,7 @@ extern "C" int32_t lr3d_ncc_backward(lr3d_stereo *st, const lr3d_solve_desc *des
     lr3d_ctx *ctx = st->ctx;
     std::lock_guard<std::mutex> lock(ctx->mu);
     HIP_TRY(hipSetDevice(ctx->device));
-    rc = adjoint_impl(st, desc, 1, weights, true, 0.0f, 0.0f, 0, grad_pose, loss);
+    rc = adjoint_impl(st, desc, 1, weights, true, 0.0f, 0.0f, 0, grad_pose, loss, nullptr);
     if (rc) return rc;
     const lr3d_view &v = st->views[desc->ref];
     const size_t px = (size_t)v.width * v.height, off = st->off[desc->ref];
@@ -851,12 +882,16 @@ extern "C" int32_t lr3d_ncc_backward(lr3d_stereo *st, const lr3d_solve_desc *des
 }
 
 extern "C" int32_t lr3d_ncc_refine_step(lr3d_stereo *st, const lr3d_solve_desc *descs, int32_t n,
-                                        float step_depth, float step_normal, double *grad_pose, double *loss) {
+                                        float step_depth, float step_normal, double *grad_pose,
+                                        double *loss_before, double *loss_after) {
     if (!st || !descs || n <= 0) return fail(LR3D_ERR_INVALID, "lr3d_ncc_refine_step: NULL argument or n <= 0");
     lr3d_ctx *ctx = st->ctx;
     std::lock_guard<std::mutex> lock(ctx->mu);
     HIP_TRY(hipSetDevice(ctx->device));
-    return adjoint_impl(st, descs, n, nullptr, false, step_depth, step_normal, 1, grad_pose, loss);
+    if (!(step_depth >= 0.0f) || !(step_normal >= 0.0f))
+        return fail(LR3D_ERR_INVALID, "lr3d_ncc_refine_step: steps must be >= 0");
+    return adjoint_impl(st, descs, n, nullptr, false, step_depth, step_normal, 1, grad_pose, loss_before,
+                        loss_after);
 }
 
 // ---- TSDF ---------------------------------------------------------------------------------------------

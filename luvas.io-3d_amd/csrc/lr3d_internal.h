@@ -76,6 +76,19 @@ struct PmPlan {
     int32_t geometric;          // any solve of the batch is a geometric pass
 };
 
+// FP64 camera geometry of one solve, read by the adjoint's backward passes: reference intrinsics
+// and, per source, intrinsics and the relative pose (CAM:73-77) before rounding to FP32.
+struct AdjGeom {
+    double kr[4];
+    double k[kMaxSources][4];
+    double R[kMaxSources][9];
+    double t[kMaxSources][3];
+};
+
+// Doubles per block in the adjoint's partial sums: loss before, loss after the plane step, then
+// dL/dR (9) and dL/dt (3) of each source.
+constexpr int32_t kAdjPartial = 2 + kMaxSources * 12;
+
 // One view of a TSDF integration pass (the 128-byte uniform of TV:244-253 plus culling data).
 struct TsdfView {
     float k[4];
@@ -110,12 +123,12 @@ hipError_t pm_launch_propagate(hipStream_t s, const PmPlan &plan, const PmPools 
 hipError_t pm_launch_pack(hipStream_t s, const PmPlan &plan, const PmPools &pools);
 hipError_t pm_launch_consistency(hipStream_t s, const PmPlan &plan, const PmPools &pools,
                                  uint32_t needed);
-// NCC adjoint: costs, dL/d(depth, normal) per pixel and per-block pose partials (1 + 8*12
-// doubles per block: loss, then dL/dR (9) and dL/dt (3) of each source); apply != 0 also takes one
-// gradient step on the planes.
+// NCC adjoint: costs, dL/d(depth, normal) per pixel and per-block pose partials (kAdjPartial
+// doubles per block); apply != 0 also takes one gradient step on the planes, kept where it lowers
+// the pixel's cost.
 hipError_t ncc_launch_backward(hipStream_t s, const PmPlan &plan, const PmPools &pools,
-                               const float *weights, float *cost, float *gd, float *gn,
-                               double *pose_partials, float step_depth, float step_normal,
+                               const AdjGeom *geom, const float *weights, float *cost, float *gd,
+                               float *gn, double *pose_partials, float step_depth, float step_normal,
                                int32_t apply);
 hipError_t tsdf_launch_visibility(hipStream_t s, const TsdfGridDev &g, const int4 *coords,
                                   const TsdfView *views, int32_t n_views, uint32_t *vis,

@@ -5,97 +5,156 @@
 // of PM:192-233 (mean of the k smallest bilateral-NCC view costs).  The kernel returns
 // dL/dd_p, dL/dn_p and, reduced per block, dL/dR_s and dL/dt_s of the relative pose of every
 // source view s, for L = sum_p weight_p C_p.  Windows and taps are recomputed (nothing from the
-// forward pass is stored): pass 1 evaluates every view's cost exactly as the forward kernels do
-// and selects the k smallest; for each selected view whose cost is differentiable (all taps inside,
-// variance above the floor, cost strictly inside (0, 2)), pass 2 re-accumulates the NCC sums and
-// pass 3 accumulates dC/dH over the 36 taps, which is chained through
-// H = Ks (R + t n^T / rho) Kr^-1, rho = d (n . ray).  The top-k selection, INVALID views and the
+// forward pass is stored):
+//   pass 1  every view's cost exactly as the forward kernels compute it (FP32), and the k
+//           smallest by (cost, index);
+//   pass 2  for each selected view whose cost is differentiable (all taps inside, variance above
+//           the floor, cost strictly inside (0, 2)), the NCC sums;
+//   pass 3  dC/dH accumulated over the 36 taps and chained through
+//           H = Ks (R + t n^T / rho) Kr^-1, rho = d (n . ray).
+// Passes 2 and 3 run in FP64 from the FP32 plane, the u8 images and the FP64 relative poses: in
+// FP32 the taps q = H (X, Y, 1) at pixel coordinates in the hundreds carry ~1e-4 px of rounding,
+// which moves single-pixel gradients by up to ~1 %.  The top-k selection, INVALID views and the
 // clamp contribute no gradient (the cost is piecewise constant there).
 //
-// Optionally the same launch applies one gradient step to the planes (refinement).
+// Optionally the same launch takes one gradient step on every plane and keeps it where the
+// step's FP32 aggregated cost is lower than the current one (refinement never raises a pixel's
+// cost for fixed poses).
 #include "pm_device.h"
 
 namespace lr3d {
 
 namespace {
 
-// NCC sums of one source view in FP32 (the forward arithmetic) for the adjoint passes.
-struct ViewSums {
-    float ss, sss, srs;
+// The bilateral window of every lane, staged in LDS as [sample][lane] (bank-conflict free), so
+// that the FP64 tap loops below can stay rolled without spilling the window to scratch.
+struct WindowLds {
+    float v[kSamples][64];
+    float w[kSamples][64];
 };
 
-__device__ __forceinline__ ViewSums view_sums(const PmSrc &s, const Mat3 &H, float px, float py,
-                                              const Window &win, const uint32_t *__restrict__ quad) {
-    const uint32_t *__restrict__ img = quad + s.off;
-    const float wmax = (float)s.W - 1.0f, hmax = (float)s.H - 1.0f;
-    ViewSums r{0.0f, 0.0f, 0.0f};
-#pragma unroll
+// Reference-window statistics in FP64 from the FP32 window (values and bilateral weights).  The
+// forward pass forms them in FP32, where E[x^2] - E[x]^2 cancels.
+struct RefStats {
+    double sw, mean, var;
+};
+
+__device__ __forceinline__ RefStats ref_stats(const WindowLds &win, int lane) {
+    double sw = 0.0, sr = 0.0;
+#pragma unroll 6
     for (int i = 0; i < kSamples; i++) {
-        const float X = px + (float)((i % 6) * 2 - 5), Y = py + (float)((i / 6) * 2 - 5);
-        const float qx = fmaf(H.m[0], X, fmaf(H.m[1], Y, H.m[2]));
-        const float qy = fmaf(H.m[3], X, fmaf(H.m[4], Y, H.m[5]));
-        const float qz = fmaf(H.m[6], X, fmaf(H.m[7], Y, H.m[8]));
-        const float rz = 1.0f / qz;
-        const float u = fminf(fmaxf(qx * rz, 0.0f), wmax), v = fminf(fmaxf(qy * rz, 0.0f), hmax);
-        const float x0 = floorf(u), y0 = floorf(v);
-        const float fx = u - x0, fy = v - y0;
-        const uint32_t q = img[__umul24((uint32_t)y0, (uint32_t)s.W) + (uint32_t)x0];
-        const float a = (float)(q & 0xffu), b = (float)((q >> 8) & 0xffu);
-        const float c = (float)((q >> 16) & 0xffu), e = (float)(q >> 24);
-        const float top = fmaf(fx, b - a, a), bot = fmaf(fx, e - c, c);
-        const float val = fmaf(fy, bot - top, top);
-        const float wv = win.w[i] * val;
-        r.ss = r.ss + wv;
-        r.sss = fmaf(wv, val, r.sss);
-        r.srs = fmaf(wv, win.v[i], r.srs);
+        sw += (double)win.w[i][lane];
+        sr = fma((double)win.w[i][lane], (double)win.v[i][lane], sr);
     }
-    return r;
+    const double mean = sr / sw;
+    double var = 0.0;
+#pragma unroll 6
+    for (int i = 0; i < kSamples; i++) {
+        const double dv = (double)win.v[i][lane] - mean;
+        var = fma((double)win.w[i][lane] * dv, dv, var);
+    }
+    return RefStats{sw, mean, var / sw};
 }
 
-// dC_view/dH (row-major) of a differentiable view, given its sums.
-__device__ __forceinline__ void view_grad_h(const PmSrc &s, const Mat3 &H, float px, float py,
-                                            const Window &win, const uint32_t *__restrict__ quad,
-                                            const ViewSums &sums, float G[9]) {
+struct Mat3d {
+    double m[9];
+};
+
+// H = Ks (R + t n^T / rho) Kr^-1 in FP64.
+__device__ __forceinline__ void homography64(const AdjGeom &g, int s, const double n[3], double rho,
+                                             Mat3d &H) {
+    double M[9];
+#pragma unroll
+    for (int i = 0; i < 3; i++)
+#pragma unroll
+        for (int j = 0; j < 3; j++) M[i * 3 + j] = g.R[s][i * 3 + j] + g.t[s][i] * n[j] / rho;
+    const double *k = g.k[s];
+    double A[9];
+#pragma unroll
+    for (int j = 0; j < 3; j++) {
+        A[j] = k[0] * M[j] + k[2] * M[6 + j];
+        A[3 + j] = k[1] * M[3 + j] + k[3] * M[6 + j];
+        A[6 + j] = M[6 + j];
+    }
+    const double *kr = g.kr;
+#pragma unroll
+    for (int i = 0; i < 3; i++) {
+        H.m[i * 3 + 0] = A[i * 3 + 0] / kr[0];
+        H.m[i * 3 + 1] = A[i * 3 + 1] / kr[1];
+        H.m[i * 3 + 2] = A[i * 3 + 2] - A[i * 3 + 0] * kr[2] / kr[0] - A[i * 3 + 1] * kr[3] / kr[1];
+    }
+}
+
+// Tap of a source view in FP64: position (clamped into the image, so reads stay inside), the
+// bilinear sample and its texel-space gradient.
+struct Tap {
+    double u, v, rz, val, gu, gv;
+};
+
+__device__ __forceinline__ Tap source_tap(const uint32_t *__restrict__ img, int32_t W, double wmax, double hmax,
+                                          const Mat3d &H, double X, double Y) {
+    const double qx = fma(H.m[0], X, fma(H.m[1], Y, H.m[2]));
+    const double qy = fma(H.m[3], X, fma(H.m[4], Y, H.m[5]));
+    const double qz = fma(H.m[6], X, fma(H.m[7], Y, H.m[8]));
+    Tap t;
+    t.rz = 1.0 / qz;
+    t.u = fmin(fmax(qx * t.rz, 0.0), wmax);
+    t.v = fmin(fmax(qy * t.rz, 0.0), hmax);
+    const double x0 = floor(t.u), y0 = floor(t.v);
+    const double fx = t.u - x0, fy = t.v - y0;
+    const uint32_t q = img[__umul24((uint32_t)y0, (uint32_t)W) + (uint32_t)x0];
+    const double a = (double)(q & 0xffu), b = (double)((q >> 8) & 0xffu);
+    const double c = (double)((q >> 16) & 0xffu), e = (double)(q >> 24);
+    const double top = fma(fx, b - a, a), bot = fma(fx, e - c, c);
+    t.val = fma(fy, bot - top, top);
+    t.gu = (b - a) * (1.0 - fy) + (e - c) * fy;
+    t.gv = bot - top;
+    return t;
+}
+
+// dC_view/dH (row-major) of a differentiable view (passes 2 and 3).
+//   dC/dval_i = (w_i / S) [B (val_i - mean_s) - (r_i - mean_r) A],  A = 1/den, B = cov var_r / den^3
+//   du/dH_0j = x_j / q_z, dv/dH_1j = x_j / q_z, d(u, v)/dH_2j = -(u, v) x_j / q_z
+__device__ __forceinline__ void view_grad_h(const PmSrc &s, const Mat3d &H, double px, double py,
+                                            const WindowLds &win, int lane, const RefStats &rs,
+                                            const uint32_t *__restrict__ quad, double G[9]) {
     const uint32_t *__restrict__ img = quad + s.off;
-    const float wmax = (float)s.W - 1.0f, hmax = (float)s.H - 1.0f;
-    const float inv_sw = 1.0f / win.sw;
-    const float mean = sums.ss * inv_sw;
-    const float var_s = fmaxf(sums.sss * inv_sw - mean * mean, 0.0f);
-    const float den = sqrtf(win.var * var_s);
-    const float cov = sums.srs * inv_sw - win.mean * mean;
-    const float A = 1.0f / den;
-    const float B = cov * win.var * A * A * A;
-#pragma unroll
-    for (int k = 0; k < 9; k++) G[k] = 0.0f;
-#pragma unroll
+    const double wmax = (double)s.W - 1.0, hmax = (double)s.H - 1.0;
+    double ss = 0.0, sss = 0.0, srs = 0.0;
+#pragma unroll 1
     for (int i = 0; i < kSamples; i++) {
-        const float X = px + (float)((i % 6) * 2 - 5), Y = py + (float)((i / 6) * 2 - 5);
-        const float qx = fmaf(H.m[0], X, fmaf(H.m[1], Y, H.m[2]));
-        const float qy = fmaf(H.m[3], X, fmaf(H.m[4], Y, H.m[5]));
-        const float qz = fmaf(H.m[6], X, fmaf(H.m[7], Y, H.m[8]));
-        const float rz = 1.0f / qz;
-        const float u = fminf(fmaxf(qx * rz, 0.0f), wmax), v = fminf(fmaxf(qy * rz, 0.0f), hmax);
-        const float x0 = floorf(u), y0 = floorf(v);
-        const float fx = u - x0, fy = v - y0;
-        const uint32_t q = img[__umul24((uint32_t)y0, (uint32_t)s.W) + (uint32_t)x0];
-        const float a = (float)(q & 0xffu), b = (float)((q >> 8) & 0xffu);
-        const float c = (float)((q >> 16) & 0xffu), e = (float)(q >> 24);
-        const float top = fmaf(fx, b - a, a), bot = fmaf(fx, e - c, c);
-        const float val = fmaf(fy, bot - top, top);
-        // d cost / d val_i
-        const float dval = win.w[i] * inv_sw * (B * (val - mean) - (win.v[i] - win.mean) * A);
-        const float gu = (b - a) * (1.0f - fy) + (e - c) * fy;
-        const float gv = bot - top;
-        const float gx = dval * gu * rz, gy = dval * gv * rz;
-        const float gz = -(gx * u + gy * v);
-        G[0] = fmaf(gx, X, G[0]);
-        G[1] = fmaf(gx, Y, G[1]);
+        const double X = px + (double)((i % 6) * 2 - 5), Y = py + (double)((i / 6) * 2 - 5);
+        const Tap t = source_tap(img, s.W, wmax, hmax, H, X, Y);
+        const double wv = (double)win.w[i][lane] * t.val;
+        ss += wv;
+        sss = fma(wv, t.val, sss);
+        srs = fma(wv, (double)win.v[i][lane], srs);
+    }
+    const double inv_sw = 1.0 / rs.sw;
+    const double mean = ss * inv_sw;
+    const double var_s = fmax(sss * inv_sw - mean * mean, 0.0);
+    const double den = sqrt(rs.var * var_s);
+    const double cov = srs * inv_sw - rs.mean * mean;
+    const double A = den > 0.0 ? 1.0 / den : 0.0;
+    const double B = cov * rs.var * A * A * A;
+#pragma unroll
+    for (int k = 0; k < 9; k++) G[k] = 0.0;
+#pragma unroll 1
+    for (int i = 0; i < kSamples; i++) {
+        const double X = px + (double)((i % 6) * 2 - 5), Y = py + (double)((i / 6) * 2 - 5);
+        const Tap t = source_tap(img, s.W, wmax, hmax, H, X, Y);
+        const double dval = (double)win.w[i][lane] * inv_sw *
+                            (B * (t.val - mean) - ((double)win.v[i][lane] - rs.mean) * A);
+        const double gx = dval * t.gu * t.rz, gy = dval * t.gv * t.rz;
+        const double gz = -(gx * t.u + gy * t.v);
+        G[0] = fma(gx, X, G[0]);
+        G[1] = fma(gx, Y, G[1]);
         G[2] += gx;
-        G[3] = fmaf(gy, X, G[3]);
-        G[4] = fmaf(gy, Y, G[4]);
+        G[3] = fma(gy, X, G[3]);
+        G[4] = fma(gy, Y, G[4]);
         G[5] += gy;
-        G[6] = fmaf(gz, X, G[6]);
-        G[7] = fmaf(gz, Y, G[7]);
+        G[6] = fma(gz, X, G[6]);
+        G[7] = fma(gz, Y, G[7]);
         G[8] += gz;
     }
 }
@@ -110,6 +169,7 @@ __device__ __forceinline__ double wave_sum(double v) {
 // One wavefront per block (8 x 8 threads): pose gradients are reduced per wavefront with a fixed
 // shuffle tree and written as one partial per block (the host adds them in order).
 __global__ void __launch_bounds__(64) ncc_backward_kernel(PmPlan plan, PmPools pools,
+                                                          const AdjGeom *__restrict__ geom,
                                                           const float *__restrict__ weights,
                                                           float *__restrict__ out_cost,
                                                           float *__restrict__ out_gd,
@@ -119,6 +179,7 @@ __global__ void __launch_bounds__(64) ncc_backward_kernel(PmPlan plan, PmPools p
                                                           int32_t apply) {
     PixelTask task = locate(plan, (int32_t)blockDim.x);
     const PmDesc &d = plan.descs[task.view];
+    const AdjGeom &g = geom[task.view];
     const int32_t x = task.gx, y = task.gy;
     const bool inside = x < d.W && y < d.H;
     const uint32_t index = inside ? (uint32_t)y * (uint32_t)d.W + (uint32_t)x : 0u;
@@ -130,16 +191,28 @@ __global__ void __launch_bounds__(64) ncc_backward_kernel(PmPlan plan, PmPools p
     const float nr = dot3(h.x, h.y, h.z, rx, ry, 1.0f);
     const bool valid = inside && h.w > 0.0f && !(inv > d.inv_min) && !(inv < d.inv_max) &&
                        nr < -1e-3f && d.n_src > 0;
-    Window win;
-    float rho = 0.0f;
+    __shared__ WindowLds lds;
+    const int lane = (int)(threadIdx.y * blockDim.x + threadIdx.x);
+    RefStats rs{1.0, 0.0, 0.0};
+    float wsw = 1.0f, wmean = 0.0f, wvar = 0.0f;  // the FP32 window statistics of the forward pass
     float costs[kMaxSources];
     uint32_t selected = 0u;
     float cost = kInvalid;
 #pragma unroll
     for (int s = 0; s < kMaxSources; s++) costs[s] = INFINITY;
     if (valid) {
+        Window win;
         prepare_window(pools.quad + d.off, d.W, d.H, x, y, win);
-        rho = dot3(h.x, h.y, h.z, rx * h.w, ry * h.w, 1.0f * h.w);
+#pragma unroll
+        for (int i = 0; i < kSamples; i++) {
+            lds.v[i][lane] = win.v[i];
+            lds.w[i][lane] = win.w[i];
+        }
+        wsw = win.sw;
+        wmean = win.mean;
+        wvar = win.var;
+        rs = ref_stats(lds, lane);
+        const float rho = dot3(h.x, h.y, h.z, rx * h.w, ry * h.w, 1.0f * h.w);
         // pass 1: view costs, exactly as the forward pass, and the k smallest by (cost, index)
 #pragma unroll 1
         for (int s = 0; s < d.n_src; s++) {
@@ -166,12 +239,17 @@ __global__ void __launch_bounds__(64) ncc_backward_kernel(PmPlan plan, PmPools p
         }
         cost = total / (float)d.topk;
     }
+    // FP64 plane: X = d ray, rho = n . X
+    const double n[3] = {(double)h.x, (double)h.y, (double)h.z};
+    const double dep = (double)h.w;
+    const double rayd[3] = {((double)x - g.kr[2]) / g.kr[0], ((double)y - g.kr[3]) / g.kr[1], 1.0};
+    const double Xp[3] = {rayd[0] * dep, rayd[1] * dep, dep};
+    const double nrd = n[0] * rayd[0] + n[1] * rayd[1] + n[2] * rayd[2];
+    const double rhod = nrd * dep, r2 = rhod * rhod;
     const double scale = (double)weight / (double)d.topk;
-    const double X0 = (double)rx * h.w, X1 = (double)ry * h.w, X2 = h.w;
-    const double n0 = h.x, n1 = h.y, n2 = h.z;
     double dd = 0.0, dn0 = 0.0, dn1 = 0.0, dn2 = 0.0;
     const uint32_t b = xcd_remap(blockIdx.x, gridDim.x);
-    double *partial = pose_partials ? pose_partials + (size_t)b * (kMaxSources * 12 + 1) : nullptr;
+    double *partial = pose_partials ? pose_partials + (size_t)b * kAdjPartial : nullptr;
     const bool lead = (threadIdx.x | threadIdx.y) == 0;
     // passes 2 and 3 per source view; the loop is uniform over the wavefront for the reduction
     for (int s = 0; s < d.n_src; s++) {
@@ -182,70 +260,59 @@ __global__ void __launch_bounds__(64) ncc_backward_kernel(PmPlan plan, PmPools p
 #pragma unroll
         for (int q = 0; q < kMaxSources; q++)
             if (q == s) cs = costs[q];
-        // INVALID, degenerate (exactly 1) or clamped views carry no gradient
-        bool smooth = valid && (selected & (1u << s)) && cs > 0.0f && cs < 2.0f && cs != 1.0f;
+        // INVALID (2), degenerate (exactly 1) or clamped (0, 2) views carry no gradient: the cost
+        // is constant there.  A non-degenerate view whose FP32 cost is exactly 1 is skipped as well.
+        const bool smooth = valid && (selected & (1u << s)) && cs > 0.0f && cs < 2.0f && cs != 1.0f;
         if (smooth) {
             const PmSrc &src = d.src[s];
-            Mat3 H;
-            homography(d, src, h.x, h.y, h.z, rho, H);
-            const ViewSums sums = view_sums(src, H, px, py, win, pools.quad);
-            const float mean = sums.ss / win.sw;
-            const float var_s = fmaxf(sums.sss / win.sw - mean * mean, 0.0f);
-            if (!(sqrtf(win.var * var_s) < kDegenerateDen)) {
-                float G[9];
-                view_grad_h(src, H, px, py, win, pools.quad, sums, G);
-                // G_M = Ks^T G Kr^-T
-                double T[9];
-                for (int j = 0; j < 3; j++) {
-                    T[j] = (double)src.k[0] * G[j];
-                    T[3 + j] = (double)src.k[1] * G[3 + j];
-                    T[6 + j] = (double)src.k[2] * G[j] + (double)src.k[3] * G[3 + j] + G[6 + j];
-                }
-                double GM[9];
-                for (int i = 0; i < 3; i++) {
-                    GM[i * 3 + 0] = T[i * 3 + 0] * d.ikx + T[i * 3 + 2] * d.kx2;
-                    GM[i * 3 + 1] = T[i * 3 + 1] * d.iky + T[i * 3 + 2] * d.ky2;
-                    GM[i * 3 + 2] = T[i * 3 + 2];
-                }
-                const double t0 = src.t[0], t1 = src.t[1], t2 = src.t[2];
-                const double rhod = rho, r2 = rhod * rhod;
-                double gmn[3];
-                for (int i = 0; i < 3; i++) gmn[i] = GM[i * 3] * n0 + GM[i * 3 + 1] * n1 + GM[i * 3 + 2] * n2;
-                const double tGn = t0 * gmn[0] + t1 * gmn[1] + t2 * gmn[2];
-                dd += -tGn * (double)nr / r2;
-                dn0 += (t0 * GM[0] + t1 * GM[3] + t2 * GM[6]) / rhod - tGn * X0 / r2;
-                dn1 += (t0 * GM[1] + t1 * GM[4] + t2 * GM[7]) / rhod - tGn * X1 / r2;
-                dn2 += (t0 * GM[2] + t1 * GM[5] + t2 * GM[8]) / rhod - tGn * X2 / r2;
+            Mat3d H;
+            homography64(g, s, n, rhod, H);
+            double G[9];
+            view_grad_h(src, H, (double)x, (double)y, lds, lane, rs, pools.quad, G);
+            // G_M = Ks^T G Kr^-T
+            const double *k = g.k[s];
+            double T[9];
 #pragma unroll
-                for (int j = 0; j < 9; j++) gpose[j] = scale * GM[j];
-#pragma unroll
-                for (int j = 0; j < 3; j++) gpose[9 + j] = scale * gmn[j] / rhod;
+            for (int j = 0; j < 3; j++) {
+                T[j] = k[0] * G[j];
+                T[3 + j] = k[1] * G[3 + j];
+                T[6 + j] = k[2] * G[j] + k[3] * G[3 + j] + G[6 + j];
             }
+            double GM[9];
+#pragma unroll
+            for (int i = 0; i < 3; i++) {
+                GM[i * 3 + 0] = (T[i * 3 + 0] - T[i * 3 + 2] * g.kr[2]) / g.kr[0];
+                GM[i * 3 + 1] = (T[i * 3 + 1] - T[i * 3 + 2] * g.kr[3]) / g.kr[1];
+                GM[i * 3 + 2] = T[i * 3 + 2];
+            }
+            const double *t = g.t[s];
+            double gmn[3];
+#pragma unroll
+            for (int i = 0; i < 3; i++) gmn[i] = GM[i * 3] * n[0] + GM[i * 3 + 1] * n[1] + GM[i * 3 + 2] * n[2];
+            const double tGn = t[0] * gmn[0] + t[1] * gmn[1] + t[2] * gmn[2];
+            dd += -tGn * nrd / r2;
+            dn0 += (t[0] * GM[0] + t[1] * GM[3] + t[2] * GM[6]) / rhod - tGn * Xp[0] / r2;
+            dn1 += (t[0] * GM[1] + t[1] * GM[4] + t[2] * GM[7]) / rhod - tGn * Xp[1] / r2;
+            dn2 += (t[0] * GM[2] + t[1] * GM[5] + t[2] * GM[8]) / rhod - tGn * Xp[2] / r2;
+#pragma unroll
+            for (int j = 0; j < 9; j++) gpose[j] = scale * GM[j];
+#pragma unroll
+            for (int j = 0; j < 3; j++) gpose[9 + j] = scale * gmn[j] / rhod;
         }
         if (partial) {
 #pragma unroll
             for (int j = 0; j < 12; j++) {
                 const double sum = wave_sum(gpose[j]);
-                if (lead) partial[1 + s * 12 + j] = sum;
+                if (lead) partial[2 + s * 12 + j] = sum;
             }
         }
     }
-    if (partial) {
-        const double ls = wave_sum(valid ? (double)cost * weight : 0.0);
-        if (lead) partial[0] = ls;
-    }
-    if (!inside) return;
     const float gd = (float)(scale * dd);
     const float gn0 = (float)(scale * dn0), gn1 = (float)(scale * dn1), gn2 = (float)(scale * dn2);
-    if (out_cost) out_cost[d.off + index] = cost;
-    if (out_gd) out_gd[d.off + index] = gd;
-    if (out_gn) {
-        out_gn[(size_t)(d.off + index) * 3 + 0] = gn0;
-        out_gn[(size_t)(d.off + index) * 3 + 1] = gn1;
-        out_gn[(size_t)(d.off + index) * 3 + 2] = gn2;
-    }
+    float after = cost;
     if (apply && valid) {
-        // one gradient step on the plane, kept valid (depth range, facing the camera)
+        // one gradient step on the plane (depth kept in range and within 0.5 % of the current
+        // value, normal kept unit and facing the camera), kept if its aggregated cost is lower
         const float lo = 1.0f / d.inv_min, hi = 1.0f / d.inv_max;
         float step = -step_depth * gd;
         step = fminf(fmaxf(step, -0.005f * h.w), 0.005f * h.w);
@@ -260,20 +327,50 @@ __global__ void __launch_bounds__(64) ncc_backward_kernel(PmPlan plan, PmPools p
             my = h.y;
             mz = h.z;
         }
-        pools.hyp[d.off + index] = make_float4(mx, my, mz, nd);
+        Window win;
+#pragma unroll
+        for (int i = 0; i < kSamples; i++) {
+            win.v[i] = lds.v[i][lane];
+            win.w[i] = lds.w[i][lane];
+        }
+        win.sw = wsw;
+        win.mean = wmean;
+        win.var = wvar;
+        const float c = hypothesis_cost<kMaxSources, false>(d, pools, px, py, rx, ry, win, mx, my, mz, nd);
+        if (c < cost) {
+            after = c;
+            pools.hyp[d.off + index] = make_float4(mx, my, mz, nd);
+            pools.cost[d.off + index] = c;
+        }
+    }
+    if (partial) {
+        const double l0 = wave_sum(valid ? (double)cost * weight : 0.0);
+        const double l1 = wave_sum(valid ? (double)after * weight : 0.0);
+        if (lead) {
+            partial[0] = l0;
+            partial[1] = l1;
+        }
+    }
+    if (!inside) return;
+    if (out_cost) out_cost[d.off + index] = cost;
+    if (out_gd) out_gd[d.off + index] = gd;
+    if (out_gn) {
+        out_gn[(size_t)(d.off + index) * 3 + 0] = gn0;
+        out_gn[(size_t)(d.off + index) * 3 + 1] = gn1;
+        out_gn[(size_t)(d.off + index) * 3 + 2] = gn2;
     }
 }
 
 }  // namespace
 
 hipError_t ncc_launch_backward(hipStream_t s, const PmPlan &plan, const PmPools &pools,
-                               const float *weights, float *cost, float *gd, float *gn,
-                               double *pose_partials, float step_depth, float step_normal,
+                               const AdjGeom *geom, const float *weights, float *cost, float *gd,
+                               float *gn, double *pose_partials, float step_depth, float step_normal,
                                int32_t apply) {
     if (plan.tx * plan.ty != 64) return hipErrorInvalidConfiguration;
     dim3 block(plan.tx, plan.ty), grid(plan.blocks);
-    hipLaunchKernelGGL(ncc_backward_kernel, grid, block, 0, s, plan, pools, weights, cost, gd, gn,
-                       pose_partials, step_depth, step_normal, apply);
+    hipLaunchKernelGGL(ncc_backward_kernel, grid, block, 0, s, plan, pools, geom, weights, cost, gd,
+                       gn, pose_partials, step_depth, step_normal, apply);
     return hipGetLastError();
 }
 

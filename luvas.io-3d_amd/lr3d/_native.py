@@ -103,7 +103,7 @@ def load() -> C.CDLL:
         "lr3d_ncc_backward": (i32, [P, C.POINTER(SolveDesc), C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
                                     C.c_void_p, C.c_void_p]),
         "lr3d_ncc_refine_step": (i32, [P, C.POINTER(SolveDesc), i32, C.c_float, C.c_float, C.c_void_p,
-                                       C.c_void_p]),
+                                       C.c_void_p, C.c_void_p]),
         "lr3d_tsdf_create": (i32, [P, C.POINTER(dbl), dbl, C.POINTER(i32), C.c_void_p, i32, dbl,
                                    C.POINTER(P)]),
         "lr3d_tsdf_destroy": (i32, [P]),

@@ -53,28 +53,50 @@ def perturb(pose: Pose, w, tau) -> Pose:
     return Pose(E @ pose.R, E @ pose.t + np.asarray(tau, dtype=np.float64))
 
 
+def _pose_totals(stereo, solves, grads) -> dict:
+    total = {}
+    for (ref, o), g in zip(solves, grads):
+        srcs = list(o.sources)[:8]
+        for lay, (w, tau) in world_pose_gradients(stereo.views, ref, srcs, g[:len(srcs)]).items():
+            a, b = total.get(lay, (np.zeros(3), np.zeros(3)))
+            total[lay] = (a + w, b + tau)
+    return total
+
+
 def refine(stereo, solves, steps: int = 50, step_depth: float = 2e-4, step_normal: float = 2e-3,
            step_rot: float = 2e-4, step_trans: float = 2e-4, fixed=(0,), log=None):
     """`steps` joint gradient steps on the planes of every (ref, options) and on the poses of every
-    view not in `fixed` (gauge).  Pose steps are normalised per view: each moves by at most
-    step_rot radians and step_trans metres along its negative gradient.  Returns the loss history
-    (sum over views, before each step)."""
+    view not in `fixed` (gauge).
+
+    Each step: one plane step on the GPU (kept per pixel where it lowers the cost), then one pose
+    step, normalised per view (each view turns by step_rot radians and moves by step_trans metres
+    along its negative gradient).  The pose step is kept if the total loss after it is not higher
+    (the step then grows by 1.25), else undone and halved.  The loss is therefore non-increasing.
+    Returns the loss history: the loss before the first step, then after each step."""
     history = []
+    rot, trans = step_rot, step_trans
     for it in range(steps):
-        grads, losses = stereo.refine_step(solves, step_depth, step_normal)
-        history.append(float(losses.sum()))
-        total = {}
-        for (ref, o), g in zip(solves, grads):
-            srcs = list(o.sources)[:8]
-            for lay, (w, tau) in world_pose_gradients(stereo.views, ref, srcs, g[:len(srcs)]).items():
-                a, b = total.get(lay, (np.zeros(3), np.zeros(3)))
-                total[lay] = (a + w, b + tau)
+        grads, before, after = stereo.refine_step(solves, step_depth, step_normal)
+        if not history:
+            history.append(float(before.sum()))
+        loss = float(after.sum())
+        total = _pose_totals(stereo, solves, grads)
+        saved = {lay: stereo.views[lay].pose for lay in total if lay not in fixed}
         for lay, (w, tau) in total.items():
             if lay in fixed:
                 continue
-            dw = -step_rot * w / (np.linalg.norm(w) + 1e-30)
-            dt = -step_trans * tau / (np.linalg.norm(tau) + 1e-30)
+            dw = -rot * w / (np.linalg.norm(w) + 1e-30)
+            dt = -trans * tau / (np.linalg.norm(tau) + 1e-30)
             stereo.set_pose(lay, perturb(stereo.views[lay].pose, dw, dt))
+        _, trial, _ = stereo.refine_step(solves, 0.0, 0.0)
+        if float(trial.sum()) <= loss:
+            loss = float(trial.sum())
+            rot, trans = rot * 1.25, trans * 1.25
+        else:
+            for lay, pose in saved.items():
+                stereo.set_pose(lay, pose)
+            rot, trans = rot * 0.5, trans * 0.5
+        history.append(loss)
         if log:
-            log(it, history[-1])
+            log(it, loss)
     return history

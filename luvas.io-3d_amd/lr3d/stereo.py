@@ -177,15 +177,17 @@ class HipDenseStereo:
                 "loss": float(loss[0])}
 
     def refine_step(self, solves, step_depth: float, step_normal: float):
-        """One gradient step on the planes of every (ref, options); returns the relative-pose
-        gradients (n, n_src, 12) and the losses before the step (n,)."""
+        """One gradient step on the planes of every (ref, options), kept per pixel where it lowers
+        the cost; returns the relative-pose gradients (n, 8, 12) at the planes before the step and
+        the losses before and after it (n,) each.  Steps of 0 only evaluate."""
         descs = [self._desc(r, o)[0] for r, o in solves]
         arr = (N.SolveDesc * len(descs))(*descs)
         gp = np.zeros((len(descs), N.MAX_SOURCES, 12), dtype=np.float64)
-        loss = np.zeros(len(descs), dtype=np.float64)
+        before = np.zeros(len(descs), dtype=np.float64)
+        after = np.zeros(len(descs), dtype=np.float64)
         N.check(self._lib.lr3d_ncc_refine_step(self.handle, arr, len(descs), float(step_depth),
-                                               float(step_normal), N.ptr(gp), N.ptr(loss)))
-        return gp, loss
+                                               float(step_normal), N.ptr(gp), N.ptr(before), N.ptr(after)))
+        return gp, before, after
 
     def set_coarse(self, coarse: Optional["HipDenseStereo"]) -> None:
         N.check(self._lib.lr3d_stereo_set_coarse(self.handle, coarse.handle if coarse else None))

@@ -2,10 +2,19 @@
 FP64 CPU reference gradient (oracle/adjoint_oracle.c, itself pinned by central differences in
 tests/test_oracle_adjoint.py) and against FP64 central differences of 16 world-pose parameters.
 
-Bars: per pixel, |g_gpu - g_fp64| <= 1e-3 * max(1, |g_fp64|) for dL/dd and dL/dn on >= 99 % of the
-1024 seeded pixels whose FP32 and FP64 costs agree (the top-k choice is the same); the summed
-relative-pose gradient within 1e-3 relative; world-pose gradients within 2e-2 of FP64 central
-differences (the summed loss crosses texel edges inside any step, see test_oracle_adjoint.py)."""
+Compared pixels are the well-conditioned ones among 1024 seeded pixels: FP32 and FP64 costs
+agree within 1e-4 (in low-texture windows the forward's FP32 variance cancels, and FP32 can see a
+cost where FP64 sees a degenerate window, or the reverse) and no tap of a selected view lies
+within 1e-4 px of a texel edge (FP32 and FP64 tap positions can fall in different bilinear cells
+there, where the gradient jumps).
+
+The backward passes run in FP64 from the FP32 planes, so the bars are tight: every output finite
+over the whole view; per compared pixel, |g_gpu - g_fp64| <= 1e-4 * max(1, |g_fp64|) for dL/dd and
+dL/dn on >= 99 % of them and <= 1e-3 on all; the summed relative-pose gradient within 1e-4
+relative; world-pose gradients within 2e-2 of FP64 central differences, relative to the norm of
+the view's gradient (the summed loss crosses texel edges inside any step, see
+test_oracle_adjoint.py)."""
+import os
 import numpy as np
 import pytest
 
@@ -40,66 +49,87 @@ def seeded_pixels(cost, n=1024, seed=12):
     return np.sort(rng.choice(ok, size=min(n, ok.size), replace=False))
 
 
-def test_pixel_gradients_match_fp64_reference(solved):
+@pytest.fixture(scope="module")
+def reference(solved):
+    """FP64 cost and gradient of the seeded pixels; the well-conditioned ones."""
     w, st, o, hyp, cost = solved
+    imgs = O.Images(w.images)
+    p = O.params64(w.views, 0, w.sources[0], *w.ranges[0], topk=w.topk)
+    res = {}
+    for i in seeded_pixels(cost):
+        y, x = divmod(int(i), 320)
+        h = hyp[i].astype(np.float64)
+        res[int(i)] = O.cost_grad64(p, imgs, 0, x, y, h[:3], h[3])
+    clean = [i for i, r in res.items() if abs(r.cost - cost[i]) <= 1e-4 and r.min_kink >= 1e-4]
+    return imgs, res, np.array(clean)
+
+
+def finite_or_report(out, tag):
+    bad = ~np.isfinite(out["grad_depth"]) | ~np.isfinite(out["grad_normal"]).all(axis=1)
+    if bad.any() or not np.isfinite(out["grad_pose"]).all():
+        os.makedirs("gpurun_out", exist_ok=True)
+        np.savez(f"gpurun_out/adjoint_{tag}.npz", **{k: v for k, v in out.items() if k != "loss"})
+        px = [divmod(int(i), 320)[::-1] for i in np.flatnonzero(bad)[:8]]
+        say(f"{tag}: {int(bad.sum())} pixels with non-finite gradients, first (x, y) {px}; "
+            f"pose finite {bool(np.isfinite(out['grad_pose']).all())}")
+    return not bad.any() and np.isfinite(out["grad_pose"]).all()
+
+
+def test_pixel_gradients_match_fp64_reference(solved, reference):
+    w, st, o, hyp, cost = solved
+    imgs, res, clean = reference
     say("case: NCC adjoint per pixel, view 0 of sphere 8x320x240, 1024 seeded pixels")
     out = st.ncc_backward(0, o)
     assert np.array_equal(out["cost"].view(np.uint32), cost.view(np.uint32))  # forward is exact
-    imgs = O.Images(w.images)
-    p = O.params64(w.views, 0, w.sources[0], *w.ranges[0], topk=w.topk)
-    idx = seeded_pixels(cost)
-    good = total = 0
-    for i in idx:
-        y, x = divmod(int(i), 320)
-        h = hyp[i].astype(np.float64)
-        r = O.cost_grad64(p, imgs, 0, x, y, h[:3], h[3])
-        if abs(r.cost - cost[i]) > 1e-4:
-            continue  # FP32 and FP64 chose different views
-        total += 1
-        ok = abs(out["grad_depth"][i] - r.d_depth) <= 1e-3 * max(1.0, abs(r.d_depth))
-        for k in range(3):
-            ok &= abs(out["grad_normal"][i, k] - r.d_normal[k]) <= 1e-3 * max(1.0, abs(r.d_normal[k]))
-        good += ok
-    say(f"adjoint: {good}/{total} pixels within 1e-3 of the FP64 reference")
-    assert total >= 900 and good >= 0.99 * total
+    assert finite_or_report(out, "unweighted")
+    good = 0
+    worst = (0.0, None)
+    for i in clean:
+        r = res[int(i)]
+        ref = np.array([r.d_depth, *r.d_normal])
+        got = np.array([out["grad_depth"][i], *out["grad_normal"][i]], dtype=np.float64)
+        e = np.max(np.abs(got - ref) / np.maximum(1.0, np.abs(ref)))
+        good += e <= 1e-4
+        worst = max(worst, (e, (int(i) % 320, int(i) // 320)), key=lambda a: a[0])
+    say(f"adjoint: {good}/{len(clean)} compared pixels within 1e-4 of the FP64 reference (of {len(res)}), "
+        f"worst {worst[0]:.2e} at {worst[1]}")
+    assert len(clean) >= 512 and good >= 0.99 * len(clean) and worst[0] <= 1e-3
 
 
-def test_pose_gradient_matches_fp64_reference(solved):
+def test_pose_gradient_matches_fp64_reference(solved, reference):
     w, st, o, hyp, cost = solved
-    say("case: NCC adjoint, relative-pose gradient summed over 1024 pixels")
-    idx = seeded_pixels(cost)
+    imgs, res, clean = reference
+    say(f"case: NCC adjoint, relative-pose gradient summed over {len(clean)} compared pixels")
     weights = np.zeros(320 * 240, np.float32)
-    weights[idx] = 1.0
+    weights[clean] = 1.0
     out = st.ncc_backward(0, o, weights)
-    imgs = O.Images(w.images)
-    p = O.params64(w.views, 0, w.sources[0], *w.ranges[0], topk=w.topk)
+    assert finite_or_report(out, "weighted")
     ref = np.zeros((len(w.sources[0]), 12))
     loss = 0.0
-    for i in idx:
-        y, x = divmod(int(i), 320)
-        h = hyp[i].astype(np.float64)
-        r = O.cost_grad64(p, imgs, 0, x, y, h[:3], h[3])
+    for i in clean:
+        r = res[int(i)]
         loss += r.cost
         for s in range(len(w.sources[0])):
             ref[s, :9] += list(r.d_R[s])
             ref[s, 9:] += list(r.d_t[s])
     err = np.abs(out["grad_pose"] - ref).max() / np.abs(ref).max()
     say(f"pose gradient: max relative error {err:.2e}, loss gpu {out['loss']:.6f} fp64 {loss:.6f}")
-    assert err <= 1e-3
+    assert err <= 1e-4
     assert abs(out["loss"] - loss) <= 1e-4 * loss
 
 
-def test_world_pose_gradients_match_central_differences(solved):
+def test_world_pose_gradients_match_central_differences(solved, reference):
     from lr3d.refine import perturb, world_pose_gradients
     w, st, o, hyp, cost = solved
     say("case: 16 world-pose parameters, FP64 central differences of the summed cost")
-    idx = seeded_pixels(cost, n=256, seed=3)
+    imgs, res, clean = reference
+    idx = clean[::max(1, len(clean) // 256)][:256]
     weights = np.zeros(320 * 240, np.float32)
     weights[idx] = 1.0
     out = st.ncc_backward(0, o, weights)
+    assert finite_or_report(out, "fd")
     srcs = w.sources[0]
     grads = world_pose_gradients(w.views, 0, srcs, out["grad_pose"])
-    imgs = O.Images(w.images)
     planes = [(divmod(int(i), 320), hyp[i].astype(np.float64)) for i in idx]
 
     def loss(views):
@@ -120,7 +150,8 @@ def test_world_pose_gradients_match_central_differences(solved):
             vals.append(loss(views))
         fd = (vals[0] - vals[1]) / (2 * h)
         g = grads[layer][0][k] if k < 3 else grads[layer][1][k - 3]
-        errs.append(abs(fd - g) / max(1.0, abs(g)))
+        norm = np.linalg.norm(np.concatenate(grads[layer]))
+        errs.append(abs(fd - g) / max(1.0, norm))
     say(f"world-pose gradients vs FD: median relative error {np.median(errs):.2e}, max {max(errs):.2e}")
     assert len(params) == 16
     assert np.median(errs) <= 5e-3 and max(errs) <= 2e-2
@@ -129,7 +160,7 @@ def test_world_pose_gradients_match_central_differences(solved):
 def test_refinement_lowers_the_cost_with_noisy_poses(ctx):
     import lr3d
     from lr3d.refine import perturb, refine
-    say("case: joint plane + pose refinement, 8 views, poses noised (0.5 deg, 5 mm), 20 steps")
+    say("case: joint plane + pose refinement, sphere 8x320x240, poses noised (0.5 deg, 5 mm), 50 steps")
     w = scenes.sphere_small().render()
     rng = np.random.default_rng(4)
     noisy = []
@@ -145,11 +176,17 @@ def test_refinement_lowers_the_cost_with_noisy_poses(ctx):
     st.solve_batched(solves)
 
     def pose_err():
-        return np.mean([np.linalg.norm(st.views[i].pose.center - w.views[i].pose.center) for i in range(1, 8)])
+        c = np.mean([np.linalg.norm(st.views[i].pose.center - w.views[i].pose.center) for i in range(1, 8)])
+        r = np.mean([np.degrees(np.arccos(np.clip((np.trace(st.views[i].pose.R @ w.views[i].pose.R.T) - 1) / 2, -1, 1)))
+                     for i in range(1, 8)])
+        return c, r
 
-    e0 = pose_err()
-    hist = refine(st, solves, steps=20)
-    e1 = pose_err()
-    say(f"refinement: loss {hist[0]:.1f} -> {hist[-1]:.1f}, mean centre error {e0 * 1000:.2f} -> {e1 * 1000:.2f} mm")
+    (c0, r0) = pose_err()
+    hist = refine(st, solves, steps=50)
+    (c1, r1) = pose_err()
+    say(f"refinement: loss {hist[0]:.1f} -> {hist[-1]:.1f}; mean centre error {c0 * 1000:.2f} -> {c1 * 1000:.2f} mm, "
+        f"rotation error {r0:.3f} -> {r1:.3f} deg")
     st.destroy()
-    assert hist[-1] < hist[0]
+    assert np.isfinite(hist).all(), hist
+    assert all(b <= a for a, b in zip(hist, hist[1:]))
+    assert hist[-1] < 0.99 * hist[0]

@@ -173,6 +173,21 @@ int32_t lr3d_tsdf_integrate_host(lr3d_tsdf *ts, const uint32_t *packed,
  * equals n calls of lr3d_tsdf_integrate in that order. */
 int32_t lr3d_tsdf_integrate_batched(lr3d_tsdf *ts, lr3d_stereo *st, const int32_t *layers,
                                     int32_t map, const lr3d_integration_view *views, int32_t n);
+/* Adjoint of TSDF integration (no counterpart in the reference).  tsdf_q is the value voxel q
+ * (pool index brick*512 + x + 8y + 64z) holds after integrating views[0..n) of the given layers'
+ * maps in order into a fresh volume (tsdf 1, weight 0), with the inputs, culling and arithmetic of
+ * lr3d_tsdf_integrate_batched (the volume itself is neither read nor modified);
+ * L = sum_q (tsdf_q - targets[q])^2.  Outputs (each may be NULL): tsdf_out (n_voxels), grad_depth
+ * = dL/d(depth used at each pixel: the map's depth where the map has one, else the prior), per
+ * view in list order (sum of width*height floats), grad_pose = dL/dR (9, row-major) and dL/dt (3)
+ * of each view's world -> camera pose (n*12 doubles), loss.  The nearest-pixel choice, the skips
+ * and the clamps carry no gradient.  Gradients are accumulated with FP64 atomics: runs agree to
+ * FP64 rounding, not bit for bit. */
+int32_t lr3d_tsdf_backward(lr3d_tsdf *ts, lr3d_stereo *st, const int32_t *layers, int32_t map,
+                           const lr3d_integration_view *views, int32_t n, const uint32_t *voxels,
+                           const float *targets, int64_t n_voxels, float *tsdf_out, float *grad_depth,
+                           double *grad_pose, double *loss);
+
 /* TsdfVolume.read (TV:270-277): n_bricks*512 floats each (either may be NULL). */
 int32_t lr3d_tsdf_read(lr3d_tsdf *ts, float *tsdf, float *weight);
 /* Voxel updates ((voxel, view) pairs written) of the integrate calls since the last reset. */

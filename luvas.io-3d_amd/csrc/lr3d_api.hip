@@ -908,6 +908,11 @@ struct lr3d_tsdf {
     DevBuf<uint32_t> host_map;
     DevBuf<unsigned long long> counter;
     bool counting = false;
+    // adjoint scratch
+    DevBuf<uint32_t> adj_voxels, adj_pixel_off;
+    DevBuf<float> adj_targets, adj_tsdf, adj_gdf;
+    DevBuf<float2> adj_ckpt;
+    DevBuf<double> adj_gd, adj_gp, adj_loss;
 };
 
 __global__ void fill_kernel(float *p, size_t n, float v) {
@@ -1094,6 +1099,89 @@ extern "C" int32_t lr3d_tsdf_integrate_host(lr3d_tsdf *ts, const uint32_t *packe
     HIP_TRY(ts->host_map.reserve(px));
     HIP_TRY(hipMemcpyAsync(ts->host_map.ptr, packed, px * 4, hipMemcpyHostToDevice, ctx->stream));
     return integrate_impl(ts, ts->host_map.ptr, tv, view);
+}
+
+extern "C" int32_t lr3d_tsdf_backward(lr3d_tsdf *ts, lr3d_stereo *st, const int32_t *layers, int32_t map,
+                                      const lr3d_integration_view *views, int32_t n, const uint32_t *voxels,
+                                      const float *targets, int64_t n_voxels, float *tsdf_out, float *grad_depth,
+                                      double *grad_pose, double *loss) {
+    if (!ts || !layers || !views || (n_voxels > 0 && (!voxels || !targets)))
+        return fail(LR3D_ERR_INVALID, "lr3d_tsdf_backward: NULL argument");
+    if (n <= 0) return fail(LR3D_ERR_INVALID, "lr3d_tsdf_backward: n must be > 0");
+    if (n_voxels < 0 || n_voxels > (int64_t)INT32_MAX)
+        return fail(LR3D_ERR_INVALID, "lr3d_tsdf_backward: n_voxels must be in [0, 2^31)");
+    const uint64_t pool = (uint64_t)ts->grid.n_bricks * 512u;
+    for (int64_t i = 0; i < n_voxels; i++)
+        if (voxels[i] >= pool) return fail(LR3D_ERR_INVALID, "lr3d_tsdf_backward: voxel %lld is outside the pool", (long long)i);
+    lr3d_ctx *ctx = ts->ctx;
+    std::lock_guard<std::mutex> lock(ctx->mu);
+    HIP_TRY(hipSetDevice(ctx->device));
+    std::vector<TsdfView> tv;
+    const uint32_t *maps = nullptr;
+    int32_t rc = stereo_views(ts, st, layers, map, views, n, tv, &maps);
+    if (rc) return rc;
+    const int32_t words = (n + 31) / 32;
+    std::vector<uint32_t> pixel_off(n + 1, 0);
+    for (int32_t i = 0; i < n; i++) pixel_off[i + 1] = pixel_off[i] + (uint32_t)(views[i].width * views[i].height);
+    const size_t total_px = pixel_off[n];
+    size_t prior_total = 0;
+    for (int32_t i = 0; i < n; i++)
+        if (views[i].prior) prior_total += (size_t)views[i].width * views[i].height;
+    if (prior_total) {
+        HIP_TRY(ts->priors.reserve(prior_total));
+        for (int32_t i = 0; i < n; i++)
+            if (views[i].prior)
+                HIP_TRY(hipMemcpyAsync(ts->priors.ptr + tv[i].prior_off, views[i].prior,
+                                       (size_t)views[i].width * views[i].height * 4, hipMemcpyHostToDevice, ctx->stream));
+    }
+    HIP_TRY(ts->views.reserve(n));
+    HIP_TRY(hipMemcpyAsync(ts->views.ptr, tv.data(), sizeof(TsdfView) * n, hipMemcpyHostToDevice, ctx->stream));
+    HIP_TRY(ts->vis.reserve((size_t)ts->grid.n_bricks * words));
+    HIP_TRY(timed(ctx, "tsdf_visibility", [&] {
+        return tsdf_launch_visibility(ctx->stream, ts->grid, ts->coords.ptr, ts->views.ptr, n, ts->vis.ptr, words);
+    }));
+    HIP_TRY(ts->adj_pixel_off.reserve(n + 1));
+    HIP_TRY(hipMemcpyAsync(ts->adj_pixel_off.ptr, pixel_off.data(), 4 * (n + 1), hipMemcpyHostToDevice, ctx->stream));
+    HIP_TRY(ts->adj_gd.reserve(std::max<size_t>(total_px, 1)));
+    HIP_TRY(ts->adj_gp.reserve((size_t)n * 12));
+    HIP_TRY(hipMemsetAsync(ts->adj_gd.ptr, 0, total_px * 8, ctx->stream));
+    HIP_TRY(hipMemsetAsync(ts->adj_gp.ptr, 0, (size_t)n * 12 * 8, ctx->stream));
+    const uint32_t nv = (uint32_t)n_voxels;
+    const uint32_t waves = (nv + 63u) / 64u;
+    std::vector<double> partials(waves);
+    if (nv > 0) {
+        const int32_t n_seg = (n + 15) / 16;
+        HIP_TRY(ts->adj_voxels.reserve(nv));
+        HIP_TRY(ts->adj_targets.reserve(nv));
+        HIP_TRY(ts->adj_tsdf.reserve(nv));
+        HIP_TRY(ts->adj_ckpt.reserve((size_t)n_seg * nv));
+        HIP_TRY(ts->adj_loss.reserve(((nv + 255u) / 256u) * 4u));
+        HIP_TRY(hipMemcpyAsync(ts->adj_voxels.ptr, voxels, (size_t)nv * 4, hipMemcpyHostToDevice, ctx->stream));
+        HIP_TRY(hipMemcpyAsync(ts->adj_targets.ptr, targets, (size_t)nv * 4, hipMemcpyHostToDevice, ctx->stream));
+        HIP_TRY(timed(ctx, "tsdf_backward", [&] {
+            return tsdf_launch_backward(ctx->stream, ts->grid, ts->views.ptr, n, ts->vis.ptr, words, ts->coords.ptr,
+                                        maps, ts->priors.ptr, ts->adj_voxels.ptr, ts->adj_targets.ptr, nv,
+                                        ts->adj_pixel_off.ptr, ts->adj_ckpt.ptr, ts->adj_tsdf.ptr, ts->adj_gd.ptr,
+                                        ts->adj_gp.ptr, ts->adj_loss.ptr);
+        }));
+        HIP_TRY(hipMemcpyAsync(partials.data(), ts->adj_loss.ptr, (size_t)waves * 8, hipMemcpyDeviceToHost, ctx->stream));
+        if (tsdf_out)
+            HIP_TRY(hipMemcpyAsync(tsdf_out, ts->adj_tsdf.ptr, (size_t)nv * 4, hipMemcpyDeviceToHost, ctx->stream));
+    }
+    if (grad_depth && total_px) {
+        HIP_TRY(ts->adj_gdf.reserve(total_px));
+        HIP_TRY(launch_to_float(ctx->stream, ts->adj_gd.ptr, ts->adj_gdf.ptr, total_px));
+        HIP_TRY(hipMemcpyAsync(grad_depth, ts->adj_gdf.ptr, total_px * 4, hipMemcpyDeviceToHost, ctx->stream));
+    }
+    if (grad_pose) HIP_TRY(hipMemcpyAsync(grad_pose, ts->adj_gp.ptr, (size_t)n * 12 * 8, hipMemcpyDeviceToHost, ctx->stream));
+    rc = finish(ctx);
+    if (rc) return rc;
+    if (loss) {
+        double l = 0.0;
+        for (uint32_t i = 0; i < waves; i++) l += partials[i];
+        *loss = l;
+    }
+    return LR3D_OK;
 }
 
 extern "C" int32_t lr3d_tsdf_read(lr3d_tsdf *ts, float *tsdf, float *weight) {

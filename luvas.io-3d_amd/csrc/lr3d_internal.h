@@ -137,6 +137,15 @@ hipError_t tsdf_launch_integrate(hipStream_t s, const TsdfGridDev &g, const Tsdf
                                  int32_t n_views, const uint32_t *vis, int32_t words,
                                  const int4 *coords, const uint32_t *maps, const float *priors,
                                  float *tsdf, float *weight, unsigned long long *counter);
+// TSDF adjoint over listed voxels (pool indices); n_voxels > 0.  ckpt holds
+// ceil(n_views / 16) * n_voxels float2; grad_depth (per pixel of each view, at pixel_off[v]) and
+// grad_pose (n_views * 12) are accumulated (zero them first); loss_partials: one per wavefront.
+hipError_t tsdf_launch_backward(hipStream_t s, const TsdfGridDev &g, const TsdfView *views, int32_t n_views,
+                                const uint32_t *vis, int32_t words, const int4 *coords, const uint32_t *maps,
+                                const float *priors, const uint32_t *voxels, const float *targets,
+                                uint32_t n_voxels, const uint32_t *pixel_off, float2 *ckpt, float *tsdf_out,
+                                double *grad_depth, double *grad_pose, double *loss_partials);
+hipError_t launch_to_float(hipStream_t s, const double *in, float *out, size_t n);
 
 // Launch shapes (x threads, y threads) picked per (kernel, image size).
 struct TileShape {

@@ -181,6 +181,209 @@ __global__ void __launch_bounds__(256) integrate_kernel(TsdfGridDev g, const Tsd
     }
 }
 
+// ---- adjoint (no counterpart in the reference) ------------------------------------------------
+//
+// L = sum_q (tsdf_q - target_q)^2 over listed voxels q, where tsdf_q is the value after applying
+// views 0..n-1 in order to a fresh voxel (tsdf 1, weight 0) exactly as integrate_kernel does
+// (same culling bits, same FP32 arithmetic).  One thread per listed voxel:
+//   forward   replays every view, storing (tsdf, weight) before every 16th view (a checkpoint) in
+//             `ckpt` ([segment][voxel]);
+//   reverse   per segment, last first: replays the segment from its checkpoint with the state
+//             before each view in LDS, then walks it backwards with a = dL/dtsdf, b = dL/dweight
+//             (FP64):  D = W + w, u = [D < 96],
+//               dL/do = a w / D,  dL/dw = a (o - tsdf) / D + u b,
+//               a <- a W / D,     b <- a (tsdf_prev - tsdf) / D + u b.
+// dL/do and dL/dw are chained to the depth used at the pixel (map depth, or the prior where the
+// map has none) and to cam = R world + t.  Depth gradients are added per pixel and pose gradients
+// per view (wavefront sums first) with FP64 atomics: the result does not depend on the order
+// beyond FP64 rounding.  The nearest-pixel choice, the skips and the clamps carry no gradient.
+
+constexpr int kSegment = 16;
+
+struct StepOut {
+    float o, w;
+    uint32_t pixel;      // index into the view's map
+    double do_dd, dw_dd; // partial derivatives of (o, w) with respect to the depth used
+    double do_dz;        // d o / d cam.z
+    double dw_dc[3];     // d w / d cam
+    float cam[3];
+};
+
+// One view applied to one voxel (TV:110-153 in integrate_kernel's arithmetic); false if skipped.
+template <bool DERIV>
+__device__ __forceinline__ bool tsdf_step(const TsdfView &V, const uint32_t *__restrict__ maps,
+                                          const float *__restrict__ priors, float mu, float wx, float wy,
+                                          float wz, StepOut &out) {
+    const float c0 = fmaf(V.R[1], wy, fmaf(V.R[0], wx, V.t[0]));
+    const float c1 = fmaf(V.R[4], wy, fmaf(V.R[3], wx, V.t[1]));
+    const float c2 = fmaf(V.R[7], wy, fmaf(V.R[6], wx, V.t[2]));
+    const float cam0 = fmaf(V.R[2], wz, c0);
+    const float cam1 = fmaf(V.R[5], wz, c1);
+    const float cam2 = fmaf(V.R[8], wz, c2);
+    if (!(cam2 > 0.01f)) return false;
+    const int32_t px = (int32_t)rintf((V.k[0] * cam0) / cam2 + V.k[2]);
+    const int32_t py = (int32_t)rintf((V.k[1] * cam1) / cam2 + V.k[3]);
+    if (px < 0 || py < 0 || px >= V.W || py >= V.H) return false;
+    const uint32_t i = (uint32_t)py * (uint32_t)V.W + (uint32_t)px;
+    const uint32_t packed = maps[V.map_off + i];
+    float d = f16_bits_to_f32(packed);
+    const bool from_map = d > 0.0f;
+    if (!from_map) d = V.has_prior ? priors[V.prior_off + i] : 0.0f;
+    if (!(d > 0.0f)) return false;
+    const float sdf = d - cam2;
+    if (sdf < -mu) return false;
+    float w;
+    double dbase[3] = {0.0, 0.0, 0.0}, f = 1.0, df_dd = 0.0, base_d = 0.0;
+    if (from_map) {
+        float nx, ny, nz;
+        unpack_normal(packed, nx, ny, nz);
+        const float inv = 1.0f / sqrtf(cam0 * cam0 + cam1 * cam1 + cam2 * cam2);
+        const float nc = nx * (cam0 * inv) + ny * (cam1 * inv) + nz * (cam2 * inv);
+        w = fmaxf(0.2f, fabsf(nc));
+        if (DERIV) {
+            base_d = (double)w;
+            if (fabsf(nc) > 0.2f) {
+                const double len = sqrt((double)cam0 * cam0 + (double)cam1 * cam1 + (double)cam2 * cam2);
+                const double cx = cam0 / len, cy = cam1 / len, cz = cam2 / len;
+                const double ncd = nx * cx + ny * cy + nz * cz;
+                const double sg = ncd >= 0.0 ? 1.0 : -1.0;
+                dbase[0] = sg * (nx - ncd * cx) / len;
+                dbase[1] = sg * (ny - ncd * cy) / len;
+                dbase[2] = sg * (nz - ncd * cz) / len;
+            }
+        }
+        if (V.confidence > 0.0f) {
+            const float cc = (float)(packed >> 28u) / 8.0f;
+            const float ratio = V.confidence / d;
+            const float r = fminf(fmaxf(ratio, 0.5f), 2.0f);
+            const float q = fmaxf(0.15f, (1.0f - cc) * (1.0f - cc));
+            w = w * ((q * r) * r);
+            if (DERIV) {
+                f = (double)q * r * r;
+                if (ratio > 0.5f && ratio < 2.0f)
+                    df_dd = (double)q * 2.0 * r * (-(double)V.confidence / ((double)d * d));
+            }
+        }
+    } else {
+        w = kPriorWeight;
+    }
+    out.o = fminf(1.0f, sdf / mu);
+    out.w = w;
+    out.pixel = i;
+    if (DERIV) {
+        const bool open = sdf / mu < 1.0f;
+        out.do_dd = open ? 1.0 / (double)mu : 0.0;
+        out.do_dz = open ? -1.0 / (double)mu : 0.0;
+        out.dw_dd = base_d * df_dd;
+        for (int k = 0; k < 3; k++) out.dw_dc[k] = dbase[k] * f;
+        out.cam[0] = cam0;
+        out.cam[1] = cam1;
+        out.cam[2] = cam2;
+    }
+    return true;
+}
+
+__device__ __forceinline__ double wave_sum_d(double v) {
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
+    return v;
+}
+
+__global__ void __launch_bounds__(256) tsdf_backward_kernel(
+    TsdfGridDev g, const TsdfView *__restrict__ views, int32_t n_views, const uint32_t *__restrict__ vis,
+    int32_t words, const int4 *__restrict__ coords, const uint32_t *__restrict__ maps,
+    const float *__restrict__ priors, const uint32_t *__restrict__ voxels, const float *__restrict__ targets,
+    uint32_t n_voxels, const uint32_t *__restrict__ pixel_off, float2 *__restrict__ ckpt,
+    float *__restrict__ tsdf_out, double *__restrict__ grad_depth, double *__restrict__ grad_pose,
+    double *__restrict__ loss_partials) {
+    __shared__ float2 seg_state[kSegment][256];
+    const uint32_t q = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t lane = threadIdx.x & 63u;
+    const bool active = q < n_voxels;
+    const uint32_t index = active ? voxels[q] : 0u;
+    const uint32_t brick = index >> 9, local = index & 511u;
+    const int4 c = coords[brick];
+    const float wx = g.origin[0] + ((float)(c.x * 8 + (int32_t)(local & 7u)) + 0.5f) * g.voxel;
+    const float wy = g.origin[1] + ((float)(c.y * 8 + (int32_t)((local >> 3) & 7u)) + 0.5f) * g.voxel;
+    const float wz = g.origin[2] + ((float)(c.z * 8 + (int32_t)(local >> 6)) + 0.5f) * g.voxel;
+    const float mu = g.mu;
+    const int32_t n_seg = (n_views + kSegment - 1) / kSegment;
+    auto visible = [&](int32_t v) -> bool {
+        return active && ((vis[(size_t)brick * words + (v >> 5)] >> (v & 31)) & 1u);
+    };
+    // forward with checkpoints
+    float ts = 1.0f, ws = 0.0f;
+    for (int32_t v = 0; v < n_views; v++) {
+        if ((v % kSegment) == 0 && active) ckpt[(size_t)(v / kSegment) * n_voxels + q] = make_float2(ts, ws);
+        StepOut st;
+        if (visible(v) && tsdf_step<false>(views[v], maps, priors, mu, wx, wy, wz, st)) {
+            const float w0 = ws;
+            ts = (ts * w0 + st.o * st.w) / (w0 + st.w);
+            ws = fminf(w0 + st.w, kMaxWeight);
+        }
+    }
+    const double r = active ? (double)ts - (double)targets[q] : 0.0;
+    if (active && tsdf_out) tsdf_out[q] = ts;
+    const double l = wave_sum_d(r * r);
+    if (lane == 0) loss_partials[blockIdx.x * (blockDim.x / 64u) + (threadIdx.x >> 6)] = l;
+    // reverse
+    double a = 2.0 * r, b = 0.0;
+    for (int32_t sgm = n_seg - 1; sgm >= 0; sgm--) {
+        const int32_t v0 = sgm * kSegment, v1 = min(n_views, v0 + kSegment);
+        float2 state = active ? ckpt[(size_t)sgm * n_voxels + q] : make_float2(1.0f, 0.0f);
+        for (int32_t v = v0; v < v1; v++) {
+            seg_state[v - v0][threadIdx.x] = state;
+            StepOut st;
+            if (visible(v) && tsdf_step<false>(views[v], maps, priors, mu, wx, wy, wz, st)) {
+                const float w0 = state.y;
+                state.x = (state.x * w0 + st.o * st.w) / (w0 + st.w);
+                state.y = fminf(w0 + st.w, kMaxWeight);
+            }
+        }
+        for (int32_t v = v1 - 1; v >= v0; v--) {
+            StepOut st;
+            const bool applied = visible(v) && tsdf_step<true>(views[v], maps, priors, mu, wx, wy, wz, st);
+            double gc[3] = {0.0, 0.0, 0.0};
+            if (applied) {
+                const float2 prev = seg_state[v - v0][threadIdx.x];
+                const double W0 = prev.y, w = st.w, o = st.o, D = W0 + w;
+                const double tsk = ((double)prev.x * W0 + o * w) / D;
+                const double u = (prev.y + st.w) < kMaxWeight ? 1.0 : 0.0;
+                const double g_o = a * w / D;
+                const double g_w = a * (o - tsk) / D + u * b;
+                const double a_prev = a * W0 / D;
+                const double b_prev = a * ((double)prev.x - tsk) / D + u * b;
+                const double gdep = g_o * st.do_dd + g_w * st.dw_dd;
+                if (gdep != 0.0) atomicAdd(&grad_depth[pixel_off[v] + st.pixel], gdep);
+                gc[0] = g_w * st.dw_dc[0];
+                gc[1] = g_w * st.dw_dc[1];
+                gc[2] = g_o * st.do_dz + g_w * st.dw_dc[2];
+                a = a_prev;
+                b = b_prev;
+            }
+            if (__any(applied)) {
+                const double world[3] = {wx, wy, wz};
+                double part[12];
+#pragma unroll
+                for (int i = 0; i < 3; i++) {
+#pragma unroll
+                    for (int j = 0; j < 3; j++) part[i * 3 + j] = gc[i] * world[j];
+                    part[9 + i] = gc[i];
+                }
+#pragma unroll
+                for (int j = 0; j < 12; j++) {
+                    const double sum = wave_sum_d(part[j]);
+                    if (lane == 0 && sum != 0.0) atomicAdd(&grad_pose[(size_t)v * 12 + j], sum);
+                }
+            }
+        }
+    }
+}
+
+__global__ void to_float_kernel(const double *__restrict__ in, float *__restrict__ out, size_t n) {
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) out[i] = (float)in[i];
+}
+
 }  // namespace
 
 hipError_t tsdf_launch_visibility(hipStream_t s, const TsdfGridDev &g, const int4 *coords,
@@ -203,6 +406,25 @@ hipError_t tsdf_launch_integrate(hipStream_t s, const TsdfGridDev &g, const Tsdf
     else
         hipLaunchKernelGGL((integrate_kernel<false>), grid, block, 0, s, g, views, n_views, vis, words,
                            coords, maps, priors, tsdf, weight, counter);
+    return hipGetLastError();
+}
+
+hipError_t tsdf_launch_backward(hipStream_t s, const TsdfGridDev &g, const TsdfView *views, int32_t n_views,
+                                const uint32_t *vis, int32_t words, const int4 *coords, const uint32_t *maps,
+                                const float *priors, const uint32_t *voxels, const float *targets,
+                                uint32_t n_voxels, const uint32_t *pixel_off, float2 *ckpt, float *tsdf_out,
+                                double *grad_depth, double *grad_pose, double *loss_partials) {
+    dim3 block(256), grid((n_voxels + 255u) / 256u);
+    hipLaunchKernelGGL(tsdf_backward_kernel, grid, block, 0, s, g, views, n_views, vis, words, coords, maps,
+                       priors, voxels, targets, n_voxels, pixel_off, ckpt, tsdf_out, grad_depth, grad_pose,
+                       loss_partials);
+    return hipGetLastError();
+}
+
+hipError_t launch_to_float(hipStream_t s, const double *in, float *out, size_t n) {
+    if (n == 0) return hipSuccess;
+    dim3 block(256), grid((unsigned)((n + 255) / 256));
+    hipLaunchKernelGGL(to_float_kernel, grid, block, 0, s, in, out, n);
     return hipGetLastError();
 }
 

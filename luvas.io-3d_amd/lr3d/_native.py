@@ -32,7 +32,7 @@ EXPORTED = (
     "lr3d_ncc_backward", "lr3d_ncc_refine_step",
     "lr3d_tsdf_create", "lr3d_tsdf_destroy", "lr3d_tsdf_integrate", "lr3d_tsdf_integrate_host",
     "lr3d_tsdf_integrate_batched", "lr3d_tsdf_read", "lr3d_tsdf_set_counting",
-    "lr3d_tsdf_update_count",
+    "lr3d_tsdf_update_count", "lr3d_tsdf_backward",
 )
 
 
@@ -107,6 +107,8 @@ def load() -> C.CDLL:
         "lr3d_tsdf_create": (i32, [P, C.POINTER(dbl), dbl, C.POINTER(i32), C.c_void_p, i32, dbl,
                                    C.POINTER(P)]),
         "lr3d_tsdf_destroy": (i32, [P]),
+        "lr3d_tsdf_backward": (i32, [P, P, C.c_void_p, i32, C.c_void_p, i32, C.c_void_p, C.c_void_p, C.c_int64,
+                                     C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]),
         "lr3d_tsdf_integrate": (i32, [P, P, i32, i32, C.POINTER(IntegrationView)]),
         "lr3d_tsdf_integrate_host": (i32, [P, C.c_void_p, C.POINTER(IntegrationView)]),
         "lr3d_tsdf_integrate_batched": (i32, [P, P, C.POINTER(i32), i32,

@@ -141,6 +141,31 @@ class HipTsdfVolume:
         N.check(self._lib.lr3d_tsdf_integrate_batched(self.handle, stereo.handle, ls, map, arr,
                                                       len(structs)))
 
+    def backward(self, stereo, layers: Sequence[int], views: Sequence[IntegrationView], voxels, targets,
+                 map: int = N.MAP_RAW) -> dict:
+        """Replay of integrate_batched into a fresh volume at the listed voxels (pool indices) and the
+        adjoint of L = sum (tsdf - target)^2: dict(tsdf, grad_depth (list per view, H x W), grad_pose
+        (n, 12): dL/dR row-major and dL/dt per view, loss)."""
+        structs, keep = [], []
+        for v in views:
+            s, k = _iv_struct(v)
+            structs.append(s)
+            keep.append(k)
+        arr = (N.IntegrationView * len(structs))(*structs)
+        ls = (C.c_int32 * len(layers))(*layers)
+        vox = np.ascontiguousarray(voxels, dtype=np.uint32).reshape(-1)
+        tgt = np.ascontiguousarray(targets, dtype=np.float32).reshape(-1)
+        sizes = [v.width * v.height for v in views]
+        tsdf = np.empty(vox.size, dtype=np.float32)
+        gd = np.empty(sum(sizes), dtype=np.float32)
+        gp = np.empty((len(views), 12), dtype=np.float64)
+        loss = np.zeros(1, dtype=np.float64)
+        N.check(self._lib.lr3d_tsdf_backward(self.handle, stereo.handle, ls, map, arr, len(structs), N.ptr(vox),
+                                             N.ptr(tgt), vox.size, N.ptr(tsdf), N.ptr(gd), N.ptr(gp), N.ptr(loss)))
+        parts = np.split(gd, np.cumsum(sizes)[:-1])
+        return {"tsdf": tsdf, "grad_depth": [p.reshape(v.height, v.width) for p, v in zip(parts, views)],
+                "grad_pose": gp, "loss": float(loss[0])}
+
     def read(self) -> tuple[np.ndarray, np.ndarray]:
         """TV:270-277"""
         n = self.count * BRICK_VOXELS

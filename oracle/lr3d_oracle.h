@@ -166,6 +166,23 @@ int32_t orc_visible_bricks(const orc_tsdf_grid *g, const orc_integration_view *v
 int64_t orc_tsdf_integrate(const orc_tsdf_grid *g, const orc_integration_view *v,
                            const uint32_t *depth, const float *prior, float *tsdf, float *weight);
 
+/* ---- FP64 TSDF integration adjoint (tsdf_adjoint_oracle.c) ---- */
+typedef struct {
+    const double *depth;   /* W*H decoded map depth, 0 where the map has none */
+    const double *normal;  /* W*H*3 decoded normals */
+    const uint8_t *cost;   /* W*H packed cost codes (0..15) */
+    const double *prior;   /* W*H or NULL */
+} orc64_depth_input;
+
+/* Integrate views[0..n) in order into a fresh volume (tsdf 1, weight 0) at the listed voxels
+ * (pool indices) in FP64; L = sum (tsdf - target)^2.  Outputs (any may be NULL): tsdf_out and
+ * kink_out (distance of the nearest projection to a rounding boundary, px) per voxel; grad_depth
+ * per pixel of every view, view-major (dL/d depth used); grad_pose n*12 (dL/dR row-major, dL/dt). */
+int32_t orc64_tsdf_backward(const orc_tsdf_grid *g, const orc_integration_view *views,
+                            const orc64_depth_input *inputs, int32_t n_views, const uint32_t *voxels,
+                            const double *targets, int64_t n_voxels, double *tsdf_out,
+                            double *grad_depth, double *grad_pose, double *kink_out, double *loss);
+
 /* ---- FP64 cost and its gradient (adjoint_oracle.c) ---- */
 typedef struct {
     double k[4];

@@ -387,3 +387,63 @@ def cost_grad64(p: Orc64Params, images: Images, ref: int, x: int, y: int, n, d: 
     nn = (C.c_double * 3)(*[float(v) for v in n])
     lib.orc64_cost_grad(C.byref(p), C.byref(images.struct), ref, x, y, nn, float(d), C.byref(r))
     return r
+
+
+# ---- FP64 TSDF integration adjoint (tsdf_adjoint_oracle.c) -------------------------------------
+
+class Orc64DepthInput(C.Structure):
+    _fields_ = [("depth", C.c_void_p), ("normal", C.c_void_p), ("cost", C.c_void_p), ("prior", C.c_void_p)]
+
+
+def decode_maps(packed_maps):
+    """Packed u32 maps -> (depth f64, normal f64 (W*H, 3), cost code u8) with the decoders of PK."""
+    out = []
+    lib = load()
+    for m in packed_maps:
+        m = np.ascontiguousarray(m, dtype=np.uint32).reshape(-1)
+        depth = np.zeros(m.size, dtype=np.float64)
+        normal = np.zeros((m.size, 3), dtype=np.float64)
+        nz = np.flatnonzero(m)
+        f16 = (m[nz] & 0xFFFF).astype(np.uint16).view(np.float16).astype(np.float64)
+        depth[nz] = np.where(f16 > 0, f16, 0.0)
+        nn = (C.c_float * 3)()
+        for i in nz:
+            lib.orc_unpack_normal(C.c_uint32(int(m[i])), nn)
+            normal[i] = list(nn)
+        out.append((depth, normal, (m >> 28).astype(np.uint8)))
+    return out
+
+
+def tsdf_backward64(grid: Grid, views, decoded, priors, voxels, targets):
+    """FP64 replay + adjoint; returns dict(tsdf, kink, grad_depth (list per view), grad_pose (n, 12), loss)."""
+    lib = load()
+    if not hasattr(lib, "_tb64"):
+        lib.orc64_tsdf_backward.restype = C.c_int32
+        lib.orc64_tsdf_backward.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int32, C.c_void_p, C.c_void_p,
+                                            C.c_int64, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]
+        lib._tb64 = True
+    n = len(views)
+    varr = (OrcIntegrationView * n)(*views)
+    keep = []
+    inputs = (Orc64DepthInput * n)()
+    for i, (dep, nor, cost) in enumerate(decoded):
+        pr = None if priors is None or priors[i] is None else np.ascontiguousarray(priors[i], dtype=np.float64).reshape(-1)
+        dep, nor, cost = (np.ascontiguousarray(dep, dtype=np.float64), np.ascontiguousarray(nor, dtype=np.float64),
+                          np.ascontiguousarray(cost, dtype=np.uint8))
+        keep += [dep, nor, cost, pr]
+        inputs[i] = Orc64DepthInput(dep.ctypes.data, nor.ctypes.data, cost.ctypes.data,
+                                    pr.ctypes.data if pr is not None else None)
+    vox = np.ascontiguousarray(voxels, dtype=np.uint32)
+    tgt = np.ascontiguousarray(targets, dtype=np.float64)
+    sizes = [v.width * v.height for v in views]
+    tsdf = np.zeros(vox.size)
+    kink = np.zeros(vox.size)
+    gd = np.zeros(sum(sizes))
+    gp = np.zeros((n, 12))
+    loss = np.zeros(1)
+    rc = lib.orc64_tsdf_backward(C.addressof(grid.struct), C.addressof(varr), C.addressof(inputs), n,
+                                 vox.ctypes.data, tgt.ctypes.data, vox.size, tsdf.ctypes.data, gd.ctypes.data,
+                                 gp.ctypes.data, kink.ctypes.data, loss.ctypes.data)
+    assert rc == 0
+    splits = np.split(gd, np.cumsum(sizes)[:-1])
+    return {"tsdf": tsdf, "kink": kink, "grad_depth": splits, "grad_pose": gp, "loss": float(loss[0])}

@@ -41,7 +41,22 @@ export declare class HipDenseStereo {
     filteredMap(ref: number, sources: number[], needed: number): Promise<Uint32Array>;
     depthMap(ref: number): Promise<Uint32Array>;
     planes(ref: number): Promise<{ hyp: Float32Array; cost: Float32Array }>;
+    nccBackward(ref: number, options: SolveOptions, weights?: ArrayLike<number> | null): Promise<NccGradients>;
+    refineStep(solves: Array<[number, SolveOptions]>, stepDepth: number, stepNormal: number):
+        Promise<{ gradPose: Float64Array; lossBefore: Float64Array; lossAfter: Float64Array }>;
+    setPose(layer: number, pose: Pose): void;
     destroy(): void;
+}
+
+/** Adjoint of the aggregated NCC cost of one reference view's planes. */
+export interface NccGradients {
+    cost: Float32Array;
+    gradDepth: Float32Array;
+    /** 3 per pixel */
+    gradNormal: Float32Array;
+    /** 8 source slots x (dL/dR row-major 9, dL/dt 3) of the relative pose ref -> source */
+    gradPose: Float64Array;
+    loss: number;
 }
 
 export declare class BrickGrid {
@@ -71,7 +86,31 @@ export declare class HipTsdfVolume {
     integrate(view: IntegrationView): Promise<void>;
     integrateLayers(layers: number[], views: IntegrationView[], map?: number): Promise<void>;
     read(): Promise<{ tsdf: Float32Array; weight: Float32Array }>;
+    backward(layers: number[], views: IntegrationView[], voxels: ArrayLike<number>, targets: ArrayLike<number>,
+             map?: number): Promise<TsdfGradients>;
     destroy(): void;
+}
+
+/** Adjoint of integrating views into a fresh volume, at listed voxels, for sum (tsdf - target)^2. */
+export interface TsdfGradients {
+    tsdf: Float32Array;
+    /** per view in list order, width*height each, concatenated */
+    gradDepth: Float32Array;
+    /** per view: dL/dR (9, row-major), dL/dt (3) of the world -> camera pose */
+    gradPose: Float64Array;
+    loss: number;
+}
+
+export declare class NccCostFunction {
+    constructor(stereo: HipDenseStereo, ref: number, options: SolveOptions);
+    forward(weights?: ArrayLike<number> | null): Promise<{ loss: number; cost: Float32Array }>;
+    backward(): Promise<{ gradDepth: Float32Array; gradNormal: Float32Array; gradPose: Float64Array }>;
+}
+
+export declare class TsdfIntegrationFunction {
+    constructor(volume: HipTsdfVolume, layers: number[], views: IntegrationView[], map?: number);
+    forward(): Promise<void>;
+    backward(voxels: ArrayLike<number>, targets: ArrayLike<number>): Promise<TsdfGradients>;
 }
 
 export interface StereoInput {

@@ -121,6 +121,32 @@ class HipDenseStereo {
         return native.stereoReadHyp(this.handle, ref, c.width, c.height);
     }
 
+    /**
+     * Adjoint of L = sum_p weights[p] * C_p (the aggregated NCC cost of the current planes of `ref`):
+     * {cost, gradDepth, gradNormal (3 per pixel), gradPose (8 sources x (dL/dR 9, dL/dt 3)), loss}.
+     */
+    async nccBackward(ref, options, weights) {
+        const c = this.cameras[ref];
+        const w = weights === undefined || weights === null ? null : Float32Array.from(weights);
+        return native.nccBackward(this.handle, HipDenseStereo.descOf(ref, options), w, c.width * c.height);
+    }
+
+    /**
+     * One gradient step on the planes of every [ref, options] (kept per pixel where the cost drops):
+     * {gradPose (per solve, 8 x 12), lossBefore, lossAfter}.  Steps of 0 only evaluate.
+     */
+    async refineStep(solves, stepDepth, stepNormal) {
+        return native.nccRefineStep(this.handle, solves.map((s) => HipDenseStereo.descOf(s[0], s[1])),
+            stepDepth, stepNormal);
+    }
+
+    /** Replaces a layer's world -> camera pose {R (9, row-major), t (3)}. */
+    setPose(layer, pose) {
+        native.stereoSetPose(this.handle, layer, Array.from(pose.R), Array.from(pose.t));
+        this.views[layer] = Object.assign({}, this.views[layer], { pose });
+        this.cameras[layer] = cameraOf(this.views[layer], this.width, this.height);
+    }
+
     destroy() {
         if (this.handle) native.stereoDestroy(this.handle);
         this.handle = null;
@@ -215,9 +241,69 @@ class HipTsdfVolume {
         return native.tsdfRead(this.handle, this.layout.count);
     }
 
+    /**
+     * Adjoint of integrating `views` (of `layers`' maps, in order, into a fresh volume) at the listed
+     * voxels (pool indices) for L = sum (tsdf - target)^2: {tsdf, gradDepth (per view, concatenated),
+     * gradPose (per view: dL/dR 9, dL/dt 3), loss}.  The volume is not modified.
+     */
+    async backward(layers, views, voxels, targets, map) {
+        return native.tsdfBackward(this.handle, this.stereo.handle, layers, map === undefined ? MAP_RAW : map,
+            views.map((v) => HipTsdfVolume.viewOf(v, this.width, this.height)), Uint32Array.from(voxels),
+            Float32Array.from(targets));
+    }
+
     destroy() {
         if (this.handle) native.tsdfDestroy(this.handle);
         this.handle = null;
+    }
+}
+
+/**
+ * The aggregated NCC cost as a forward/backward pair over the planes and relative poses of one
+ * reference view.  forward() evaluates L = sum_p w_p C_p; backward() returns its gradients.  Both
+ * come from one GPU pass that recomputes the NCC windows (nothing is stored between the calls
+ * on the device; the pass's results are kept for backward()).
+ */
+class NccCostFunction {
+    constructor(stereo, ref, options) {
+        this.stereo = stereo;
+        this.ref = ref;
+        this.options = options;
+        this.saved = null;
+    }
+
+    async forward(weights) {
+        this.saved = await this.stereo.nccBackward(this.ref, this.options, weights);
+        return { loss: this.saved.loss, cost: this.saved.cost };
+    }
+
+    async backward() {
+        if (!this.saved) throw new Error('NccCostFunction.backward: call forward first');
+        const { gradDepth, gradNormal, gradPose } = this.saved;
+        return { gradDepth, gradNormal, gradPose };
+    }
+}
+
+/**
+ * TSDF integration as a forward/backward pair: forward() integrates the layers' maps into the
+ * volume (as integrateLayers); backward() replays that integration from a fresh volume at the
+ * listed voxels and returns the gradients of sum (tsdf - target)^2 with respect to the depths used
+ * and the view poses.
+ */
+class TsdfIntegrationFunction {
+    constructor(volume, layers, views, map) {
+        this.volume = volume;
+        this.layers = layers;
+        this.views = views;
+        this.map = map === undefined ? MAP_FILTERED : map;
+    }
+
+    async forward() {
+        await this.volume.integrateLayers(this.layers, this.views, this.map);
+    }
+
+    async backward(voxels, targets) {
+        return this.volume.backward(this.layers, this.views, voxels, targets, this.map);
     }
 }
 
@@ -298,5 +384,5 @@ const STEREO_METHODS = {
 
 module.exports = {
     Context, HipDenseStereo, HipTsdfVolume, BrickGrid, STEREO_METHODS, MAP_RAW, MAP_FILTERED,
-    MAX_SOURCES, BRICK, native,
+    MAX_SOURCES, BRICK, NccCostFunction, TsdfIntegrationFunction, native,
 };

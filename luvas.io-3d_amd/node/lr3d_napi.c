@@ -106,7 +106,7 @@ static int read_view(napi_env env, napi_value o, lr3d_view *v) {
 /* ---- async work -------------------------------------------------------------------------------- */
 
 enum { OP_SOLVE, OP_SOLVE_BATCHED, OP_FILTER, OP_INTEGRATE, OP_READ_DEPTH, OP_READ_FILTERED,
-       OP_READ_HYP, OP_READ_TSDF };
+       OP_READ_HYP, OP_READ_TSDF, OP_NCC_BACKWARD, OP_NCC_REFINE, OP_TSDF_BACKWARD };
 
 typedef struct {
     int op;
@@ -125,6 +125,14 @@ typedef struct {
     lr3d_integration_view *iviews;
     void *out_a, *out_b;
     size_t bytes_a, bytes_b;
+    /* adjoints */
+    float *weights, *targets;
+    uint32_t *voxels;
+    int64_t n_voxels;
+    float step_depth, step_normal;
+    void *out_c, *out_d, *out_e;
+    size_t bytes_c, bytes_d, bytes_e;
+    double loss;
 } job_t;
 
 static void job_free(job_t *j) {
@@ -137,6 +145,12 @@ static void job_free(job_t *j) {
     free(j->iviews);
     free(j->out_a);
     free(j->out_b);
+    free(j->out_c);
+    free(j->out_d);
+    free(j->out_e);
+    free(j->weights);
+    free(j->targets);
+    free(j->voxels);
     free(j);
 }
 
@@ -154,6 +168,18 @@ static void job_execute(napi_env env, void *data) {
     case OP_READ_FILTERED: j->status = lr3d_stereo_read_filtered(j->st, j->layer, (uint32_t *)j->out_a); break;
     case OP_READ_HYP: j->status = lr3d_stereo_read_hyp(j->st, j->layer, (float *)j->out_a, (float *)j->out_b); break;
     case OP_READ_TSDF: j->status = lr3d_tsdf_read(j->ts, (float *)j->out_a, (float *)j->out_b); break;
+    case OP_NCC_BACKWARD:
+        j->status = lr3d_ncc_backward(j->st, &j->descs[0], j->weights, (float *)j->out_a, (float *)j->out_b,
+                                      (float *)j->out_c, (double *)j->out_d, &j->loss);
+        break;
+    case OP_NCC_REFINE:
+        j->status = lr3d_ncc_refine_step(j->st, j->descs, j->n, j->step_depth, j->step_normal, (double *)j->out_a,
+                                         (double *)j->out_b, (double *)j->out_c);
+        break;
+    case OP_TSDF_BACKWARD:
+        j->status = lr3d_tsdf_backward(j->ts, j->st, j->layers, j->map, j->iviews, j->n, j->voxels, j->targets,
+                                       j->n_voxels, (float *)j->out_a, (float *)j->out_b, (double *)j->out_c, &j->loss);
+        break;
     }
     if (j->status != LR3D_OK) snprintf(j->error, sizeof(j->error), "lr3d error %d: %s", (int)j->status, lr3d_last_error());
 }
@@ -179,6 +205,28 @@ static void job_complete(napi_env env, napi_status status, void *data) {
         napi_value result = undefined(env);
         if (j->op == OP_READ_DEPTH || j->op == OP_READ_FILTERED) {
             result = typed_from(env, j->out_a, j->bytes_a, napi_uint32_array, 4);
+        } else if (j->op == OP_NCC_BACKWARD) {
+            napi_value loss;
+            napi_create_object(env, &result);
+            napi_set_named_property(env, result, "cost", typed_from(env, j->out_a, j->bytes_a, napi_float32_array, 4));
+            napi_set_named_property(env, result, "gradDepth", typed_from(env, j->out_b, j->bytes_b, napi_float32_array, 4));
+            napi_set_named_property(env, result, "gradNormal", typed_from(env, j->out_c, j->bytes_c, napi_float32_array, 4));
+            napi_set_named_property(env, result, "gradPose", typed_from(env, j->out_d, j->bytes_d, napi_float64_array, 8));
+            napi_create_double(env, j->loss, &loss);
+            napi_set_named_property(env, result, "loss", loss);
+        } else if (j->op == OP_NCC_REFINE) {
+            napi_create_object(env, &result);
+            napi_set_named_property(env, result, "gradPose", typed_from(env, j->out_a, j->bytes_a, napi_float64_array, 8));
+            napi_set_named_property(env, result, "lossBefore", typed_from(env, j->out_b, j->bytes_b, napi_float64_array, 8));
+            napi_set_named_property(env, result, "lossAfter", typed_from(env, j->out_c, j->bytes_c, napi_float64_array, 8));
+        } else if (j->op == OP_TSDF_BACKWARD) {
+            napi_value loss;
+            napi_create_object(env, &result);
+            napi_set_named_property(env, result, "tsdf", typed_from(env, j->out_a, j->bytes_a, napi_float32_array, 4));
+            napi_set_named_property(env, result, "gradDepth", typed_from(env, j->out_b, j->bytes_b, napi_float32_array, 4));
+            napi_set_named_property(env, result, "gradPose", typed_from(env, j->out_c, j->bytes_c, napi_float64_array, 8));
+            napi_create_double(env, j->loss, &loss);
+            napi_set_named_property(env, result, "loss", loss);
         } else if (j->op == OP_READ_HYP || j->op == OP_READ_TSDF) {
             napi_create_object(env, &result);
             napi_set_named_property(env, result, j->op == OP_READ_HYP ? "hyp" : "tsdf",
@@ -484,6 +532,141 @@ static napi_value tsdf_read(napi_env env, napi_callback_info info) {
     return queue(env, j);
 }
 
+/* nccBackward(stereo, solveOptions, weights | null, pixels) -> {cost, gradDepth, gradNormal, gradPose, loss} */
+static napi_value ncc_backward(napi_env env, napi_callback_info info) {
+    ARGS(4);
+    job_t *j = new_job(OP_NCC_BACKWARD);
+    j->st = (lr3d_stereo *)get_external(env, argv[0]);
+    j->n = 1;
+    j->descs = (lr3d_solve_desc *)calloc(1, sizeof(lr3d_solve_desc));
+    j->priors = (float **)calloc(1, sizeof(float *));
+    if (!read_solve(env, argv[1], &j->descs[0], &j->priors[0])) {
+        job_free(j);
+        napi_throw_type_error(env, NULL, "lr3d.node: solve options need ref and sources");
+        return NULL;
+    }
+    int32_t px = 0;
+    napi_get_value_int32(env, argv[3], &px);
+    napi_valuetype t = napi_undefined;
+    napi_typeof(env, argv[2], &t);
+    if (t == napi_object) {
+        size_t count = 0;
+        j->weights = (float *)copy_typed(env, argv[2], 4, &count);
+        if (!j->weights || count != (size_t)px) {
+            job_free(j);
+            napi_throw_range_error(env, NULL, "lr3d.node: weights must be a Float32Array of width*height");
+            return NULL;
+        }
+    }
+    j->bytes_a = j->bytes_b = (size_t)px * 4u;
+    j->bytes_c = (size_t)px * 12u;
+    j->bytes_d = (size_t)LR3D_MAX_SOURCES * 12u * 8u;
+    j->out_a = malloc(j->bytes_a + 1);
+    j->out_b = malloc(j->bytes_b + 1);
+    j->out_c = malloc(j->bytes_c + 1);
+    j->out_d = malloc(j->bytes_d);
+    return queue(env, j);
+}
+
+/* nccRefineStep(stereo, [solveOptions], stepDepth, stepNormal) -> {gradPose, lossBefore, lossAfter} */
+static napi_value ncc_refine_step(napi_env env, napi_callback_info info) {
+    ARGS(4);
+    job_t *j = new_job(OP_NCC_REFINE);
+    j->st = (lr3d_stereo *)get_external(env, argv[0]);
+    uint32_t n = 0;
+    napi_get_array_length(env, argv[1], &n);
+    j->n = (int32_t)n;
+    j->descs = (lr3d_solve_desc *)calloc(n ? n : 1, sizeof(lr3d_solve_desc));
+    j->priors = (float **)calloc(n ? n : 1, sizeof(float *));
+    for (uint32_t i = 0; i < n; i++) {
+        napi_value o;
+        napi_get_element(env, argv[1], i, &o);
+        if (!read_solve(env, o, &j->descs[i], &j->priors[i])) {
+            job_free(j);
+            napi_throw_type_error(env, NULL, "lr3d.node: solve options need ref and sources");
+            return NULL;
+        }
+    }
+    double sd = 0, sn = 0;
+    napi_get_value_double(env, argv[2], &sd);
+    napi_get_value_double(env, argv[3], &sn);
+    j->step_depth = (float)sd;
+    j->step_normal = (float)sn;
+    j->bytes_a = (size_t)(n ? n : 1) * LR3D_MAX_SOURCES * 12u * 8u;
+    j->bytes_b = j->bytes_c = (size_t)(n ? n : 1) * 8u;
+    j->out_a = malloc(j->bytes_a);
+    j->out_b = malloc(j->bytes_b);
+    j->out_c = malloc(j->bytes_c);
+    return queue(env, j);
+}
+
+/* stereoSetPose(stereo, layer, R[9], t[3]) */
+static napi_value stereo_set_pose(napi_env env, napi_callback_info info) {
+    ARGS(4);
+    lr3d_stereo *st = (lr3d_stereo *)get_external(env, argv[0]);
+    int32_t layer = 0;
+    napi_get_value_int32(env, argv[1], &layer);
+    double R[9], t[3];
+    for (uint32_t i = 0; i < 9; i++) {
+        napi_value e;
+        napi_get_element(env, argv[2], i, &e);
+        napi_get_value_double(env, e, &R[i]);
+    }
+    for (uint32_t i = 0; i < 3; i++) {
+        napi_value e;
+        napi_get_element(env, argv[3], i, &e);
+        napi_get_value_double(env, e, &t[i]);
+    }
+    int32_t rc = lr3d_stereo_set_pose(st, layer, R, t);
+    if (rc != LR3D_OK) return throw_status(env, rc);
+    return undefined(env);
+}
+
+/* tsdfBackward(tsdf, stereo, layers, map, views, voxels: Uint32Array, targets: Float32Array)
+ *   -> {tsdf, gradDepth, gradPose, loss} */
+static napi_value tsdf_backward(napi_env env, napi_callback_info info) {
+    ARGS(7);
+    job_t *j = new_job(OP_TSDF_BACKWARD);
+    j->ts = (lr3d_tsdf *)get_external(env, argv[0]);
+    j->st = (lr3d_stereo *)get_external(env, argv[1]);
+    uint32_t n = 0;
+    napi_get_array_length(env, argv[2], &n);
+    napi_get_value_int32(env, argv[3], &j->map);
+    j->n = (int32_t)n;
+    j->layers = (int32_t *)calloc(n ? n : 1, sizeof(int32_t));
+    j->iviews = (lr3d_integration_view *)calloc(n ? n : 1, sizeof(lr3d_integration_view));
+    j->priors = (float **)calloc(n ? n : 1, sizeof(float *));
+    size_t px = 0;
+    for (uint32_t i = 0; i < n; i++) {
+        napi_value e, o;
+        napi_get_element(env, argv[2], i, &e);
+        napi_get_value_int32(env, e, &j->layers[i]);
+        napi_get_element(env, argv[4], i, &o);
+        if (!read_iview(env, o, &j->iviews[i], &j->priors[i])) {
+            job_free(j);
+            napi_throw_type_error(env, NULL, "lr3d.node: an integration view needs intrinsics, R[9], t[3]");
+            return NULL;
+        }
+        px += (size_t)j->iviews[i].width * j->iviews[i].height;
+    }
+    size_t nv = 0, nt = 0;
+    j->voxels = (uint32_t *)copy_typed(env, argv[5], 4, &nv);
+    j->targets = (float *)copy_typed(env, argv[6], 4, &nt);
+    if (nv != nt) {
+        job_free(j);
+        napi_throw_range_error(env, NULL, "lr3d.node: voxels and targets differ in length");
+        return NULL;
+    }
+    j->n_voxels = (int64_t)nv;
+    j->bytes_a = nv * 4u;
+    j->bytes_b = px * 4u;
+    j->bytes_c = (size_t)(n ? n : 1) * 12u * 8u;
+    j->out_a = malloc(j->bytes_a + 1);
+    j->out_b = malloc(j->bytes_b + 1);
+    j->out_c = malloc(j->bytes_c);
+    return queue(env, j);
+}
+
 static napi_value last_error(napi_env env, napi_callback_info info) {
     (void)info;
     napi_value s;
@@ -504,6 +687,8 @@ static napi_value init(napi_env env, napi_value exports) {
         {"stereoSetCoarse", stereo_set_coarse}, {"tsdfCreate", tsdf_create},
         {"tsdfDestroy", tsdf_destroy},       {"tsdfIntegrate", tsdf_integrate},
         {"tsdfRead", tsdf_read},             {"lastError", last_error},
+        {"nccBackward", ncc_backward},       {"nccRefineStep", ncc_refine_step},
+        {"stereoSetPose", stereo_set_pose},  {"tsdfBackward", tsdf_backward},
     };
     for (size_t i = 0; i < sizeof(fns) / sizeof(fns[0]); i++) {
         napi_value f;

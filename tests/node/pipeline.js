@@ -31,6 +31,26 @@ async function main() {
         fs.writeFileSync(path.join(dir, `filtered${layer}.u32`), Buffer.from((await run.stereo.filteredMap(layer, job.sources[layer], job.needed)).buffer));
         await volume.integrate(Object.assign({}, job.views[layer], { maxDepth: job.ranges[layer][1], prior: null, confidence: 0 }));
     }
+    // adjoints through the forward/backward pairs
+    const opts = {
+        sources: job.sources[0], minDepth: job.ranges[0][0], maxDepth: job.ranges[0][1], prior: null,
+        geometric: false, iterations: job.iterations,
+    };
+    const ncc = new lr3d.NccCostFunction(run.stereo, 0, opts);
+    const f = await ncc.forward(null);
+    const g = await ncc.backward();
+    fs.writeFileSync(path.join(dir, 'ncc_cost.f32'), Buffer.from(f.cost.buffer));
+    fs.writeFileSync(path.join(dir, 'ncc_gd.f32'), Buffer.from(g.gradDepth.buffer));
+    fs.writeFileSync(path.join(dir, 'ncc_gn.f32'), Buffer.from(g.gradNormal.buffer));
+    fs.writeFileSync(path.join(dir, 'ncc_gp.f64'), Buffer.from(g.gradPose.buffer));
+    const layers = job.views.map((v, i) => i);
+    const ivs = job.views.map((v, i) => Object.assign({}, v, { maxDepth: job.ranges[i][1], prior: null, confidence: 0 }));
+    const integration = new lr3d.TsdfIntegrationFunction(volume, layers, ivs, lr3d.MAP_FILTERED);
+    const tb = await integration.backward(job.voxels, job.targets);
+    fs.writeFileSync(path.join(dir, 'tsdf_bw.f32'), Buffer.from(tb.tsdf.buffer));
+    fs.writeFileSync(path.join(dir, 'tsdf_gd.f32'), Buffer.from(tb.gradDepth.buffer));
+    fs.writeFileSync(path.join(dir, 'tsdf_gp.f64'), Buffer.from(tb.gradPose.buffer));
+    fs.writeFileSync(path.join(dir, 'losses.json'), JSON.stringify({ ncc: f.loss, tsdf: tb.loss }));
     const { tsdf, weight } = await volume.read();
     fs.writeFileSync(path.join(dir, 'tsdf.f32'), Buffer.from(tsdf.buffer));
     fs.writeFileSync(path.join(dir, 'weight.f32'), Buffer.from(weight.buffer));

@@ -17,7 +17,8 @@ pytestmark = pytest.mark.skipif(NODE is None, reason="node is not installed")
 
 EXPORTS = ["ctxCreate", "ctxDestroy", "stereoCreate", "stereoDestroy", "stereoSetImage", "stereoSolve",
            "stereoFilter", "stereoReadDepth", "stereoReadFiltered", "stereoReadHyp", "stereoSetCoarse",
-           "tsdfCreate", "tsdfDestroy", "tsdfIntegrate", "tsdfRead", "lastError"]
+           "tsdfCreate", "tsdfDestroy", "tsdfIntegrate", "tsdfRead", "lastError", "nccBackward",
+           "nccRefineStep", "stereoSetPose", "tsdfBackward"]
 
 
 def node(code, timeout=60):
@@ -70,6 +71,12 @@ def test_node_pipeline_matches_python_host(tmp_path, ctx):
     job = {"method": "hip-single-scale", "width": 320, "height": 240, "views": views, "sources": w.sources,
            "ranges": w.ranges, "iterations": 3, "needed": 2, "origin": [-0.512] * 3, "voxel": 0.008,
            "extent": [1.024] * 3, "centre": [0.0, 0.0, 0.0], "radius": 0.6, "truncation": 4}
+    g0 = lr3d.BrickGrid((-0.512,) * 3, 0.008, (1.024,) * 3)
+    g0.mark(0.0, 0.0, 0.0, 0.6)
+    rng = np.random.default_rng(3)
+    voxels = rng.integers(0, g0.build()[2] * 512, 20000)
+    job["voxels"] = [int(v) for v in voxels]
+    job["targets"] = [float(v) for v in rng.uniform(-1, 1, voxels.size).astype(np.float32)]
     for i, im in enumerate(w.images):
         im.tofile(tmp_path / f"gray{i}.u8")
     (tmp_path / "job.json").write_text(json.dumps(job))
@@ -94,6 +101,23 @@ def test_node_pipeline_matches_python_host(tmp_path, ctx):
         assert np.array_equal(st.read_filtered(i), np.fromfile(tmp_path / f"filtered{i}.u32", np.uint32))
         v = w.views[i]
         vol.integrate(st, i, lr3d.IntegrationView(v.intrinsics, v.pose, w.ranges[i][1], 320, 240), map=lr3d.MAP_FILTERED)
+    # adjoints: the Node forward/backward pairs against the Python host
+    losses = json.loads((tmp_path / "losses.json").read_text())
+    o = lr3d.SolveOptions(sources=w.sources[0], min_depth=w.ranges[0][0], max_depth=w.ranges[0][1], iterations=3)
+    nb = st.ncc_backward(0, o)
+    assert np.array_equal(nb["cost"].view(np.uint32), np.fromfile(tmp_path / "ncc_cost.f32", np.uint32))
+    assert np.array_equal(nb["grad_depth"].view(np.uint32), np.fromfile(tmp_path / "ncc_gd.f32", np.uint32))
+    assert np.array_equal(nb["grad_normal"].reshape(-1).view(np.uint32), np.fromfile(tmp_path / "ncc_gn.f32", np.uint32))
+    assert np.array_equal(nb["grad_pose"], np.fromfile(tmp_path / "ncc_gp.f64", np.float64).reshape(8, 12)[:len(w.sources[0])])
+    assert nb["loss"] == losses["ncc"]
+    vox = np.array(job["voxels"], dtype=np.uint32)
+    ivs = [lr3d.IntegrationView(v.intrinsics, v.pose, w.ranges[i][1], 320, 240) for i, v in enumerate(w.views)]
+    tb = vol.backward(st, list(range(8)), ivs, vox, np.array(job["targets"], np.float32), map=lr3d.MAP_FILTERED)
+    assert np.array_equal(tb["tsdf"].view(np.uint32), np.fromfile(tmp_path / "tsdf_bw.f32", np.uint32))
+    gd = np.concatenate([x.reshape(-1) for x in tb["grad_depth"]])
+    assert np.allclose(gd, np.fromfile(tmp_path / "tsdf_gd.f32", np.float32), rtol=1e-6, atol=1e-9)
+    assert np.allclose(tb["grad_pose"], np.fromfile(tmp_path / "tsdf_gp.f64", np.float64).reshape(8, 12), rtol=1e-10, atol=1e-12)
+    assert abs(tb["loss"] - losses["tsdf"]) <= 1e-12 * max(1.0, abs(tb["loss"]))
     tsdf, weight = vol.read()
     assert np.array_equal(np.fromfile(tmp_path / "coords.i32", np.int32), layout[0])
     assert np.array_equal(tsdf.view(np.uint32), np.fromfile(tmp_path / "tsdf.f32", np.uint32))

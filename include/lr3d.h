@@ -133,6 +133,16 @@ int32_t lr3d_stereo_set_pose(lr3d_stereo *st, int32_t layer, const double R[9], 
 /* Overwrite a layer's planes (n.xyz, depth per pixel, width*height*4 floats). */
 int32_t lr3d_stereo_write_hyp(lr3d_stereo *st, int32_t layer, const float *hyp);
 
+/* ---- Launch-shape autotuner (no counterpart in the reference) -----------------------------
+ * Times the candidate block shapes of init, propagate (first iteration) and pack/consistency on
+ * this batch (`repeats` launches each, 1..16) and keeps the fastest per kernel for later calls on
+ * the set.  The set is left as found (planes, costs and packed maps are restored); maps do not
+ * depend on the shapes.  kernel names: "init", "propagate", "pack"; shapes are threads (x, y)
+ * with x*y a multiple of 64 up to 256. */
+int32_t lr3d_stereo_autotune(lr3d_stereo *st, const lr3d_solve_desc *descs, int32_t n, int32_t repeats);
+int32_t lr3d_stereo_get_tile(lr3d_stereo *st, const char *kernel, int32_t *tx, int32_t *ty);
+int32_t lr3d_stereo_set_tile(lr3d_stereo *st, const char *kernel, int32_t tx, int32_t ty);
+
 /* ---- Adjoint of the aggregated NCC cost (no counterpart in the reference) -------------------
  * C_p = the aggregated cost (PM:192-233) of the current plane of pixel p of desc->ref, evaluated as
  * the forward kernels do; L = sum_p weights[p] * C_p (weights NULL = 1).  Windows and taps are
@@ -187,6 +197,13 @@ int32_t lr3d_tsdf_backward(lr3d_tsdf *ts, lr3d_stereo *st, const int32_t *layers
                            const lr3d_integration_view *views, int32_t n, const uint32_t *voxels,
                            const float *targets, int64_t n_voxels, float *tsdf_out, float *grad_depth,
                            double *grad_pose, double *loss);
+
+/* Times integrate passes of these views with 1, 2 and 4 bricks (wavefronts) per block and keeps
+ * the fastest for later passes; the volume is restored.  No counterpart in the reference. */
+int32_t lr3d_tsdf_autotune(lr3d_tsdf *ts, lr3d_stereo *st, const int32_t *layers, int32_t map,
+                           const lr3d_integration_view *views, int32_t n, int32_t repeats);
+int32_t lr3d_tsdf_get_bricks_per_block(lr3d_tsdf *ts, int32_t *bricks);
+int32_t lr3d_tsdf_set_bricks_per_block(lr3d_tsdf *ts, int32_t bricks);
 
 /* TsdfVolume.read (TV:270-277): n_bricks*512 floats each (either may be NULL). */
 int32_t lr3d_tsdf_read(lr3d_tsdf *ts, float *tsdf, float *weight);

@@ -250,6 +250,9 @@ struct lr3d_stereo {
     DevBuf<AdjGeom> adj_geom;
     lr3d_stereo *coarse = nullptr;
     std::vector<GraphEntry> graphs;
+    // launch shapes (threads x, y) of init, propagate and pack/consistency; set by the autotuner
+    TileShape tiles[3] = {pm_tile_for("init", 0, 0, 0), pm_tile_for("propagate", 0, 0, 0),
+                          pm_tile_for("pack", 0, 0, 0)};
 
     ~lr3d_stereo() {
         for (auto &g : graphs) destroy_graph(g);
@@ -452,12 +455,14 @@ struct Plans {
     PmPlan init, prop, pack;
 };
 
+// tiles: launch shapes of init, propagate, pack (the stereo's own when NULL).
 static int32_t upload_plans(lr3d_stereo *st, const std::vector<PmDesc> &descs,
-                            const std::vector<uint8_t> &pack, Plans &plans, std::vector<int64_t> &key) {
+                            const std::vector<uint8_t> &pack, Plans &plans, std::vector<int64_t> &key,
+                            const TileShape *tiles = nullptr) {
     lr3d_ctx *ctx = st->ctx;
     const int32_t n = (int32_t)descs.size();
-    const TileShape ti = pm_tile_for("init", 0, 0, 0), tp = pm_tile_for("propagate", 0, 0, 0),
-                    tk = pm_tile_for("pack", 0, 0, 0);
+    if (!tiles) tiles = st->tiles;
+    const TileShape ti = tiles[0], tp = tiles[1], tk = tiles[2];
     std::vector<uint32_t> bi(n + 1), bp(n + 1), bk(n + 1);
     int32_t kmax = 1, geometric = 0;
     key.clear();
@@ -497,6 +502,10 @@ static int32_t upload_plans(lr3d_stereo *st, const std::vector<PmDesc> &descs,
     key.push_back(kmax);
     key.push_back(geometric);
     key.push_back((int64_t)(uintptr_t)st->plan_buf.ptr);
+    for (int i = 0; i < 3; i++) {
+        key.push_back(tiles[i].tx);
+        key.push_back(tiles[i].ty);
+    }
     return LR3D_OK;
 }
 
@@ -645,6 +654,156 @@ extern "C" int32_t lr3d_stereo_solve_batched(lr3d_stereo *st, const lr3d_solve_d
     std::lock_guard<std::mutex> lock(st->ctx->mu);
     HIP_TRY(hipSetDevice(st->ctx->device));
     return solve_impl(st, descs, n, true);
+}
+
+// ---- launch-shape autotuner (no counterpart in the reference) ----------------------------------------
+//
+// Candidate block shapes (threads x, y; whole wavefronts, at most 256 threads).  Every pixel's
+// result is independent of the tiling, so any choice gives identical maps.
+
+static const TileShape kTileCandidates[3][10] = {
+    {{8, 8}, {16, 4}, {32, 2}, {64, 1}, {16, 8}, {64, 4}},                                   // init
+    {{8, 8}, {16, 4}, {4, 16}, {32, 2}, {64, 1}, {16, 8}, {8, 16}, {32, 4}, {16, 16}, {64, 4}},  // propagate
+    {{64, 4}, {64, 1}, {64, 2}, {32, 8}, {256, 1}},                                          // pack, consistency
+};
+static const int kTileCandidateCount[3] = {6, 10, 5};
+
+static int tile_slot(const char *kernel) {
+    if (!kernel) return -1;
+    if (strcmp(kernel, "init") == 0) return 0;
+    if (strcmp(kernel, "propagate") == 0) return 1;
+    if (strcmp(kernel, "pack") == 0) return 2;
+    return -1;
+}
+
+static bool valid_tile(int32_t tx, int32_t ty) {
+    return tx > 0 && ty > 0 && tx * ty <= 256 && (tx * ty) % 64 == 0;
+}
+
+extern "C" int32_t lr3d_stereo_get_tile(lr3d_stereo *st, const char *kernel, int32_t *tx, int32_t *ty) {
+    if (!st || !tx || !ty) return fail(LR3D_ERR_INVALID, "lr3d_stereo_get_tile: NULL argument");
+    const int slot = tile_slot(kernel);
+    if (slot < 0) return fail(LR3D_ERR_INVALID, "lr3d_stereo_get_tile: kernel must be init, propagate or pack");
+    std::lock_guard<std::mutex> lock(st->ctx->mu);
+    *tx = st->tiles[slot].tx;
+    *ty = st->tiles[slot].ty;
+    return LR3D_OK;
+}
+
+extern "C" int32_t lr3d_stereo_set_tile(lr3d_stereo *st, const char *kernel, int32_t tx, int32_t ty) {
+    if (!st) return fail(LR3D_ERR_INVALID, "lr3d_stereo_set_tile: NULL argument");
+    const int slot = tile_slot(kernel);
+    if (slot < 0) return fail(LR3D_ERR_INVALID, "lr3d_stereo_set_tile: kernel must be init, propagate or pack");
+    if (!valid_tile(tx, ty))
+        return fail(LR3D_ERR_INVALID, "lr3d_stereo_set_tile: %d x %d is not 64, 128, 192 or 256 threads", tx, ty);
+    std::lock_guard<std::mutex> lock(st->ctx->mu);
+    st->tiles[slot] = {tx, ty};
+    return LR3D_OK;
+}
+
+struct EventPair {
+    hipEvent_t a = nullptr, b = nullptr;
+    ~EventPair() {
+        if (a) (void)hipEventDestroy(a);
+        if (b) (void)hipEventDestroy(b);
+    }
+};
+
+extern "C" int32_t lr3d_stereo_autotune(lr3d_stereo *st, const lr3d_solve_desc *descs, int32_t n, int32_t repeats) {
+    if (!st || !descs || n <= 0) return fail(LR3D_ERR_INVALID, "lr3d_stereo_autotune: NULL argument or n <= 0");
+    lr3d_ctx *ctx = st->ctx;
+    std::lock_guard<std::mutex> lock(ctx->mu);
+    HIP_TRY(hipSetDevice(ctx->device));
+    repeats = std::max(1, std::min(repeats, 16));
+    for (int32_t i = 0; i < n; i++) {
+        int32_t rc = check_layer(st, descs[i].ref, "lr3d_stereo_autotune");
+        if (rc) return rc;
+        for (int32_t j = 0; j < i; j++)
+            if (descs[j].ref == descs[i].ref)
+                return fail(LR3D_ERR_INVALID, "lr3d_stereo_autotune: layer %d appears twice", descs[i].ref);
+        if (descs[i].from_coarse && !st->coarse)
+            return fail(LR3D_ERR_STATE, "lr3d_stereo_autotune: from_coarse without a coarse level");
+    }
+    std::vector<uint32_t> prior_off;
+    int32_t rc = stage_priors(st, descs, n, prior_off);
+    if (rc) return rc;
+    std::vector<PmDesc> dv;
+    std::vector<uint8_t> packs;
+    bool any_iteration = false;
+    for (int32_t i = 0; i < n; i++) {
+        if (std::min(descs[i].n_sources, (int32_t)kMaxSources) <= 0) continue;
+        PmDesc d;
+        rc = build_desc(st, descs[i], prior_off[i], true, d);
+        if (rc) return rc;
+        d.iterations = std::min(d.iterations, 1);  // propagate is timed on the first iteration
+        any_iteration |= d.iterations > 0;
+        dv.push_back(d);
+        packs.push_back(1);
+    }
+    if (dv.empty()) return LR3D_OK;
+    // Tuning leaves the set as it found it: planes, costs and packed maps are restored.
+    DevBuf<uint32_t> sdepth;
+    DevBuf<float4> shyp;
+    DevBuf<float> scost;
+    HIP_TRY(sdepth.alloc(st->total_px));
+    HIP_TRY(shyp.alloc(st->total_px));
+    HIP_TRY(scost.alloc(st->total_px));
+    HIP_TRY(hipMemcpyAsync(sdepth.ptr, st->depth.ptr, st->total_px * 4, hipMemcpyDeviceToDevice, ctx->stream));
+    HIP_TRY(hipMemcpyAsync(shyp.ptr, st->hyp.ptr, st->total_px * 16, hipMemcpyDeviceToDevice, ctx->stream));
+    HIP_TRY(hipMemcpyAsync(scost.ptr, st->cost.ptr, st->total_px * 4, hipMemcpyDeviceToDevice, ctx->stream));
+    EventPair ev;
+    HIP_TRY(hipEventCreate(&ev.a));
+    HIP_TRY(hipEventCreate(&ev.b));
+    const PmPools pools = pools_of(st);
+    // Each candidate starts from the same state: the packed maps restored (geometric passes read
+    // them) and, for propagate and pack, a fresh init.
+    auto time_slot = [&](int slot, TileShape cand, float &avg_ms) -> int32_t {
+        TileShape tl[3] = {st->tiles[0], st->tiles[1], st->tiles[2]};
+        tl[slot] = cand;
+        Plans plans;
+        std::vector<int64_t> key;
+        int32_t r = upload_plans(st, dv, packs, plans, key, tl);
+        if (r) return r;
+        float total = 0.0f;
+        for (int32_t k = 0; k < repeats; k++) {
+            HIP_TRY(hipMemcpyAsync(st->depth.ptr, sdepth.ptr, st->total_px * 4, hipMemcpyDeviceToDevice, ctx->stream));
+            if (slot != 0) HIP_TRY(pm_launch_init(ctx->stream, plans.init, pools));
+            HIP_TRY(hipEventRecord(ev.a, ctx->stream));
+            if (slot == 0) HIP_TRY(pm_launch_init(ctx->stream, plans.init, pools));
+            if (slot == 1)
+                for (uint32_t phase = 0; phase < 2; phase++)
+                    HIP_TRY(pm_launch_propagate(ctx->stream, plans.prop, pools, 0u, phase, 1.0f));
+            if (slot == 2) HIP_TRY(pm_launch_pack(ctx->stream, plans.pack, pools));
+            HIP_TRY(hipEventRecord(ev.b, ctx->stream));
+            HIP_TRY(hipEventSynchronize(ev.b));
+            float ms = 0.0f;
+            HIP_TRY(hipEventElapsedTime(&ms, ev.a, ev.b));
+            total += ms;
+        }
+        avg_ms = total / (float)repeats;
+        return LR3D_OK;
+    };
+    const int order[3] = {1, 0, 2};
+    for (int o = 0; o < 3; o++) {
+        const int slot = order[o];
+        if (slot == 1 && !any_iteration) continue;
+        float best_ms = 0.0f;
+        TileShape best = st->tiles[slot];
+        for (int c = 0; c < kTileCandidateCount[slot]; c++) {
+            float ms = 0.0f;
+            rc = time_slot(slot, kTileCandidates[slot][c], ms);
+            if (rc) return rc;
+            if (c == 0 || ms < best_ms) {
+                best_ms = ms;
+                best = kTileCandidates[slot][c];
+            }
+        }
+        st->tiles[slot] = best;
+    }
+    HIP_TRY(hipMemcpyAsync(st->depth.ptr, sdepth.ptr, st->total_px * 4, hipMemcpyDeviceToDevice, ctx->stream));
+    HIP_TRY(hipMemcpyAsync(st->hyp.ptr, shyp.ptr, st->total_px * 16, hipMemcpyDeviceToDevice, ctx->stream));
+    HIP_TRY(hipMemcpyAsync(st->cost.ptr, scost.ptr, st->total_px * 4, hipMemcpyDeviceToDevice, ctx->stream));
+    return finish(ctx);
 }
 
 extern "C" int32_t lr3d_stereo_filter(lr3d_stereo *st, int32_t ref, const int32_t *checks,
@@ -803,9 +962,11 @@ static int32_t adjoint_impl(lr3d_stereo *st, const lr3d_solve_desc *descs, int32
     }
     Plans plans;
     std::vector<int64_t> key;
-    int32_t rc = upload_plans(st, dv, packs, plans, key);
+    // the backward kernel runs one 8 x 8 wavefront per block
+    const TileShape tiles[3] = {{8, 8}, st->tiles[1], st->tiles[2]};
+    int32_t rc = upload_plans(st, dv, packs, plans, key, tiles);
     if (rc) return rc;
-    const PmPlan &plan = plans.init;  // 8 x 8 threads per block
+    const PmPlan &plan = plans.init;
     HIP_TRY(st->adj_geom.reserve((size_t)n));
     HIP_TRY(hipMemcpyAsync(st->adj_geom.ptr, geo.data(), sizeof(AdjGeom) * n, hipMemcpyHostToDevice, ctx->stream));
     HIP_TRY(st->adj_partials.reserve((size_t)plan.blocks * kAdjPartial));
@@ -908,6 +1069,7 @@ struct lr3d_tsdf {
     DevBuf<uint32_t> host_map;
     DevBuf<unsigned long long> counter;
     bool counting = false;
+    int32_t bricks_per_block = 4;  // integrate_kernel: wavefronts (bricks) per block; set by the autotuner
     // adjoint scratch
     DevBuf<uint32_t> adj_voxels, adj_pixel_off;
     DevBuf<float> adj_targets, adj_tsdf, adj_gdf;
@@ -1012,13 +1174,11 @@ static int32_t make_tsdf_view(const lr3d_integration_view &v, uint32_t map_off, 
     return LR3D_OK;
 }
 
-static int32_t integrate_impl(lr3d_tsdf *ts, const uint32_t *maps, const std::vector<TsdfView> &tv,
-                              const lr3d_integration_view *views) {
+// Priors, view table and culling bits of a pass (stream-ordered).
+static int32_t stage_pass(lr3d_tsdf *ts, const std::vector<TsdfView> &tv, const lr3d_integration_view *views) {
     lr3d_ctx *ctx = ts->ctx;
     const int32_t n = (int32_t)tv.size();
-    if (n == 0) return finish(ctx);
     const int32_t words = (n + 31) / 32;
-    // priors
     size_t prior_total = 0;
     for (int32_t i = 0; i < n; i++)
         if (views[i].prior) prior_total += (size_t)views[i].width * views[i].height;
@@ -1036,10 +1196,21 @@ static int32_t integrate_impl(lr3d_tsdf *ts, const uint32_t *maps, const std::ve
     HIP_TRY(timed(ctx, "tsdf_visibility", [&] {
         return tsdf_launch_visibility(ctx->stream, ts->grid, ts->coords.ptr, ts->views.ptr, n, ts->vis.ptr, words);
     }));
+    return LR3D_OK;
+}
+
+static int32_t integrate_impl(lr3d_tsdf *ts, const uint32_t *maps, const std::vector<TsdfView> &tv,
+                              const lr3d_integration_view *views) {
+    lr3d_ctx *ctx = ts->ctx;
+    const int32_t n = (int32_t)tv.size();
+    if (n == 0) return finish(ctx);
+    const int32_t words = (n + 31) / 32;
+    int32_t rc = stage_pass(ts, tv, views);
+    if (rc) return rc;
     HIP_TRY(timed(ctx, "tsdf_integrate", [&] {
         return tsdf_launch_integrate(ctx->stream, ts->grid, ts->views.ptr, n, ts->vis.ptr, words,
                                      ts->coords.ptr, maps, ts->priors.ptr, ts->tsdf.ptr, ts->weight.ptr,
-                                     ts->counting ? ts->counter.ptr : nullptr);
+                                     ts->counting ? ts->counter.ptr : nullptr, ts->bricks_per_block);
     }));
     return finish(ctx);
 }
@@ -1124,22 +1295,8 @@ extern "C" int32_t lr3d_tsdf_backward(lr3d_tsdf *ts, lr3d_stereo *st, const int3
     std::vector<uint32_t> pixel_off(n + 1, 0);
     for (int32_t i = 0; i < n; i++) pixel_off[i + 1] = pixel_off[i] + (uint32_t)(views[i].width * views[i].height);
     const size_t total_px = pixel_off[n];
-    size_t prior_total = 0;
-    for (int32_t i = 0; i < n; i++)
-        if (views[i].prior) prior_total += (size_t)views[i].width * views[i].height;
-    if (prior_total) {
-        HIP_TRY(ts->priors.reserve(prior_total));
-        for (int32_t i = 0; i < n; i++)
-            if (views[i].prior)
-                HIP_TRY(hipMemcpyAsync(ts->priors.ptr + tv[i].prior_off, views[i].prior,
-                                       (size_t)views[i].width * views[i].height * 4, hipMemcpyHostToDevice, ctx->stream));
-    }
-    HIP_TRY(ts->views.reserve(n));
-    HIP_TRY(hipMemcpyAsync(ts->views.ptr, tv.data(), sizeof(TsdfView) * n, hipMemcpyHostToDevice, ctx->stream));
-    HIP_TRY(ts->vis.reserve((size_t)ts->grid.n_bricks * words));
-    HIP_TRY(timed(ctx, "tsdf_visibility", [&] {
-        return tsdf_launch_visibility(ctx->stream, ts->grid, ts->coords.ptr, ts->views.ptr, n, ts->vis.ptr, words);
-    }));
+    rc = stage_pass(ts, tv, views);
+    if (rc) return rc;
     HIP_TRY(ts->adj_pixel_off.reserve(n + 1));
     HIP_TRY(hipMemcpyAsync(ts->adj_pixel_off.ptr, pixel_off.data(), 4 * (n + 1), hipMemcpyHostToDevice, ctx->stream));
     HIP_TRY(ts->adj_gd.reserve(std::max<size_t>(total_px, 1)));
@@ -1182,6 +1339,75 @@ extern "C" int32_t lr3d_tsdf_backward(lr3d_tsdf *ts, lr3d_stereo *st, const int3
         *loss = l;
     }
     return LR3D_OK;
+}
+
+extern "C" int32_t lr3d_tsdf_get_bricks_per_block(lr3d_tsdf *ts, int32_t *out) {
+    if (!ts || !out) return fail(LR3D_ERR_INVALID, "lr3d_tsdf_get_bricks_per_block: NULL argument");
+    std::lock_guard<std::mutex> lock(ts->ctx->mu);
+    *out = ts->bricks_per_block;
+    return LR3D_OK;
+}
+
+extern "C" int32_t lr3d_tsdf_set_bricks_per_block(lr3d_tsdf *ts, int32_t bricks) {
+    if (!ts) return fail(LR3D_ERR_INVALID, "lr3d_tsdf_set_bricks_per_block: NULL argument");
+    if (bricks != 1 && bricks != 2 && bricks != 4)
+        return fail(LR3D_ERR_INVALID, "lr3d_tsdf_set_bricks_per_block: %d is not 1, 2 or 4", bricks);
+    std::lock_guard<std::mutex> lock(ts->ctx->mu);
+    ts->bricks_per_block = bricks;
+    return LR3D_OK;
+}
+
+extern "C" int32_t lr3d_tsdf_autotune(lr3d_tsdf *ts, lr3d_stereo *st, const int32_t *layers, int32_t map,
+                                      const lr3d_integration_view *views, int32_t n, int32_t repeats) {
+    if (!ts || !layers || !views || n <= 0) return fail(LR3D_ERR_INVALID, "lr3d_tsdf_autotune: NULL argument or n <= 0");
+    lr3d_ctx *ctx = ts->ctx;
+    std::lock_guard<std::mutex> lock(ctx->mu);
+    HIP_TRY(hipSetDevice(ctx->device));
+    repeats = std::max(1, std::min(repeats, 16));
+    std::vector<TsdfView> tv;
+    const uint32_t *maps = nullptr;
+    int32_t rc = stereo_views(ts, st, layers, map, views, n, tv, &maps);
+    if (rc) return rc;
+    rc = stage_pass(ts, tv, views);
+    if (rc) return rc;
+    const int32_t words = (n + 31) / 32;
+    const size_t voxels = (size_t)ts->grid.n_bricks * 512u;
+    // every candidate integrates into the same starting volume; the volume is restored at the end
+    DevBuf<float> s_tsdf, s_weight;
+    HIP_TRY(s_tsdf.alloc(voxels));
+    HIP_TRY(s_weight.alloc(voxels));
+    HIP_TRY(hipMemcpyAsync(s_tsdf.ptr, ts->tsdf.ptr, voxels * 4, hipMemcpyDeviceToDevice, ctx->stream));
+    HIP_TRY(hipMemcpyAsync(s_weight.ptr, ts->weight.ptr, voxels * 4, hipMemcpyDeviceToDevice, ctx->stream));
+    EventPair ev;
+    HIP_TRY(hipEventCreate(&ev.a));
+    HIP_TRY(hipEventCreate(&ev.b));
+    const int32_t candidates[3] = {1, 2, 4};
+    float best_ms = 0.0f;
+    int32_t best = ts->bricks_per_block;
+    for (int c = 0; c < 3; c++) {
+        float total = 0.0f;
+        for (int32_t k = 0; k < repeats; k++) {
+            HIP_TRY(hipMemcpyAsync(ts->tsdf.ptr, s_tsdf.ptr, voxels * 4, hipMemcpyDeviceToDevice, ctx->stream));
+            HIP_TRY(hipMemcpyAsync(ts->weight.ptr, s_weight.ptr, voxels * 4, hipMemcpyDeviceToDevice, ctx->stream));
+            HIP_TRY(hipEventRecord(ev.a, ctx->stream));
+            HIP_TRY(tsdf_launch_integrate(ctx->stream, ts->grid, ts->views.ptr, n, ts->vis.ptr, words, ts->coords.ptr,
+                                          maps, ts->priors.ptr, ts->tsdf.ptr, ts->weight.ptr, nullptr,
+                                          candidates[c]));
+            HIP_TRY(hipEventRecord(ev.b, ctx->stream));
+            HIP_TRY(hipEventSynchronize(ev.b));
+            float ms = 0.0f;
+            HIP_TRY(hipEventElapsedTime(&ms, ev.a, ev.b));
+            total += ms;
+        }
+        if (c == 0 || total < best_ms) {
+            best_ms = total;
+            best = candidates[c];
+        }
+    }
+    ts->bricks_per_block = best;
+    HIP_TRY(hipMemcpyAsync(ts->tsdf.ptr, s_tsdf.ptr, voxels * 4, hipMemcpyDeviceToDevice, ctx->stream));
+    HIP_TRY(hipMemcpyAsync(ts->weight.ptr, s_weight.ptr, voxels * 4, hipMemcpyDeviceToDevice, ctx->stream));
+    return finish(ctx);
 }
 
 extern "C" int32_t lr3d_tsdf_read(lr3d_tsdf *ts, float *tsdf, float *weight) {

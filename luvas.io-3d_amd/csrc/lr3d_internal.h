@@ -136,7 +136,8 @@ hipError_t tsdf_launch_visibility(hipStream_t s, const TsdfGridDev &g, const int
 hipError_t tsdf_launch_integrate(hipStream_t s, const TsdfGridDev &g, const TsdfView *views,
                                  int32_t n_views, const uint32_t *vis, int32_t words,
                                  const int4 *coords, const uint32_t *maps, const float *priors,
-                                 float *tsdf, float *weight, unsigned long long *counter);
+                                 float *tsdf, float *weight, unsigned long long *counter,
+                                 int32_t bricks_per_block);
 // TSDF adjoint over listed voxels (pool indices); n_voxels > 0.  ckpt holds
 // ceil(n_views / 16) * n_voxels float2; grad_depth (per pixel of each view, at pixel_off[v]) and
 // grad_pose (n_views * 12) are accumulated (zero them first); loss_partials: one per wavefront.

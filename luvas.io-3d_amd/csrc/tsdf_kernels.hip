@@ -398,8 +398,11 @@ hipError_t tsdf_launch_visibility(hipStream_t s, const TsdfGridDev &g, const int
 hipError_t tsdf_launch_integrate(hipStream_t s, const TsdfGridDev &g, const TsdfView *views,
                                  int32_t n_views, const uint32_t *vis, int32_t words,
                                  const int4 *coords, const uint32_t *maps, const float *priors,
-                                 float *tsdf, float *weight, unsigned long long *counter) {
-    dim3 block(256), grid(((uint32_t)g.n_bricks + 3u) / 4u);
+                                 float *tsdf, float *weight, unsigned long long *counter,
+                                 int32_t bricks_per_block) {
+    if (bricks_per_block != 1 && bricks_per_block != 2 && bricks_per_block != 4) return hipErrorInvalidValue;
+    const uint32_t bpb = (uint32_t)bricks_per_block;
+    dim3 block(64u * bpb), grid(((uint32_t)g.n_bricks + bpb - 1u) / bpb);
     if (counter)
         hipLaunchKernelGGL((integrate_kernel<true>), grid, block, 0, s, g, views, n_views, vis, words,
                            coords, maps, priors, tsdf, weight, counter);

@@ -32,7 +32,9 @@ EXPORTED = (
     "lr3d_ncc_backward", "lr3d_ncc_refine_step",
     "lr3d_tsdf_create", "lr3d_tsdf_destroy", "lr3d_tsdf_integrate", "lr3d_tsdf_integrate_host",
     "lr3d_tsdf_integrate_batched", "lr3d_tsdf_read", "lr3d_tsdf_set_counting",
-    "lr3d_tsdf_update_count", "lr3d_tsdf_backward",
+    "lr3d_tsdf_update_count", "lr3d_tsdf_backward", "lr3d_stereo_autotune", "lr3d_stereo_get_tile",
+    "lr3d_stereo_set_tile", "lr3d_tsdf_autotune", "lr3d_tsdf_get_bricks_per_block",
+    "lr3d_tsdf_set_bricks_per_block",
 )
 
 
@@ -107,6 +109,12 @@ def load() -> C.CDLL:
         "lr3d_tsdf_create": (i32, [P, C.POINTER(dbl), dbl, C.POINTER(i32), C.c_void_p, i32, dbl,
                                    C.POINTER(P)]),
         "lr3d_tsdf_destroy": (i32, [P]),
+        "lr3d_stereo_autotune": (i32, [P, C.POINTER(SolveDesc), i32, i32]),
+        "lr3d_stereo_get_tile": (i32, [P, C.c_char_p, C.POINTER(i32), C.POINTER(i32)]),
+        "lr3d_stereo_set_tile": (i32, [P, C.c_char_p, i32, i32]),
+        "lr3d_tsdf_autotune": (i32, [P, P, C.c_void_p, i32, C.c_void_p, i32, i32]),
+        "lr3d_tsdf_get_bricks_per_block": (i32, [P, C.POINTER(i32)]),
+        "lr3d_tsdf_set_bricks_per_block": (i32, [P, i32]),
         "lr3d_tsdf_backward": (i32, [P, P, C.c_void_p, i32, C.c_void_p, i32, C.c_void_p, C.c_void_p, C.c_int64,
                                      C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]),
         "lr3d_tsdf_integrate": (i32, [P, P, i32, i32, C.POINTER(IntegrationView)]),

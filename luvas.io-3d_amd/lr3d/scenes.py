@@ -10,8 +10,13 @@ Scene choices that the workload descriptions leave open, fixed here:
     y = -0.5; cameras look at the origin from an orbit at 0.4 m height, world +y up;
   * the TSDF cubes are centred on the origin;
   * room: the inside of a 4 x 2.5 x 3 m box (x, y, z), floor at y = 0, with 12 seeded
-    axis-aligned boxes standing on the floor; cameras at 1.2-1.8 m height inside the room look
-    at seeded points on the far walls.
+    axis-aligned boxes (0.3-0.8 m wide, 0.3-1.0 m tall) standing on the floor; 64 cameras on a
+    seeded loop (radii 1.1 x 0.8 m around the room centre, 1.2-1.8 m height) look inward across
+    the room, 10-25 degrees down and up to 20 degrees off the radial direction, with a horizontal field of view
+    of 70 degrees; resolutions cycle 640x480, 960x540, 1280x720, 1920x1080 along the loop, so
+    neighbouring views differ in size; neighbour counts are drawn from 4-10 per view and sliced
+    to 8 (DS:232); depth 0.3-6 m; the 384^3 x 1 cm TSDF cube is centred on the room centre (the
+    two x walls, 2 m from the centre, fall 8 cm outside it).
 """
 from __future__ import annotations
 
@@ -236,7 +241,53 @@ def sphere_phone(seed: int = 2, n: int = 32) -> Workload:
                     tsdf=TsdfSpec((-0.512, -0.512, -0.512), 0.004, 256, 4.0))
 
 
+def room_scene(seed: int, freq: float = 0.3) -> SceneSpec:
+    """The room and its 12 boxes (mulberry32 seeded)."""
+    rnd = mulberry32(seed * 104729 + 7)
+    prims = [Primitive(SG_ROOM, (-2.0, 0.0, -1.5), (2.0, 2.5, 1.5))]
+    for _ in range(12):
+        sx, sz = 0.3 + 0.5 * rnd(), 0.3 + 0.5 * rnd()
+        h = 0.3 + 0.7 * rnd()
+        x = -2.0 + sx / 2 + (4.0 - sx) * rnd()
+        z = -1.5 + sz / 2 + (3.0 - sz) * rnd()
+        prims.append(Primitive(SG_BOX, (x - sx / 2, 0.0, z - sz / 2), (x + sx / 2, h, z + sz / 2)))
+    return SceneSpec(prims, seed=seed, freq=freq)
+
+
+ROOM_SIZES = [(640, 480), (960, 540), (1280, 720), (1920, 1080)]
+
+
+def room_batched(seed: int = 3, n: int = 64, scale: float = 1.0, iterations: int = 8) -> Workload:
+    """Batched configuration: 64 views of the procedural room at mixed resolutions (16 each at
+    640x480, 960x540, 1280x720, 1920x1080), neighbour counts drawn from 4-10 (sliced to 8),
+    8 iterations, top-k by the reference rule; TSDF 384^3 x 1 cm, truncation 4 voxels.
+    `scale` shrinks every image (intrinsics scaled alike) for CPU-checkable variants."""
+    rnd = mulberry32(seed * 7919 + 29)
+    views = []
+    counts = []
+    for i in range(n):
+        W0, H0 = ROOM_SIZES[i % 4]
+        W, H = max(16, int(round(W0 * scale))), max(16, int(round(H0 * scale)))
+        f = (W / 2) / math.tan(math.radians(35.0))
+        k = Intrinsics(f, f, (W - 1) / 2, (H - 1) / 2)
+        a = 2 * math.pi * (i + 0.4 * (rnd() - 0.5)) / n
+        height = 1.2 + 0.6 * rnd()
+        c = (1.1 * math.cos(a), height, 0.8 * math.sin(a))
+        yaw = a + math.pi + math.radians(20.0) * (2 * rnd() - 1)
+        pitch = math.radians(10.0 + 15.0 * rnd())
+        d = (math.cos(yaw) * math.cos(pitch), -math.sin(pitch), math.sin(yaw) * math.cos(pitch))
+        pose = look_at(c, (c[0] + d[0], c[1] + d[1], c[2] + d[2]))
+        views.append(StereoView(k, pose, W, H))
+        counts.append(4 + int(7 * rnd()))
+    sources = [ring[:cnt] for ring, cnt in zip(_ring_sources(n, 10), counts)]
+    name = f"room-{n}-mixed" + ("" if scale == 1.0 else f"-x{scale:g}")
+    return Workload(name, room_scene(seed), views, sources, [(0.3, 6.0)] * n,
+                    iterations=iterations, topk=0,
+                    tsdf=TsdfSpec((-1.92, 1.25 - 1.92, -1.92), 0.01, 384, 4.0))
+
+
 WORKLOADS = {
     "sphere-small": sphere_small,
     "sphere-phone": sphere_phone,
+    "room-batched": room_batched,
 }

@@ -112,6 +112,24 @@ class HipDenseStereo:
         arr = (N.SolveDesc * len(descs))(*descs)
         N.check(self._lib.lr3d_stereo_solve_batched(self.handle, arr, len(descs)))
 
+    def autotune(self, solves: Sequence[tuple[int, SolveOptions]], repeats: int = 3) -> dict:
+        """Times the candidate launch shapes on this batch and keeps the fastest; returns them."""
+        descs = [self._desc(r, o)[0] for r, o in solves]
+        arr = (N.SolveDesc * len(descs))(*descs)
+        N.check(self._lib.lr3d_stereo_autotune(self.handle, arr, len(descs), int(repeats)))
+        return self.tiles()
+
+    def tiles(self) -> dict:
+        out = {}
+        for k in ("init", "propagate", "pack"):
+            tx, ty = C.c_int32(), C.c_int32()
+            N.check(self._lib.lr3d_stereo_get_tile(self.handle, k.encode(), C.byref(tx), C.byref(ty)))
+            out[k] = (tx.value, ty.value)
+        return out
+
+    def set_tile(self, kernel: str, tx: int, ty: int) -> None:
+        N.check(self._lib.lr3d_stereo_set_tile(self.handle, kernel.encode(), int(tx), int(ty)))
+
     def filter(self, ref: int, sources: Sequence[int], needed: int) -> None:
         """DS:296-309: consistency check of `ref` into its filtered map (stays on the GPU)."""
         checked = list(sources)[: N.MAX_SOURCES]

@@ -141,6 +141,29 @@ class HipTsdfVolume:
         N.check(self._lib.lr3d_tsdf_integrate_batched(self.handle, stereo.handle, ls, map, arr,
                                                       len(structs)))
 
+    def autotune(self, stereo, layers: Sequence[int], views: Sequence[IntegrationView], map: int = N.MAP_RAW,
+                 repeats: int = 3) -> int:
+        """Times integrate passes with 1, 2 and 4 bricks per block, keeps the fastest (returned)."""
+        structs, keep = [], []
+        for v in views:
+            s, k = _iv_struct(v)
+            structs.append(s)
+            keep.append(k)
+        arr = (N.IntegrationView * len(structs))(*structs)
+        ls = (C.c_int32 * len(layers))(*layers)
+        N.check(self._lib.lr3d_tsdf_autotune(self.handle, stereo.handle, ls, map, arr, len(structs), int(repeats)))
+        return self.bricks_per_block
+
+    @property
+    def bricks_per_block(self) -> int:
+        v = C.c_int32()
+        N.check(self._lib.lr3d_tsdf_get_bricks_per_block(self.handle, C.byref(v)))
+        return v.value
+
+    @bricks_per_block.setter
+    def bricks_per_block(self, bricks: int) -> None:
+        N.check(self._lib.lr3d_tsdf_set_bricks_per_block(self.handle, int(bricks)))
+
     def backward(self, stereo, layers: Sequence[int], views: Sequence[IntegrationView], voxels, targets,
                  map: int = N.MAP_RAW) -> dict:
         """Replay of integrate_batched into a fresh volume at the listed voxels (pool indices) and the

@@ -1,0 +1,100 @@
+"""Launch-shape autotuner (lr3d_stereo_autotune, lr3d_tsdf_autotune): every candidate shape gives
+bit-identical maps and volumes, tuning leaves the set and the volume as it found them, and the
+chosen shapes are among the candidates."""
+import numpy as np
+import pytest
+
+from lr3d import scenes
+
+pytestmark = pytest.mark.gpu
+
+PROPAGATE = [(8, 8), (16, 4), (4, 16), (32, 2), (64, 1), (16, 8), (8, 16), (32, 4), (16, 16), (64, 4)]
+INIT = [(8, 8), (16, 4), (32, 2), (64, 1), (16, 8), (64, 4)]
+PACK = [(64, 4), (64, 1), (64, 2), (32, 8), (256, 1)]
+
+
+def say(msg):
+    print(msg, flush=True)
+
+
+@pytest.fixture(scope="module")
+def work():
+    return scenes.sphere_small().render()
+
+
+def _stereo(ctx, w):
+    import lr3d
+    st = lr3d.HipDenseStereo(ctx, w.views)
+    for i, im in enumerate(w.images):
+        st.set_image(i, im)
+    solves = [(r, lr3d.SolveOptions(sources=w.sources[r], min_depth=w.ranges[r][0], max_depth=w.ranges[r][1],
+                                    iterations=w.iterations, topk=w.topk)) for r in range(len(w.views))]
+    return st, solves
+
+
+def test_every_shape_gives_identical_maps(ctx, work):
+    say("case: sphere 8x320x240 solved with every candidate shape of init, propagate and pack")
+    ref, solves = _stereo(ctx, work)
+    ref.solve_batched(solves)
+    expect = [ref.depth_map(r) for r in range(8)]
+    ref.destroy()
+    st, _ = _stereo(ctx, work)
+    for kernel, shapes in (("propagate", PROPAGATE), ("init", INIT), ("pack", PACK)):
+        for tx, ty in shapes:
+            st.set_tile(kernel, tx, ty)
+            st.solve_batched(solves)
+            for r in range(8):
+                assert np.array_equal(st.depth_map(r), expect[r]), (kernel, tx, ty, r)
+        st.set_tile(kernel, *shapes[0])
+    st.destroy()
+
+
+def test_autotune_keeps_the_set_and_picks_a_candidate(ctx, work):
+    say("case: stereo autotune on the sphere batch")
+    st, solves = _stereo(ctx, work)
+    st.solve_batched(solves)
+    before = [st.depth_map(r) for r in range(8)]
+    hyp_before = st.hypotheses(3)
+    tiles = st.autotune(solves, repeats=2)
+    say(f"autotune: {tiles}")
+    assert tiles["propagate"] in PROPAGATE and tiles["init"] in INIT and tiles["pack"] in PACK
+    for r in range(8):
+        assert np.array_equal(st.depth_map(r), before[r])
+    hyp_after = st.hypotheses(3)
+    assert np.array_equal(hyp_after[0], hyp_before[0]) and np.array_equal(hyp_after[1], hyp_before[1])
+    st.solve_batched(solves)
+    for r in range(8):
+        assert np.array_equal(st.depth_map(r), before[r])
+    st.destroy()
+
+
+def test_tsdf_autotune_and_every_block_size(ctx, work):
+    import lr3d
+    say("case: TSDF 128^3 x 8 mm, bricks per block 1 / 2 / 4 and autotune")
+    st, solves = _stereo(ctx, work)
+    st.solve_batched(solves)
+    spec = work.tsdf
+    grid, layout = lr3d.dense_grid(spec.origin, spec.voxel, spec.voxels)
+    views = [lr3d.IntegrationView(v.intrinsics, v.pose, work.ranges[i][1], v.width, v.height)
+             for i, v in enumerate(work.views)]
+    results = []
+    for bricks in (1, 2, 4):
+        vol = lr3d.HipTsdfVolume(ctx, grid, layout, spec.truncation_voxels)
+        vol.bricks_per_block = bricks
+        vol.integrate_batched(st, list(range(8)), views)
+        results.append(vol.read())
+        vol.destroy()
+    for t, w in results[1:]:
+        assert np.array_equal(t.view(np.uint32), results[0][0].view(np.uint32))
+        assert np.array_equal(w.view(np.uint32), results[0][1].view(np.uint32))
+    vol = lr3d.HipTsdfVolume(ctx, grid, layout, spec.truncation_voxels)
+    vol.integrate_batched(st, list(range(4)), views[:4])
+    before = vol.read()
+    chosen = vol.autotune(st, list(range(8)), views, repeats=2)
+    say(f"tsdf autotune: {chosen} bricks per block")
+    assert chosen in (1, 2, 4)
+    after = vol.read()
+    assert np.array_equal(before[0].view(np.uint32), after[0].view(np.uint32))
+    assert np.array_equal(before[1].view(np.uint32), after[1].view(np.uint32))
+    vol.destroy()
+    st.destroy()

@@ -316,3 +316,26 @@ def test_mixed_resolution_batch_matches_oracle(ctx, work):
         say(f"mixed: view {r} ({views[r].width}x{views[r].height}, {len(s)} sources) bit-exact {exact:.6f}")
         assert exact >= 0.99
     st.destroy()
+
+
+def test_room_mixed_batch_matches_oracle(ctx):
+    """The batched room configuration at quarter scale: 64 views at 160x120, 240x135, 320x180 and
+    480x270 with 4-8 neighbours each, all solved in one batch; one view of each size is checked
+    against the oracle."""
+    from lr3d import HipDenseStereo, SolveOptions, scenes
+    w = scenes.room_batched(scale=0.25, iterations=2).render()
+    say(f"case: room batch x0.25, {len(w.views)} views, {w.megapixels:.2f} Mpx, 2 iterations")
+    st = HipDenseStereo(ctx, w.views)
+    for i, im in enumerate(w.images):
+        st.set_image(i, im)
+    st.solve_batched([(r, SolveOptions(sources=w.sources[r], min_depth=w.ranges[r][0], max_depth=w.ranges[r][1],
+                                       iterations=w.iterations, topk=w.topk)) for r in range(len(w.views))])
+    imgs = O.Images(w.images)
+    for r in (0, 1, 2, 3):
+        hyp, _ = st.hypotheses(r)
+        _, ohyp, _ = O.solve(w.views, imgs, r, w.sources[r], *w.ranges[r], iterations=w.iterations, topk=w.topk)
+        exact = np.all(hyp.view(np.uint32) == ohyp.view(np.uint32), axis=1).mean()
+        say(f"room: view {r} ({w.views[r].width}x{w.views[r].height}, {len(w.sources[r][:8])} sources) "
+            f"planes bit-exact {exact:.6f}")
+        assert exact >= 0.99
+    st.destroy()

@@ -63,6 +63,8 @@ def main():
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-profile", action="store_true", help="time without kernel events")
     ap.add_argument("--no-autotune", action="store_true", help="keep the default launch shapes")
+    ap.add_argument("--tiles", default=None,
+                    help="pin launch shapes instead of tuning, e.g. init=16x8,propagate=8x8,pack=64x4,bricks=4")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU only; --gpus must be 1",
@@ -100,13 +102,24 @@ def main():
 
     # launch shapes: timed on this batch before the warm-up (leaves the set and volume unchanged)
     tiles = None
-    if not args.no_autotune:
+    if args.tiles:
+        for item in args.tiles.split(","):
+            k, v = item.split("=")
+            if k == "bricks":
+                vol.bricks_per_block = int(v)
+            else:
+                tx, ty = (int(x) for x in v.split("x"))
+                st.set_tile(k, tx, ty)
+        tiles = {k: list(v) for k, v in st.tiles().items()}
+        tiles["tsdf_bricks_per_block"] = vol.bricks_per_block
+        log(f"pinned launch shapes: {tiles}")
+    elif not args.no_autotune:
         t_tune = time.perf_counter()
-        tiles = st.autotune(solves, repeats=2)
+        tiles = st.autotune(solves, repeats=3)
         st.solve_batched(solves)
         for r in range(n):
             st.filter(r, work.sources[r], needed[r])
-        bricks = vol.autotune(st, list(range(n)), iviews, map=lr3d.MAP_FILTERED, repeats=2)
+        bricks = vol.autotune(st, list(range(n)), iviews, map=lr3d.MAP_FILTERED, repeats=3)
         tiles = {k: list(v) for k, v in tiles.items()}
         tiles["tsdf_bricks_per_block"] = bricks
         log(f"autotune ({time.perf_counter() - t_tune:.1f}s): {tiles}")
@@ -175,6 +188,14 @@ def main():
                     "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s", "frac": round(achieved / FP32_PEAK_TFLOPS, 4),
                     "traffic": None, "avg_launch_ms": round(prop_ms / prop_launches, 3),
                     "flop_per_launch": flops}
+    # HBM bytes per launch measured by rocprofv3 counters for this workload (profiles/traffic.json)
+    measured = {}
+    try:
+        measured = json.load(open(os.path.join(ROOT, "profiles", "traffic.json"))).get(work.name, {})
+    except (OSError, ValueError):
+        pass
+    if roofline is not None:
+        roofline["traffic"] = measured.get("pm_propagate")
     tsdf_launches, tsdf_ms = stats["tsdf_integrate"]
     tsdf = {"voxel_updates_per_step": updates, "voxels": layout[2] * 512}
     if tsdf_launches:
@@ -186,7 +207,7 @@ def main():
                      "roofline": {"kernel": "tsdf_integrate", "bound": "hbm",
                                   "achieved": round(bytes_alg / t_avg / 1e9, 1), "peak": HBM_PEAK_GBS,
                                   "unit": "GB/s", "frac": round(bytes_alg / t_avg / 1e9 / HBM_PEAK_GBS, 4),
-                                  "bytes_per_launch": bytes_alg}})
+                                  "bytes_per_launch": bytes_alg, "traffic": measured.get("tsdf_integrate")}})
 
     plain = None
     if not args.no_plain:

@@ -134,7 +134,7 @@ int32_t lr3d_stereo_set_pose(lr3d_stereo *st, int32_t layer, const double R[9], 
 int32_t lr3d_stereo_write_hyp(lr3d_stereo *st, int32_t layer, const float *hyp);
 
 /* ---- Launch-shape autotuner (no counterpart in the reference) -----------------------------
- * Times the candidate block shapes of init, propagate (first iteration) and pack/consistency on
+ * Times the candidate block shapes of init, propagate (second iteration) and pack/consistency on
  * this batch (`repeats` launches each, 1..16) and keeps the fastest per kernel for later calls on
  * the set.  The set is left as found (planes, costs and packed maps are restored); maps do not
  * depend on the shapes.  kernel names: "init", "propagate", "pack"; shapes are threads (x, y)

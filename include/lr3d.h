@@ -205,6 +205,22 @@ int32_t lr3d_tsdf_autotune(lr3d_tsdf *ts, lr3d_stereo *st, const int32_t *layers
 int32_t lr3d_tsdf_get_bricks_per_block(lr3d_tsdf *ts, int32_t *bricks);
 int32_t lr3d_tsdf_set_bricks_per_block(lr3d_tsdf *ts, int32_t bricks);
 
+/* Overwrite the volume's tsdf and weight pools (n_bricks*512 floats each). */
+int32_t lr3d_tsdf_write(lr3d_tsdf *ts, const float *tsdf, const float *weight);
+
+/* ---- Surface extraction (replaces extractSurface, surface-nets.ts:47-166, with computeNormals,
+ * mesh.ts:9-32; and removeSmallComponents, mesh.ts:38-90) --------------------------------------
+ * The mesh equals the reference's array for array: vertices numbered in pool order, triangles in
+ * voxel / axis order, normals accumulated face by face into floats.  min_weight as the
+ * reference's minWeight (0.6 in tsdf-meshing.ts:23).  Meshes live on the device until read. */
+typedef struct lr3d_mesh lr3d_mesh;
+int32_t lr3d_tsdf_extract_surface(lr3d_tsdf *ts, double min_weight, lr3d_mesh **out);
+int32_t lr3d_mesh_remove_small_components(lr3d_mesh *in, double fraction, int32_t min_faces, lr3d_mesh **out);
+int32_t lr3d_mesh_size(lr3d_mesh *m, int64_t *vertices, int64_t *indices);
+/* positions, normals: vertices*3 floats; indices: indices uint32 (any may be NULL) */
+int32_t lr3d_mesh_read(lr3d_mesh *m, float *positions, float *normals, uint32_t *indices);
+int32_t lr3d_mesh_destroy(lr3d_mesh *m);
+
 /* TsdfVolume.read (TV:270-277): n_bricks*512 floats each (either may be NULL). */
 int32_t lr3d_tsdf_read(lr3d_tsdf *ts, float *tsdf, float *weight);
 /* Voxel updates ((voxel, view) pairs written) of the integrate calls since the last reset. */

@@ -15,11 +15,14 @@
 
 #include <algorithm>
 #include <map>
+#include <memory>
 #include <mutex>
 #include <string>
 #include <vector>
 
 #include "../../include/lr3d.h"
+
+#include <rocprim/device/device_scan.hpp>
 
 using namespace lr3d;
 
@@ -1078,6 +1081,8 @@ struct lr3d_tsdf {
     DevBuf<unsigned long long> counter;
     bool counting = false;
     int32_t bricks_per_block = 4;  // integrate_kernel: wavefronts (bricks) per block; set by the autotuner
+    std::vector<int32_t> coords_host;  // brick grid coordinates (4 per brick), for the mesher's lookup
+    DevBuf<int32_t> lookup;            // grid index -> pool brick or -1 (built on first use)
     // adjoint scratch
     DevBuf<uint32_t> adj_voxels, adj_pixel_off;
     DevBuf<float> adj_targets, adj_tsdf, adj_gdf;
@@ -1110,6 +1115,7 @@ extern "C" int32_t lr3d_tsdf_create(lr3d_ctx *ctx, const double origin[3], doubl
     HIP_TRY(hipSetDevice(ctx->device));
     lr3d_tsdf *ts = new lr3d_tsdf();
     ts->ctx = ctx;
+    ts->coords_host.assign(coords, coords + (size_t)n_bricks * 4);
     for (int a = 0; a < 3; a++) {
         ts->dims[a] = dims[a];
         ts->grid.origin[a] = (float)origin[a];
@@ -1416,6 +1422,219 @@ extern "C" int32_t lr3d_tsdf_autotune(lr3d_tsdf *ts, lr3d_stereo *st, const int3
     HIP_TRY(hipMemcpyAsync(ts->tsdf.ptr, s_tsdf.ptr, voxels * 4, hipMemcpyDeviceToDevice, ctx->stream));
     HIP_TRY(hipMemcpyAsync(ts->weight.ptr, s_weight.ptr, voxels * 4, hipMemcpyDeviceToDevice, ctx->stream));
     return finish(ctx);
+}
+
+extern "C" int32_t lr3d_tsdf_write(lr3d_tsdf *ts, const float *tsdf, const float *weight) {
+    if (!ts || !tsdf || !weight) return fail(LR3D_ERR_INVALID, "lr3d_tsdf_write: NULL argument");
+    lr3d_ctx *ctx = ts->ctx;
+    std::lock_guard<std::mutex> lock(ctx->mu);
+    HIP_TRY(hipSetDevice(ctx->device));
+    const size_t bytes = (size_t)ts->grid.n_bricks * 512u * 4u;
+    HIP_TRY(hipMemcpyAsync(ts->tsdf.ptr, tsdf, bytes, hipMemcpyHostToDevice, ctx->stream));
+    HIP_TRY(hipMemcpyAsync(ts->weight.ptr, weight, bytes, hipMemcpyHostToDevice, ctx->stream));
+    return finish(ctx);
+}
+
+// ---- surface nets and mesh post-processing (SN = surface-nets.ts, MS = mesh.ts) ----------------------
+
+struct lr3d_mesh {
+    lr3d_ctx *ctx = nullptr;
+    DevBuf<float> positions, normals;
+    DevBuf<uint32_t> indices;
+    uint32_t n_vertices = 0, n_indices = 0;
+};
+
+// Device-wide exclusive prefix sum of n uint32 values (rocPRIM); returns the total in *total.
+struct Scanner {
+    DevBuf<uint8_t> tmp;
+    uint32_t last[2] = {0, 0};
+    hipError_t run(hipStream_t s, const uint32_t *in, uint32_t *out, uint32_t n, uint32_t *total) {
+        *total = 0;
+        if (n == 0) return hipSuccess;
+        size_t bytes = 0;
+        hipError_t e = rocprim::exclusive_scan(nullptr, bytes, in, out, 0u, (size_t)n, rocprim::plus<uint32_t>(), s);
+        if (e != hipSuccess) return e;
+        if ((e = tmp.reserve(std::max<size_t>(bytes, 1))) != hipSuccess) return e;
+        e = rocprim::exclusive_scan(tmp.ptr, bytes, in, out, 0u, (size_t)n, rocprim::plus<uint32_t>(), s);
+        if (e != hipSuccess) return e;
+        if ((e = hipMemcpyAsync(&last[0], in + n - 1, 4, hipMemcpyDeviceToHost, s)) != hipSuccess) return e;
+        if ((e = hipMemcpyAsync(&last[1], out + n - 1, 4, hipMemcpyDeviceToHost, s)) != hipSuccess) return e;
+        if ((e = hipStreamSynchronize(s)) != hipSuccess) return e;
+        *total = last[0] + last[1];
+        return hipSuccess;
+    }
+};
+
+// computeNormals (MS:9-32) of a mesh whose positions and indices are set.
+static int32_t mesh_normals(lr3d_mesh *m, Scanner &scan) {
+    hipStream_t s = m->ctx->stream;
+    const uint32_t V = m->n_vertices, N = m->n_indices;
+    HIP_TRY(m->normals.alloc((size_t)V * 3));
+    if (V == 0) return LR3D_OK;
+    DevBuf<uint32_t> deg, off, cursor, adj;
+    HIP_TRY(deg.alloc(V));
+    HIP_TRY(off.alloc(V));
+    HIP_TRY(cursor.alloc(V));
+    HIP_TRY(adj.alloc(std::max<uint32_t>(N, 1)));
+    HIP_TRY(hipMemsetAsync(deg.ptr, 0, (size_t)V * 4, s));
+    HIP_TRY(hipMemsetAsync(cursor.ptr, 0, (size_t)V * 4, s));
+    HIP_TRY(mesh_launch_degree(s, m->indices.ptr, N, deg.ptr));
+    uint32_t total = 0;
+    HIP_TRY(scan.run(s, deg.ptr, off.ptr, V, &total));
+    HIP_TRY(mesh_launch_adjacency(s, m->indices.ptr, N, off.ptr, cursor.ptr, adj.ptr));
+    HIP_TRY(mesh_launch_normal_sum(s, m->positions.ptr, m->indices.ptr, off.ptr, deg.ptr, adj.ptr, V, m->normals.ptr));
+    return finish(m->ctx);
+}
+
+extern "C" int32_t lr3d_tsdf_extract_surface(lr3d_tsdf *ts, double min_weight, lr3d_mesh **out) {
+    if (!ts || !out) return fail(LR3D_ERR_INVALID, "lr3d_tsdf_extract_surface: NULL argument");
+    *out = nullptr;
+    lr3d_ctx *ctx = ts->ctx;
+    std::lock_guard<std::mutex> lock(ctx->mu);
+    HIP_TRY(hipSetDevice(ctx->device));
+    hipStream_t s = ctx->stream;
+    const int64_t grid_cells = (int64_t)ts->dims[0] * ts->dims[1] * ts->dims[2];
+    if (!ts->lookup.ptr) {
+        // SN:67-74's inverse: grid index -> pool brick (BrickGrid.build, TV:54-64)
+        std::vector<int32_t> lookup((size_t)grid_cells, -1);
+        for (int32_t b = 0; b < ts->grid.n_bricks; b++) {
+            const int32_t *c = &ts->coords_host[(size_t)b * 4];
+            lookup[((size_t)c[2] * ts->dims[1] + c[1]) * ts->dims[0] + c[0]] = b;
+        }
+        HIP_TRY(ts->lookup.alloc((size_t)grid_cells));
+        HIP_TRY(hipMemcpyAsync(ts->lookup.ptr, lookup.data(), lookup.size() * 4, hipMemcpyHostToDevice, s));
+        HIP_TRY(hipStreamSynchronize(s));
+    }
+    MeshVolume v;
+    for (int a = 0; a < 3; a++) {
+        v.origin[a] = ts->grid.origin_d[a];
+        v.dims[a] = ts->dims[a];
+    }
+    v.voxel = ts->grid.voxel_d;
+    v.min_weight = min_weight;
+    v.lookup = ts->lookup.ptr;
+    v.coords = ts->coords.ptr;
+    v.tsdf = ts->tsdf.ptr;
+    v.weight = ts->weight.ptr;
+    const uint32_t cells = (uint32_t)((uint64_t)ts->grid.n_bricks * 512u);
+    std::unique_ptr<lr3d_mesh> m(new lr3d_mesh());
+    m->ctx = ctx;
+    Scanner scan;
+    DevBuf<uint32_t> flag, vidx, qcount, qoff;
+    DevBuf<int32_t> cell_vertex;
+    DevBuf<double> pos_d;
+    HIP_TRY(flag.alloc(cells));
+    HIP_TRY(vidx.alloc(cells));
+    HIP_TRY(cell_vertex.alloc(cells));
+    // SN:87-115: active cells, numbered in pool order
+    HIP_TRY(timed(ctx, "mesh_active", [&] { return mesh_launch_active(s, v, cells, flag.ptr); }));
+    uint32_t V = 0;
+    HIP_TRY(scan.run(s, flag.ptr, vidx.ptr, cells, &V));
+    m->n_vertices = V;
+    HIP_TRY(pos_d.alloc((size_t)std::max<uint32_t>(V, 1) * 3));
+    HIP_TRY(m->positions.alloc((size_t)std::max<uint32_t>(V, 1) * 3));
+    HIP_TRY(timed(ctx, "mesh_vertices", [&] {
+        return mesh_launch_vertices(s, v, cells, vidx.ptr, cell_vertex.ptr, pos_d.ptr, m->positions.ptr);
+    }));
+    flag.release();
+    vidx.release();
+    // SN:117-156: quads counted per voxel, written at their sequential offsets
+    HIP_TRY(qcount.alloc(cells));
+    HIP_TRY(qoff.alloc(cells));
+    HIP_TRY(timed(ctx, "mesh_quads", [&] {
+        return mesh_launch_quads(s, v, cells, cell_vertex.ptr, pos_d.ptr, nullptr, qcount.ptr, nullptr);
+    }));
+    uint32_t Q = 0;
+    HIP_TRY(scan.run(s, qcount.ptr, qoff.ptr, cells, &Q));
+    if ((uint64_t)Q * 6 >= (1ull << 32)) return fail(LR3D_ERR_INVALID, "lr3d_tsdf_extract_surface: more than 2^32 indices");
+    m->n_indices = Q * 6;
+    HIP_TRY(m->indices.alloc((size_t)std::max<uint32_t>(m->n_indices, 1)));
+    HIP_TRY(timed(ctx, "mesh_quads", [&] {
+        return mesh_launch_quads(s, v, cells, cell_vertex.ptr, pos_d.ptr, qoff.ptr, nullptr, m->indices.ptr);
+    }));
+    int32_t rc = finish(ctx);
+    if (rc) return rc;
+    rc = mesh_normals(m.get(), scan);
+    if (rc) return rc;
+    *out = m.release();
+    return LR3D_OK;
+}
+
+extern "C" int32_t lr3d_mesh_remove_small_components(lr3d_mesh *in, double fraction, int32_t min_faces,
+                                                     lr3d_mesh **out) {
+    if (!in || !out) return fail(LR3D_ERR_INVALID, "lr3d_mesh_remove_small_components: NULL argument");
+    *out = nullptr;
+    lr3d_ctx *ctx = in->ctx;
+    std::lock_guard<std::mutex> lock(ctx->mu);
+    HIP_TRY(hipSetDevice(ctx->device));
+    hipStream_t s = ctx->stream;
+    const uint32_t V = in->n_vertices, F = in->n_indices / 3;
+    std::unique_ptr<lr3d_mesh> m(new lr3d_mesh());
+    m->ctx = ctx;
+    Scanner scan;
+    DevBuf<uint32_t> parent, changed, faces, keep, koff, first, sflag, rank;
+    HIP_TRY(parent.alloc(std::max<uint32_t>(V, 1)));
+    HIP_TRY(changed.alloc(1));
+    uint32_t changed_host = 0;
+    HIP_TRY(mesh_launch_components(s, in->indices.ptr, F, V, parent.ptr, changed.ptr, &changed_host));
+    HIP_TRY(faces.alloc(std::max<uint32_t>(V, 1)));
+    HIP_TRY(keep.alloc(std::max<uint32_t>(F, 1)));
+    HIP_TRY(koff.alloc(std::max<uint32_t>(F, 1)));
+    HIP_TRY(hipMemsetAsync(faces.ptr, 0, (size_t)std::max<uint32_t>(V, 1) * 4, s));
+    const double limit = std::max((double)min_faces, (double)F * fraction);  // MS:65
+    HIP_TRY(mesh_launch_keep(s, in->indices.ptr, F, parent.ptr, faces.ptr, limit, keep.ptr));
+    uint32_t K = 0;
+    HIP_TRY(scan.run(s, keep.ptr, koff.ptr, F, &K));
+    const uint32_t stream_len = K * 3;
+    HIP_TRY(first.alloc(std::max<uint32_t>(V, 1)));
+    HIP_TRY(sflag.alloc(std::max<uint32_t>(stream_len, 1)));
+    HIP_TRY(rank.alloc(std::max<uint32_t>(stream_len, 1)));
+    HIP_TRY(hipMemsetAsync(first.ptr, 0xff, (size_t)std::max<uint32_t>(V, 1) * 4, s));
+    HIP_TRY(mesh_launch_first_use(s, in->indices.ptr, F, keep.ptr, koff.ptr, first.ptr, sflag.ptr));
+    uint32_t NV = 0;
+    HIP_TRY(scan.run(s, sflag.ptr, rank.ptr, stream_len, &NV));
+    m->n_vertices = NV;
+    m->n_indices = stream_len;
+    HIP_TRY(m->positions.alloc((size_t)std::max<uint32_t>(NV, 1) * 3));
+    HIP_TRY(m->normals.alloc((size_t)std::max<uint32_t>(NV, 1) * 3));
+    HIP_TRY(m->indices.alloc(std::max<uint32_t>(stream_len, 1)));
+    HIP_TRY(mesh_launch_compact(s, in->indices.ptr, F, keep.ptr, koff.ptr, first.ptr, rank.ptr, in->positions.ptr,
+                                in->normals.ptr, m->indices.ptr, m->positions.ptr, m->normals.ptr));
+    int32_t rc = finish(ctx);
+    if (rc) return rc;
+    *out = m.release();
+    return LR3D_OK;
+}
+
+extern "C" int32_t lr3d_mesh_size(lr3d_mesh *m, int64_t *vertices, int64_t *indices) {
+    if (!m || !vertices || !indices) return fail(LR3D_ERR_INVALID, "lr3d_mesh_size: NULL argument");
+    *vertices = m->n_vertices;
+    *indices = m->n_indices;
+    return LR3D_OK;
+}
+
+extern "C" int32_t lr3d_mesh_read(lr3d_mesh *m, float *positions, float *normals, uint32_t *indices) {
+    if (!m) return fail(LR3D_ERR_INVALID, "lr3d_mesh_read: NULL argument");
+    lr3d_ctx *ctx = m->ctx;
+    std::lock_guard<std::mutex> lock(ctx->mu);
+    HIP_TRY(hipSetDevice(ctx->device));
+    if (positions && m->n_vertices)
+        HIP_TRY(hipMemcpyAsync(positions, m->positions.ptr, (size_t)m->n_vertices * 12, hipMemcpyDeviceToHost, ctx->stream));
+    if (normals && m->n_vertices)
+        HIP_TRY(hipMemcpyAsync(normals, m->normals.ptr, (size_t)m->n_vertices * 12, hipMemcpyDeviceToHost, ctx->stream));
+    if (indices && m->n_indices)
+        HIP_TRY(hipMemcpyAsync(indices, m->indices.ptr, (size_t)m->n_indices * 4, hipMemcpyDeviceToHost, ctx->stream));
+    return finish(ctx);
+}
+
+extern "C" int32_t lr3d_mesh_destroy(lr3d_mesh *m) {
+    if (!m) return LR3D_OK;
+    lr3d_ctx *ctx = m->ctx;
+    std::lock_guard<std::mutex> lock(ctx->mu);
+    (void)hipSetDevice(ctx->device);
+    (void)hipStreamSynchronize(ctx->stream);
+    delete m;
+    return LR3D_OK;
 }
 
 extern "C" int32_t lr3d_tsdf_read(lr3d_tsdf *ts, float *tsdf, float *weight) {

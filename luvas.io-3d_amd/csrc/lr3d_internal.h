@@ -148,6 +148,39 @@ hipError_t tsdf_launch_backward(hipStream_t s, const TsdfGridDev &g, const TsdfV
                                 double *grad_depth, double *grad_pose, double *loss_partials);
 hipError_t launch_to_float(hipStream_t s, const double *in, float *out, size_t n);
 
+// Brick TSDF as the surface extractor reads it (the reference's SdfBricks, surface-nets.ts:4-12).
+struct MeshVolume {
+    double origin[3];
+    double voxel;
+    double min_weight;
+    int32_t dims[3];            // grid size in bricks
+    const int32_t *lookup;      // grid index -> pool brick or -1
+    const int4 *coords;         // pool brick -> grid coordinates
+    const float *tsdf, *weight;
+};
+
+hipError_t mesh_launch_active(hipStream_t s, const MeshVolume &v, uint32_t cells, uint32_t *flag);
+hipError_t mesh_launch_vertices(hipStream_t s, const MeshVolume &v, uint32_t cells, const uint32_t *vidx,
+                                int32_t *cell_vertex, double *pos_d, float *pos_f);
+// indices == NULL: count the quads of every voxel into qcount; else write their triangles at qoff
+hipError_t mesh_launch_quads(hipStream_t s, const MeshVolume &v, uint32_t cells, const int32_t *cell_vertex,
+                             const double *pos, const uint32_t *qoff, uint32_t *qcount, uint32_t *indices);
+hipError_t mesh_launch_degree(hipStream_t s, const uint32_t *indices, uint32_t n_indices, uint32_t *deg);
+hipError_t mesh_launch_adjacency(hipStream_t s, const uint32_t *indices, uint32_t n_indices, const uint32_t *off,
+                                 uint32_t *cursor, uint32_t *adj);
+hipError_t mesh_launch_normal_sum(hipStream_t s, const float *pos, const uint32_t *indices, const uint32_t *off,
+                                  const uint32_t *deg, const uint32_t *adj, uint32_t n_vertices, float *normals);
+// union-find labels (synchronises the stream once per round)
+hipError_t mesh_launch_components(hipStream_t s, const uint32_t *indices, uint32_t n_faces, uint32_t n_vertices,
+                                  uint32_t *parent, uint32_t *changed_dev, uint32_t *changed_host);
+hipError_t mesh_launch_keep(hipStream_t s, const uint32_t *indices, uint32_t n_faces, const uint32_t *parent,
+                            uint32_t *faces, double limit, uint32_t *keep);
+hipError_t mesh_launch_first_use(hipStream_t s, const uint32_t *indices, uint32_t n_faces, const uint32_t *keep,
+                                 const uint32_t *koff, uint32_t *first, uint32_t *flag);
+hipError_t mesh_launch_compact(hipStream_t s, const uint32_t *indices, uint32_t n_faces, const uint32_t *keep,
+                               const uint32_t *koff, const uint32_t *first, const uint32_t *rank, const float *pos,
+                               const float *nrm, uint32_t *out_idx, float *out_pos, float *out_nrm);
+
 // Launch shapes (x threads, y threads) picked per (kernel, image size).
 struct TileShape {
     int32_t tx, ty;

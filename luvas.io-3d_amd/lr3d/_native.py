@@ -34,7 +34,8 @@ EXPORTED = (
     "lr3d_tsdf_integrate_batched", "lr3d_tsdf_read", "lr3d_tsdf_set_counting",
     "lr3d_tsdf_update_count", "lr3d_tsdf_backward", "lr3d_stereo_autotune", "lr3d_stereo_get_tile",
     "lr3d_stereo_set_tile", "lr3d_tsdf_autotune", "lr3d_tsdf_get_bricks_per_block",
-    "lr3d_tsdf_set_bricks_per_block",
+    "lr3d_tsdf_set_bricks_per_block", "lr3d_tsdf_write", "lr3d_tsdf_extract_surface",
+    "lr3d_mesh_remove_small_components", "lr3d_mesh_size", "lr3d_mesh_read", "lr3d_mesh_destroy",
 )
 
 
@@ -115,6 +116,12 @@ def load() -> C.CDLL:
         "lr3d_tsdf_autotune": (i32, [P, P, C.c_void_p, i32, C.c_void_p, i32, i32]),
         "lr3d_tsdf_get_bricks_per_block": (i32, [P, C.POINTER(i32)]),
         "lr3d_tsdf_set_bricks_per_block": (i32, [P, i32]),
+        "lr3d_tsdf_write": (i32, [P, C.c_void_p, C.c_void_p]),
+        "lr3d_tsdf_extract_surface": (i32, [P, dbl, C.POINTER(P)]),
+        "lr3d_mesh_remove_small_components": (i32, [P, dbl, i32, C.POINTER(P)]),
+        "lr3d_mesh_size": (i32, [P, C.POINTER(i64), C.POINTER(i64)]),
+        "lr3d_mesh_read": (i32, [P, C.c_void_p, C.c_void_p, C.c_void_p]),
+        "lr3d_mesh_destroy": (i32, [P]),
         "lr3d_tsdf_backward": (i32, [P, P, C.c_void_p, i32, C.c_void_p, i32, C.c_void_p, C.c_void_p, C.c_int64,
                                      C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]),
         "lr3d_tsdf_integrate": (i32, [P, P, i32, i32, C.POINTER(IntegrationView)]),

@@ -189,6 +189,20 @@ class HipTsdfVolume:
         return {"tsdf": tsdf, "grad_depth": [p.reshape(v.height, v.width) for p, v in zip(parts, views)],
                 "grad_pose": gp, "loss": float(loss[0])}
 
+    def write(self, tsdf: np.ndarray, weight: np.ndarray) -> None:
+        """Overwrite the tsdf and weight pools (count * 512 floats each)."""
+        n = self.count * BRICK_VOXELS
+        t = np.ascontiguousarray(tsdf, dtype=np.float32).reshape(n)
+        w = np.ascontiguousarray(weight, dtype=np.float32).reshape(n)
+        N.check(self._lib.lr3d_tsdf_write(self.handle, N.ptr(t), N.ptr(w)))
+
+    def extract_surface(self, min_weight: float = 0.6):
+        """surface-nets.ts extractSurface (with computeNormals) on the GPU; returns a HipMesh."""
+        from .mesh import HipMesh
+        out = C.c_void_p()
+        N.check(self._lib.lr3d_tsdf_extract_surface(self.handle, float(min_weight), C.byref(out)))
+        return HipMesh(out)
+
     def read(self) -> tuple[np.ndarray, np.ndarray]:
         """TV:270-277"""
         n = self.count * BRICK_VOXELS

@@ -166,6 +166,32 @@ int32_t orc_visible_bricks(const orc_tsdf_grid *g, const orc_integration_view *v
 int64_t orc_tsdf_integrate(const orc_tsdf_grid *g, const orc_integration_view *v,
                            const uint32_t *depth, const float *prior, float *tsdf, float *weight);
 
+/* ---- surface nets and mesh post-processing (mesh_oracle.c) ---- */
+typedef struct {
+    double origin[3];
+    double voxel;
+    int32_t dims[3];        /* grid size in bricks */
+    int32_t n_bricks;
+    const int32_t *lookup;  /* dims product: grid index -> pool brick or -1 */
+    const float *tsdf;      /* n_bricks * 512 */
+    const float *weight;
+} orc_sdf_bricks;
+
+typedef struct {
+    float *positions;  /* n_vertices * 3 */
+    float *normals;    /* n_vertices * 3 */
+    uint32_t *indices; /* n_indices */
+    int64_t n_vertices, n_indices;
+} orc_mesh;
+
+/* extractSurface (surface-nets.ts:47-166) with computeNormals (mesh.ts:9-32) */
+int32_t orc_extract_surface(const orc_sdf_bricks *vol, double min_weight, orc_mesh *out);
+/* removeSmallComponents (mesh.ts:38-90) */
+int32_t orc_remove_small_components(const orc_mesh *in, double fraction, int32_t min_faces, orc_mesh *out);
+void orc_compute_normals(const float *positions, const uint32_t *indices, int64_t n_indices,
+                         int64_t n_vertices, float *normals);
+void orc_mesh_free(orc_mesh *m);
+
 /* ---- FP64 TSDF integration adjoint (tsdf_adjoint_oracle.c) ---- */
 typedef struct {
     const double *depth;   /* W*H decoded map depth, 0 where the map has none */

@@ -447,3 +447,70 @@ def tsdf_backward64(grid: Grid, views, decoded, priors, voxels, targets):
     assert rc == 0
     splits = np.split(gd, np.cumsum(sizes)[:-1])
     return {"tsdf": tsdf, "kink": kink, "grad_depth": splits, "grad_pose": gp, "loss": float(loss[0])}
+
+
+# ---- surface nets and mesh post-processing (mesh_oracle.c) ----------------------------------------
+
+class OrcSdfBricks(C.Structure):
+    _fields_ = [("origin", C.c_double * 3), ("voxel", C.c_double), ("dims", C.c_int32 * 3), ("n_bricks", C.c_int32),
+                ("lookup", C.c_void_p), ("tsdf", C.c_void_p), ("weight", C.c_void_p)]
+
+
+class OrcMesh(C.Structure):
+    _fields_ = [("positions", C.POINTER(C.c_float)), ("normals", C.POINTER(C.c_float)),
+                ("indices", C.POINTER(C.c_uint32)), ("n_vertices", C.c_int64), ("n_indices", C.c_int64)]
+
+
+def _mesh_lib():
+    lib = load()
+    if not hasattr(lib, "_mesh"):
+        lib.orc_extract_surface.restype = C.c_int32
+        lib.orc_extract_surface.argtypes = [C.POINTER(OrcSdfBricks), C.c_double, C.POINTER(OrcMesh)]
+        lib.orc_remove_small_components.restype = C.c_int32
+        lib.orc_remove_small_components.argtypes = [C.POINTER(OrcMesh), C.c_double, C.c_int32, C.POINTER(OrcMesh)]
+        lib.orc_mesh_free.restype = None
+        lib.orc_mesh_free.argtypes = [C.POINTER(OrcMesh)]
+        lib._mesh = True
+    return lib
+
+
+def _take(lib, m):
+    nv, ni = m.n_vertices, m.n_indices
+    out = {"positions": np.ctypeslib.as_array(m.positions, (max(nv, 1) * 3,))[:nv * 3].reshape(-1, 3).copy(),
+           "normals": np.ctypeslib.as_array(m.normals, (max(nv, 1) * 3,))[:nv * 3].reshape(-1, 3).copy(),
+           "indices": np.ctypeslib.as_array(m.indices, (max(ni, 1),))[:ni].copy()}
+    return out
+
+
+def extract_surface(origin, voxel, dims, lookup, tsdf, weight, min_weight):
+    """surface-nets.ts extractSurface: dict(positions (V,3) f32, normals (V,3) f32, indices u32)."""
+    lib = _mesh_lib()
+    lookup = np.ascontiguousarray(lookup, dtype=np.int32)
+    tsdf = np.ascontiguousarray(tsdf, dtype=np.float32)
+    weight = np.ascontiguousarray(weight, dtype=np.float32)
+    v = OrcSdfBricks()
+    v.origin[:] = list(origin)
+    v.voxel = voxel
+    v.dims[:] = list(dims)
+    v.n_bricks = tsdf.size // 512
+    v.lookup, v.tsdf, v.weight = lookup.ctypes.data, tsdf.ctypes.data, weight.ctypes.data
+    m = OrcMesh()
+    assert lib.orc_extract_surface(C.byref(v), float(min_weight), C.byref(m)) == 0
+    out = _take(lib, m)
+    lib.orc_mesh_free(C.byref(m))
+    return out
+
+
+def remove_small_components(mesh, fraction, min_faces):
+    """mesh.ts removeSmallComponents on a dict mesh."""
+    lib = _mesh_lib()
+    p = np.ascontiguousarray(mesh["positions"], dtype=np.float32).reshape(-1)
+    n = np.ascontiguousarray(mesh["normals"], dtype=np.float32).reshape(-1)
+    i = np.ascontiguousarray(mesh["indices"], dtype=np.uint32).reshape(-1)
+    m = OrcMesh(p.ctypes.data_as(C.POINTER(C.c_float)), n.ctypes.data_as(C.POINTER(C.c_float)),
+                i.ctypes.data_as(C.POINTER(C.c_uint32)), p.size // 3, i.size)
+    o = OrcMesh()
+    assert lib.orc_remove_small_components(C.byref(m), float(fraction), int(min_faces), C.byref(o)) == 0
+    out = _take(lib, o)
+    lib.orc_mesh_free(C.byref(o))
+    return out

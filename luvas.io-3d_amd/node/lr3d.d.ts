@@ -88,6 +88,9 @@ export declare class HipTsdfVolume {
     read(): Promise<{ tsdf: Float32Array; weight: Float32Array }>;
     backward(layers: number[], views: IntegrationView[], voxels: ArrayLike<number>, targets: ArrayLike<number>,
              map?: number): Promise<TsdfGradients>;
+    /** extractSurface + removeSmallComponents on the GPU (defaults 0.6, 0.01, 200; fraction < 0 keeps all). */
+    extractSurface(minWeight?: number, fraction?: number, minFaces?: number):
+        Promise<{ positions: Float32Array; normals: Float32Array; indices: Uint32Array }>;
     destroy(): void;
 }
 

@@ -242,6 +242,16 @@ class HipTsdfVolume {
     }
 
     /**
+     * extractSurface(read(), minWeight) followed by removeSmallComponents(mesh, fraction, minFaces)
+     * (surface-nets.ts:47-166, mesh.ts:38-90; tsdf-meshing.ts:84-90 uses 0.6, 0.01, 200), on the
+     * GPU: {positions, normals, indices}.  fraction < 0 keeps every component.
+     */
+    async extractSurface(minWeight, fraction, minFaces) {
+        return native.tsdfExtractSurface(this.handle, minWeight === undefined ? 0.6 : minWeight,
+            fraction === undefined ? 0.01 : fraction, minFaces === undefined ? 200 : minFaces);
+    }
+
+    /**
      * Adjoint of integrating `views` (of `layers`' maps, in order, into a fresh volume) at the listed
      * voxels (pool indices) for L = sum (tsdf - target)^2: {tsdf, gradDepth (per view, concatenated),
      * gradPose (per view: dL/dR 9, dL/dt 3), loss}.  The volume is not modified.

@@ -106,7 +106,7 @@ static int read_view(napi_env env, napi_value o, lr3d_view *v) {
 /* ---- async work -------------------------------------------------------------------------------- */
 
 enum { OP_SOLVE, OP_SOLVE_BATCHED, OP_FILTER, OP_INTEGRATE, OP_READ_DEPTH, OP_READ_FILTERED,
-       OP_READ_HYP, OP_READ_TSDF, OP_NCC_BACKWARD, OP_NCC_REFINE, OP_TSDF_BACKWARD };
+       OP_READ_HYP, OP_READ_TSDF, OP_NCC_BACKWARD, OP_NCC_REFINE, OP_TSDF_BACKWARD, OP_MESH };
 
 typedef struct {
     int op;
@@ -133,6 +133,9 @@ typedef struct {
     void *out_c, *out_d, *out_e;
     size_t bytes_c, bytes_d, bytes_e;
     double loss;
+    /* mesh extraction */
+    double min_weight, fraction;
+    int32_t min_faces;
 } job_t;
 
 static void job_free(job_t *j) {
@@ -176,6 +179,28 @@ static void job_execute(napi_env env, void *data) {
         j->status = lr3d_ncc_refine_step(j->st, j->descs, j->n, j->step_depth, j->step_normal, (double *)j->out_a,
                                          (double *)j->out_b, (double *)j->out_c);
         break;
+    case OP_MESH: {
+        /* extractSurface, then removeSmallComponents when fraction >= 0 (tsdf-meshing.ts:84-90) */
+        lr3d_mesh *m = NULL, *c = NULL;
+        j->status = lr3d_tsdf_extract_surface(j->ts, j->min_weight, &m);
+        if (j->status == LR3D_OK && j->fraction >= 0.0) {
+            j->status = lr3d_mesh_remove_small_components(m, j->fraction, j->min_faces, &c);
+            lr3d_mesh_destroy(m);
+            m = c;
+        }
+        int64_t nv = 0, ni = 0;
+        if (j->status == LR3D_OK) j->status = lr3d_mesh_size(m, &nv, &ni);
+        if (j->status == LR3D_OK) {
+            j->bytes_a = j->bytes_b = (size_t)nv * 12u;
+            j->bytes_c = (size_t)ni * 4u;
+            j->out_a = malloc(j->bytes_a + 1);
+            j->out_b = malloc(j->bytes_b + 1);
+            j->out_c = malloc(j->bytes_c + 1);
+            j->status = lr3d_mesh_read(m, (float *)j->out_a, (float *)j->out_b, (uint32_t *)j->out_c);
+        }
+        lr3d_mesh_destroy(m);
+        break;
+    }
     case OP_TSDF_BACKWARD:
         j->status = lr3d_tsdf_backward(j->ts, j->st, j->layers, j->map, j->iviews, j->n, j->voxels, j->targets,
                                        j->n_voxels, (float *)j->out_a, (float *)j->out_b, (double *)j->out_c, &j->loss);
@@ -219,6 +244,11 @@ static void job_complete(napi_env env, napi_status status, void *data) {
             napi_set_named_property(env, result, "gradPose", typed_from(env, j->out_a, j->bytes_a, napi_float64_array, 8));
             napi_set_named_property(env, result, "lossBefore", typed_from(env, j->out_b, j->bytes_b, napi_float64_array, 8));
             napi_set_named_property(env, result, "lossAfter", typed_from(env, j->out_c, j->bytes_c, napi_float64_array, 8));
+        } else if (j->op == OP_MESH) {
+            napi_create_object(env, &result);
+            napi_set_named_property(env, result, "positions", typed_from(env, j->out_a, j->bytes_a, napi_float32_array, 4));
+            napi_set_named_property(env, result, "normals", typed_from(env, j->out_b, j->bytes_b, napi_float32_array, 4));
+            napi_set_named_property(env, result, "indices", typed_from(env, j->out_c, j->bytes_c, napi_uint32_array, 4));
         } else if (j->op == OP_TSDF_BACKWARD) {
             napi_value loss;
             napi_create_object(env, &result);
@@ -667,6 +697,18 @@ static napi_value tsdf_backward(napi_env env, napi_callback_info info) {
     return queue(env, j);
 }
 
+/* tsdfExtractSurface(tsdf, minWeight, fraction, minFaces) -> {positions, normals, indices};
+ * fraction < 0 skips the small-component removal */
+static napi_value tsdf_extract_surface(napi_env env, napi_callback_info info) {
+    ARGS(4);
+    job_t *j = new_job(OP_MESH);
+    j->ts = (lr3d_tsdf *)get_external(env, argv[0]);
+    napi_get_value_double(env, argv[1], &j->min_weight);
+    napi_get_value_double(env, argv[2], &j->fraction);
+    napi_get_value_int32(env, argv[3], &j->min_faces);
+    return queue(env, j);
+}
+
 static napi_value last_error(napi_env env, napi_callback_info info) {
     (void)info;
     napi_value s;
@@ -689,6 +731,7 @@ static napi_value init(napi_env env, napi_value exports) {
         {"tsdfRead", tsdf_read},             {"lastError", last_error},
         {"nccBackward", ncc_backward},       {"nccRefineStep", ncc_refine_step},
         {"stereoSetPose", stereo_set_pose},  {"tsdfBackward", tsdf_backward},
+        {"tsdfExtractSurface", tsdf_extract_surface},
     };
     for (size_t i = 0; i < sizeof(fns) / sizeof(fns[0]); i++) {
         napi_value f;

@@ -51,6 +51,8 @@ async function main() {
     fs.writeFileSync(path.join(dir, 'tsdf_gd.f32'), Buffer.from(tb.gradDepth.buffer));
     fs.writeFileSync(path.join(dir, 'tsdf_gp.f64'), Buffer.from(tb.gradPose.buffer));
     fs.writeFileSync(path.join(dir, 'losses.json'), JSON.stringify({ ncc: f.loss, tsdf: tb.loss }));
+    const mesh = await volume.extractSurface(0.6, 0.01, 200);
+    for (const k of ['positions', 'normals', 'indices']) fs.writeFileSync(path.join(dir, `mesh_${k}.bin`), Buffer.from(mesh[k].buffer));
     const { tsdf, weight } = await volume.read();
     fs.writeFileSync(path.join(dir, 'tsdf.f32'), Buffer.from(tsdf.buffer));
     fs.writeFileSync(path.join(dir, 'weight.f32'), Buffer.from(weight.buffer));

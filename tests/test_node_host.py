@@ -18,7 +18,7 @@ pytestmark = pytest.mark.skipif(NODE is None, reason="node is not installed")
 EXPORTS = ["ctxCreate", "ctxDestroy", "stereoCreate", "stereoDestroy", "stereoSetImage", "stereoSolve",
            "stereoFilter", "stereoReadDepth", "stereoReadFiltered", "stereoReadHyp", "stereoSetCoarse",
            "tsdfCreate", "tsdfDestroy", "tsdfIntegrate", "tsdfRead", "lastError", "nccBackward",
-           "nccRefineStep", "stereoSetPose", "tsdfBackward"]
+           "nccRefineStep", "stereoSetPose", "tsdfBackward", "tsdfExtractSurface"]
 
 
 def node(code, timeout=60):
@@ -118,6 +118,15 @@ def test_node_pipeline_matches_python_host(tmp_path, ctx):
     assert np.allclose(gd, np.fromfile(tmp_path / "tsdf_gd.f32", np.float32), rtol=1e-6, atol=1e-9)
     assert np.allclose(tb["grad_pose"], np.fromfile(tmp_path / "tsdf_gp.f64", np.float64).reshape(8, 12), rtol=1e-10, atol=1e-12)
     assert abs(tb["loss"] - losses["tsdf"]) <= 1e-12 * max(1.0, abs(tb["loss"]))
+    # mesh: the Node call against the Python host's extraction
+    m = vol.extract_surface(0.6)
+    c = m.remove_small_components(0.01, 200)
+    mesh = c.read()
+    for k, dt in (("positions", np.float32), ("normals", np.float32), ("indices", np.uint32)):
+        assert np.array_equal(np.asarray(mesh[k]).reshape(-1).view(np.uint32),
+                              np.fromfile(tmp_path / f"mesh_{k}.bin", dt).view(np.uint32)), k
+    c.destroy()
+    m.destroy()
     tsdf, weight = vol.read()
     assert np.array_equal(np.fromfile(tmp_path / "coords.i32", np.int32), layout[0])
     assert np.array_equal(tsdf.view(np.uint32), np.fromfile(tmp_path / "tsdf.f32", np.uint32))

@@ -221,6 +221,27 @@ int32_t lr3d_mesh_size(lr3d_mesh *m, int64_t *vertices, int64_t *indices);
 int32_t lr3d_mesh_read(lr3d_mesh *m, float *positions, float *normals, uint32_t *indices);
 int32_t lr3d_mesh_destroy(lr3d_mesh *m);
 
+/* ---- Dense-cloud fusion (replaces fuseDepthMaps, depth-fusion.ts:23-74, and PointAccumulator,
+ * point-accumulator.ts) ----------------------------------------------------------------------
+ * Decodes the layers' packed maps (map selector) with the stereo set's cameras, colours each
+ * sample from its view's RGB thumbnail, and averages position, normal and colour per cell of
+ * `cell` metres, growing the cell by 1.4 while more than cell_budget cells are occupied (after
+ * each view).  Per-cell sums equal the reference's (float sums in sample order); points come out in
+ * cell-key order (the reference's follow its hash table), and a coarsening merges cells in key
+ * order (the reference: table order).  final_cell (may be NULL) receives the cell size used. */
+typedef struct {
+    int32_t width, height;
+    const uint8_t *rgb;  /* width*height*3 */
+} lr3d_thumb;
+typedef struct lr3d_cloud lr3d_cloud;
+int32_t lr3d_fuse_depth_maps(lr3d_stereo *st, const int32_t *layers, int32_t map, int32_t n,
+                             const lr3d_thumb *thumbs, double cell, int64_t cell_budget, int32_t min_support,
+                             lr3d_cloud **out, double *final_cell);
+int32_t lr3d_cloud_size(lr3d_cloud *c, int64_t *points);
+/* positions, normals: points*3 floats; colors: points*3 bytes; support: points (any may be NULL) */
+int32_t lr3d_cloud_read(lr3d_cloud *c, float *positions, float *normals, uint8_t *colors, uint16_t *support);
+int32_t lr3d_cloud_destroy(lr3d_cloud *c);
+
 /* TsdfVolume.read (TV:270-277): n_bricks*512 floats each (either may be NULL). */
 int32_t lr3d_tsdf_read(lr3d_tsdf *ts, float *tsdf, float *weight);
 /* Voxel updates ((voxel, view) pairs written) of the integrate calls since the last reset. */

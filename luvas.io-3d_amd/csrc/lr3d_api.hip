@@ -22,6 +22,7 @@
 
 #include "../../include/lr3d.h"
 
+#include <rocprim/device/device_radix_sort.hpp>
 #include <rocprim/device/device_scan.hpp>
 
 using namespace lr3d;
@@ -1603,6 +1604,251 @@ extern "C" int32_t lr3d_mesh_remove_small_components(lr3d_mesh *in, double fract
     int32_t rc = finish(ctx);
     if (rc) return rc;
     *out = m.release();
+    return LR3D_OK;
+}
+
+// ---- dense-cloud fusion (DF = depth-fusion.ts, PA = point-accumulator.ts) ---------------------------
+
+struct lr3d_cloud {
+    lr3d_ctx *ctx = nullptr;
+    DevBuf<float> positions, normals;
+    DevBuf<uint8_t> colors;
+    DevBuf<uint16_t> support;
+    uint32_t n = 0;
+};
+
+// Stable LSD radix sort of (63-bit key, value) pairs.
+static hipError_t sort_pairs(hipStream_t s, DevBuf<uint8_t> &tmp, const uint64_t *kin, uint64_t *kout,
+                             const uint32_t *vin, uint32_t *vout, uint32_t n) {
+    if (n == 0) return hipSuccess;
+    size_t bytes = 0;
+    hipError_t e = rocprim::radix_sort_pairs(nullptr, bytes, kin, kout, vin, vout, (size_t)n, 0, 63, s);
+    if (e != hipSuccess) return e;
+    if ((e = tmp.reserve(std::max<size_t>(bytes, 1))) != hipSuccess) return e;
+    return rocprim::radix_sort_pairs(tmp.ptr, bytes, kin, kout, vin, vout, (size_t)n, 0, 63, s);
+}
+
+__global__ void iota_u32_kernel(uint32_t *a, uint32_t n) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) a[i] = i;
+}
+
+// The accumulator's cells (sorted by key): keys, 9 float sums, counts.
+struct CellSet {
+    DevBuf<uint64_t> keys;
+    DevBuf<float> sums;
+    DevBuf<uint32_t> counts;
+    uint32_t n = 0;
+    hipError_t alloc(uint32_t count) {
+        hipError_t e = keys.alloc(std::max<uint32_t>(count, 1));
+        if (e == hipSuccess) e = sums.alloc((size_t)std::max<uint32_t>(count, 1) * 9);
+        if (e == hipSuccess) e = counts.alloc(std::max<uint32_t>(count, 1));
+        n = count;
+        return e;
+    }
+};
+
+extern "C" int32_t lr3d_fuse_depth_maps(lr3d_stereo *st, const int32_t *layers, int32_t map, int32_t n,
+                                        const lr3d_thumb *thumbs, double cell, int64_t cell_budget,
+                                        int32_t min_support, lr3d_cloud **out, double *final_cell) {
+    if (!st || !layers || !thumbs || !out) return fail(LR3D_ERR_INVALID, "lr3d_fuse_depth_maps: NULL argument");
+    *out = nullptr;
+    if (n <= 0) return fail(LR3D_ERR_INVALID, "lr3d_fuse_depth_maps: n must be > 0");
+    if (!(cell > 0.0)) return fail(LR3D_ERR_INVALID, "lr3d_fuse_depth_maps: cell must be > 0");
+    if (cell_budget < 1) return fail(LR3D_ERR_INVALID, "lr3d_fuse_depth_maps: cell_budget must be >= 1");
+    if (map != LR3D_MAP_RAW && map != LR3D_MAP_FILTERED)
+        return fail(LR3D_ERR_INVALID, "lr3d_fuse_depth_maps: unknown map selector %d", map);
+    size_t thumb_bytes = 0;
+    for (int32_t i = 0; i < n; i++) {
+        int32_t rc = check_layer(st, layers[i], "lr3d_fuse_depth_maps");
+        if (rc) return rc;
+        if (thumbs[i].width <= 0 || thumbs[i].height <= 0 || !thumbs[i].rgb)
+            return fail(LR3D_ERR_INVALID, "lr3d_fuse_depth_maps: thumbnail %d is empty", i);
+        thumb_bytes += (size_t)thumbs[i].width * thumbs[i].height * 3;
+    }
+    lr3d_ctx *ctx = st->ctx;
+    std::lock_guard<std::mutex> lock(ctx->mu);
+    HIP_TRY(hipSetDevice(ctx->device));
+    hipStream_t s = ctx->stream;
+    const uint32_t *maps = map == LR3D_MAP_RAW ? st->depth.ptr : st->filt.ptr;
+    DevBuf<uint8_t> dthumbs, sort_tmp;
+    HIP_TRY(dthumbs.alloc(thumb_bytes));
+    std::vector<FusionView> fv(n);
+    uint64_t toff = 0;
+    for (int32_t i = 0; i < n; i++) {
+        const lr3d_view &v = st->views[layers[i]];
+        FusionView &f = fv[i];
+        f.W = v.width;
+        f.H = v.height;
+        f.k[0] = v.fx;
+        f.k[1] = v.fy;
+        f.k[2] = v.cx;
+        f.k[3] = v.cy;
+        for (int j = 0; j < 9; j++) f.R[j] = v.R[j];
+        for (int j = 0; j < 3; j++) f.t[j] = v.t[j];
+        f.map_off = st->off[layers[i]];
+        f.tw = thumbs[i].width;
+        f.th = thumbs[i].height;
+        f.thumb_off = toff;
+        const size_t bytes = (size_t)f.tw * f.th * 3;
+        HIP_TRY(hipMemcpyAsync(dthumbs.ptr + toff, thumbs[i].rgb, bytes, hipMemcpyHostToDevice, s));
+        toff += bytes;
+    }
+    DevBuf<uint32_t> overflow;
+    HIP_TRY(overflow.alloc(1));
+    HIP_TRY(hipMemsetAsync(overflow.ptr, 0, 4, s));
+    Scanner scan;
+    CellSet state;
+    HIP_TRY(state.alloc(0));
+    for (int32_t l = 0; l < n; l++) {
+        const FusionView &v = fv[l];
+        const uint32_t P = (uint32_t)(v.W * v.H);
+        DevBuf<uint32_t> flag, pos;
+        HIP_TRY(flag.alloc(P));
+        HIP_TRY(pos.alloc(P));
+        HIP_TRY(fusion_launch_valid(s, v, maps, flag.ptr));
+        uint32_t M = 0;
+        HIP_TRY(scan.run(s, flag.ptr, pos.ptr, P, &M));
+        if (M > 0) {
+            // DF:52-71 + PA:37-48: samples keyed by cell, grouped by a stable sort (pixel order kept)
+            DevBuf<uint64_t> keys, skeys;
+            DevBuf<uint32_t> idx, sidx, head, seg, start;
+            HIP_TRY(keys.alloc(M));
+            HIP_TRY(skeys.alloc(M));
+            HIP_TRY(idx.alloc(M));
+            HIP_TRY(sidx.alloc(M));
+            HIP_TRY(fusion_launch_keys(s, v, maps, dthumbs.ptr, flag.ptr, pos.ptr, cell, keys.ptr, idx.ptr, overflow.ptr));
+            HIP_TRY(sort_pairs(s, sort_tmp, keys.ptr, skeys.ptr, idx.ptr, sidx.ptr, M));
+            HIP_TRY(head.alloc(M));
+            HIP_TRY(seg.alloc(M));
+            HIP_TRY(fusion_launch_heads(s, skeys.ptr, M, head.ptr));
+            uint32_t G = 0;
+            HIP_TRY(scan.run(s, head.ptr, seg.ptr, M, &G));
+            HIP_TRY(start.alloc(G));
+            HIP_TRY(fusion_launch_starts(s, head.ptr, seg.ptr, M, start.ptr));
+            DevBuf<int32_t> exist;
+            DevBuf<uint32_t> is_new, new_pos;
+            HIP_TRY(exist.alloc(G));
+            HIP_TRY(is_new.alloc(G));
+            HIP_TRY(new_pos.alloc(G));
+            HIP_TRY(fusion_launch_lookup(s, skeys.ptr, start.ptr, G, state.keys.ptr, state.n, exist.ptr, is_new.ptr));
+            uint32_t Nn = 0;
+            HIP_TRY(scan.run(s, is_new.ptr, new_pos.ptr, G, &Nn));
+            // merged set: the old cells, then the new ones, re-sorted by key below
+            CellSet cat;
+            HIP_TRY(cat.alloc(state.n + Nn));
+            if (state.n) {
+                HIP_TRY(hipMemcpyAsync(cat.keys.ptr, state.keys.ptr, (size_t)state.n * 8, hipMemcpyDeviceToDevice, s));
+                HIP_TRY(hipMemcpyAsync(cat.sums.ptr, state.sums.ptr, (size_t)state.n * 36, hipMemcpyDeviceToDevice, s));
+                HIP_TRY(hipMemcpyAsync(cat.counts.ptr, state.counts.ptr, (size_t)state.n * 4, hipMemcpyDeviceToDevice, s));
+            }
+            // PA:49-62: running sums of existing cells updated in place, new cells appended
+            HIP_TRY(fusion_launch_reduce(s, v, maps, dthumbs.ptr, skeys.ptr, sidx.ptr, start.ptr, G, M, exist.ptr,
+                                         new_pos.ptr, cat.sums.ptr, cat.counts.ptr, cat.keys.ptr + state.n,
+                                         cat.sums.ptr + (size_t)state.n * 9, cat.counts.ptr + state.n));
+            const uint32_t T = state.n + Nn;
+            DevBuf<uint64_t> tk;
+            DevBuf<uint32_t> perm0, perm;
+            HIP_TRY(tk.alloc(T));
+            HIP_TRY(perm0.alloc(T));
+            HIP_TRY(perm.alloc(T));
+            (void)hipGetLastError();
+            hipLaunchKernelGGL(iota_u32_kernel, dim3((T + 255) / 256), dim3(256), 0, s, perm0.ptr, T);
+            HIP_TRY(hipGetLastError());
+            HIP_TRY(sort_pairs(s, sort_tmp, cat.keys.ptr, tk.ptr, perm0.ptr, perm.ptr, T));
+            CellSet next;
+            HIP_TRY(next.alloc(T));
+            HIP_TRY(fusion_launch_gather(s, perm.ptr, T, cat.keys.ptr, cat.sums.ptr, cat.counts.ptr, next.keys.ptr,
+                                         next.sums.ptr, next.counts.ptr));
+            std::swap(state.keys, next.keys);
+            std::swap(state.sums, next.sums);
+            std::swap(state.counts, next.counts);
+            state.n = T;
+        }
+        // DF:36-37 / PA:98-131: coarsen by 1.4 while the cloud is over budget
+        while ((int64_t)state.n > cell_budget) {
+            cell = cell * 1.4;
+            const uint32_t S = state.n;
+            DevBuf<uint64_t> keys, skeys;
+            DevBuf<uint32_t> idx, sidx, head, seg, start;
+            HIP_TRY(keys.alloc(S));
+            HIP_TRY(skeys.alloc(S));
+            HIP_TRY(idx.alloc(S));
+            HIP_TRY(sidx.alloc(S));
+            HIP_TRY(head.alloc(S));
+            HIP_TRY(seg.alloc(S));
+            HIP_TRY(fusion_launch_rekey(s, state.sums.ptr, state.counts.ptr, S, cell, keys.ptr, idx.ptr, overflow.ptr));
+            HIP_TRY(sort_pairs(s, sort_tmp, keys.ptr, skeys.ptr, idx.ptr, sidx.ptr, S));
+            HIP_TRY(fusion_launch_heads(s, skeys.ptr, S, head.ptr));
+            uint32_t G = 0;
+            HIP_TRY(scan.run(s, head.ptr, seg.ptr, S, &G));
+            HIP_TRY(start.alloc(G));
+            HIP_TRY(fusion_launch_starts(s, head.ptr, seg.ptr, S, start.ptr));
+            CellSet next;
+            HIP_TRY(next.alloc(G));
+            HIP_TRY(fusion_launch_merge(s, skeys.ptr, sidx.ptr, start.ptr, G, S, state.sums.ptr, state.counts.ptr,
+                                        next.keys.ptr, next.sums.ptr, next.counts.ptr));
+            std::swap(state.keys, next.keys);
+            std::swap(state.sums, next.sums);
+            std::swap(state.counts, next.counts);
+            state.n = G;
+            if (G == S && cell > 1e300) break;  // cannot merge further
+        }
+    }
+    uint32_t ovf = 0;
+    HIP_TRY(hipMemcpyAsync(&ovf, overflow.ptr, 4, hipMemcpyDeviceToHost, s));
+    int32_t rc = finish(ctx);
+    if (rc) return rc;
+    if (ovf) return fail(LR3D_ERR_INVALID, "lr3d_fuse_depth_maps: a point lies more than 2^20 cells from the origin");
+    // PA:108-143 cloud(minSupport), in key order
+    std::unique_ptr<lr3d_cloud> c(new lr3d_cloud());
+    c->ctx = ctx;
+    DevBuf<uint32_t> flag, pos;
+    HIP_TRY(flag.alloc(std::max<uint32_t>(state.n, 1)));
+    HIP_TRY(pos.alloc(std::max<uint32_t>(state.n, 1)));
+    HIP_TRY(fusion_launch_support(s, state.counts.ptr, state.n, (uint32_t)std::max(min_support, 0), flag.ptr));
+    uint32_t total = 0;
+    HIP_TRY(scan.run(s, flag.ptr, pos.ptr, state.n, &total));
+    c->n = total;
+    HIP_TRY(c->positions.alloc((size_t)std::max<uint32_t>(total, 1) * 3));
+    HIP_TRY(c->normals.alloc((size_t)std::max<uint32_t>(total, 1) * 3));
+    HIP_TRY(c->colors.alloc((size_t)std::max<uint32_t>(total, 1) * 3));
+    HIP_TRY(c->support.alloc(std::max<uint32_t>(total, 1)));
+    HIP_TRY(fusion_launch_cloud(s, state.sums.ptr, state.counts.ptr, state.n, flag.ptr, pos.ptr, c->positions.ptr,
+                                c->normals.ptr, c->colors.ptr, c->support.ptr));
+    rc = finish(ctx);
+    if (rc) return rc;
+    if (final_cell) *final_cell = cell;
+    *out = c.release();
+    return LR3D_OK;
+}
+
+extern "C" int32_t lr3d_cloud_size(lr3d_cloud *c, int64_t *points) {
+    if (!c || !points) return fail(LR3D_ERR_INVALID, "lr3d_cloud_size: NULL argument");
+    *points = c->n;
+    return LR3D_OK;
+}
+
+extern "C" int32_t lr3d_cloud_read(lr3d_cloud *c, float *positions, float *normals, uint8_t *colors, uint16_t *support) {
+    if (!c) return fail(LR3D_ERR_INVALID, "lr3d_cloud_read: NULL argument");
+    lr3d_ctx *ctx = c->ctx;
+    std::lock_guard<std::mutex> lock(ctx->mu);
+    HIP_TRY(hipSetDevice(ctx->device));
+    const size_t n = c->n;
+    if (n && positions) HIP_TRY(hipMemcpyAsync(positions, c->positions.ptr, n * 12, hipMemcpyDeviceToHost, ctx->stream));
+    if (n && normals) HIP_TRY(hipMemcpyAsync(normals, c->normals.ptr, n * 12, hipMemcpyDeviceToHost, ctx->stream));
+    if (n && colors) HIP_TRY(hipMemcpyAsync(colors, c->colors.ptr, n * 3, hipMemcpyDeviceToHost, ctx->stream));
+    if (n && support) HIP_TRY(hipMemcpyAsync(support, c->support.ptr, n * 2, hipMemcpyDeviceToHost, ctx->stream));
+    return finish(ctx);
+}
+
+extern "C" int32_t lr3d_cloud_destroy(lr3d_cloud *c) {
+    if (!c) return LR3D_OK;
+    lr3d_ctx *ctx = c->ctx;
+    std::lock_guard<std::mutex> lock(ctx->mu);
+    (void)hipSetDevice(ctx->device);
+    (void)hipStreamSynchronize(ctx->stream);
+    delete c;
     return LR3D_OK;
 }
 

@@ -181,6 +181,40 @@ hipError_t mesh_launch_compact(hipStream_t s, const uint32_t *indices, uint32_t 
                                const uint32_t *koff, const uint32_t *first, const uint32_t *rank, const float *pos,
                                const float *nrm, uint32_t *out_idx, float *out_pos, float *out_nrm);
 
+// One view of a dense-cloud fusion pass (DF:44-74): camera in double (JavaScript numbers).
+struct FusionView {
+    int32_t W, H;
+    double k[4];
+    double R[9], t[3];
+    uint32_t map_off;      // pixel offset of the layer's packed map
+    int32_t tw, th;        // thumbnail size
+    uint64_t thumb_off;    // byte offset of the thumbnail (RGB)
+};
+
+hipError_t fusion_launch_valid(hipStream_t s, const FusionView &v, const uint32_t *maps, uint32_t *flag);
+hipError_t fusion_launch_keys(hipStream_t s, const FusionView &v, const uint32_t *maps, const uint8_t *thumbs,
+                              const uint32_t *flag, const uint32_t *pos, double cell, uint64_t *keys, uint32_t *idx,
+                              uint32_t *overflow);
+hipError_t fusion_launch_heads(hipStream_t s, const uint64_t *keys, uint32_t n, uint32_t *head);
+hipError_t fusion_launch_starts(hipStream_t s, const uint32_t *head, const uint32_t *seg, uint32_t n, uint32_t *start);
+hipError_t fusion_launch_lookup(hipStream_t s, const uint64_t *keys, const uint32_t *start, uint32_t segments,
+                                const uint64_t *state_keys, uint32_t state_n, int32_t *exist, uint32_t *is_new);
+hipError_t fusion_launch_reduce(hipStream_t s, const FusionView &v, const uint32_t *maps, const uint8_t *thumbs,
+                                const uint64_t *keys, const uint32_t *idx, const uint32_t *start, uint32_t segments,
+                                uint32_t n, const int32_t *exist, const uint32_t *new_pos, float *state_sums,
+                                uint32_t *state_counts, uint64_t *new_keys, float *new_sums, uint32_t *new_counts);
+hipError_t fusion_launch_gather(hipStream_t s, const uint32_t *perm, uint32_t n, const uint64_t *keys, const float *sums,
+                                const uint32_t *counts, uint64_t *out_keys, float *out_sums, uint32_t *out_counts);
+hipError_t fusion_launch_rekey(hipStream_t s, const float *sums, const uint32_t *counts, uint32_t n, double cell,
+                               uint64_t *keys, uint32_t *idx, uint32_t *overflow);
+hipError_t fusion_launch_merge(hipStream_t s, const uint64_t *keys, const uint32_t *idx, const uint32_t *start,
+                               uint32_t segments, uint32_t n, const float *sums, const uint32_t *counts,
+                               uint64_t *out_keys, float *out_sums, uint32_t *out_counts);
+hipError_t fusion_launch_support(hipStream_t s, const uint32_t *counts, uint32_t n, uint32_t min_support, uint32_t *flag);
+hipError_t fusion_launch_cloud(hipStream_t s, const float *sums, const uint32_t *counts, uint32_t n, const uint32_t *flag,
+                               const uint32_t *pos, float *positions, float *normals, uint8_t *colors,
+                               uint16_t *support);
+
 // Launch shapes (x threads, y threads) picked per (kernel, image size).
 struct TileShape {
     int32_t tx, ty;

@@ -295,6 +295,7 @@ inline dim3 grid_for(uint64_t n) { return dim3((unsigned)((n + 255) / 256)); }
 }  // namespace
 
 hipError_t mesh_launch_active(hipStream_t s, const MeshVolume &v, uint32_t cells, uint32_t *flag) {
+    (void)hipGetLastError();  // report this launch's status, not an earlier call's
     if (cells == 0) return hipSuccess;
     hipLaunchKernelGGL(active_kernel, grid_for(cells), dim3(256), 0, s, v, cells, flag);
     return hipGetLastError();
@@ -302,12 +303,14 @@ hipError_t mesh_launch_active(hipStream_t s, const MeshVolume &v, uint32_t cells
 
 hipError_t mesh_launch_vertices(hipStream_t s, const MeshVolume &v, uint32_t cells, const uint32_t *vidx,
                                 int32_t *cell_vertex, double *pos_d, float *pos_f) {
+    (void)hipGetLastError();  // report this launch's status, not an earlier call's
     hipLaunchKernelGGL(vertex_kernel, grid_for(cells), dim3(256), 0, s, v, cells, vidx, cell_vertex, pos_d, pos_f);
     return hipGetLastError();
 }
 
 hipError_t mesh_launch_quads(hipStream_t s, const MeshVolume &v, uint32_t cells, const int32_t *cell_vertex,
                              const double *pos, const uint32_t *qoff, uint32_t *qcount, uint32_t *indices) {
+    (void)hipGetLastError();  // report this launch's status, not an earlier call's
     if (!indices)
         hipLaunchKernelGGL(quad_count_kernel, grid_for(cells), dim3(256), 0, s, v, cells, cell_vertex, qcount);
     else
@@ -316,12 +319,14 @@ hipError_t mesh_launch_quads(hipStream_t s, const MeshVolume &v, uint32_t cells,
 }
 
 hipError_t mesh_launch_degree(hipStream_t s, const uint32_t *indices, uint32_t n_indices, uint32_t *deg) {
+    (void)hipGetLastError();  // report this launch's status, not an earlier call's
     if (n_indices) hipLaunchKernelGGL(degree_kernel, grid_for(n_indices), dim3(256), 0, s, indices, n_indices, deg);
     return hipGetLastError();
 }
 
 hipError_t mesh_launch_adjacency(hipStream_t s, const uint32_t *indices, uint32_t n_indices, const uint32_t *off,
                                  uint32_t *cursor, uint32_t *adj) {
+    (void)hipGetLastError();  // report this launch's status, not an earlier call's
     if (n_indices)
         hipLaunchKernelGGL(adjacency_kernel, grid_for(n_indices), dim3(256), 0, s, indices, n_indices, off, cursor, adj);
     return hipGetLastError();
@@ -329,6 +334,7 @@ hipError_t mesh_launch_adjacency(hipStream_t s, const uint32_t *indices, uint32_
 
 hipError_t mesh_launch_normal_sum(hipStream_t s, const float *pos, const uint32_t *indices, const uint32_t *off,
                                   const uint32_t *deg, const uint32_t *adj, uint32_t n_vertices, float *normals) {
+    (void)hipGetLastError();  // report this launch's status, not an earlier call's
     if (n_vertices)
         hipLaunchKernelGGL(normal_kernel, grid_for(n_vertices), dim3(256), 0, s, pos, indices, off, deg, adj, n_vertices,
                            normals);
@@ -337,6 +343,7 @@ hipError_t mesh_launch_normal_sum(hipStream_t s, const float *pos, const uint32_
 
 hipError_t mesh_launch_components(hipStream_t s, const uint32_t *indices, uint32_t n_faces, uint32_t n_vertices,
                                   uint32_t *parent, uint32_t *changed_dev, uint32_t *changed_host) {
+    (void)hipGetLastError();  // report this launch's status, not an earlier call's
     if (n_vertices == 0) return hipSuccess;
     hipLaunchKernelGGL(iota_kernel, grid_for(n_vertices), dim3(256), 0, s, parent, n_vertices);
     hipError_t e = hipGetLastError();
@@ -354,6 +361,7 @@ hipError_t mesh_launch_components(hipStream_t s, const uint32_t *indices, uint32
 
 hipError_t mesh_launch_keep(hipStream_t s, const uint32_t *indices, uint32_t n_faces, const uint32_t *parent,
                             uint32_t *faces, double limit, uint32_t *keep) {
+    (void)hipGetLastError();  // report this launch's status, not an earlier call's
     if (n_faces == 0) return hipSuccess;
     hipLaunchKernelGGL(component_faces_kernel, grid_for(n_faces), dim3(256), 0, s, indices, n_faces, parent, faces);
     hipLaunchKernelGGL(keep_kernel, grid_for(n_faces), dim3(256), 0, s, indices, n_faces, parent, faces, limit, keep);
@@ -362,6 +370,7 @@ hipError_t mesh_launch_keep(hipStream_t s, const uint32_t *indices, uint32_t n_f
 
 hipError_t mesh_launch_first_use(hipStream_t s, const uint32_t *indices, uint32_t n_faces, const uint32_t *keep,
                                  const uint32_t *koff, uint32_t *first, uint32_t *flag) {
+    (void)hipGetLastError();  // report this launch's status, not an earlier call's
     if (n_faces == 0) return hipSuccess;
     hipLaunchKernelGGL(first_use_kernel, grid_for(n_faces), dim3(256), 0, s, indices, n_faces, keep, koff, first);
     hipLaunchKernelGGL(first_flag_kernel, grid_for(n_faces), dim3(256), 0, s, indices, n_faces, keep, koff, first, flag);
@@ -371,6 +380,7 @@ hipError_t mesh_launch_first_use(hipStream_t s, const uint32_t *indices, uint32_
 hipError_t mesh_launch_compact(hipStream_t s, const uint32_t *indices, uint32_t n_faces, const uint32_t *keep,
                                const uint32_t *koff, const uint32_t *first, const uint32_t *rank, const float *pos,
                                const float *nrm, uint32_t *out_idx, float *out_pos, float *out_nrm) {
+    (void)hipGetLastError();  // report this launch's status, not an earlier call's
     if (n_faces == 0) return hipSuccess;
     hipLaunchKernelGGL(compact_kernel, grid_for(n_faces), dim3(256), 0, s, indices, n_faces, keep, koff, first, rank,
                        pos, nrm, out_idx, out_pos, out_nrm);

@@ -13,9 +13,10 @@ from .stereo import HipDenseStereo, SolveOptions
 from .tsdf import BRICK, BrickGrid, HipTsdfVolume, IntegrationView, dense_grid
 from . import refine
 from .mesh import HipMesh
+from .fusion import fuse_depth_maps
 
 __all__ = [
     "Lr3dError", "MAP_FILTERED", "MAP_RAW", "MAX_SOURCES", "Intrinsics", "Pose", "StereoView",
     "look_at", "relative_pose", "scale_intrinsics", "Context", "HipDenseStereo", "SolveOptions",
-    "BRICK", "BrickGrid", "HipTsdfVolume", "IntegrationView", "dense_grid", "refine", "HipMesh",
+    "BRICK", "BrickGrid", "HipTsdfVolume", "IntegrationView", "dense_grid", "refine", "HipMesh", "fuse_depth_maps",
 ]

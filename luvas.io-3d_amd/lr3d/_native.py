@@ -36,6 +36,7 @@ EXPORTED = (
     "lr3d_stereo_set_tile", "lr3d_tsdf_autotune", "lr3d_tsdf_get_bricks_per_block",
     "lr3d_tsdf_set_bricks_per_block", "lr3d_tsdf_write", "lr3d_tsdf_extract_surface",
     "lr3d_mesh_remove_small_components", "lr3d_mesh_size", "lr3d_mesh_read", "lr3d_mesh_destroy",
+    "lr3d_fuse_depth_maps", "lr3d_cloud_size", "lr3d_cloud_read", "lr3d_cloud_destroy",
 )
 
 
@@ -122,6 +123,11 @@ def load() -> C.CDLL:
         "lr3d_mesh_size": (i32, [P, C.POINTER(i64), C.POINTER(i64)]),
         "lr3d_mesh_read": (i32, [P, C.c_void_p, C.c_void_p, C.c_void_p]),
         "lr3d_mesh_destroy": (i32, [P]),
+        "lr3d_fuse_depth_maps": (i32, [P, C.c_void_p, i32, i32, C.c_void_p, dbl, i64, i32, C.POINTER(P),
+                                       C.POINTER(dbl)]),
+        "lr3d_cloud_size": (i32, [P, C.POINTER(i64)]),
+        "lr3d_cloud_read": (i32, [P, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]),
+        "lr3d_cloud_destroy": (i32, [P]),
         "lr3d_tsdf_backward": (i32, [P, P, C.c_void_p, i32, C.c_void_p, i32, C.c_void_p, C.c_void_p, C.c_int64,
                                      C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]),
         "lr3d_tsdf_integrate": (i32, [P, P, i32, i32, C.POINTER(IntegrationView)]),

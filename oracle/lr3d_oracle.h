@@ -192,6 +192,36 @@ void orc_compute_normals(const float *positions, const uint32_t *indices, int64_
                          int64_t n_vertices, float *normals);
 void orc_mesh_free(orc_mesh *m);
 
+/* ---- dense-cloud fusion (fusion_oracle.c) ---- */
+typedef struct {
+    int32_t width, height;
+    double fx, fy, cx, cy;
+    double R[9], t[3];       /* world -> camera */
+    const uint32_t *packed;  /* width*height packed map */
+    int32_t thumb_width, thumb_height;
+    const uint8_t *rgb;      /* thumb_width*thumb_height*3 */
+} orc_fusion_view;
+
+typedef struct {
+    int64_t n;
+    float *positions, *normals;
+    uint8_t *colors;
+    uint16_t *support;
+} orc_cloud;
+
+/* depth-maps.ts:65-85: out = depth, nx, ny, nz (camera frame); 0 for an empty sample */
+int32_t orc_unpack_depth_sample(uint32_t value, double out[4]);
+/* fuseDepthMaps (depth-fusion.ts:23-74) with PointAccumulator (point-accumulator.ts), cloud(min_support) */
+int32_t orc_fuse_depth_maps(const orc_fusion_view *views, int32_t n_views, double cell, int64_t cell_budget,
+                            int32_t min_support, orc_cloud *out, double *final_cell);
+void orc_cloud_free(orc_cloud *c);
+void *orc_acc_create(double cell);
+void orc_acc_add(void *acc, const double v[9]);
+void orc_acc_coarsen(void *acc, double factor);
+int64_t orc_acc_size(void *acc);
+int64_t orc_acc_cloud(void *acc, int32_t min_support, float *positions, uint8_t *colors, uint16_t *support);
+void orc_acc_destroy(void *acc);
+
 /* ---- FP64 TSDF integration adjoint (tsdf_adjoint_oracle.c) ---- */
 typedef struct {
     const double *depth;   /* W*H decoded map depth, 0 where the map has none */

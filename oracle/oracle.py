@@ -514,3 +514,51 @@ def remove_small_components(mesh, fraction, min_faces):
     out = _take(lib, o)
     lib.orc_mesh_free(C.byref(o))
     return out
+
+
+# ---- dense-cloud fusion (fusion_oracle.c) -------------------------------------------------------
+
+class OrcFusionView(C.Structure):
+    _fields_ = [("width", C.c_int32), ("height", C.c_int32), ("fx", C.c_double), ("fy", C.c_double),
+                ("cx", C.c_double), ("cy", C.c_double), ("R", C.c_double * 9), ("t", C.c_double * 3),
+                ("packed", C.c_void_p), ("thumb_width", C.c_int32), ("thumb_height", C.c_int32), ("rgb", C.c_void_p)]
+
+
+class OrcCloud(C.Structure):
+    _fields_ = [("n", C.c_int64), ("positions", C.POINTER(C.c_float)), ("normals", C.POINTER(C.c_float)),
+                ("colors", C.POINTER(C.c_uint8)), ("support", C.POINTER(C.c_uint16))]
+
+
+def fuse_depth_maps(views, maps, thumbs, cell, cell_budget, min_support=1):
+    """depth-fusion.ts fuseDepthMaps: views (StereoView), maps (packed u32 per view), thumbs (H, W, 3)
+    u8; returns (dict(positions, normals, colors, support), final cell) in the accumulator's slot order."""
+    lib = load()
+    if not hasattr(lib, "_fuse"):
+        lib.orc_fuse_depth_maps.restype = C.c_int32
+        lib.orc_fuse_depth_maps.argtypes = [C.c_void_p, C.c_int32, C.c_double, C.c_int64, C.c_int32,
+                                            C.POINTER(OrcCloud), C.POINTER(C.c_double)]
+        lib.orc_cloud_free.restype = None
+        lib.orc_cloud_free.argtypes = [C.POINTER(OrcCloud)]
+        lib._fuse = True
+    keep = []
+    arr = (OrcFusionView * len(views))()
+    for i, (v, m, th) in enumerate(zip(views, maps, thumbs)):
+        m = np.ascontiguousarray(m, dtype=np.uint32)
+        th = np.ascontiguousarray(th, dtype=np.uint8)
+        keep += [m, th]
+        k = v.intrinsics
+        arr[i] = OrcFusionView(v.width, v.height, k.fx, k.fy, k.cx, k.cy,
+                               (C.c_double * 9)(*np.asarray(v.pose.R, float).reshape(9)),
+                               (C.c_double * 3)(*np.asarray(v.pose.t, float).reshape(3)),
+                               m.ctypes.data, th.shape[1], th.shape[0], th.ctypes.data)
+    out = OrcCloud()
+    final = C.c_double()
+    assert lib.orc_fuse_depth_maps(arr, len(views), float(cell), int(cell_budget), int(min_support), C.byref(out),
+                                   C.byref(final)) == 0
+    n = out.n
+    res = {"positions": np.ctypeslib.as_array(out.positions, (max(n, 1) * 3,))[:n * 3].reshape(-1, 3).copy(),
+           "normals": np.ctypeslib.as_array(out.normals, (max(n, 1) * 3,))[:n * 3].reshape(-1, 3).copy(),
+           "colors": np.ctypeslib.as_array(out.colors, (max(n, 1) * 3,))[:n * 3].reshape(-1, 3).copy(),
+           "support": np.ctypeslib.as_array(out.support, (max(n, 1),))[:n].copy()}
+    lib.orc_cloud_free(C.byref(out))
+    return res, final.value

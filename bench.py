@@ -136,7 +136,7 @@ def main():
     ctx.synchronize()
     elapsed = time.perf_counter() - t0
     stats = {k: ctx.kernel_stats(k) for k in ("pm_init", "pm_propagate", "pm_pack", "pm_consistency",
-                                            "tsdf_visibility", "tsdf_integrate")}
+                                            "tsdf_tiles", "tsdf_visibility", "tsdf_integrate")}
     ctx.set_profiling(False)
     ms_per_step = elapsed * 1e3 / args.steps
     # The same steps replayed from the captured graph (HIP cannot time events inside graphs, so
@@ -204,6 +204,9 @@ def main():
         bytes_alg = layout[2] * 512 * 16 + map_bytes
         tsdf.update({"avg_launch_ms": round(t_avg * 1e3, 3),
                      "voxel_upd_per_s": updates / t_avg,
+                     # the bytes a one-view-per-pass schedule moves for the same updates
+                     # (16 B per voxel update), per second of this pass
+                     "plain_schedule_gbs": round(updates * 16 / t_avg / 1e9, 1),
                      "roofline": {"kernel": "tsdf_integrate", "bound": "hbm",
                                   "achieved": round(bytes_alg / t_avg / 1e9, 1), "peak": HBM_PEAK_GBS,
                                   "unit": "GB/s", "frac": round(bytes_alg / t_avg / 1e9 / HBM_PEAK_GBS, 4),

@@ -205,6 +205,14 @@ int32_t lr3d_tsdf_autotune(lr3d_tsdf *ts, lr3d_stereo *st, const int32_t *layers
 int32_t lr3d_tsdf_get_bricks_per_block(lr3d_tsdf *ts, int32_t *bricks);
 int32_t lr3d_tsdf_set_bricks_per_block(lr3d_tsdf *ts, int32_t bricks);
 
+/* Depth culling (on by default): before a pass, every view's map is reduced to the largest usable
+ * depth per 16x16 and 128x128 pixels, and a (brick, view) pair that the reference's culling keeps
+ * (TV:220-234) is dropped when no voxel of the brick can be updated by that view (all projections
+ * outside the image, or all voxels more than the truncation behind every depth they can see).
+ * Volumes, update counts and adjoints are identical with it on or off.  No counterpart in the
+ * reference. */
+int32_t lr3d_tsdf_set_depth_culling(lr3d_tsdf *ts, int32_t on);
+
 /* Overwrite the volume's tsdf and weight pools (n_bricks*512 floats each). */
 int32_t lr3d_tsdf_write(lr3d_tsdf *ts, const float *tsdf, const float *weight);
 

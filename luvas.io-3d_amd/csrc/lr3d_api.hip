@@ -1076,6 +1076,8 @@ struct lr3d_tsdf {
     DevBuf<int4> coords;
     DevBuf<float> tsdf, weight;
     DevBuf<uint32_t> vis;
+    DevBuf<float> tiles;               // depth tiles of the views of the last pass
+    bool depth_culling = true;         // drop (brick, view) pairs that cannot update a voxel
     DevBuf<TsdfView> views;
     DevBuf<float> priors;
     DevBuf<uint32_t> host_map;
@@ -1189,8 +1191,9 @@ static int32_t make_tsdf_view(const lr3d_integration_view &v, uint32_t map_off, 
     return LR3D_OK;
 }
 
-// Priors, view table and culling bits of a pass (stream-ordered).
-static int32_t stage_pass(lr3d_tsdf *ts, const std::vector<TsdfView> &tv, const lr3d_integration_view *views) {
+// Priors, view table, depth tiles and culling bits of a pass (stream-ordered).
+static int32_t stage_pass(lr3d_tsdf *ts, std::vector<TsdfView> &tv, const lr3d_integration_view *views,
+                          const uint32_t *maps) {
     lr3d_ctx *ctx = ts->ctx;
     const int32_t n = (int32_t)tv.size();
     const int32_t words = (n + 31) / 32;
@@ -1205,22 +1208,47 @@ static int32_t stage_pass(lr3d_tsdf *ts, const std::vector<TsdfView> &tv, const 
                                        (size_t)views[i].width * views[i].height * 4, hipMemcpyHostToDevice,
                                        ctx->stream));
     }
+    size_t tiles = 0;
+    int32_t max_tw = 0, max_th = 0;
+    for (int32_t i = 0; i < n; i++) {
+        TsdfView &v = tv[i];
+        v.tw = (v.W + kTsdfTile - 1) / kTsdfTile;
+        v.th = (v.H + kTsdfTile - 1) / kTsdfTile;
+        v.tw2 = (v.tw + kTsdfTile2 - 1) / kTsdfTile2;
+        v.th2 = (v.th + kTsdfTile2 - 1) / kTsdfTile2;
+        v.tile_off = (uint32_t)tiles;
+        tiles += (size_t)v.tw * v.th;
+        v.tile2_off = (uint32_t)tiles;
+        tiles += (size_t)v.tw2 * v.th2;
+        max_tw = std::max(max_tw, v.tw);
+        max_th = std::max(max_th, v.th);
+    }
+    if (tiles > 0xffffffffull) return fail(LR3D_ERR_INVALID, "lr3d_tsdf_integrate: too many depth tiles");
+    const bool cull = ts->depth_culling && n <= 65535;
     HIP_TRY(ts->views.reserve(n));
     HIP_TRY(hipMemcpyAsync(ts->views.ptr, tv.data(), sizeof(TsdfView) * n, hipMemcpyHostToDevice, ctx->stream));
+    if (cull) {
+        HIP_TRY(ts->tiles.reserve(tiles));
+        HIP_TRY(timed(ctx, "tsdf_tiles", [&] {
+            return tsdf_launch_depth_tiles(ctx->stream, ts->views.ptr, n, max_tw, max_th, maps, ts->priors.ptr,
+                                           ts->tiles.ptr);
+        }));
+    }
     HIP_TRY(ts->vis.reserve((size_t)ts->grid.n_bricks * words));
     HIP_TRY(timed(ctx, "tsdf_visibility", [&] {
-        return tsdf_launch_visibility(ctx->stream, ts->grid, ts->coords.ptr, ts->views.ptr, n, ts->vis.ptr, words);
+        return tsdf_launch_visibility(ctx->stream, ts->grid, ts->coords.ptr, ts->views.ptr, n, ts->vis.ptr, words,
+                                      cull ? ts->tiles.ptr : nullptr);
     }));
     return LR3D_OK;
 }
 
-static int32_t integrate_impl(lr3d_tsdf *ts, const uint32_t *maps, const std::vector<TsdfView> &tv,
+static int32_t integrate_impl(lr3d_tsdf *ts, const uint32_t *maps, std::vector<TsdfView> &tv,
                               const lr3d_integration_view *views) {
     lr3d_ctx *ctx = ts->ctx;
     const int32_t n = (int32_t)tv.size();
     if (n == 0) return finish(ctx);
     const int32_t words = (n + 31) / 32;
-    int32_t rc = stage_pass(ts, tv, views);
+    int32_t rc = stage_pass(ts, tv, views, maps);
     if (rc) return rc;
     HIP_TRY(timed(ctx, "tsdf_integrate", [&] {
         return tsdf_launch_integrate(ctx->stream, ts->grid, ts->views.ptr, n, ts->vis.ptr, words,
@@ -1310,7 +1338,7 @@ extern "C" int32_t lr3d_tsdf_backward(lr3d_tsdf *ts, lr3d_stereo *st, const int3
     std::vector<uint32_t> pixel_off(n + 1, 0);
     for (int32_t i = 0; i < n; i++) pixel_off[i + 1] = pixel_off[i] + (uint32_t)(views[i].width * views[i].height);
     const size_t total_px = pixel_off[n];
-    rc = stage_pass(ts, tv, views);
+    rc = stage_pass(ts, tv, views, maps);
     if (rc) return rc;
     HIP_TRY(ts->adj_pixel_off.reserve(n + 1));
     HIP_TRY(hipMemcpyAsync(ts->adj_pixel_off.ptr, pixel_off.data(), 4 * (n + 1), hipMemcpyHostToDevice, ctx->stream));
@@ -1356,6 +1384,13 @@ extern "C" int32_t lr3d_tsdf_backward(lr3d_tsdf *ts, lr3d_stereo *st, const int3
     return LR3D_OK;
 }
 
+extern "C" int32_t lr3d_tsdf_set_depth_culling(lr3d_tsdf *ts, int32_t on) {
+    if (!ts) return fail(LR3D_ERR_INVALID, "lr3d_tsdf_set_depth_culling: NULL argument");
+    std::lock_guard<std::mutex> lock(ts->ctx->mu);
+    ts->depth_culling = on != 0;
+    return LR3D_OK;
+}
+
 extern "C" int32_t lr3d_tsdf_get_bricks_per_block(lr3d_tsdf *ts, int32_t *out) {
     if (!ts || !out) return fail(LR3D_ERR_INVALID, "lr3d_tsdf_get_bricks_per_block: NULL argument");
     std::lock_guard<std::mutex> lock(ts->ctx->mu);
@@ -1383,7 +1418,7 @@ extern "C" int32_t lr3d_tsdf_autotune(lr3d_tsdf *ts, lr3d_stereo *st, const int3
     const uint32_t *maps = nullptr;
     int32_t rc = stereo_views(ts, st, layers, map, views, n, tv, &maps);
     if (rc) return rc;
-    rc = stage_pass(ts, tv, views);
+    rc = stage_pass(ts, tv, views, maps);
     if (rc) return rc;
     const int32_t words = (n + 31) / 32;
     const size_t voxels = (size_t)ts->grid.n_bricks * 512u;

@@ -99,11 +99,18 @@ struct TsdfView {
     uint32_t map_off;   // word offset of the packed map
     int32_t has_prior;
     uint32_t prior_off;
+    // depth tiles of the view's map (max usable depth per kTsdfTile^2 pixels, and per
+    // kTsdfTile2^2 of those): word offsets and tile-grid sizes
+    uint32_t tile_off, tile2_off;
+    int32_t tw, th, tw2, th2;
     double kd[4];       // double copies for the brick culling (TV:220-234 runs in double)
     double Rd[9];
     double td[3];
     double max_depth;
 };
+
+constexpr int32_t kTsdfTile = 16;   // pixels per depth-tile side
+constexpr int32_t kTsdfTile2 = 8;   // depth tiles per coarse-tile side (128 pixels)
 
 struct TsdfGridDev {
     float origin[3];
@@ -130,9 +137,15 @@ hipError_t ncc_launch_backward(hipStream_t s, const PmPlan &plan, const PmPools 
                                const AdjGeom *geom, const float *weights, float *cost, float *gd,
                                float *gn, double *pose_partials, float step_depth, float step_normal,
                                int32_t apply);
+// Depth tiles of every view of a pass (tiles: sum of tw*th + tw2*th2 floats, at the views'
+// offsets); max_tw/max_th: the largest tile grid among the views.
+hipError_t tsdf_launch_depth_tiles(hipStream_t s, const TsdfView *views, int32_t n_views, int32_t max_tw,
+                                   int32_t max_th, const uint32_t *maps, const float *priors, float *tiles);
+// Culling bits: the reference's brick test, then a depth test against the view's depth tiles
+// (tiles == nullptr: the reference's test alone).
 hipError_t tsdf_launch_visibility(hipStream_t s, const TsdfGridDev &g, const int4 *coords,
                                   const TsdfView *views, int32_t n_views, uint32_t *vis,
-                                  int32_t words);
+                                  int32_t words, const float *tiles);
 hipError_t tsdf_launch_integrate(hipStream_t s, const TsdfGridDev &g, const TsdfView *views,
                                  int32_t n_views, const uint32_t *vis, int32_t words,
                                  const int4 *coords, const uint32_t *maps, const float *priors,

@@ -48,24 +48,118 @@ __device__ __forceinline__ void unpack_normal(uint32_t packed, float &nx, float 
     nz = nz * inv;
 }
 
-// TV:220-234: one thread per (brick, 32-view word); bit j of the word = view 32*word + j visible.
+// Largest depth pixel i of view V can give a voxel: the map's depth where positive, else the prior
+// where the view has one (TV:122-126); 0 where no update can come from the pixel.
+__device__ __forceinline__ float usable_depth(const TsdfView &V, const uint32_t *__restrict__ maps,
+                                              const float *__restrict__ priors, uint32_t i) {
+    float d = f16_bits_to_f32(maps[V.map_off + i]);
+    if (!(d > 0.0f)) d = V.has_prior ? priors[V.prior_off + i] : 0.0f;
+    return d > 0.0f ? d : 0.0f;
+}
+
+// Depth tiles: one 16x16 block per tile of kTsdfTile^2 pixels, grid (max tw, max th, views).
+__global__ void __launch_bounds__(256) depth_tile_kernel(const TsdfView *__restrict__ views,
+                                                         const uint32_t *__restrict__ maps,
+                                                         const float *__restrict__ priors,
+                                                         float *__restrict__ tiles) {
+    const TsdfView &V = views[blockIdx.z];
+    if ((int32_t)blockIdx.x >= V.tw || (int32_t)blockIdx.y >= V.th) return;  // whole block
+    const int32_t x = (int32_t)(blockIdx.x * kTsdfTile + threadIdx.x);
+    const int32_t y = (int32_t)(blockIdx.y * kTsdfTile + threadIdx.y);
+    float d = 0.0f;
+    if (x < V.W && y < V.H) d = usable_depth(V, maps, priors, (uint32_t)y * (uint32_t)V.W + (uint32_t)x);
+    for (int off = 32; off > 0; off >>= 1) d = fmaxf(d, __shfl_xor(d, off, 64));
+    __shared__ float part[4];
+    const uint32_t t = threadIdx.y * kTsdfTile + threadIdx.x;
+    if ((t & 63u) == 0) part[t >> 6] = d;
+    __syncthreads();
+    if (t == 0)
+        tiles[V.tile_off + blockIdx.y * (uint32_t)V.tw + blockIdx.x] =
+            fmaxf(fmaxf(part[0], part[1]), fmaxf(part[2], part[3]));
+}
+
+// Coarse tiles: one block per view, one thread per coarse tile (max of kTsdfTile2^2 tiles).
+__global__ void __launch_bounds__(256) depth_tile2_kernel(const TsdfView *__restrict__ views,
+                                                          float *__restrict__ tiles) {
+    const TsdfView &V = views[blockIdx.x];
+    for (int32_t j = (int32_t)threadIdx.x; j < V.tw2 * V.th2; j += (int32_t)blockDim.x) {
+        const int32_t cx = j % V.tw2, cy = j / V.tw2;
+        float m = 0.0f;
+        for (int32_t ty = cy * kTsdfTile2; ty < min(V.th, (cy + 1) * kTsdfTile2); ty++)
+            for (int32_t tx = cx * kTsdfTile2; tx < min(V.tw, (cx + 1) * kTsdfTile2); tx++)
+                m = fmaxf(m, tiles[V.tile_off + (uint32_t)(ty * V.tw + tx)]);
+        tiles[V.tile2_off + (uint32_t)j] = m;
+    }
+}
+
+// Whether any voxel of brick c can take an update from view V.  False only when the brick's voxel
+// centres all lie at camera depth >= 0.02 and either their projections miss the image or the
+// largest usable depth over the tiles their pixels fall in is below the brick's nearest depth
+// minus the truncation, so that every voxel meets sdf < -mu (TV:140).  Computed in double over the
+// 8 corners of the box of voxel centres (the projection of the box lies in the hull of the
+// projected corners), with margins of one pixel and 1e-4 m + 1e-6 z, far above the FP32 rounding of
+// a voxel's projection and depth in integrate_kernel.
+__device__ bool depth_reachable(const TsdfGridDev &g, const TsdfView &V, int4 c, const float *__restrict__ tiles) {
+    double zmin = 1e300, umin = 1e300, umax = -1e300, vmin = 1e300, vmax = -1e300;
+#pragma unroll 1
+    for (int k = 0; k < 8; k++) {
+        const double x = g.origin_d[0] + ((double)(c.x * 8) + ((k & 1) ? 7.5 : 0.5)) * g.voxel_d;
+        const double y = g.origin_d[1] + ((double)(c.y * 8) + ((k & 2) ? 7.5 : 0.5)) * g.voxel_d;
+        const double z = g.origin_d[2] + ((double)(c.z * 8) + ((k & 4) ? 7.5 : 0.5)) * g.voxel_d;
+        const double p0 = (V.Rd[0] * x + V.Rd[1] * y + V.Rd[2] * z) + V.td[0];
+        const double p1 = (V.Rd[3] * x + V.Rd[4] * y + V.Rd[5] * z) + V.td[1];
+        const double p2 = (V.Rd[6] * x + V.Rd[7] * y + V.Rd[8] * z) + V.td[2];
+        if (!(p2 >= 0.02)) return true;
+        const double u = (V.kd[0] * p0) / p2 + V.kd[2];
+        const double v = (V.kd[1] * p1) / p2 + V.kd[3];
+        zmin = fmin(zmin, p2);
+        umin = fmin(umin, u);
+        umax = fmax(umax, u);
+        vmin = fmin(vmin, v);
+        vmax = fmax(vmax, v);
+    }
+    const double fx0 = fmax(0.0, floor(umin) - 1.0), fx1 = fmin((double)(V.W - 1), ceil(umax) + 1.0);
+    const double fy0 = fmax(0.0, floor(vmin) - 1.0), fy1 = fmin((double)(V.H - 1), ceil(vmax) + 1.0);
+    if (fx0 > fx1 || fy0 > fy1) return false;  // every projection is outside the image
+    int32_t tx0 = (int32_t)fx0 / kTsdfTile, tx1 = (int32_t)fx1 / kTsdfTile;
+    int32_t ty0 = (int32_t)fy0 / kTsdfTile, ty1 = (int32_t)fy1 / kTsdfTile;
+    float dmax = 0.0f;
+    if ((tx1 - tx0 + 1) * (ty1 - ty0 + 1) <= 64) {
+        for (int32_t ty = ty0; ty <= ty1; ty++)
+            for (int32_t tx = tx0; tx <= tx1; tx++) dmax = fmaxf(dmax, tiles[V.tile_off + (uint32_t)(ty * V.tw + tx)]);
+    } else {
+        tx0 /= kTsdfTile2;
+        tx1 /= kTsdfTile2;
+        ty0 /= kTsdfTile2;
+        ty1 /= kTsdfTile2;
+        if ((tx1 - tx0 + 1) * (ty1 - ty0 + 1) > 64) return true;
+        for (int32_t ty = ty0; ty <= ty1; ty++)
+            for (int32_t tx = tx0; tx <= tx1; tx++) dmax = fmaxf(dmax, tiles[V.tile2_off + (uint32_t)(ty * V.tw2 + tx)]);
+    }
+    if (!(dmax > 0.0f)) return false;
+    return (double)dmax >= zmin - g.mu_d - (1e-4 + 1e-6 * zmin);
+}
+
+// TV:220-234, one lane per (brick, view); a half-wavefront holds one 32-view word of one brick
+// (bit j of the word = view 32*word + j visible).  With depth tiles the reference's test is
+// followed by depth_reachable, which drops only (brick, view) pairs that update no voxel.
 __global__ void __launch_bounds__(256) visibility_kernel(TsdfGridDev g, const int4 *__restrict__ coords,
                                                          const TsdfView *__restrict__ views,
                                                          int32_t n_views, uint32_t *__restrict__ vis,
-                                                         int32_t words) {
+                                                         int32_t words, const float *__restrict__ tiles) {
     const uint32_t gid = blockIdx.x * blockDim.x + threadIdx.x;
-    const uint32_t brick = gid / (uint32_t)words;
-    const int32_t word = (int32_t)(gid % (uint32_t)words);
-    if (brick >= (uint32_t)g.n_bricks) return;
-    const int4 c = coords[brick];
-    const double side = g.voxel_d * 8.0;
-    const double radius = g.voxel_d * 8.0 * 0.87 + g.mu_d;
-    const double cx = g.origin_d[0] + ((double)c.x + 0.5) * side;
-    const double cy = g.origin_d[1] + ((double)c.y + 0.5) * side;
-    const double cz = g.origin_d[2] + ((double)c.z + 0.5) * side;
-    uint32_t bits = 0;
-    const int32_t v_end = min(n_views, word * 32 + 32);
-    for (int32_t v = word * 32; v < v_end; v++) {
+    const uint32_t pair = gid >> 5;
+    const uint32_t brick = pair / (uint32_t)words;
+    const int32_t word = (int32_t)(pair % (uint32_t)words);
+    const int32_t v = word * 32 + (int32_t)(threadIdx.x & 31u);
+    bool visible = false;
+    if (brick < (uint32_t)g.n_bricks && v < n_views) {
+        const int4 c = coords[brick];
+        const double side = g.voxel_d * 8.0;
+        const double radius = g.voxel_d * 8.0 * 0.87 + g.mu_d;
+        const double cx = g.origin_d[0] + ((double)c.x + 0.5) * side;
+        const double cy = g.origin_d[1] + ((double)c.y + 0.5) * side;
+        const double cz = g.origin_d[2] + ((double)c.z + 0.5) * side;
         const TsdfView &V = views[v];
         const double p0 = (V.Rd[0] * cx + V.Rd[1] * cy + V.Rd[2] * cz) + V.td[0];
         const double p1 = (V.Rd[3] * cx + V.Rd[4] * cy + V.Rd[5] * cz) + V.td[1];
@@ -73,12 +167,16 @@ __global__ void __launch_bounds__(256) visibility_kernel(TsdfGridDev g, const in
         const double u = (V.kd[0] * p0) / p2 + V.kd[2];
         const double w = (V.kd[1] * p1) / p2 + V.kd[3];
         const double z = p2;
-        if (z < 0.02 - radius || z > V.max_depth + radius) continue;
-        const double margin = (V.kd[0] * radius) / fmax(0.02, z);
-        if (u < -margin || w < -margin || u > (double)V.W + margin || w > (double)V.H + margin) continue;
-        bits |= 1u << (v - word * 32);
+        visible = !(z < 0.02 - radius || z > V.max_depth + radius);
+        if (visible) {
+            const double margin = (V.kd[0] * radius) / fmax(0.02, z);
+            visible = !(u < -margin || w < -margin || u > (double)V.W + margin || w > (double)V.H + margin);
+        }
+        if (visible && tiles) visible = depth_reachable(g, V, c, tiles);
     }
-    vis[(size_t)brick * words + word] = bits;
+    const uint64_t bits = __ballot(visible);
+    if ((threadIdx.x & 31u) == 0 && brick < (uint32_t)g.n_bricks)
+        vis[(size_t)brick * words + word] = (threadIdx.x & 32u) ? (uint32_t)(bits >> 32) : (uint32_t)bits;
 }
 
 // TV:103-154, one wavefront per brick, all visible views of the pass in order.
@@ -386,12 +484,26 @@ __global__ void to_float_kernel(const double *__restrict__ in, float *__restrict
 
 }  // namespace
 
+hipError_t tsdf_launch_depth_tiles(hipStream_t s, const TsdfView *views, int32_t n_views, int32_t max_tw,
+                                   int32_t max_th, const uint32_t *maps, const float *priors, float *tiles) {
+    if (n_views <= 0 || max_tw <= 0 || max_th <= 0) return hipSuccess;
+    if (n_views > 65535) return hipErrorInvalidValue;
+    dim3 block(kTsdfTile, kTsdfTile), grid((uint32_t)max_tw, (uint32_t)max_th, (uint32_t)n_views);
+    hipLaunchKernelGGL(depth_tile_kernel, grid, block, 0, s, views, maps, priors, tiles);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(depth_tile2_kernel, dim3((uint32_t)n_views), dim3(256), 0, s, views, tiles);
+    return hipGetLastError();
+}
+
 hipError_t tsdf_launch_visibility(hipStream_t s, const TsdfGridDev &g, const int4 *coords,
                                   const TsdfView *views, int32_t n_views, uint32_t *vis,
-                                  int32_t words) {
-    const uint32_t threads = (uint32_t)g.n_bricks * (uint32_t)words;
-    dim3 block(256), grid((threads + 255u) / 256u);
-    hipLaunchKernelGGL(visibility_kernel, grid, block, 0, s, g, coords, views, n_views, vis, words);
+                                  int32_t words, const float *tiles) {
+    const uint64_t threads = (uint64_t)g.n_bricks * (uint64_t)words * 32u;
+    if (threads == 0) return hipSuccess;
+    if (threads > 0xffffff00ull) return hipErrorInvalidValue;
+    dim3 block(256), grid((uint32_t)((threads + 255u) / 256u));
+    hipLaunchKernelGGL(visibility_kernel, grid, block, 0, s, g, coords, views, n_views, vis, words, tiles);
     return hipGetLastError();
 }
 

@@ -164,6 +164,10 @@ class HipTsdfVolume:
     def bricks_per_block(self, bricks: int) -> None:
         N.check(self._lib.lr3d_tsdf_set_bricks_per_block(self.handle, int(bricks)))
 
+    def set_depth_culling(self, on: bool) -> None:
+        """Drop (brick, view) pairs that cannot update a voxel (on by default; results unchanged)."""
+        N.check(self._lib.lr3d_tsdf_set_depth_culling(self.handle, int(bool(on))))
+
     def backward(self, stereo, layers: Sequence[int], views: Sequence[IntegrationView], voxels, targets,
                  map: int = N.MAP_RAW) -> dict:
         """Replay of integrate_batched into a fresh volume at the listed voxels (pool indices) and the

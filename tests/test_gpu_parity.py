@@ -218,6 +218,48 @@ def test_tsdf_batched_equals_sequential(ctx, work, oracle_maps):
     assert np.array_equal(wa.view(np.uint32), wb.view(np.uint32))
 
 
+def test_depth_culling_changes_nothing(ctx, work, oracle_maps):
+    """Depth culling drops only (brick, view) pairs that update no voxel: volumes and update
+    counts are identical with it on and off, for raw maps, for views with a prior (dense, and
+    holed with zeros and NaNs) and with confidence weighting, batched and one view per pass."""
+    say("case: TSDF depth culling on vs off (8 views, priors on views 2 and 5, confidence on view 3)")
+    from dataclasses import replace
+    from lr3d import HipTsdfVolume
+    depths = oracle_maps[0]
+    grid, layout, views = _tsdf_setup(work)
+    n = 320 * 240
+    truth2 = work.truth[2].astype(np.float32).reshape(-1).copy()
+    holed5 = work.truth[5].astype(np.float32).reshape(-1).copy()
+    holed5[::7] = 0.0
+    holed5[3::11] = np.nan
+    views[2] = replace(views[2], prior=truth2)
+    views[5] = replace(views[5], prior=holed5)
+    views[3] = replace(views[3], confidence=1.2)
+    st = make_stereo(ctx, work)
+    for i in range(len(views)):
+        st.write_depth(i, depths[i * n:(i + 1) * n])
+    out = {}
+    for cull in (True, False):
+        for batched in (True, False):
+            v = HipTsdfVolume(ctx, grid, layout, work.tsdf.truncation_voxels)
+            v.set_depth_culling(cull)
+            v.set_counting(True)
+            if batched:
+                v.integrate_batched(st, list(range(len(views))), views)
+            else:
+                for i, iv in enumerate(views):
+                    v.integrate(st, i, iv)
+            out[cull, batched] = (*v.read(), v.update_count())
+            v.destroy()
+    st.destroy()
+    ref = out[False, False]
+    say(f"culling: {ref[2]} voxel updates")
+    for key, (t, w, c) in out.items():
+        assert np.array_equal(t.view(np.uint32), ref[0].view(np.uint32)), key
+        assert np.array_equal(w.view(np.uint32), ref[1].view(np.uint32)), key
+        assert c == ref[2], key
+
+
 def test_geometric_pass_matches_oracle(ctx, work, oracle_maps):
     say("case: geometric (ACMM) pass of view 0 over the oracle's photometric maps")
     depths = oracle_maps[0].copy()

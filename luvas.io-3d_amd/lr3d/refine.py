@@ -64,22 +64,33 @@ def _pose_totals(stereo, solves, grads) -> dict:
 
 
 def refine(stereo, solves, steps: int = 50, step_depth: float = 2e-4, step_normal: float = 2e-3,
-           step_rot: float = 2e-4, step_trans: float = 2e-4, fixed=(0,), log=None):
+           step_rot: float = 1e-3, step_trans: float = 1e-3, fixed=(0,), resolve: bool = True, log=None):
     """`steps` joint gradient steps on the planes of every (ref, options) and on the poses of every
-    view not in `fixed` (gauge).
+    view not in `fixed` (gauge).  The planes must be solved at the current poses beforehand.
 
-    Each step: one plane step on the GPU (kept per pixel where it lowers the cost), then one pose
-    step, normalised per view (each view turns by step_rot radians and moves by step_trans metres
-    along its negative gradient).  The pose step is kept if the total loss after it is not higher
-    (the step then grows by 1.25), else undone and halved.  The loss is therefore non-increasing.
-    Returns the loss history: the loss before the first step, then after each step."""
+    Every step takes one plane step on the GPU (lr3d_ncc_refine_step: a gradient step on each
+    pixel's plane, kept where it lowers the pixel's cost, with the pose gradients at the planes
+    before it) and one pose step: each view turns by step_rot radians and moves by step_trans
+    metres along its negative gradient.
+
+    resolve=True: after the pose step every view's planes are solved again at the new poses
+    (PatchMatch with the solves' options), followed by a plane step, and the pose step is kept if
+    that loss is lower (the steps then grow by 1.5); otherwise the poses are restored and the
+    planes solved again, which reproduces them (the solve is deterministic), and the steps are
+    halved.  Planes fitted at wrong poses hold those poses in place: with the planes fixed, a move
+    toward the true poses raises the loss; re-solved planes follow the poses, so the comparison is
+    between planes fitted at each.
+    resolve=False: the planes are kept; the pose step is kept if the loss of the current planes
+    does not rise (steps grow by 1.25), else undone and halved.
+
+    The loss is non-increasing.  Returns the history: the loss before the first step, then the
+    loss after each step."""
     history = []
     rot, trans = step_rot, step_trans
+    grads, before, after = stereo.refine_step(solves, step_depth, step_normal)
+    history.append(float(before.sum()))
+    loss = float(after.sum())
     for it in range(steps):
-        grads, before, after = stereo.refine_step(solves, step_depth, step_normal)
-        if not history:
-            history.append(float(before.sum()))
-        loss = float(after.sum())
         total = _pose_totals(stereo, solves, grads)
         saved = {lay: stereo.views[lay].pose for lay in total if lay not in fixed}
         for lay, (w, tau) in total.items():
@@ -88,14 +99,31 @@ def refine(stereo, solves, steps: int = 50, step_depth: float = 2e-4, step_norma
             dw = -rot * w / (np.linalg.norm(w) + 1e-30)
             dt = -trans * tau / (np.linalg.norm(tau) + 1e-30)
             stereo.set_pose(lay, perturb(stereo.views[lay].pose, dw, dt))
-        _, trial, _ = stereo.refine_step(solves, 0.0, 0.0)
-        if float(trial.sum()) <= loss:
-            loss = float(trial.sum())
-            rot, trans = rot * 1.25, trans * 1.25
+        if resolve:
+            stereo.solve_batched(solves)
+            g2, _, a2 = stereo.refine_step(solves, step_depth, step_normal)
+            if float(a2.sum()) < loss:
+                loss, grads = float(a2.sum()), g2
+                rot, trans = rot * 1.5, trans * 1.5
+            else:
+                for lay, pose in saved.items():
+                    stereo.set_pose(lay, pose)
+                stereo.solve_batched(solves)
+                grads, _, a3 = stereo.refine_step(solves, step_depth, step_normal)
+                loss = min(loss, float(a3.sum()))
+                rot, trans = rot * 0.5, trans * 0.5
         else:
-            for lay, pose in saved.items():
-                stereo.set_pose(lay, pose)
-            rot, trans = rot * 0.5, trans * 0.5
+            _, trial, _ = stereo.refine_step(solves, 0.0, 0.0)
+            if float(trial.sum()) <= loss:
+                loss = float(trial.sum())
+                rot, trans = rot * 1.25, trans * 1.25
+            else:
+                for lay, pose in saved.items():
+                    stereo.set_pose(lay, pose)
+                rot, trans = rot * 0.5, trans * 0.5
+            if it + 1 < steps:
+                grads, _, a2 = stereo.refine_step(solves, step_depth, step_normal)
+                loss = min(loss, float(a2.sum()))
         history.append(loss)
         if log:
             log(it, loss)

@@ -241,6 +241,41 @@ def sphere_phone(seed: int = 2, n: int = 32) -> Workload:
                     tsdf=TsdfSpec((-0.512, -0.512, -0.512), 0.004, 256, 4.0))
 
 
+def sphere_diff(seed: int = 4, n: int = 16) -> Workload:
+    """Differentiation of the NCC cost: 16 views 960x540 of the sphere scene on a seeded jittered
+    orbit (radius 1.2-1.8 m), f = 750 (the phone field of view), depth 0.3-4 m; 4 neighbours
+    (+-45 degrees on the ring), top-2, 6 iterations; TSDF 256^3 x 4 mm, truncation 4 voxels.
+    Poses as rendered; `noisy_views` perturbs them."""
+    W, H = 960, 540
+    k = Intrinsics(750.0, 750.0, (W - 1) / 2, (H - 1) / 2)
+    rnd = mulberry32(seed * 7919 + 41)
+    views = []
+    for i in range(n):
+        radius = 1.2 + 0.6 * rnd()
+        a = 2 * math.pi * (i + 0.6 * (rnd() - 0.5)) / n
+        height = 0.4 + 0.2 * (rnd() - 0.5)
+        pose = look_at((radius * math.sin(a), height, radius * math.cos(a)))
+        views.append(StereoView(k, pose, W, H))
+    return Workload(f"sphere-diff-{n}x960x540", sphere_scene(seed, freq=0.75), views, _ring_sources(n, 4),
+                    [(0.3, 4.0)] * n, iterations=6, topk=2,
+                    tsdf=TsdfSpec((-0.512, -0.512, -0.512), 0.004, 256, 4.0))
+
+
+def noisy_views(views: List[StereoView], seed: int = 4, rotation_deg: float = 0.5,
+                translation: float = 0.005, fixed: int = 0) -> List[StereoView]:
+    """Poses perturbed by seeded noise: rotation about a random axis with per-axis sigma
+    rotation_deg / sqrt(3) (RMS angle rotation_deg), translation with per-axis sigma
+    translation / sqrt(3) (RMS translation); view `fixed` is left exact (the gauge)."""
+    from .refine import perturb
+    rng = np.random.default_rng(seed)
+    out = []
+    for i, v in enumerate(views):
+        w = rng.normal(0, math.radians(rotation_deg) / math.sqrt(3), 3)
+        tau = rng.normal(0, translation / math.sqrt(3), 3)
+        out.append(v if i == fixed else StereoView(v.intrinsics, perturb(v.pose, w, tau), v.width, v.height))
+    return out
+
+
 def room_scene(seed: int, freq: float = 0.3) -> SceneSpec:
     """The room and its 12 boxes (mulberry32 seeded)."""
     rnd = mulberry32(seed * 104729 + 7)
@@ -286,8 +321,34 @@ def room_batched(seed: int = 3, n: int = 64, scale: float = 1.0, iterations: int
                     tsdf=TsdfSpec((-1.92, 1.25 - 1.92, -1.92), 0.01, 384, 4.0))
 
 
+def room_large(seed: int = 5, n: int = 200, iterations: int = 8) -> Workload:
+    """Room-scale forward + TSDF adjoint: 200 views 1280x720 of the procedural room (the batched
+    configuration's trajectory and neighbour rule), 8 iterations, top-k by the reference rule;
+    TSDF 512^3 x 8 mm (1.07 GB of tsdf + weight), truncation 4 voxels, centred on the room."""
+    rnd = mulberry32(seed * 7919 + 29)
+    W, H = 1280, 720
+    f = (W / 2) / math.tan(math.radians(35.0))
+    k = Intrinsics(f, f, (W - 1) / 2, (H - 1) / 2)
+    views, counts = [], []
+    for i in range(n):
+        a = 2 * math.pi * (i + 0.4 * (rnd() - 0.5)) / n
+        height = 1.2 + 0.6 * rnd()
+        c = (1.1 * math.cos(a), height, 0.8 * math.sin(a))
+        yaw = a + math.pi + math.radians(20.0) * (2 * rnd() - 1)
+        pitch = math.radians(10.0 + 15.0 * rnd())
+        d = (math.cos(yaw) * math.cos(pitch), -math.sin(pitch), math.sin(yaw) * math.cos(pitch))
+        views.append(StereoView(k, look_at(c, (c[0] + d[0], c[1] + d[1], c[2] + d[2])), W, H))
+        counts.append(4 + int(7 * rnd()))
+    sources = [ring[:cnt] for ring, cnt in zip(_ring_sources(n, 10), counts)]
+    return Workload(f"room-{n}x1280x720", room_scene(seed), views, sources, [(0.3, 6.0)] * n,
+                    iterations=iterations, topk=0,
+                    tsdf=TsdfSpec((-2.048, 1.25 - 2.048, -2.048), 0.008, 512, 4.0))
+
+
 WORKLOADS = {
     "sphere-small": sphere_small,
     "sphere-phone": sphere_phone,
     "room-batched": room_batched,
+    "sphere-diff": sphere_diff,
+    "room-large": room_large,
 }

@@ -50,7 +50,8 @@ class HipDenseStereo:
     def __init__(self, ctx: Context, views: Sequence[StereoView]):
         self._lib = N.load()
         self.ctx = ctx
-        self.views = list(views)
+        # own copies: set_pose replaces poses here, never in the caller's objects
+        self.views = [StereoView(v.intrinsics, v.pose, v.width, v.height) for v in views]
         arr = (N.View * len(self.views))(*[_view_struct(v) for v in self.views])
         handle = C.c_void_p()
         N.check(self._lib.lr3d_stereo_create(ctx.handle, arr, len(self.views), 0, C.byref(handle)))

@@ -85,6 +85,7 @@ float orc_random(uint32_t *rng);
 uint32_t orc_pack_sample(float depth, const float normal[3], float cost);
 float orc_unpack_depth(uint32_t packed);
 void orc_unpack_normal(uint32_t packed, float out[3]);
+void orc_decode_map(const uint32_t *packed, int64_t n, double *depth, double *normal, uint8_t *cost);
 uint32_t orc_unpack_cost(uint32_t packed);
 uint16_t orc_f32_to_f16(float v);
 float orc_f16_to_f32(uint16_t h);
@@ -232,12 +233,16 @@ typedef struct {
 
 /* Integrate views[0..n) in order into a fresh volume (tsdf 1, weight 0) at the listed voxels
  * (pool indices) in FP64; L = sum (tsdf - target)^2.  Outputs (any may be NULL): tsdf_out and
- * kink_out (distance of the nearest projection to a rounding boundary, px) per voxel; grad_depth
+ * kink_out per voxel (distance to the nearest decision boundary over all views: the nearest-pixel
+ * rounding in pixels, the cam.z > 0.01 and sdf >= -mu tests in units of mu); grad_depth
  * per pixel of every view, view-major (dL/d depth used); grad_pose n*12 (dL/dR row-major, dL/dt). */
 int32_t orc64_tsdf_backward(const orc_tsdf_grid *g, const orc_integration_view *views,
                             const orc64_depth_input *inputs, int32_t n_views, const uint32_t *voxels,
                             const double *targets, int64_t n_voxels, double *tsdf_out,
                             double *grad_depth, double *grad_pose, double *kink_out, double *loss);
+int32_t orc64_tsdf_loss(const orc_tsdf_grid *g, const orc_integration_view *views, const orc64_depth_input *inputs,
+                        int32_t n_views, const uint32_t *voxels, const double *targets, int64_t n_voxels,
+                        const uint8_t *vis, double *loss);
 
 /* ---- FP64 cost and its gradient (adjoint_oracle.c) ---- */
 typedef struct {

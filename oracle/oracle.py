@@ -399,19 +399,32 @@ def decode_maps(packed_maps):
     """Packed u32 maps -> (depth f64, normal f64 (W*H, 3), cost code u8) with the decoders of PK."""
     out = []
     lib = load()
+    if not hasattr(lib, "_dm"):
+        lib.orc_decode_map.restype = None
+        lib.orc_decode_map.argtypes = [C.c_void_p, C.c_int64, C.c_void_p, C.c_void_p, C.c_void_p]
+        lib._dm = True
     for m in packed_maps:
         m = np.ascontiguousarray(m, dtype=np.uint32).reshape(-1)
         depth = np.zeros(m.size, dtype=np.float64)
         normal = np.zeros((m.size, 3), dtype=np.float64)
-        nz = np.flatnonzero(m)
-        f16 = (m[nz] & 0xFFFF).astype(np.uint16).view(np.float16).astype(np.float64)
-        depth[nz] = np.where(f16 > 0, f16, 0.0)
-        nn = (C.c_float * 3)()
-        for i in nz:
-            lib.orc_unpack_normal(C.c_uint32(int(m[i])), nn)
-            normal[i] = list(nn)
-        out.append((depth, normal, (m >> 28).astype(np.uint8)))
+        cost = np.zeros(m.size, dtype=np.uint8)
+        lib.orc_decode_map(m.ctypes.data, m.size, depth.ctypes.data, normal.ctypes.data, cost.ctypes.data)
+        out.append((depth, normal, cost))
     return out
+
+
+def _depth_inputs(decoded, priors):
+    n = len(decoded)
+    keep = []
+    inputs = (Orc64DepthInput * n)()
+    for i, (dep, nor, cost) in enumerate(decoded):
+        pr = None if priors is None or priors[i] is None else np.ascontiguousarray(priors[i], dtype=np.float64).reshape(-1)
+        dep, nor, cost = (np.ascontiguousarray(dep, dtype=np.float64), np.ascontiguousarray(nor, dtype=np.float64),
+                          np.ascontiguousarray(cost, dtype=np.uint8))
+        keep += [dep, nor, cost, pr]
+        inputs[i] = Orc64DepthInput(dep.ctypes.data, nor.ctypes.data, cost.ctypes.data,
+                                    pr.ctypes.data if pr is not None else None)
+    return inputs, keep
 
 
 def tsdf_backward64(grid: Grid, views, decoded, priors, voxels, targets):
@@ -424,15 +437,7 @@ def tsdf_backward64(grid: Grid, views, decoded, priors, voxels, targets):
         lib._tb64 = True
     n = len(views)
     varr = (OrcIntegrationView * n)(*views)
-    keep = []
-    inputs = (Orc64DepthInput * n)()
-    for i, (dep, nor, cost) in enumerate(decoded):
-        pr = None if priors is None or priors[i] is None else np.ascontiguousarray(priors[i], dtype=np.float64).reshape(-1)
-        dep, nor, cost = (np.ascontiguousarray(dep, dtype=np.float64), np.ascontiguousarray(nor, dtype=np.float64),
-                          np.ascontiguousarray(cost, dtype=np.uint8))
-        keep += [dep, nor, cost, pr]
-        inputs[i] = Orc64DepthInput(dep.ctypes.data, nor.ctypes.data, cost.ctypes.data,
-                                    pr.ctypes.data if pr is not None else None)
+    inputs, keep = _depth_inputs(decoded, priors)
     vox = np.ascontiguousarray(voxels, dtype=np.uint32)
     tgt = np.ascontiguousarray(targets, dtype=np.float64)
     sizes = [v.width * v.height for v in views]
@@ -447,6 +452,33 @@ def tsdf_backward64(grid: Grid, views, decoded, priors, voxels, targets):
     assert rc == 0
     splits = np.split(gd, np.cumsum(sizes)[:-1])
     return {"tsdf": tsdf, "kink": kink, "grad_depth": splits, "grad_pose": gp, "loss": float(loss[0])}
+
+
+def visibility_flags(grid: Grid, views) -> np.ndarray:
+    """Culling flags of every view, [view][brick] (uint8)."""
+    return np.stack([visible_bricks(grid, v) for v in views]).astype(np.uint8)
+
+
+def tsdf_loss64(grid: Grid, views, decoded, priors, voxels, targets, vis=None) -> float:
+    """The FP64 loss of tsdf_backward64 alone (OpenMP over voxels); vis from visibility_flags or None."""
+    lib = load()
+    if not hasattr(lib, "_tl64"):
+        lib.orc64_tsdf_loss.restype = C.c_int32
+        lib.orc64_tsdf_loss.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int32, C.c_void_p, C.c_void_p,
+                                        C.c_int64, C.c_void_p, C.c_void_p]
+        lib._tl64 = True
+    n = len(views)
+    varr = (OrcIntegrationView * n)(*views)
+    inputs, keep = _depth_inputs(decoded, priors)
+    vox = np.ascontiguousarray(voxels, dtype=np.uint32)
+    tgt = np.ascontiguousarray(targets, dtype=np.float64)
+    flags = None if vis is None else np.ascontiguousarray(vis, dtype=np.uint8)
+    loss = np.zeros(1)
+    rc = lib.orc64_tsdf_loss(C.addressof(grid.struct), C.addressof(varr), C.addressof(inputs), n, vox.ctypes.data,
+                             tgt.ctypes.data, vox.size, flags.ctypes.data if flags is not None else None,
+                             loss.ctypes.data)
+    assert rc == 0
+    return float(loss[0])
 
 
 # ---- surface nets and mesh post-processing (mesh_oracle.c) ----------------------------------------

@@ -178,6 +178,24 @@ void orc_unpack_normal(uint32_t packed, float out[3]) {
 /* PK:17-19 */
 uint32_t orc_unpack_cost(uint32_t packed) { return packed >> 28u; }
 
+/* A whole packed map: depth (f16 -> double, 0 where the sample is 0 or not positive), the normal
+ * (orc_unpack_normal, widened to double) and the cost code, for the FP64 TSDF adjoint. */
+void orc_decode_map(const uint32_t *packed, int64_t n, double *depth, double *normal, uint8_t *cost) {
+#pragma omp parallel for schedule(static)
+    for (int64_t i = 0; i < n; i++) {
+        const uint32_t m = packed[i];
+        depth[i] = 0.0;
+        normal[i * 3] = normal[i * 3 + 1] = normal[i * 3 + 2] = 0.0;
+        cost[i] = (uint8_t)(m >> 28u);
+        if (m == 0u) continue;
+        const float d = orc_unpack_depth(m);
+        depth[i] = d > 0.0f ? (double)d : 0.0;
+        float nn[3];
+        orc_unpack_normal(m, nn);
+        for (int k = 0; k < 3; k++) normal[i * 3 + k] = (double)nn[k];
+    }
+}
+
 /* PK:21-33 */
 uint32_t orc_pack_sample(float depth, const float normal[3], float cost) {
     if (!(depth > 0.0f) || cost >= 1.0f) return 0u;

@@ -40,23 +40,30 @@ static int view_step(const orc_tsdf_grid *g, const orc_integration_view *v, cons
                      const double world[3], step64 *s) {
     double cam[3];
     for (int i = 0; i < 3; i++) cam[i] = v->R[i * 3] * world[0] + v->R[i * 3 + 1] * world[1] + v->R[i * 3 + 2] * world[2] + v->t[i];
-    if (!(cam[2] > 0.01)) return 0;
+    const double mu = g->truncation;
+    /* distance to the nearest decision boundary, recorded before any skip: the cam.z > 0.01 test
+     * and the sdf >= -mu test (in units of mu) and the nearest-pixel rounding (pixels); a voxel this
+     * close to one can take the other branch in lower precision */
+    const double kz = fabs(cam[2] - 0.01) / mu;
+    if (!(cam[2] > 0.01)) {
+        s->kink = kz;
+        return 0;
+    }
     const double u = (v->fx * cam[0]) / cam[2] + v->cx, vv = (v->fy * cam[1]) / cam[2] + v->cy;
-    /* distance to a rounding boundary, recorded before any skip: a projection this close to one can
-     * pick another pixel (and another skip decision) in lower precision */
-    s->kink = fmin(fabs(fabs(u - floor(u)) - 0.5), fabs(fabs(vv - floor(vv)) - 0.5));
+    s->kink = fmin(kz, fmin(fabs(fabs(u - floor(u)) - 0.5), fabs(fabs(vv - floor(vv)) - 0.5)));
     const double ru = rint(u), rv = rint(vv);
     if (ru < 0 || rv < 0 || ru >= v->width || rv >= v->height) return 0;
     const size_t i = (size_t)rv * v->width + (size_t)ru;
     double d = in->depth[i];
     const int from_map = d > 0.0;
-    const double mu = g->truncation;
-    const double kink = s->kink;
+    double kink = s->kink;
     memset(s, 0, sizeof(*s));
     s->kink = kink;
     if (!from_map) d = in->prior ? in->prior[i] : 0.0;
     if (!(d > 0.0)) return 0;
     const double sdf = d - cam[2];
+    kink = fmin(kink, fabs(sdf + mu) / mu);
+    s->kink = kink;
     if (sdf < -mu) return 0;
     s->pixel = i;
     if (sdf / mu < 1.0) {
@@ -180,5 +187,48 @@ int32_t orc64_tsdf_backward(const orc_tsdf_grid *g, const orc_integration_view *
     free(vis);
     free(steps);
     free(offs);
+    return 0;
+}
+
+/* The loss alone, L = sum_i (tsdf_i - target_i)^2 (the forward of orc64_tsdf_backward), with the
+ * voxels split over OpenMP threads; for central differences over large voxel lists.  vis: the
+ * culling flags [view][brick] (orc_visible_bricks), or NULL to compute them. */
+int32_t orc64_tsdf_loss(const orc_tsdf_grid *g, const orc_integration_view *views, const orc64_depth_input *inputs,
+                        int32_t n_views, const uint32_t *voxels, const double *targets, int64_t n_voxels,
+                        const uint8_t *vis, double *loss) {
+    uint8_t *own = NULL;
+    if (!vis) {
+        own = (uint8_t *)malloc((size_t)n_views * g->n_bricks + 1);
+        if (!own) return -1;
+#pragma omp parallel for schedule(static)
+        for (int32_t v = 0; v < n_views; v++) orc_visible_bricks(g, &views[v], own + (size_t)v * g->n_bricks);
+        vis = own;
+    }
+    double total = 0.0;
+#pragma omp parallel for schedule(static) reduction(+ : total)
+    for (int64_t q = 0; q < n_voxels; q++) {
+        const uint32_t index = voxels[q];
+        const int32_t b = (int32_t)(index / 512u), local = (int32_t)(index % 512u);
+        const int32_t *c = &g->coords[b * 4];
+        const double world[3] = {
+            g->origin[0] + ((double)(c[0] * 8 + local % 8) + 0.5) * g->voxel,
+            g->origin[1] + ((double)(c[1] * 8 + (local / 8) % 8) + 0.5) * g->voxel,
+            g->origin[2] + ((double)(c[2] * 8 + local / 64) + 0.5) * g->voxel,
+        };
+        double ts = 1.0, W = 0.0;
+        for (int32_t v = 0; v < n_views; v++) {
+            if (!vis[(size_t)v * g->n_bricks + b]) continue;
+            step64 s;
+            s.kink = 1e9;
+            if (!view_step(g, &views[v], &inputs[v], world, &s)) continue;
+            const double D = W + s.w;
+            ts = (ts * W + s.o * s.w) / D;
+            W = fmin(D, MAX_WEIGHT);
+        }
+        const double r = ts - targets[q];
+        total += r * r;
+    }
+    *loss = total;
+    free(own);
     return 0;
 }

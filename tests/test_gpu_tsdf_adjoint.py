@@ -4,8 +4,9 @@ test_oracle_tsdf_adjoint.py) on the sphere workload: 8 views at 320x240 solved a
 GPU, TSDF 128^3 x 8 mm, truncation 3 voxels, two views with a prior, four with confidence weighting.
 
 Bars: the replayed tsdf equals lr3d_tsdf_integrate_batched's bit for bit; against FP64 (on voxels
-whose projections stay 1e-3 px away from a rounding boundary in every view, so that FP32 and FP64
-pick the same pixels): tsdf within 1e-4 (in units of the truncation), loss within 1e-5 relative,
+that stay 1e-3 away from every decision boundary in every view -- the nearest-pixel rounding in
+pixels, the cam.z > 0.01 and sdf >= -mu tests in units of mu -- so that FP32 and FP64 take the
+same branches): tsdf within 1e-4 (in units of the truncation), loss within 1e-5 relative,
 per-view pose gradients within 1e-3 of the view's largest entry (a sum over ~6e4 voxels of terms
 that carry the FP32 state's rounding; measured 1.5e-4), and per-pixel depth gradients
 within 1e-3 + 1e-4 |g| (L per metre) on >= 99.9 % of the pixels that receive one: the FP32 state
@@ -84,7 +85,7 @@ def test_gradients_match_fp64_reference(ctx, setup):
     import lr3d
     w, st, grid, layout, views, vox, tgt, ref, g, ivs, decoded, priors = setup
     keep = ref["kink"] >= 1e-3
-    say(f"case: TSDF adjoint vs FP64 reference, {int(keep.sum())} of {len(vox)} voxels clear of rounding boundaries")
+    say(f"case: TSDF adjoint vs FP64 reference, {int(keep.sum())} of {len(vox)} voxels clear of decision boundaries")
     vk, tk = vox[keep], tgt[keep]
     ref = O.tsdf_backward64(g, ivs, decoded, priors, vk, tk.astype(np.float64))
     vol = lr3d.HipTsdfVolume(ctx, grid, layout, w.tsdf.truncation_voxels)

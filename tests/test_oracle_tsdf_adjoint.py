@@ -218,3 +218,19 @@ def test_sphere_pose_gradients_against_central_differences(sphere):
                 assert f == pytest.approx(gk, rel=1e-5, abs=1e-6 * max(1.0, np.abs(r["grad_pose"][vi]).max())), (vi, k)
                 checked += 1
     assert checked >= 20
+
+
+def test_parallel_loss_equals_the_replay_loss(sphere):
+    """orc64_tsdf_loss (OpenMP over voxels, for difference quotients over large voxel lists) is the
+    forward of orc64_tsdf_backward: the same loss up to FP64 summation order, with the culling flags
+    computed inside or passed in."""
+    g, views, inputs, priors, vox, tgt = sphere
+    r = O.tsdf_backward64(g, views, inputs, priors, vox, tgt)
+    direct = float(((r["tsdf"] - tgt) ** 2).sum())
+    a = O.tsdf_loss64(g, views, inputs, priors, vox, tgt)
+    b = O.tsdf_loss64(g, views, inputs, priors, vox, tgt, vis=O.visibility_flags(g, views))
+    assert a == pytest.approx(r["loss"], rel=1e-12) and a == pytest.approx(direct, rel=1e-12)
+    assert b == pytest.approx(a, rel=1e-12)  # OpenMP's reduction order varies
+    # a voxel list in another order gives the same loss
+    perm = np.random.default_rng(3).permutation(len(vox))
+    assert O.tsdf_loss64(g, views, inputs, priors, vox[perm], tgt[perm]) == pytest.approx(a, rel=1e-12)

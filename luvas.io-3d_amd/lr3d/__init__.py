@@ -14,9 +14,11 @@ from .tsdf import BRICK, BrickGrid, HipTsdfVolume, IntegrationView, dense_grid
 from . import refine
 from .mesh import HipMesh
 from .fusion import fuse_depth_maps
+from .packing import upsample_depth
 
 __all__ = [
     "Lr3dError", "MAP_FILTERED", "MAP_RAW", "MAX_SOURCES", "Intrinsics", "Pose", "StereoView",
     "look_at", "relative_pose", "scale_intrinsics", "Context", "HipDenseStereo", "SolveOptions",
     "BRICK", "BrickGrid", "HipTsdfVolume", "IntegrationView", "dense_grid", "refine", "HipMesh", "fuse_depth_maps",
+    "upsample_depth",
 ]

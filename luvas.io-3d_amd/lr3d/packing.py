@@ -1,4 +1,5 @@
-"""Host decode of the packed per-pixel depth format (frontend/src/app/vision/mvs/depth-maps.ts).
+"""Host decode of the packed per-pixel depth format and the phone-depth prior
+(frontend/src/app/vision/mvs/depth-maps.ts).
 
 u32 = f16 depth (bits 0-15) | octahedral normal q.x, q.y (6 bits each, bits 16-27) |
 min(15, floor(15 * cost)) (bits 28-31); 0 = no depth.
@@ -43,3 +44,37 @@ def unpack_depth_sample(value: int):
     if not d > 0:
         return None
     return d, unpack_normal(value)
+
+
+def upsample_depth(meters, grid_width: int, grid_height: int, width: int, height: int) -> np.ndarray:
+    """depth-maps.ts:9-41 upsampleDepth: bilinear upsampling of a phone depth grid (0 = no data) to
+    width x height, ignoring cells without data; a pixel is 0 unless the weights of the cells with
+    data sum to more than 0.5.  The prior input of a solve (float32, row-major).
+
+    Computed in float64 with the JavaScript's operations in its order (the per-pixel sums add the
+    four samples in order, skipping those without data) and rounded to float32 like the
+    Float32Array it fills."""
+    g = np.asarray(meters, dtype=np.float32).reshape(grid_height, grid_width).astype(np.float64)
+    gw, gh = int(grid_width), int(grid_height)
+
+    def axis(n, gn):
+        c = np.minimum(gn - 1, np.maximum(0.0, ((np.arange(n) + 0.5) / n) * gn - 0.5))
+        c0 = np.floor(c)
+        c1 = np.minimum(gn - 1, c0 + 1)
+        return c0.astype(np.int64), c1.astype(np.int64), c - c0
+
+    y0, y1, wy = axis(int(height), gh)
+    x0, x1, wx = axis(int(width), gw)
+    wy, wx = wy[:, None], wx[None, :]
+    samples = [(g[y0][:, x0], (1 - wx) * (1 - wy)), (g[y0][:, x1], wx * (1 - wy)),
+               (g[y1][:, x0], (1 - wx) * wy), (g[y1][:, x1], wx * wy)]
+    total = np.zeros((int(height), int(width)))
+    weight = np.zeros((int(height), int(width)))
+    with np.errstate(invalid="ignore"):
+        for d, w in samples:
+            w = np.broadcast_to(w, total.shape)
+            ok = d > 0
+            total = np.where(ok, total + d * w, total)
+            weight = np.where(ok, weight + w, weight)
+        out = np.where(weight > 0.5, total / np.where(weight > 0.5, weight, 1.0), 0.0)
+    return out.astype(np.float32).reshape(-1)

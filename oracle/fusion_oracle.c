@@ -103,6 +103,32 @@ static double half_to_double(uint32_t h) {
     return sign * ldexp(1.0, exponent - 15) * (1.0 + mantissa / 1024.0);
 }
 
+/* depth-maps.ts:9-41 upsampleDepth (double arithmetic, stored as float like the Float32Array) */
+void orc_upsample_depth(const float *meters, int32_t gw, int32_t gh, int32_t width, int32_t height, float *out) {
+    for (int32_t y = 0; y < height; y++) {
+        const double gy = fmin(gh - 1, fmax(0.0, ((y + 0.5) / height) * gh - 0.5));
+        const int32_t y0 = (int32_t)floor(gy);
+        const int32_t y1 = y0 + 1 < gh - 1 ? y0 + 1 : gh - 1;
+        const double wy = gy - y0;
+        for (int32_t x = 0; x < width; x++) {
+            const double gx = fmin(gw - 1, fmax(0.0, ((x + 0.5) / width) * gw - 0.5));
+            const int32_t x0 = (int32_t)floor(gx);
+            const int32_t x1 = x0 + 1 < gw - 1 ? x0 + 1 : gw - 1;
+            const double wx = gx - x0;
+            const double d[4] = {meters[y0 * gw + x0], meters[y0 * gw + x1], meters[y1 * gw + x0], meters[y1 * gw + x1]};
+            const double w[4] = {(1 - wx) * (1 - wy), wx * (1 - wy), (1 - wx) * wy, wx * wy};
+            double sum = 0, weight = 0;
+            for (int k = 0; k < 4; k++) {
+                if (d[k] > 0) {
+                    sum += d[k] * w[k];
+                    weight += w[k];
+                }
+            }
+            out[(size_t)y * width + x] = weight > 0.5 ? (float)(sum / weight) : 0.0f;
+        }
+    }
+}
+
 /* depth-maps.ts:65-85 unpackDepthSample */
 int32_t orc_unpack_depth_sample(uint32_t value, double out[4]) {
     if (value == 0) return 0;

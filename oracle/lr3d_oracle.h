@@ -211,6 +211,7 @@ typedef struct {
 } orc_cloud;
 
 /* depth-maps.ts:65-85: out = depth, nx, ny, nz (camera frame); 0 for an empty sample */
+void orc_upsample_depth(const float *meters, int32_t gw, int32_t gh, int32_t width, int32_t height, float *out);
 int32_t orc_unpack_depth_sample(uint32_t value, double out[4]);
 /* fuseDepthMaps (depth-fusion.ts:23-74) with PointAccumulator (point-accumulator.ts), cloud(min_support) */
 int32_t orc_fuse_depth_maps(const orc_fusion_view *views, int32_t n_views, double cell, int64_t cell_budget,

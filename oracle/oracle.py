@@ -395,6 +395,18 @@ class Orc64DepthInput(C.Structure):
     _fields_ = [("depth", C.c_void_p), ("normal", C.c_void_p), ("cost", C.c_void_p), ("prior", C.c_void_p)]
 
 
+def upsample_depth(meters, gw, gh, width, height) -> np.ndarray:
+    lib = load()
+    if not hasattr(lib, "_ud"):
+        lib.orc_upsample_depth.restype = None
+        lib.orc_upsample_depth.argtypes = [C.c_void_p, C.c_int32, C.c_int32, C.c_int32, C.c_int32, C.c_void_p]
+        lib._ud = True
+    m = np.ascontiguousarray(meters, dtype=np.float32).reshape(-1)
+    out = np.zeros(int(width) * int(height), dtype=np.float32)
+    lib.orc_upsample_depth(m.ctypes.data, gw, gh, width, height, out.ctypes.data)
+    return out
+
+
 def decode_maps(packed_maps):
     """Packed u32 maps -> (depth f64, normal f64 (W*H, 3), cost code u8) with the decoders of PK."""
     out = []

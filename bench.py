@@ -17,6 +17,7 @@ Default workload: 32 views at 1280x720 of the seeded sphere scene, 8 neighbours,
                  events on the library's stream over the timed region
   cpu_baseline   the C oracle (same arithmetic, OpenMP) on a bounded sample of the same workload
   plain          the per-view path (one solve / integrate call per view) for the speed-up ratio
+  adjoint        the NCC adjoint of every view in one launch and the TSDF adjoint over 1M voxels
 """
 import argparse
 import json
@@ -61,6 +62,7 @@ def main():
     ap.add_argument("--workload", default="sphere-phone")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
+    ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint legs")
     ap.add_argument("--no-profile", action="store_true", help="time without kernel events")
     ap.add_argument("--no-autotune", action="store_true", help="keep the default launch shapes")
     ap.add_argument("--tiles", default=None,
@@ -212,6 +214,40 @@ def main():
                                   "unit": "GB/s", "frac": round(bytes_alg / t_avg / 1e9 / HBM_PEAK_GBS, 4),
                                   "bytes_per_launch": bytes_alg, "traffic": measured.get("tsdf_integrate")}})
 
+    # Adjoint legs on the same batch (untimed in the step): the NCC adjoint of every view's
+    # aggregated cost in one launch (lr3d_ncc_refine_step with zero steps: cost, dL/dplane and
+    # relative-pose gradients of every pixel), and the TSDF adjoint of the filtered maps' pass over
+    # 1M seeded voxels near the surface (lr3d_tsdf_backward), each timed with HIP events.
+    adjoint = None
+    if not args.no_adjoint:
+        ctx.set_profiling(True)
+        for _ in range(3):
+            st.refine_step(solves, 0.0, 0.0)
+        ncc_launches, ncc_ms = ctx.kernel_stats("ncc_backward")
+        rng = np.random.default_rng(7)
+        coords = layout[0].reshape(-1, 4)
+        cand = np.unique(rng.integers(0, layout[2] * 512, 4_000_000).astype(np.uint32))
+        bb, ll = cand // 512, cand % 512
+        cell = coords[bb, :3] * 8 + np.stack([ll % 8, (ll // 8) % 8, ll // 64], axis=1)
+        mu = spec.voxel * spec.truncation_voxels
+        sd = work.scene.signed_distance(np.asarray(spec.origin) + (cell + 0.5) * spec.voxel)
+        pick = np.flatnonzero(np.abs(sd) < 2 * mu)[:1_000_000]
+        vox, tgt = cand[pick], np.clip(sd[pick] / mu, -1, 1).astype(np.float32)
+        for _ in range(3):
+            vol.backward(st, list(range(n)), iviews, vox, tgt, map=lr3d.MAP_FILTERED)
+        tb_launches, tb_ms = ctx.kernel_stats("tsdf_backward")
+        ctx.set_profiling(False)
+        adjoint = {
+            "ncc": {"views": n, "mpx": round(work.megapixels, 3),
+                    "avg_launch_ms": round(ncc_ms / max(ncc_launches, 1), 3),
+                    "mpx_per_s": round(work.megapixels / (ncc_ms / max(ncc_launches, 1)) * 1e3, 1)},
+            "tsdf": {"views": n, "voxels": int(len(vox)),
+                     "avg_launch_ms": round(tb_ms / max(tb_launches, 1), 3),
+                     "voxel_views_per_s": round(len(vox) * n / (tb_ms / max(tb_launches, 1)) * 1e3, 1)},
+        }
+        log(f"adjoint: NCC {adjoint['ncc']['avg_launch_ms']} ms for {n} views, TSDF "
+            f"{adjoint['tsdf']['avg_launch_ms']} ms for {len(vox)} voxels x {n} views")
+
     plain = None
     if not args.no_plain:
         log("plain path: one call per view")
@@ -285,6 +321,7 @@ def main():
         "cpu_baseline": cpu,
         "tsdf": tsdf,
         "plain": plain,
+        "adjoint": adjoint,
         "accuracy": {"filtered_bad_frac": round(bad / max(valid, 1), 5), "filtered_valid_px": valid,
                      "raw_bad_frac": round(raw_bad / max(raw_valid, 1), 5), "raw_valid_px": raw_valid,
                      "bad_threshold": "relative depth error > 1 % vs analytic depth"},

@@ -45,7 +45,21 @@ export declare class HipDenseStereo {
     refineStep(solves: Array<[number, SolveOptions]>, stepDepth: number, stepNormal: number):
         Promise<{ gradPose: Float64Array; lossBefore: Float64Array; lossAfter: Float64Array }>;
     setPose(layer: number, pose: Pose): void;
+    /** fuseDepthMaps on the GPU over these layers' maps (MAP_FILTERED by default, minSupport 1). */
+    fuseDepthMaps(layers: number[], thumbs: Thumbnail[], cell: number, cellBudget: number, map?: number,
+                  minSupport?: number): Promise<FusedCloud>;
     destroy(): void;
+}
+
+/** A view's colour thumbnail (depth-fusion.ts: `Thumbnail`). */
+export interface Thumbnail { width: number; height: number; rgb: Uint8Array; }
+/** The fused cloud: 3 floats / 3 bytes per point, points in cell-key order; `cell` = final cell size. */
+export interface FusedCloud {
+    positions: Float32Array;
+    normals: Float32Array;
+    colors: Uint8Array;
+    support: Uint16Array;
+    cell: number;
 }
 
 /** Adjoint of the aggregated NCC cost of one reference view's planes. */

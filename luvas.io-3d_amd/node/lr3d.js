@@ -147,6 +147,16 @@ class HipDenseStereo {
         this.cameras[layer] = cameraOf(this.views[layer], this.width, this.height);
     }
 
+    /**
+     * fuseDepthMaps(maps, thumbs, cell, cellBudget) (depth-fusion.ts:23-74) on the GPU, over these
+     * layers' maps (MAP_FILTERED by default): {positions, normals, colors, support, cell}.  Points
+     * come in cell-key order; per-cell values equal the reference accumulator's.
+     */
+    async fuseDepthMaps(layers, thumbs, cell, cellBudget, map, minSupport) {
+        return native.stereoFuseDepthMaps(this.handle, layers, thumbs, cell, cellBudget,
+            map === undefined ? MAP_FILTERED : map, minSupport === undefined ? 1 : minSupport);
+    }
+
     destroy() {
         if (this.handle) native.stereoDestroy(this.handle);
         this.handle = null;

@@ -106,7 +106,7 @@ static int read_view(napi_env env, napi_value o, lr3d_view *v) {
 /* ---- async work -------------------------------------------------------------------------------- */
 
 enum { OP_SOLVE, OP_SOLVE_BATCHED, OP_FILTER, OP_INTEGRATE, OP_READ_DEPTH, OP_READ_FILTERED,
-       OP_READ_HYP, OP_READ_TSDF, OP_NCC_BACKWARD, OP_NCC_REFINE, OP_TSDF_BACKWARD, OP_MESH };
+       OP_READ_HYP, OP_READ_TSDF, OP_NCC_BACKWARD, OP_NCC_REFINE, OP_TSDF_BACKWARD, OP_MESH, OP_FUSE };
 
 typedef struct {
     int op;
@@ -136,6 +136,12 @@ typedef struct {
     /* mesh extraction */
     double min_weight, fraction;
     int32_t min_faces;
+    /* dense-cloud fusion */
+    lr3d_thumb *thumbs;
+    uint8_t **rgbs;
+    double cell, final_cell;
+    int64_t cell_budget;
+    int32_t min_support;
 } job_t;
 
 static void job_free(job_t *j) {
@@ -154,6 +160,11 @@ static void job_free(job_t *j) {
     free(j->weights);
     free(j->targets);
     free(j->voxels);
+    if (j->rgbs) {
+        for (int32_t i = 0; i < j->n; i++) free(j->rgbs[i]);
+        free(j->rgbs);
+    }
+    free(j->thumbs);
     free(j);
 }
 
@@ -199,6 +210,27 @@ static void job_execute(napi_env env, void *data) {
             j->status = lr3d_mesh_read(m, (float *)j->out_a, (float *)j->out_b, (uint32_t *)j->out_c);
         }
         lr3d_mesh_destroy(m);
+        break;
+    }
+    case OP_FUSE: {
+        /* fuseDepthMaps (depth-fusion.ts:23-74): positions, normals, colors, support */
+        lr3d_cloud *c = NULL;
+        j->status = lr3d_fuse_depth_maps(j->st, j->layers, j->map, j->n, j->thumbs, j->cell, j->cell_budget,
+                                         j->min_support, &c, &j->final_cell);
+        int64_t np = 0;
+        if (j->status == LR3D_OK) j->status = lr3d_cloud_size(c, &np);
+        if (j->status == LR3D_OK) {
+            j->bytes_a = j->bytes_b = (size_t)np * 12u;
+            j->bytes_c = (size_t)np * 3u;
+            j->bytes_d = (size_t)np * 2u;
+            j->out_a = malloc(j->bytes_a + 1);
+            j->out_b = malloc(j->bytes_b + 1);
+            j->out_c = malloc(j->bytes_c + 1);
+            j->out_d = malloc(j->bytes_d + 1);
+            j->status = lr3d_cloud_read(c, (float *)j->out_a, (float *)j->out_b, (uint8_t *)j->out_c,
+                                        (uint16_t *)j->out_d);
+        }
+        lr3d_cloud_destroy(c);
         break;
     }
     case OP_TSDF_BACKWARD:
@@ -249,6 +281,15 @@ static void job_complete(napi_env env, napi_status status, void *data) {
             napi_set_named_property(env, result, "positions", typed_from(env, j->out_a, j->bytes_a, napi_float32_array, 4));
             napi_set_named_property(env, result, "normals", typed_from(env, j->out_b, j->bytes_b, napi_float32_array, 4));
             napi_set_named_property(env, result, "indices", typed_from(env, j->out_c, j->bytes_c, napi_uint32_array, 4));
+        } else if (j->op == OP_FUSE) {
+            napi_value cell;
+            napi_create_object(env, &result);
+            napi_set_named_property(env, result, "positions", typed_from(env, j->out_a, j->bytes_a, napi_float32_array, 4));
+            napi_set_named_property(env, result, "normals", typed_from(env, j->out_b, j->bytes_b, napi_float32_array, 4));
+            napi_set_named_property(env, result, "colors", typed_from(env, j->out_c, j->bytes_c, napi_uint8_array, 1));
+            napi_set_named_property(env, result, "support", typed_from(env, j->out_d, j->bytes_d, napi_uint16_array, 2));
+            napi_create_double(env, j->final_cell, &cell);
+            napi_set_named_property(env, result, "cell", cell);
         } else if (j->op == OP_TSDF_BACKWARD) {
             napi_value loss;
             napi_create_object(env, &result);
@@ -709,6 +750,49 @@ static napi_value tsdf_extract_surface(napi_env env, napi_callback_info info) {
     return queue(env, j);
 }
 
+/* stereoFuseDepthMaps(stereo, layers, thumbs [{width, height, rgb}], cell, cellBudget, map, minSupport)
+ * -> {positions, normals, colors, support, cell} */
+static napi_value stereo_fuse_depth_maps(napi_env env, napi_callback_info info) {
+    ARGS(7);
+    job_t *j = new_job(OP_FUSE);
+    j->st = (lr3d_stereo *)get_external(env, argv[0]);
+    uint32_t n = 0, nt = 0;
+    napi_get_array_length(env, argv[1], &n);
+    napi_get_array_length(env, argv[2], &nt);
+    if (nt != n) {
+        job_free(j);
+        napi_throw_range_error(env, NULL, "lr3d.node: one thumbnail per layer");
+        return NULL;
+    }
+    j->n = (int32_t)n;
+    j->layers = (int32_t *)calloc(n ? n : 1, sizeof(int32_t));
+    j->thumbs = (lr3d_thumb *)calloc(n ? n : 1, sizeof(lr3d_thumb));
+    j->rgbs = (uint8_t **)calloc(n ? n : 1, sizeof(uint8_t *));
+    for (uint32_t i = 0; i < n; i++) {
+        napi_value e, t, rgb;
+        napi_get_element(env, argv[1], i, &e);
+        napi_get_value_int32(env, e, &j->layers[i]);
+        napi_get_element(env, argv[2], i, &t);
+        j->thumbs[i].width = (int32_t)get_num(env, t, "width", 0);
+        j->thumbs[i].height = (int32_t)get_num(env, t, "height", 0);
+        size_t count = 0;
+        if (napi_get_named_property(env, t, "rgb", &rgb) == napi_ok) j->rgbs[i] = (uint8_t *)copy_typed(env, rgb, 1, &count);
+        if (!j->rgbs[i] || count < (size_t)j->thumbs[i].width * j->thumbs[i].height * 3u) {
+            job_free(j);
+            napi_throw_type_error(env, NULL, "lr3d.node: a thumbnail needs width, height and rgb (width*height*3 bytes)");
+            return NULL;
+        }
+        j->thumbs[i].rgb = j->rgbs[i];
+    }
+    double budget = 0;
+    napi_get_value_double(env, argv[3], &j->cell);
+    napi_get_value_double(env, argv[4], &budget);
+    j->cell_budget = (int64_t)budget;
+    napi_get_value_int32(env, argv[5], &j->map);
+    napi_get_value_int32(env, argv[6], &j->min_support);
+    return queue(env, j);
+}
+
 static napi_value last_error(napi_env env, napi_callback_info info) {
     (void)info;
     napi_value s;
@@ -731,7 +815,7 @@ static napi_value init(napi_env env, napi_value exports) {
         {"tsdfRead", tsdf_read},             {"lastError", last_error},
         {"nccBackward", ncc_backward},       {"nccRefineStep", ncc_refine_step},
         {"stereoSetPose", stereo_set_pose},  {"tsdfBackward", tsdf_backward},
-        {"tsdfExtractSurface", tsdf_extract_surface},
+        {"tsdfExtractSurface", tsdf_extract_surface}, {"stereoFuseDepthMaps", stereo_fuse_depth_maps},
     };
     for (size_t i = 0; i < sizeof(fns) / sizeof(fns[0]); i++) {
         napi_value f;

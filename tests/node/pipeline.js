@@ -51,6 +51,12 @@ async function main() {
     fs.writeFileSync(path.join(dir, 'tsdf_gd.f32'), Buffer.from(tb.gradDepth.buffer));
     fs.writeFileSync(path.join(dir, 'tsdf_gp.f64'), Buffer.from(tb.gradPose.buffer));
     fs.writeFileSync(path.join(dir, 'losses.json'), JSON.stringify({ ncc: f.loss, tsdf: tb.loss }));
+    // dense cloud of the filtered maps with the thumbnails written by the test
+    const thumbs = layers.map((i) => ({ width: job.thumb[0], height: job.thumb[1],
+        rgb: new Uint8Array(fs.readFileSync(path.join(dir, `thumb${i}.rgb`))) }));
+    const cloud = await run.stereo.fuseDepthMaps(layers, thumbs, job.cell, job.cellBudget);
+    for (const k of ['positions', 'normals', 'colors', 'support']) fs.writeFileSync(path.join(dir, `cloud_${k}.bin`), Buffer.from(cloud[k].buffer));
+    fs.writeFileSync(path.join(dir, 'cloud.json'), JSON.stringify({ cell: cloud.cell }));
     const mesh = await volume.extractSurface(0.6, 0.01, 200);
     for (const k of ['positions', 'normals', 'indices']) fs.writeFileSync(path.join(dir, `mesh_${k}.bin`), Buffer.from(mesh[k].buffer));
     const { tsdf, weight } = await volume.read();

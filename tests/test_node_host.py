@@ -79,6 +79,12 @@ def test_node_pipeline_matches_python_host(tmp_path, ctx):
     job["targets"] = [float(v) for v in rng.uniform(-1, 1, voxels.size).astype(np.float32)]
     for i, im in enumerate(w.images):
         im.tofile(tmp_path / f"gray{i}.u8")
+    # 80 x 60 colour thumbnails: the grey image subsampled, tinted per channel
+    thumbs = [np.stack([im[::4, ::4], im[1::4, 2::4], 255 - im[2::4, 1::4]], axis=-1).astype(np.uint8)
+              for im in w.images]
+    for i, t in enumerate(thumbs):
+        t.tofile(tmp_path / f"thumb{i}.rgb")
+    job.update({"thumb": [80, 60], "cell": 0.005, "cellBudget": 20000})
     (tmp_path / "job.json").write_text(json.dumps(job))
     print("case: node host pipeline, 8 views 320x240", flush=True)
     r = subprocess.run([NODE, os.path.join(ROOT, "tests", "node", "pipeline.js"), str(tmp_path)],
@@ -118,6 +124,12 @@ def test_node_pipeline_matches_python_host(tmp_path, ctx):
     assert np.allclose(gd, np.fromfile(tmp_path / "tsdf_gd.f32", np.float32), rtol=1e-6, atol=1e-9)
     assert np.allclose(tb["grad_pose"], np.fromfile(tmp_path / "tsdf_gp.f64", np.float64).reshape(8, 12), rtol=1e-10, atol=1e-12)
     assert abs(tb["loss"] - losses["tsdf"]) <= 1e-12 * max(1.0, abs(tb["loss"]))
+    # dense cloud: the Node call against the Python host's
+    cloud, cell = lr3d.fuse_depth_maps(st, list(range(8)), thumbs, 0.005, 20000)
+    node_cell = json.loads((tmp_path / "cloud.json").read_text())["cell"]
+    assert cell == node_cell
+    for k, dt in (("positions", np.float32), ("normals", np.float32), ("colors", np.uint8), ("support", np.uint16)):
+        assert np.array_equal(np.asarray(cloud[k]).reshape(-1), np.fromfile(tmp_path / f"cloud_{k}.bin", dt)), k
     # mesh: the Node call against the Python host's extraction
     m = vol.extract_surface(0.6)
     c = m.remove_small_components(0.01, 200)

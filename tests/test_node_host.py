@@ -18,7 +18,7 @@ pytestmark = pytest.mark.skipif(NODE is None, reason="node is not installed")
 EXPORTS = ["ctxCreate", "ctxDestroy", "stereoCreate", "stereoDestroy", "stereoSetImage", "stereoSolve",
            "stereoFilter", "stereoReadDepth", "stereoReadFiltered", "stereoReadHyp", "stereoSetCoarse",
            "tsdfCreate", "tsdfDestroy", "tsdfIntegrate", "tsdfRead", "lastError", "nccBackward",
-           "nccRefineStep", "stereoSetPose", "tsdfBackward", "tsdfExtractSurface"]
+           "nccRefineStep", "stereoSetPose", "tsdfBackward", "tsdfExtractSurface", "stereoFuseDepthMaps"]
 
 
 def node(code, timeout=60):

@@ -1242,6 +1242,14 @@ static int32_t stage_pass(lr3d_tsdf *ts, std::vector<TsdfView> &tv, const lr3d_i
     return LR3D_OK;
 }
 
+// The prior pool when a view of the pass has a prior, else nullptr (the integrate kernel without
+// prior lookups).
+static const float *pass_priors(const lr3d_tsdf *ts, const std::vector<TsdfView> &tv) {
+    for (const TsdfView &v : tv)
+        if (v.has_prior) return ts->priors.ptr;
+    return nullptr;
+}
+
 static int32_t integrate_impl(lr3d_tsdf *ts, const uint32_t *maps, std::vector<TsdfView> &tv,
                               const lr3d_integration_view *views) {
     lr3d_ctx *ctx = ts->ctx;
@@ -1252,7 +1260,7 @@ static int32_t integrate_impl(lr3d_tsdf *ts, const uint32_t *maps, std::vector<T
     if (rc) return rc;
     HIP_TRY(timed(ctx, "tsdf_integrate", [&] {
         return tsdf_launch_integrate(ctx->stream, ts->grid, ts->views.ptr, n, ts->vis.ptr, words,
-                                     ts->coords.ptr, maps, ts->priors.ptr, ts->tsdf.ptr, ts->weight.ptr,
+                                     ts->coords.ptr, maps, pass_priors(ts, tv), ts->tsdf.ptr, ts->weight.ptr,
                                      ts->counting ? ts->counter.ptr : nullptr, ts->bricks_per_block);
     }));
     return finish(ctx);
@@ -1441,7 +1449,7 @@ extern "C" int32_t lr3d_tsdf_autotune(lr3d_tsdf *ts, lr3d_stereo *st, const int3
             HIP_TRY(hipMemcpyAsync(ts->weight.ptr, s_weight.ptr, voxels * 4, hipMemcpyDeviceToDevice, ctx->stream));
             HIP_TRY(hipEventRecord(ev.a, ctx->stream));
             HIP_TRY(tsdf_launch_integrate(ctx->stream, ts->grid, ts->views.ptr, n, ts->vis.ptr, words, ts->coords.ptr,
-                                          maps, ts->priors.ptr, ts->tsdf.ptr, ts->weight.ptr, nullptr,
+                                          maps, pass_priors(ts, tv), ts->tsdf.ptr, ts->weight.ptr, nullptr,
                                           candidates[c]));
             HIP_TRY(hipEventRecord(ev.b, ctx->stream));
             HIP_TRY(hipEventSynchronize(ev.b));

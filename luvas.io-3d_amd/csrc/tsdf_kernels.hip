@@ -20,6 +20,10 @@ namespace lr3d {
 
 namespace {
 
+#ifndef LR3D_TSDF_VIEW_GROUP
+#define LR3D_TSDF_VIEW_GROUP 2
+#endif
+
 constexpr float kMaxWeight = 96.0f;   // TV:7
 constexpr float kPriorWeight = 0.12f; // TV:8
 
@@ -179,8 +183,97 @@ __global__ void __launch_bounds__(256) visibility_kernel(TsdfGridDev g, const in
         vis[(size_t)brick * words + word] = (threadIdx.x & 32u) ? (uint32_t)(bits >> 32) : (uint32_t)bits;
 }
 
-// TV:103-154, one wavefront per brick, all visible views of the pass in order.
+// Depth samples of one view for a lane's 8 voxels: the pixel index (-1 where the voxel is skipped
+// before the lookup: cam.z <= 0.01 or outside the image) and the packed map value there.
+constexpr int kViewGroup = LR3D_TSDF_VIEW_GROUP;  // views gathered before their updates are applied
+
+struct ViewGather {
+    int32_t idx[8];
+    uint32_t packed[8];
+    float prior[8];  // the prior at the pixel (views with a prior), else 0
+};
+
+// TV:110-121 up to the map lookup.  cam = R world + t, row i = fmaf(R_i2, z, fmaf(R_i1, y, fmaf(R_i0, x, t_i))).
+// Every lane loads (a skipped voxel reads pixel 0 and is marked idx = -1), so the loads of a view
+// are straight-line code and the waits before their use count only the loads issued after them.
+// PRIORS: the pass has views with a prior; the prior at the pixel is gathered for those views.
+template <bool PRIORS>
+__device__ __forceinline__ void gather_view(const TsdfView &V, const uint32_t *__restrict__ maps,
+                                            const float *__restrict__ priors, float wx, float wy, const float wz[8],
+                                            ViewGather &out) {
+    const uint32_t *__restrict__ map = maps + V.map_off;
+    const float c0 = fmaf(V.R[1], wy, fmaf(V.R[0], wx, V.t[0]));
+    const float c1 = fmaf(V.R[4], wy, fmaf(V.R[3], wx, V.t[1]));
+    const float c2 = fmaf(V.R[7], wy, fmaf(V.R[6], wx, V.t[2]));
+#pragma unroll
+    for (int z = 0; z < 8; z++) {
+        const float cam0 = fmaf(V.R[2], wz[z], c0);
+        const float cam1 = fmaf(V.R[5], wz[z], c1);
+        const float cam2 = fmaf(V.R[8], wz[z], c2);
+        const bool front = cam2 > 0.01f;
+        const float zc = front ? cam2 : 1.0f;
+        const int32_t px = (int32_t)rintf((V.k[0] * cam0) / zc + V.k[2]);
+        const int32_t py = (int32_t)rintf((V.k[1] * cam1) / zc + V.k[3]);
+        const bool in = front && px >= 0 && py >= 0 && px < V.W && py < V.H;
+        const uint32_t i = in ? (uint32_t)py * (uint32_t)V.W + (uint32_t)px : 0u;
+        out.idx[z] = in ? (int32_t)i : -1;
+        out.packed[z] = map[i];
+        out.prior[z] = 0.0f;
+        if (PRIORS && V.has_prior) out.prior[z] = priors[V.prior_off + i];  // V is wave-uniform
+    }
+}
+
+// TV:122-153 from the map lookup on: the update of a lane's 8 voxels by one view.
 template <bool COUNT>
+__device__ __forceinline__ void apply_view(const TsdfView &V, const ViewGather &gv, float mu, float wx, float wy,
+                                           const float wz[8], float ts[8], float ws[8], uint32_t &changed,
+                                           uint32_t &updates) {
+    const float c0 = fmaf(V.R[1], wy, fmaf(V.R[0], wx, V.t[0]));
+    const float c1 = fmaf(V.R[4], wy, fmaf(V.R[3], wx, V.t[1]));
+    const float c2 = fmaf(V.R[7], wy, fmaf(V.R[6], wx, V.t[2]));
+#pragma unroll
+    for (int z = 0; z < 8; z++) {
+        if (gv.idx[z] < 0) continue;
+        const float cam0 = fmaf(V.R[2], wz[z], c0);
+        const float cam1 = fmaf(V.R[5], wz[z], c1);
+        const float cam2 = fmaf(V.R[8], wz[z], c2);
+        const uint32_t packed = gv.packed[z];
+        float d = f16_bits_to_f32(packed);
+        const bool from_map = d > 0.0f;
+        if (!from_map) d = V.has_prior ? gv.prior[z] : 0.0f;
+        if (!(d > 0.0f)) continue;
+        const float sdf = d - cam2;
+        if (sdf < -mu) continue;
+        // The weight only matters for an update, so it is evaluated after the skips.
+        float w;
+        if (from_map) {
+            float nx, ny, nz;
+            unpack_normal(packed, nx, ny, nz);
+            const float inv = 1.0f / sqrtf(cam0 * cam0 + cam1 * cam1 + cam2 * cam2);
+            w = fmaxf(0.2f, fabsf(nx * (cam0 * inv) + ny * (cam1 * inv) + nz * (cam2 * inv)));
+            if (V.confidence > 0.0f) {
+                const float cc = (float)(packed >> 28u) / 8.0f;
+                const float r = fminf(fmaxf(V.confidence / d, 0.5f), 2.0f);
+                w = w * ((fmaxf(0.15f, (1.0f - cc) * (1.0f - cc)) * r) * r);
+            }
+        } else {
+            w = kPriorWeight;
+        }
+        const float observed = fminf(1.0f, sdf / mu);
+        const float w0 = ws[z];
+        ts[z] = (ts[z] * w0 + observed * w) / (w0 + w);
+        ws[z] = fminf(w0 + w, kMaxWeight);
+        changed |= 1u << z;
+        if (COUNT) updates++;
+    }
+}
+
+// TV:103-154, one wavefront per brick, all visible views of the pass in order.  Visible views are
+// taken kViewGroup at a time: the group's depth samples are all gathered before the first view's
+// updates, so the later views' gathers are in flight while the earlier ones are applied (a brick's
+// chain of views is not one gather latency per view); the updates are the same sequence as
+// applying the views one at a time.  Groups of 2 measured fastest (4 and 6 hold more registers).
+template <bool COUNT, bool PRIORS>
 __global__ void __launch_bounds__(256) integrate_kernel(TsdfGridDev g, const TsdfView *__restrict__ views,
                                                         int32_t n_views, const uint32_t *__restrict__ vis,
                                                         int32_t words, const int4 *__restrict__ coords,
@@ -188,7 +281,9 @@ __global__ void __launch_bounds__(256) integrate_kernel(TsdfGridDev g, const Tsd
                                                         const float *__restrict__ priors,
                                                         float *__restrict__ tsdf, float *__restrict__ weight,
                                                         unsigned long long *__restrict__ counter) {
-    const uint32_t brick = blockIdx.x * (blockDim.x / 64u) + (threadIdx.x >> 6);
+    // one brick per wavefront: the brick, its visibility words and the views are wave-uniform
+    // (scalar loads, and the prior branch below is decided per view, not per lane)
+    const uint32_t brick = __builtin_amdgcn_readfirstlane(blockIdx.x * (blockDim.x / 64u) + (threadIdx.x >> 6));
     const uint32_t lane = threadIdx.x & 63u;
     if (brick >= (uint32_t)g.n_bricks) return;
     const int4 c = coords[brick];
@@ -200,70 +295,47 @@ __global__ void __launch_bounds__(256) integrate_kernel(TsdfGridDev g, const Tsd
     for (int z = 0; z < 8; z++) wz[z] = g.origin[2] + ((float)(c.z * 8 + z) + 0.5f) * g.voxel;
     const float mu = g.mu;
     const size_t base = (size_t)brick * 512u + lane;
+    const uint32_t *__restrict__ bvis = vis + (size_t)brick * words;
+
+    // visible views in order (bit j of word k = view 32k + j)
+    int32_t word = 0;
+    uint32_t bits = words > 0 ? __builtin_amdgcn_readfirstlane(bvis[0]) : 0u;
+    auto next_view = [&]() -> int32_t {
+        while (bits == 0u) {
+            if (++word >= words) return -1;
+            bits = __builtin_amdgcn_readfirstlane(bvis[word]);
+        }
+        const int32_t v = __builtin_amdgcn_readfirstlane(word * 32 + (int32_t)__builtin_ctz(bits));
+        bits &= bits - 1u;
+        return v;
+    };
+    int32_t v = next_view();
+    if (v < 0) return;  // no view reaches this brick: nothing read, nothing written
 
     float ts[8], ws[8];
+#pragma unroll
+    for (int z = 0; z < 8; z++) {
+        ts[z] = tsdf[base + 64u * z];
+        ws[z] = weight[base + 64u * z];
+    }
     uint32_t changed = 0;  // bit z: voxel z of this lane was updated
-    bool loaded = false;
     uint32_t updates = 0;
-    for (int32_t word = 0; word < words; word++) {
-        uint32_t bits = vis[(size_t)brick * words + word];
-        while (bits) {
-            const int32_t v = word * 32 + __builtin_ctz(bits);
-            bits &= bits - 1u;
-            if (!loaded) {
-#pragma unroll
-                for (int z = 0; z < 8; z++) {
-                    ts[z] = tsdf[base + 64u * z];
-                    ws[z] = weight[base + 64u * z];
-                }
-                loaded = true;
-            }
-            const TsdfView &V = views[v];
-            const uint32_t *__restrict__ map = maps + V.map_off;
-            // cam = R world + t, row i = fmaf(R_i2, z, fmaf(R_i1, y, fmaf(R_i0, x, t_i)))
-            const float c0 = fmaf(V.R[1], wy, fmaf(V.R[0], wx, V.t[0]));
-            const float c1 = fmaf(V.R[4], wy, fmaf(V.R[3], wx, V.t[1]));
-            const float c2 = fmaf(V.R[7], wy, fmaf(V.R[6], wx, V.t[2]));
-#pragma unroll
-            for (int z = 0; z < 8; z++) {
-                const float cam0 = fmaf(V.R[2], wz[z], c0);
-                const float cam1 = fmaf(V.R[5], wz[z], c1);
-                const float cam2 = fmaf(V.R[8], wz[z], c2);
-                if (!(cam2 > 0.01f)) continue;
-                const int32_t px = (int32_t)rintf((V.k[0] * cam0) / cam2 + V.k[2]);
-                const int32_t py = (int32_t)rintf((V.k[1] * cam1) / cam2 + V.k[3]);
-                if (px < 0 || py < 0 || px >= V.W || py >= V.H) continue;
-                const uint32_t i = (uint32_t)py * (uint32_t)V.W + (uint32_t)px;
-                const uint32_t packed = map[i];
-                float d = f16_bits_to_f32(packed);
-                const bool from_map = d > 0.0f;
-                if (!from_map) d = V.has_prior ? priors[V.prior_off + i] : 0.0f;
-                if (!(d > 0.0f)) continue;
-                const float sdf = d - cam2;
-                if (sdf < -mu) continue;
-                // The weight only matters for an update, so it is evaluated after the skips.
-                float w;
-                if (from_map) {
-                    float nx, ny, nz;
-                    unpack_normal(packed, nx, ny, nz);
-                    const float inv = 1.0f / sqrtf(cam0 * cam0 + cam1 * cam1 + cam2 * cam2);
-                    w = fmaxf(0.2f, fabsf(nx * (cam0 * inv) + ny * (cam1 * inv) + nz * (cam2 * inv)));
-                    if (V.confidence > 0.0f) {
-                        const float cc = (float)(packed >> 28u) / 8.0f;
-                        const float r = fminf(fmaxf(V.confidence / d, 0.5f), 2.0f);
-                        w = w * ((fmaxf(0.15f, (1.0f - cc) * (1.0f - cc)) * r) * r);
-                    }
-                } else {
-                    w = kPriorWeight;
-                }
-                const float observed = fminf(1.0f, sdf / mu);
-                const float w0 = ws[z];
-                ts[z] = (ts[z] * w0 + observed * w) / (w0 + w);
-                ws[z] = fminf(w0 + w, kMaxWeight);
-                changed |= 1u << z;
-                if (COUNT) updates++;
-            }
+    while (v >= 0) {
+        int32_t vs[kViewGroup];
+        int32_t n = 1;
+        vs[0] = v;
+        for (; n < kViewGroup; n++) {
+            vs[n] = next_view();
+            if (vs[n] < 0) break;
         }
+        ViewGather grp[kViewGroup];
+#pragma unroll
+        for (int k = 0; k < kViewGroup; k++)
+            if (k < n) gather_view<PRIORS>(views[vs[k]], maps, priors, wx, wy, wz, grp[k]);
+#pragma unroll
+        for (int k = 0; k < kViewGroup; k++)
+            if (k < n) apply_view<COUNT>(views[vs[k]], grp[k], mu, wx, wy, wz, ts, ws, changed, updates);
+        v = n == kViewGroup ? next_view() : -1;
     }
 #pragma unroll
     for (int z = 0; z < 8; z++) {
@@ -515,11 +587,18 @@ hipError_t tsdf_launch_integrate(hipStream_t s, const TsdfGridDev &g, const Tsdf
     if (bricks_per_block != 1 && bricks_per_block != 2 && bricks_per_block != 4) return hipErrorInvalidValue;
     const uint32_t bpb = (uint32_t)bricks_per_block;
     dim3 block(64u * bpb), grid(((uint32_t)g.n_bricks + bpb - 1u) / bpb);
-    if (counter)
-        hipLaunchKernelGGL((integrate_kernel<true>), grid, block, 0, s, g, views, n_views, vis, words,
+    // priors != nullptr: some view of the pass has a prior (the caller passes nullptr otherwise)
+    if (counter && priors)
+        hipLaunchKernelGGL((integrate_kernel<true, true>), grid, block, 0, s, g, views, n_views, vis, words,
+                           coords, maps, priors, tsdf, weight, counter);
+    else if (counter)
+        hipLaunchKernelGGL((integrate_kernel<true, false>), grid, block, 0, s, g, views, n_views, vis, words,
+                           coords, maps, priors, tsdf, weight, counter);
+    else if (priors)
+        hipLaunchKernelGGL((integrate_kernel<false, true>), grid, block, 0, s, g, views, n_views, vis, words,
                            coords, maps, priors, tsdf, weight, counter);
     else
-        hipLaunchKernelGGL((integrate_kernel<false>), grid, block, 0, s, g, views, n_views, vis, words,
+        hipLaunchKernelGGL((integrate_kernel<false, false>), grid, block, 0, s, g, views, n_views, vis, words,
                            coords, maps, priors, tsdf, weight, counter);
     return hipGetLastError();
 }

@@ -64,14 +64,16 @@ def _pose_totals(stereo, solves, grads) -> dict:
 
 
 def refine(stereo, solves, steps: int = 50, step_depth: float = 2e-4, step_normal: float = 2e-3,
-           step_rot: float = 1e-3, step_trans: float = 1e-3, fixed=(0,), resolve: bool = True, log=None):
+           step_rot: float = 1e-3, step_trans: float = 1e-3, fixed=(0,), resolve: bool = True,
+           momentum: float = 0.7, log=None):
     """`steps` joint gradient steps on the planes of every (ref, options) and on the poses of every
     view not in `fixed` (gauge).  The planes must be solved at the current poses beforehand.
 
     Every step takes one plane step on the GPU (lr3d_ncc_refine_step: a gradient step on each
     pixel's plane, kept where it lowers the pixel's cost, with the pose gradients at the planes
     before it) and one pose step: each view turns by step_rot radians and moves by step_trans
-    metres along its negative gradient.
+    metres along its negative gradient direction, averaged with the previous steps' directions
+    (heavy-ball momentum, reset when a step is rejected).
 
     resolve=True: after the pose step every view's planes are solved again at the new poses
     (PatchMatch with the solves' options), followed by a plane step, and the pose step is kept if
@@ -90,14 +92,19 @@ def refine(stereo, solves, steps: int = 50, step_depth: float = 2e-4, step_norma
     grads, before, after = stereo.refine_step(solves, step_depth, step_normal)
     history.append(float(before.sum()))
     loss = float(after.sum())
+    velocity = {}
     for it in range(steps):
         total = _pose_totals(stereo, solves, grads)
         saved = {lay: stereo.views[lay].pose for lay in total if lay not in fixed}
         for lay, (w, tau) in total.items():
             if lay in fixed:
                 continue
-            dw = -rot * w / (np.linalg.norm(w) + 1e-30)
-            dt = -trans * tau / (np.linalg.norm(tau) + 1e-30)
+            vw, vt = velocity.get(lay, (np.zeros(3), np.zeros(3)))
+            vw = momentum * vw + w / (np.linalg.norm(w) + 1e-30)
+            vt = momentum * vt + tau / (np.linalg.norm(tau) + 1e-30)
+            velocity[lay] = (vw, vt)
+            dw = -rot * vw / (np.linalg.norm(vw) + 1e-30)
+            dt = -trans * vt / (np.linalg.norm(vt) + 1e-30)
             stereo.set_pose(lay, perturb(stereo.views[lay].pose, dw, dt))
         if resolve:
             stereo.solve_batched(solves)
@@ -112,6 +119,7 @@ def refine(stereo, solves, steps: int = 50, step_depth: float = 2e-4, step_norma
                 grads, _, a3 = stereo.refine_step(solves, step_depth, step_normal)
                 loss = min(loss, float(a3.sum()))
                 rot, trans = rot * 0.5, trans * 0.5
+                velocity = {}
         else:
             _, trial, _ = stereo.refine_step(solves, 0.0, 0.0)
             if float(trial.sum()) <= loss:
@@ -121,6 +129,7 @@ def refine(stereo, solves, steps: int = 50, step_depth: float = 2e-4, step_norma
                 for lay, pose in saved.items():
                     stereo.set_pose(lay, pose)
                 rot, trans = rot * 0.5, trans * 0.5
+                velocity = {}
             if it + 1 < steps:
                 grads, _, a2 = stereo.refine_step(solves, step_depth, step_normal)
                 loss = min(loss, float(a2.sum()))

@@ -137,3 +137,40 @@ def refine(stereo, solves, steps: int = 50, step_depth: float = 2e-4, step_norma
         if log:
             log(it, loss)
     return history
+
+
+def half_resolution(stereo, images):
+    """A half-resolution copy of a stereo set for the coarse stage of refine_pyramid: every view's
+    frame box-filtered 2 x 2 (rounded), intrinsics scaled by 0.5 keeping pixel centres aligned
+    (camera.ts:28-36), the same poses."""
+    from .camera import StereoView, scale_intrinsics
+    from .stereo import HipDenseStereo
+    views, frames = [], []
+    for v, im in zip(stereo.views, images):
+        im = np.asarray(im, dtype=np.uint16)
+        h, w = im.shape[0] // 2, im.shape[1] // 2
+        im = im[:2 * h, :2 * w]
+        frames.append(((im[0::2, 0::2] + im[1::2, 0::2] + im[0::2, 1::2] + im[1::2, 1::2] + 2) // 4).astype(np.uint8))
+        views.append(StereoView(scale_intrinsics(v.intrinsics, w / v.width, h / v.height), v.pose, w, h))
+    coarse = HipDenseStereo(stereo.ctx, views)
+    for i, f in enumerate(frames):
+        coarse.set_image(i, f)
+    return coarse
+
+
+def refine_pyramid(stereo, coarse, solves, coarse_steps: int = 25, steps: int = 25, fixed=(0,), log=None, **kw):
+    """Pose refinement coarse to fine: `coarse_steps` steps of refine() on `coarse` (a
+    half-resolution copy of the set, see half_resolution) from the poses of `stereo`, then the
+    poses copied back, the planes of `stereo` solved at them and `steps` steps of refine() on
+    `stereo`.  At half resolution a pose error moves the windows half as many pixels, so more of
+    them stay inside the NCC basin.  Returns (coarse history, fine history)."""
+    for lay in range(len(stereo.views)):
+        coarse.set_pose(lay, stereo.views[lay].pose)
+    coarse.solve_batched(solves)
+    hc = refine(coarse, solves, steps=coarse_steps, fixed=fixed, log=log, **kw)
+    for lay in range(len(stereo.views)):
+        if lay not in fixed:
+            stereo.set_pose(lay, coarse.views[lay].pose)
+    stereo.solve_batched(solves)
+    hf = refine(stereo, solves, steps=steps, fixed=fixed, log=log, **kw)
+    return hc, hf

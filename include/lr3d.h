@@ -159,7 +159,8 @@ int32_t lr3d_ncc_backward(lr3d_stereo *st, const lr3d_solve_desc *desc, const fl
  * the tangential part of dL/dn (kept facing the camera); a pixel keeps the step only where its
  * aggregated cost becomes lower, and the packed depth maps of descs with pack set are rewritten.
  * Returns, per desc, the relative-pose gradients at the planes before the step (n*8*12 doubles, as
- * lr3d_ncc_backward) and the loss before and after the step (n doubles each; any may be NULL). */
+ * lr3d_ncc_backward) and the loss before and after the step (n doubles each; any may be NULL).
+ * With both steps 0 it evaluates only: planes and packed maps are left unchanged. */
 int32_t lr3d_ncc_refine_step(lr3d_stereo *st, const lr3d_solve_desc *descs, int32_t n,
                              float step_depth, float step_normal, double *grad_pose,
                              double *loss_before, double *loss_after);

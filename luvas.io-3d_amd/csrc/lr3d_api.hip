@@ -1063,7 +1063,10 @@ extern "C" int32_t lr3d_ncc_refine_step(lr3d_stereo *st, const lr3d_solve_desc *
     HIP_TRY(hipSetDevice(ctx->device));
     if (!(step_depth >= 0.0f) || !(step_normal >= 0.0f))
         return fail(LR3D_ERR_INVALID, "lr3d_ncc_refine_step: steps must be >= 0");
-    return adjoint_impl(st, descs, n, nullptr, false, step_depth, step_normal, 1, grad_pose, loss_before,
+    // zero steps evaluate only: no plane is rewritten (a renormalised copy of a plane can score a
+    // last-bit lower cost) and the packed maps are left alone
+    const int32_t apply = (step_depth > 0.0f || step_normal > 0.0f) ? 1 : 0;
+    return adjoint_impl(st, descs, n, nullptr, false, step_depth, step_normal, apply, grad_pose, loss_before,
                         loss_after);
 }
 

@@ -157,6 +157,19 @@ def test_world_pose_gradients_match_central_differences(solved, reference):
     assert np.median(errs) <= 5e-3 and max(errs) <= 2e-2
 
 
+def test_zero_step_refine_only_evaluates(solved):
+    """lr3d_ncc_refine_step with both steps 0 evaluates: planes, costs and packed maps unchanged."""
+    w, st, o, hyp, cost = solved
+    say("case: refine step with zero steps leaves the set unchanged")
+    packed = st.depth_map(0)
+    gp, before, after = st.refine_step([(0, o)], 0.0, 0.0)
+    h2, c2 = st.hypotheses(0)
+    assert np.array_equal(h2.view(np.uint32), hyp.view(np.uint32))
+    assert np.array_equal(c2.view(np.uint32), cost.view(np.uint32))
+    assert np.array_equal(st.depth_map(0), packed)
+    assert before[0] == after[0] and np.isfinite(gp).all()
+
+
 def test_refinement_lowers_the_cost_with_noisy_poses(ctx):
     import lr3d
     from lr3d.refine import perturb, refine

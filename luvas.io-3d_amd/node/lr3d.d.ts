@@ -148,3 +148,25 @@ export declare const MAP_RAW: 0;
 export declare const MAP_FILTERED: 1;
 export declare const MAX_SOURCES: 8;
 export declare const BRICK: 8;
+
+/** Options of refine / refinePyramid (defaults as in the Python host's lr3d.refine.refine). */
+export interface RefineOptions {
+    steps?: number;
+    coarseSteps?: number;
+    stepDepth?: number;
+    stepNormal?: number;
+    stepRot?: number;
+    stepTrans?: number;
+    /** layers whose poses stay fixed (the gauge); default [0] */
+    fixed?: number[];
+    momentum?: number;
+}
+/** Joint gradient refinement of planes and poses through the NCC adjoint; returns the loss history. */
+export declare function refine(stereo: HipDenseStereo, solves: Array<[number, SolveOptions]>,
+                               options?: RefineOptions): Promise<number[]>;
+/** Half-resolution copy of a stereo set (frames box-filtered 2 x 2) for refinePyramid. */
+export declare function halfResolution(stereo: HipDenseStereo, images: Uint8Array[]): HipDenseStereo;
+/** Refinement coarse to fine: coarseSteps on `coarse`, then steps on `stereo`. */
+export declare function refinePyramid(stereo: HipDenseStereo, coarse: HipDenseStereo,
+                                      solves: Array<[number, SolveOptions]>, options?: RefineOptions):
+    Promise<{ coarse: number[]; fine: number[] }>;

@@ -49,7 +49,7 @@ class HipDenseStereo {
         this.ctx = ctx;
         this.width = width;
         this.height = height;
-        this.views = views;
+        this.views = views.slice();  // own copy: setPose replaces entries here, not in the caller's array
         this.cameras = views.map((v) => cameraOf(v, width, height));
         this.handle = native.stereoCreate(ctx.handle, this.cameras);
         this.lastFiltered = -1;
@@ -402,7 +402,190 @@ const STEREO_METHODS = {
     },
 };
 
+// ---- gradient refinement of planes and poses (no counterpart in the reference) -----------------
+//
+// The host side of the NCC adjoint's refinement, as in the Python host (lr3d/refine.py): pose
+// perturbation on the left, R' = exp([w]x) R, t' = exp([w]x) t + tau; relative pose of reference r
+// and source s R = R_s R_r^T, t = t_s - R t_r (camera.ts:73-77); matrices row-major arrays of 9.
+
+function mat3Mul(A, B) {
+    const C = new Array(9);
+    for (let i = 0; i < 3; i++)
+        for (let j = 0; j < 3; j++) C[i * 3 + j] = A[i * 3] * B[j] + A[i * 3 + 1] * B[3 + j] + A[i * 3 + 2] * B[6 + j];
+    return C;
+}
+function mat3Vec(A, v) {
+    return [0, 1, 2].map((i) => A[i * 3] * v[0] + A[i * 3 + 1] * v[1] + A[i * 3 + 2] * v[2]);
+}
+function mat3T(A) {
+    return [A[0], A[3], A[6], A[1], A[4], A[7], A[2], A[5], A[8]];
+}
+function cross(a, b) {
+    return [a[1] * b[2] - a[2] * b[1], a[2] * b[0] - a[0] * b[2], a[0] * b[1] - a[1] * b[0]];
+}
+function norm3(v) {
+    return Math.sqrt(v[0] * v[0] + v[1] * v[1] + v[2] * v[2]);
+}
+
+/** Rodrigues: exp([w]x). */
+function expSo3(w) {
+    const theta = norm3(w);
+    const K = [0, -w[2], w[1], w[2], 0, -w[0], -w[1], w[0], 0];
+    const I = [1, 0, 0, 0, 1, 0, 0, 0, 1];
+    if (theta < 1e-12) return I.map((x, i) => x + K[i]);
+    const KK = mat3Mul(K, K);
+    const a = Math.sin(theta) / theta, b = (1 - Math.cos(theta)) / (theta * theta);
+    return I.map((x, i) => x + a * K[i] + b * KK[i]);
+}
+
+/** R' = exp([w]x) R, t' = exp([w]x) t + tau. */
+function perturbPose(pose, w, tau) {
+    const E = expSo3(w);
+    const t = mat3Vec(E, Array.from(pose.t));
+    return { R: mat3Mul(E, Array.from(pose.R)), t: [t[0] + tau[0], t[1] + tau[1], t[2] + tau[2]] };
+}
+
+/**
+ * World-pose gradients {layer: {w: dL/dw, tau: dL/dtau}} from the relative-pose gradients of one
+ * reference view's sources (gradRel: per source slot dL/dR (9, row-major) then dL/dt (3)).
+ *   source:    dL/dw = sum_k R[:,k] x G[:,k] + t x g,  dL/dtau = g
+ *   reference: dL/dw = -vee(B - B^T), B = R^T G,        dL/dtau = -R^T g
+ */
+function worldPoseGradients(views, ref, sources, gradRel, out) {
+    out = out || new Map();
+    const add = (layer, w, tau) => {
+        const cur = out.get(layer) || { w: [0, 0, 0], tau: [0, 0, 0] };
+        out.set(layer, { w: cur.w.map((x, i) => x + w[i]), tau: cur.tau.map((x, i) => x + tau[i]) });
+    };
+    const Rr = Array.from(views[ref].pose.R), tr = Array.from(views[ref].pose.t);
+    sources.forEach((layer, s) => {
+        const G = Array.from(gradRel.slice(s * 12, s * 12 + 9));
+        const g = Array.from(gradRel.slice(s * 12 + 9, s * 12 + 12));
+        const R = mat3Mul(Array.from(views[layer].pose.R), mat3T(Rr));
+        const Rt = mat3Vec(R, tr);
+        const t = Array.from(views[layer].pose.t).map((x, i) => x - Rt[i]);
+        let wSrc = [0, 0, 0];
+        for (let k = 0; k < 3; k++) {
+            const c = cross([R[k], R[3 + k], R[6 + k]], [G[k], G[3 + k], G[6 + k]]);
+            wSrc = wSrc.map((x, i) => x + c[i]);
+        }
+        const tg = cross(t, g);
+        wSrc = wSrc.map((x, i) => x + tg[i]);
+        const B = mat3Mul(mat3T(R), G);
+        const wRef = [-(B[7] - B[5]), -(B[2] - B[6]), -(B[3] - B[1])];
+        const RTg = mat3Vec(mat3T(R), g);
+        add(layer, wSrc, g);
+        add(ref, wRef, RTg.map((x) => -x));
+    });
+    return out;
+}
+
+function poseTotals(stereo, solves, gradPose) {
+    const out = new Map();
+    solves.forEach(([ref, o], i) => {
+        const srcs = o.sources.slice(0, MAX_SOURCES);
+        worldPoseGradients(stereo.views, ref, srcs, gradPose.subarray(i * MAX_SOURCES * 12, (i + 1) * MAX_SOURCES * 12), out);
+    });
+    return out;
+}
+
+const sum = (a) => a.reduce((x, y) => x + y, 0);
+
+/**
+ * `steps` joint steps on the planes of every [ref, options] and the poses of every view not in
+ * options.fixed (default [0], the gauge): a plane step on the GPU (refineStep), then each view
+ * turns by stepRot radians and moves by stepTrans metres along its negative gradient direction,
+ * averaged with the previous steps' (momentum); the planes are solved again at the new poses and
+ * given a plane step, and the pose step is kept if the loss is lower (steps x 1.5), else undone
+ * (poses restored, planes solved again, which reproduces them) and halved.  The planes must be
+ * solved at the current poses.  Returns the loss history (non-increasing).
+ */
+async function refine(stereo, solves, options) {
+    const o = Object.assign({ steps: 50, stepDepth: 2e-4, stepNormal: 2e-3, stepRot: 1e-3, stepTrans: 1e-3,
+        fixed: [0], momentum: 0.7 }, options || {});
+    let rot = o.stepRot, trans = o.stepTrans;
+    let r = await stereo.refineStep(solves, o.stepDepth, o.stepNormal);
+    const history = [sum(r.lossBefore)];
+    let loss = sum(r.lossAfter);
+    let grads = r.gradPose;
+    let velocity = new Map();
+    for (let it = 0; it < o.steps; it++) {
+        const totals = poseTotals(stereo, solves, grads);
+        const saved = new Map();
+        for (const [layer, gr] of totals) {
+            if (o.fixed.includes(layer)) continue;
+            saved.set(layer, stereo.views[layer].pose);
+            const v = velocity.get(layer) || { w: [0, 0, 0], tau: [0, 0, 0] };
+            const nw = norm3(gr.w) + 1e-30, nt = norm3(gr.tau) + 1e-30;
+            const vw = v.w.map((x, i) => o.momentum * x + gr.w[i] / nw);
+            const vt = v.tau.map((x, i) => o.momentum * x + gr.tau[i] / nt);
+            velocity.set(layer, { w: vw, tau: vt });
+            const sw = -rot / (norm3(vw) + 1e-30), st = -trans / (norm3(vt) + 1e-30);
+            stereo.setPose(layer, perturbPose(stereo.views[layer].pose, vw.map((x) => x * sw), vt.map((x) => x * st)));
+        }
+        await stereo.solveBatched(solves);
+        r = await stereo.refineStep(solves, o.stepDepth, o.stepNormal);
+        if (sum(r.lossAfter) < loss) {
+            loss = sum(r.lossAfter);
+            grads = r.gradPose;
+            rot *= 1.5;
+            trans *= 1.5;
+        } else {
+            for (const [layer, pose] of saved) stereo.setPose(layer, pose);
+            await stereo.solveBatched(solves);
+            r = await stereo.refineStep(solves, o.stepDepth, o.stepNormal);
+            grads = r.gradPose;
+            loss = Math.min(loss, sum(r.lossAfter));
+            rot *= 0.5;
+            trans *= 0.5;
+            velocity = new Map();
+        }
+        history.push(loss);
+    }
+    return history;
+}
+
+/**
+ * A half-resolution copy of a stereo set for the coarse stage of refinePyramid: frames box-filtered
+ * 2 x 2 (rounded), intrinsics scaled by 0.5 with pixel centres aligned (camera.ts:28-36).
+ */
+function halfResolution(stereo, images) {
+    const views = stereo.views.map((v, i) => {
+        const c = stereo.cameras[i];
+        const w = Math.floor(c.width / 2), h = Math.floor(c.height / 2);
+        const sx = w / c.width, sy = h / c.height, k = v.intrinsics;
+        return { intrinsics: { fx: k.fx * sx, fy: k.fy * sy, cx: (k.cx + 0.5) * sx - 0.5, cy: (k.cy + 0.5) * sy - 0.5 },
+                 pose: v.pose, width: w, height: h };
+    });
+    const coarse = new HipDenseStereo(stereo.ctx, views[0].width, views[0].height, views);
+    images.forEach((im, i) => {
+        const W = stereo.cameras[i].width, w = views[i].width, h = views[i].height;
+        const out = new Uint8Array(w * h);
+        for (let y = 0; y < h; y++)
+            for (let x = 0; x < w; x++) {
+                const a = (2 * y) * W + 2 * x;
+                out[y * w + x] = (im[a] + im[a + W] + im[a + 1] + im[a + W + 1] + 2) >> 2;
+            }
+        coarse.setImage(i, out);
+    });
+    return coarse;
+}
+
+/** Coarse to fine: options.coarseSteps steps of refine on `coarse`, then options.steps on `stereo`. */
+async function refinePyramid(stereo, coarse, solves, options) {
+    const o = Object.assign({ coarseSteps: 25, steps: 25, fixed: [0] }, options || {});
+    for (let layer = 0; layer < stereo.views.length; layer++) coarse.setPose(layer, stereo.views[layer].pose);
+    await coarse.solveBatched(solves);
+    const hc = await refine(coarse, solves, Object.assign({}, o, { steps: o.coarseSteps }));
+    for (let layer = 0; layer < stereo.views.length; layer++)
+        if (!o.fixed.includes(layer)) stereo.setPose(layer, coarse.views[layer].pose);
+    await stereo.solveBatched(solves);
+    const hf = await refine(stereo, solves, o);
+    return { coarse: hc, fine: hf };
+}
+
 module.exports = {
     Context, HipDenseStereo, HipTsdfVolume, BrickGrid, STEREO_METHODS, MAP_RAW, MAP_FILTERED,
     MAX_SOURCES, BRICK, NccCostFunction, TsdfIntegrationFunction, native,
+    refine, refinePyramid, halfResolution, worldPoseGradients, perturbPose, expSo3,
 };

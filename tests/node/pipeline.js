@@ -63,6 +63,11 @@ async function main() {
     fs.writeFileSync(path.join(dir, 'tsdf.f32'), Buffer.from(tsdf.buffer));
     fs.writeFileSync(path.join(dir, 'weight.f32'), Buffer.from(weight.buffer));
     fs.writeFileSync(path.join(dir, 'coords.i32'), Buffer.from(layout.coords.buffer));
+    // gradient refinement of planes and poses through the Node host (last: it moves the poses)
+    const solves = job.views.map((v, i) => [i, { sources: job.sources[i], minDepth: job.ranges[i][0],
+        maxDepth: job.ranges[i][1], prior: null, geometric: false, iterations: job.iterations }]);
+    const history = await lr3d.refine(run.stereo, solves, { steps: 5 });
+    fs.writeFileSync(path.join(dir, 'refine.json'), JSON.stringify(history));
     volume.destroy();
     run.destroy();
     ctx.destroy();

@@ -336,7 +336,8 @@ __global__ void __launch_bounds__(64) ncc_backward_kernel(PmPlan plan, PmPools p
         win.sw = wsw;
         win.mean = wmean;
         win.var = wvar;
-        const float c = hypothesis_cost<kMaxSources, false>(d, pools, px, py, rx, ry, win, mx, my, mz, nd);
+        const float c =
+            hypothesis_cost<kMaxSources, false>(d, pools, px, py, rx, ry, win, mx, my, mz, nd, cost);
         if (c < cost) {
             after = c;
             pools.hyp[d.off + index] = make_float4(mx, my, mz, nd);

@@ -421,11 +421,19 @@ struct TopK {
     }
 };
 
-// PM:192-233
+// PM:192-233.  `bound` is the cost the caller will compare the result against with a strict
+// `<` (the pixel's best cost so far; +inf when the result is kept whatever it is).  Once fewer
+// than k views are left, the k smallest of (the costs so far, and 0 for every view left) bound
+// the final mean from below: every view cost is >= 0, the i-th smallest of a set can only fall
+// when members are lowered to 0, and rounded addition and division by k are monotone.  When
+// that bound already reaches `bound`, the candidate cannot win and the remaining views are not
+// evaluated; the bound is returned instead of the mean, and the caller discards it as it would
+// have discarded the mean.  A wavefront stops once all of its lanes have stopped.
 template <int KMAX, bool GEOMETRIC>
 __device__ __forceinline__ float hypothesis_cost(const PmDesc &d, const PmPools &pools, float px,
                                                  float py, float rx, float ry, const Window &win,
-                                                 float nx, float ny, float nz, float depth) {
+                                                 float nx, float ny, float nz, float depth,
+                                                 float bound) {
     float inv = 1.0f / depth;
     if (!(depth > 0.0f) || inv > d.inv_min || inv < d.inv_max) return kInvalid;
     if (!(dot3(nx, ny, nz, rx, ry, 1.0f) < -1e-3f)) return kInvalid;
@@ -435,7 +443,17 @@ __device__ __forceinline__ float hypothesis_cost(const PmDesc &d, const PmPools 
     if (count == 0) return kInvalid;
     TopK<KMAX> top;
     top.reset();
+    const int32_t first_check = count - d.topk + 1;  // from here on fewer than k views are left
     for (int32_t s = 0; s < count; s++) {
+        if (s >= first_check) {
+            const int32_t known = s - first_check + 1;  // k minus the views left
+            float total = 0.0f;
+#pragma unroll
+            for (int j = 0; j < KMAX; j++)
+                if (j < known) total = total + top.s[j];
+            const float low = total / (float)d.topk;
+            if (low >= bound) return low;
+        }
         const PmSrc &src = d.src[s];
         Mat3 H;
         homography(d, src, nx, ny, nz, rho, H);

@@ -51,7 +51,8 @@ __global__ void __launch_bounds__(256) pm_init_kernel(PmPlan plan, PmPools pools
     float c;
     if (d.from_coarse) {
         // PM:307-330
-        c = hypothesis_cost<KMAX, GEOMETRIC>(d, pools, px, py, rx, ry, win, nx, ny, nz, depth);
+        c = hypothesis_cost<KMAX, GEOMETRIC>(d, pools, px, py, rx, ry, win, nx, ny, nz, depth,
+                                             INFINITY);
         const float qx = ((px + 0.5f) * (float)d.coarse_W) / (float)d.W - 0.5f;
         const float qy = ((py + 0.5f) * (float)d.coarse_H) / (float)d.H - 0.5f;
         const int32_t bx = (int32_t)floorf(qx), by = (int32_t)floorf(qy);
@@ -67,7 +68,7 @@ __global__ void __launch_bounds__(256) pm_init_kernel(PmPlan plan, PmPools pools
             if (fabsf(den) < 1e-6f) continue;
             float cd = dot3(h.x, h.y, h.z, p0, p1, p2) / den;
             float cand = hypothesis_cost<KMAX, GEOMETRIC>(d, pools, px, py, rx, ry, win, h.x, h.y,
-                                                          h.z, cd);
+                                                          h.z, cd, c);
             if (cand < c) {
                 c = cand;
                 nx = h.x;
@@ -85,7 +86,8 @@ __global__ void __launch_bounds__(256) pm_init_kernel(PmPlan plan, PmPools pools
                 unpack_normal(packed, nx, ny, nz);
             }
         }
-        c = hypothesis_cost<KMAX, GEOMETRIC>(d, pools, px, py, rx, ry, win, nx, ny, nz, depth);
+        c = hypothesis_cost<KMAX, GEOMETRIC>(d, pools, px, py, rx, ry, win, nx, ny, nz, depth,
+                                             INFINITY);
         if ((d.flags & 1u) == 0u && d.has_prior) {
             float pr = pools.prior[d.prior_off + index];
             if (pr > 0.0f) {
@@ -95,7 +97,7 @@ __global__ void __launch_bounds__(256) pm_init_kernel(PmPlan plan, PmPools pools
                 pny = -pny;
                 pnz = -pnz;
                 float pc = hypothesis_cost<KMAX, GEOMETRIC>(d, pools, px, py, rx, ry, win, pnx, pny,
-                                                            pnz, pr);
+                                                            pnz, pr, c);
                 if (pc < c) {
                     c = pc;
                     nx = pnx;
@@ -224,7 +226,7 @@ pm_propagate_kernel(PmPlan plan, PmPools pools,
             td = (k == 0 || k == 2) ? d_random : ((k == 1 || k == 4) ? d_perturbed : cdepth);
         }
         const float cand = hypothesis_cost<KMAX, GEOMETRIC>(d, pools, px, py, rx, ry, win, tx, ty,
-                                                            tz, td);
+                                                            tz, td, best_cost);
         if (cand < best_cost) {
             best_cost = cand;
             best = make_float4(tx, ty, tz, td);

@@ -243,7 +243,7 @@ int32_t orc64_tsdf_backward(const orc_tsdf_grid *g, const orc_integration_view *
                             double *grad_depth, double *grad_pose, double *kink_out, double *loss);
 int32_t orc64_tsdf_loss(const orc_tsdf_grid *g, const orc_integration_view *views, const orc64_depth_input *inputs,
                         int32_t n_views, const uint32_t *voxels, const double *targets, int64_t n_voxels,
-                        const uint8_t *vis, double *loss);
+                        const uint8_t *vis, double *loss, double *terms);
 
 /* ---- FP64 cost and its gradient (adjoint_oracle.c) ---- */
 typedef struct {

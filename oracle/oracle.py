@@ -471,13 +471,14 @@ def visibility_flags(grid: Grid, views) -> np.ndarray:
     return np.stack([visible_bricks(grid, v) for v in views]).astype(np.uint8)
 
 
-def tsdf_loss64(grid: Grid, views, decoded, priors, voxels, targets, vis=None) -> float:
-    """The FP64 loss of tsdf_backward64 alone (OpenMP over voxels); vis from visibility_flags or None."""
+def tsdf_loss64(grid: Grid, views, decoded, priors, voxels, targets, vis=None, per_voxel=False):
+    """The FP64 loss of tsdf_backward64 alone (OpenMP over voxels, summed in voxel order); vis from
+    visibility_flags or None.  per_voxel=True returns each voxel's (tsdf - target)^2 instead."""
     lib = load()
     if not hasattr(lib, "_tl64"):
         lib.orc64_tsdf_loss.restype = C.c_int32
         lib.orc64_tsdf_loss.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int32, C.c_void_p, C.c_void_p,
-                                        C.c_int64, C.c_void_p, C.c_void_p]
+                                        C.c_int64, C.c_void_p, C.c_void_p, C.c_void_p]
         lib._tl64 = True
     n = len(views)
     varr = (OrcIntegrationView * n)(*views)
@@ -486,11 +487,12 @@ def tsdf_loss64(grid: Grid, views, decoded, priors, voxels, targets, vis=None) -
     tgt = np.ascontiguousarray(targets, dtype=np.float64)
     flags = None if vis is None else np.ascontiguousarray(vis, dtype=np.uint8)
     loss = np.zeros(1)
+    terms = np.zeros(vox.size)
     rc = lib.orc64_tsdf_loss(C.addressof(grid.struct), C.addressof(varr), C.addressof(inputs), n, vox.ctypes.data,
                              tgt.ctypes.data, vox.size, flags.ctypes.data if flags is not None else None,
-                             loss.ctypes.data)
+                             loss.ctypes.data, terms.ctypes.data)
     assert rc == 0
-    return float(loss[0])
+    return terms if per_voxel else float(loss[0])
 
 
 # ---- surface nets and mesh post-processing (mesh_oracle.c) ----------------------------------------

@@ -192,20 +192,30 @@ int32_t orc64_tsdf_backward(const orc_tsdf_grid *g, const orc_integration_view *
 
 /* The loss alone, L = sum_i (tsdf_i - target_i)^2 (the forward of orc64_tsdf_backward), with the
  * voxels split over OpenMP threads; for central differences over large voxel lists.  vis: the
- * culling flags [view][brick] (orc_visible_bricks), or NULL to compute them. */
+ * culling flags [view][brick] (orc_visible_bricks), or NULL to compute them.  terms (or NULL)
+ * receives each voxel's (tsdf_i - target_i)^2; the total adds them in voxel order, so it does not
+ * depend on the thread count. */
 int32_t orc64_tsdf_loss(const orc_tsdf_grid *g, const orc_integration_view *views, const orc64_depth_input *inputs,
                         int32_t n_views, const uint32_t *voxels, const double *targets, int64_t n_voxels,
-                        const uint8_t *vis, double *loss) {
+                        const uint8_t *vis, double *loss, double *terms) {
     uint8_t *own = NULL;
+    double *own_terms = NULL;
+    if (!terms) {
+        own_terms = (double *)malloc((size_t)n_voxels * sizeof(double) + 1);
+        if (!own_terms) return -1;
+        terms = own_terms;
+    }
     if (!vis) {
         own = (uint8_t *)malloc((size_t)n_views * g->n_bricks + 1);
-        if (!own) return -1;
-#pragma omp parallel for schedule(static)
+        if (!own) {
+            free(own_terms);
+            return -1;
+        }
+#pragma omp parallel for schedule(static) num_threads(orc_get_threads())
         for (int32_t v = 0; v < n_views; v++) orc_visible_bricks(g, &views[v], own + (size_t)v * g->n_bricks);
         vis = own;
     }
-    double total = 0.0;
-#pragma omp parallel for schedule(static) reduction(+ : total)
+#pragma omp parallel for schedule(static) num_threads(orc_get_threads())
     for (int64_t q = 0; q < n_voxels; q++) {
         const uint32_t index = voxels[q];
         const int32_t b = (int32_t)(index / 512u), local = (int32_t)(index % 512u);
@@ -226,9 +236,12 @@ int32_t orc64_tsdf_loss(const orc_tsdf_grid *g, const orc_integration_view *view
             W = fmin(D, MAX_WEIGHT);
         }
         const double r = ts - targets[q];
-        total += r * r;
+        terms[q] = r * r;
     }
+    double total = 0.0;
+    for (int64_t q = 0; q < n_voxels; q++) total += terms[q];
     *loss = total;
     free(own);
+    free(own_terms);
     return 0;
 }

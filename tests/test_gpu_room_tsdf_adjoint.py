@@ -33,8 +33,13 @@ def say(msg):
 
 
 def fd(f, h, agree=1e-5):
-    g1 = (f(h) - f(-h)) / (2 * h)
-    g2 = (f(h / 2) - f(-h / 2)) / h
+    """Central difference of a sum of per-voxel terms, differenced voxel by voxel (voxels the step
+    leaves alone cancel exactly instead of adding the rounding of the whole sum), checked against
+    the quotient with h/2."""
+    def diff(a, b):
+        return float(np.sum(np.asarray(a) - np.asarray(b)))
+    g1 = diff(f(h), f(-h)) / (2 * h)
+    g2 = diff(f(h / 2), f(-h / 2)) / h
     if not (np.isfinite(g1) and np.isfinite(g2)) or abs(g1 - g2) > agree * max(1.0, abs(g1)):
         return np.nan
     return g1
@@ -171,7 +176,7 @@ def test_gradients_match_fp64_reference_and_central_differences(ctx, room):
                 d = d.copy()
                 d[p] += h
                 inp[v] = (d, nn, c)
-                return O.tsdf_loss64(g, ivs, inp, None, vk[near], tk[near], vis=vis)
+                return O.tsdf_loss64(g, ivs, inp, None, vk[near], tk[near], vis=vis, per_voxel=True)
             f = fd(loss, 1e-7)
             if not np.isfinite(f):
                 continue
@@ -213,7 +218,7 @@ def test_gradients_match_fp64_reference_and_central_differences(ctx, room):
                     iv.t[k - 9] += h
                 vs = list(ivs)
                 vs[v] = iv
-                return O.tsdf_loss64(g, vs, decoded, None, vk[sub], tk[sub], vis=vis)
+                return O.tsdf_loss64(g, vs, decoded, None, vk[sub], tk[sub], vis=vis, per_voxel=True)
             # a sum over ~1e5 voxels always holds a few kinks (o = 1, w = 0.2, the weight cap)
             # within the step: the quotients with h and h/2 are asked to agree to 1e-4
             f = fd(loss, 1e-7, agree=1e-4)

@@ -222,15 +222,29 @@ def test_sphere_pose_gradients_against_central_differences(sphere):
 
 def test_parallel_loss_equals_the_replay_loss(sphere):
     """orc64_tsdf_loss (OpenMP over voxels, for difference quotients over large voxel lists) is the
-    forward of orc64_tsdf_backward: the same loss up to FP64 summation order, with the culling flags
-    computed inside or passed in."""
+    forward of orc64_tsdf_backward: the same per-voxel terms, a total that adds them in voxel order
+    whatever the thread count, with the culling flags computed inside or passed in."""
     g, views, inputs, priors, vox, tgt = sphere
     r = O.tsdf_backward64(g, views, inputs, priors, vox, tgt)
-    direct = float(((r["tsdf"] - tgt) ** 2).sum())
+    direct = (r["tsdf"] - tgt) ** 2
     a = O.tsdf_loss64(g, views, inputs, priors, vox, tgt)
     b = O.tsdf_loss64(g, views, inputs, priors, vox, tgt, vis=O.visibility_flags(g, views))
-    assert a == pytest.approx(r["loss"], rel=1e-12) and a == pytest.approx(direct, rel=1e-12)
-    assert b == pytest.approx(a, rel=1e-12)  # OpenMP's reduction order varies
-    # a voxel list in another order gives the same loss
+    terms = O.tsdf_loss64(g, views, inputs, priors, vox, tgt, per_voxel=True)
+    assert np.allclose(terms, direct, rtol=1e-12, atol=1e-15)
+    assert a == pytest.approx(r["loss"], rel=1e-12) and a == b
+    total = 0.0
+    for t in terms:
+        total += t
+    assert total == a
+    old = O.get_threads()
+    try:
+        O.set_threads(1)
+        one = O.tsdf_loss64(g, views, inputs, priors, vox, tgt)
+        O.set_threads(3)
+        three = O.tsdf_loss64(g, views, inputs, priors, vox, tgt)
+    finally:
+        O.set_threads(old)
+    assert one == three == a
+    # a voxel list in another order gives the same loss up to summation order
     perm = np.random.default_rng(3).permutation(len(vox))
     assert O.tsdf_loss64(g, views, inputs, priors, vox[perm], tgt[perm]) == pytest.approx(a, rel=1e-12)

@@ -242,6 +242,8 @@ struct lr3d_stereo {
     std::vector<lr3d_view> views;
     std::vector<uint32_t> off;  // pixel offset per layer
     size_t total_px = 0;
+    std::vector<uint32_t> qoff;  // texel offset per layer in the quad pool (rows quad_pitch(W) apart)
+    size_t total_q = 0;
     std::vector<uint8_t> has_image;
     std::vector<uint8_t> solved;
     DevBuf<uint32_t> quad, depth, filt;
@@ -274,14 +276,16 @@ extern "C" int32_t lr3d_stereo_create(lr3d_ctx *ctx, const lr3d_view *views, int
     *out = nullptr;
     if (n_views <= 0) return fail(LR3D_ERR_INVALID, "lr3d_stereo_create: n_views must be > 0");
     (void)flags;
-    size_t total = 0;
-    std::vector<uint32_t> off(n_views);
+    size_t total = 0, total_q = 0;
+    std::vector<uint32_t> off(n_views), qoff(n_views);
     for (int32_t i = 0; i < n_views; i++) {
         if (!valid_view(views[i]))
             return fail(LR3D_ERR_INVALID, "lr3d_stereo_create: view %d has an invalid size or intrinsics", i);
         off[i] = (uint32_t)total;
+        qoff[i] = (uint32_t)total_q;
         total += (size_t)views[i].width * (size_t)views[i].height;
-        if (total >= (1ull << 32))
+        total_q += (size_t)quad_pitch(views[i].width) * (size_t)views[i].height;
+        if (total_q >= (1ull << 32))
             return fail(LR3D_ERR_INVALID, "lr3d_stereo_create: more than 2^32 pixels in one set");
     }
     std::lock_guard<std::mutex> lock(ctx->mu);
@@ -291,13 +295,16 @@ extern "C" int32_t lr3d_stereo_create(lr3d_ctx *ctx, const lr3d_view *views, int
     st->views.assign(views, views + n_views);
     st->off = off;
     st->total_px = total;
+    st->qoff = qoff;
+    st->total_q = total_q;
     st->has_image.assign(n_views, 0);
     st->solved.assign(n_views, 0);
-    hipError_t e = st->quad.alloc(total);
+    hipError_t e = st->quad.alloc(total_q);
     if (e == hipSuccess) e = st->depth.alloc(total);
     if (e == hipSuccess) e = st->filt.alloc(total);
     if (e == hipSuccess) e = st->hyp.alloc(total);
     if (e == hipSuccess) e = st->cost.alloc(total);
+    if (e == hipSuccess) e = hipMemsetAsync(st->quad.ptr, 0, total_q * 4, ctx->stream);
     if (e == hipSuccess) e = hipMemsetAsync(st->depth.ptr, 0, total * 4, ctx->stream);
     if (e == hipSuccess) e = hipMemsetAsync(st->filt.ptr, 0, total * 4, ctx->stream);
     if (e == hipSuccess) e = hipMemsetAsync(st->hyp.ptr, 0, total * 16, ctx->stream);
@@ -345,7 +352,8 @@ extern "C" int32_t lr3d_stereo_set_image(lr3d_stereo *st, int32_t layer, const u
     HIP_TRY(st->staging.reserve(px));
     HIP_TRY(hipMemcpyAsync(st->staging.ptr, gray, px, hipMemcpyHostToDevice, ctx->stream));
     HIP_TRY(timed(ctx, "pm_build_quad", [&] {
-        return pm_build_quad(ctx->stream, st->staging.ptr, width, height, st->quad.ptr + st->off[layer]);
+        return pm_build_quad(ctx->stream, st->staging.ptr, width, height, quad_pitch(width),
+                             st->quad.ptr + st->qoff[layer]);
     }));
     st->has_image[layer] = 1;
     return finish(ctx);
@@ -415,6 +423,8 @@ static int32_t build_desc(const lr3d_stereo *st, const lr3d_solve_desc &sd, uint
     d.iterations = sd.iterations;
     d.seed_base = (uint32_t)(sd.ref + 1) * 0x9e3779b1u + d.flags * 7u;  // DS:379
     d.off = st->off[sd.ref];
+    d.pitch = quad_pitch(rv.width);
+    d.qoff = st->qoff[sd.ref];
     d.has_prior = sd.prior != nullptr;
     d.prior_off = prior_off;
     d.from_coarse = sd.from_coarse && st->coarse != nullptr;
@@ -450,6 +460,8 @@ static int32_t build_desc(const lr3d_stereo *st, const lr3d_solve_desc &sd, uint
         o.H = sv.height;
         o.off = st->off[layer];
         o.layer = layer;
+        o.pitch = quad_pitch(sv.width);
+        o.qoff = st->qoff[layer];
     }
     return LR3D_OK;
 }

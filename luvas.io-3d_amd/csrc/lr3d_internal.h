@@ -3,7 +3,10 @@
 // Device data layout (all views of a stereo set share one pool per kind, each layer at its own
 // pixel offset `off[layer]` = sum of the previous layers' width*height):
 //   quad   u32 per pixel: the 2x2 block I(x,y) | I(x+1,y)<<8 | I(x,y+1)<<16 | I(x+1,y+1)<<24,
-//          clamped to the image, so one 32-bit gather feeds one bilinear tap;
+//          clamped to the image, so one 32-bit gather feeds one bilinear tap.  Rows are `pitch`
+//          texels apart, an odd number of 128-byte lines (quad_pitch), and layers sit at their
+//          own offsets `qoff[layer]`: a gather spans ~8 rows, and with a pitch of an even number
+//          of lines (1280 px = 40 lines) all of them would fall in the same L1 tag bank;
 //   hyp    float4 per pixel (normal xyz, depth), cost f32 per pixel;
 //   depth  packed u32 per pixel (f16 depth | 6+6-bit octahedral normal | 4-bit cost);
 //   filt   packed u32 per pixel (consistency-filtered depth);
@@ -28,8 +31,10 @@ struct PmSrc {
     float R[9];     // relative rotation ref -> src, row-major
     float t[3];
     int32_t W, H;
-    uint32_t off;   // pixel offset of the source layer in every pool
+    uint32_t off;   // pixel offset of the source layer in the per-pixel pools
     int32_t layer;
+    int32_t pitch;  // texels per row of the layer in the quad pool
+    uint32_t qoff;  // texel offset of the layer in the quad pool
 };
 
 // One reference view's solve (the WGSL `Params`, PM:12-26, minus the per-launch state).
@@ -50,6 +55,8 @@ struct PmDesc {
     uint32_t coarse_off;       // pixel offset of this layer in the coarse stereo's hyp pool
     float ck[4];               // coarse intrinsics (scaleIntrinsics, CAM:28-36)
     int32_t tiles_x;           // tiles per row for the launch's tile shape (set per launch plan)
+    int32_t pitch;             // quad-pool row pitch and offset of the reference layer
+    uint32_t qoff;
     int32_t pad_;
     PmSrc src[kMaxSources];
 };
@@ -123,7 +130,13 @@ struct TsdfGridDev {
 };
 
 // Kernel launchers (stream-ordered; no synchronisation).
-hipError_t pm_build_quad(hipStream_t s, const uint8_t *gray, int32_t W, int32_t H, uint32_t *quad);
+hipError_t pm_build_quad(hipStream_t s, const uint8_t *gray, int32_t W, int32_t H, int32_t pitch, uint32_t *quad);
+
+// Row pitch (texels) of a W-wide layer in the quad pool: whole 128-byte lines, an odd number.
+inline int32_t quad_pitch(int32_t W) {
+    const int32_t lines = (W + 31) / 32;
+    return ((lines & 1) ? lines : lines + 1) * 32;
+}
 hipError_t pm_launch_init(hipStream_t s, const PmPlan &plan, const PmPools &pools);
 hipError_t pm_launch_propagate(hipStream_t s, const PmPlan &plan, const PmPools &pools,
                                uint32_t iteration, uint32_t phase, float step);

@@ -129,20 +129,20 @@ struct Window {
 };
 
 __device__ __forceinline__ float ref_pixel(const uint32_t *__restrict__ img, int32_t W, int32_t H,
-                                           int32_t x, int32_t y) {
+                                           int32_t pitch, int32_t x, int32_t y) {
     x = min(max(x, 0), W - 1);
     y = min(max(y, 0), H - 1);
-    return (float)(img[__umul24((uint32_t)y, (uint32_t)W) + (uint32_t)x] & 0xffu);
+    return (float)(img[__umul24((uint32_t)y, (uint32_t)pitch) + (uint32_t)x] & 0xffu);
 }
 
 __device__ __forceinline__ void prepare_window(const uint32_t *__restrict__ img, int32_t W, int32_t H,
-                                               int32_t x, int32_t y, Window &win) {
-    float center = ref_pixel(img, W, H, x, y);
+                                               int32_t pitch, int32_t x, int32_t y, Window &win) {
+    float center = ref_pixel(img, W, H, pitch, x, y);
     float sw = 0.0f, sr = 0.0f, srr = 0.0f;
 #pragma unroll
     for (int i = 0; i < kSamples; i++) {
         const int ox = (i % 6) * 2 - 5, oy = (i / 6) * 2 - 5;
-        float v = ref_pixel(img, W, H, x + ox, y + oy);
+        float v = ref_pixel(img, W, H, pitch, x + ox, y + oy);
         float dc = v - center;
         const float ds = (float)(ox * ox + oy * oy);
         float w = (float)exp((double)(-(dc * dc) * kColorK - ds * kSpaceK));
@@ -208,10 +208,11 @@ __device__ __forceinline__ float rcp_exact(float z) {
 
 template <bool FAST>
 __device__ __forceinline__ void view_taps(const uint32_t *__restrict__ img, int32_t W, int32_t H,
-                                          const Mat3 &Hm, float px, float py, const Window &win,
-                                          float &ss, float &sss, float &srs, bool &ok) {
+                                          int32_t pitch, const Mat3 &Hm, float px, float py,
+                                          const Window &win, float &ss, float &sss, float &srs,
+                                          bool &ok) {
     const float wmax = (float)W - 1.0f, hmax = (float)H - 1.0f;
-    const uint32_t stride = (uint32_t)W;
+    const uint32_t stride = (uint32_t)pitch;
 #pragma unroll
     for (int row = 0; row < 6; row++) {
         const float Y = py + (float)(row * 2 - 5);
@@ -250,13 +251,17 @@ __device__ __forceinline__ void view_taps(const uint32_t *__restrict__ img, int3
 }
 
 // Taps of a window known to lie inside the source image (see view_cost): no per-tap checks.
-// The linear index is still clamped to the image, so no tap can read outside it.
-__device__ __forceinline__ void view_taps_inside(const uint32_t *__restrict__ img, int32_t W, int32_t H,
-                                                 const Mat3 &Hm, float px, float py,
+// The texels are read through a buffer resource that spans the layer's texels, so a load can
+// never leave them (the hardware returns 0 past the end), at two address instructions per tap
+// (a 24-bit multiply-add and a shift) instead of three for a clamped 64-bit address.
+__device__ __forceinline__ void view_taps_inside(const uint32_t *__restrict__ img, int32_t H,
+                                                 int32_t pitch, const Mat3 &Hm, float px, float py,
                                                  const Window &win, float &ss, float &sss,
                                                  float &srs) {
-    const uint32_t stride = (uint32_t)W;
-    const uint32_t last = (uint32_t)W * (uint32_t)H - 1u;
+    const uint32_t stride = (uint32_t)pitch;
+    // raw buffer, 32-bit data format; num_records in bytes
+    const __amdgpu_buffer_rsrc_t texels = __builtin_amdgcn_make_buffer_rsrc(
+        (void *)img, (short)0, (int)((uint32_t)pitch * (uint32_t)H * 4u), 0x00020000);
 #pragma unroll
     for (int row = 0; row < 6; row++) {
         const float Y = py + (float)(row * 2 - 5);
@@ -274,7 +279,7 @@ __device__ __forceinline__ void view_taps_inside(const uint32_t *__restrict__ im
             const float u = qx * rz, v = qy * rz;
             const uint32_t ix = (uint32_t)u, iy = (uint32_t)v;
             const float fx = __builtin_amdgcn_fractf(u), fy = __builtin_amdgcn_fractf(v);
-            const uint32_t q = img[min(__umul24(iy, stride) + ix, last)];
+            const uint32_t q = __builtin_amdgcn_raw_buffer_load_b32(texels, (__umul24(iy, stride) + ix) << 2, 0, 0);
             const float a = (float)(q & 0xffu), b = (float)((q >> 8) & 0xffu);
             const float c = (float)((q >> 16) & 0xffu), e = (float)(q >> 24);
             const float top = fmaf(fx, b - a, a);
@@ -341,7 +346,7 @@ __device__ __forceinline__ int classify_window(const PmSrc &s, const Mat3 &H, fl
 
 __device__ __forceinline__ float view_cost(const PmSrc &s, const Mat3 &H, float px, float py,
                                            const Window &win, const uint32_t *__restrict__ quad) {
-    const uint32_t *__restrict__ img = quad + s.off;
+    const uint32_t *__restrict__ img = quad + s.qoff;
     const int cls = classify_window(s, H, px, py);
     float ss = 0.0f, sss = 0.0f, srs = 0.0f;
     bool ok = cls != kOutside;
@@ -351,12 +356,12 @@ __device__ __forceinline__ float view_cost(const PmSrc &s, const Mat3 &H, float 
                             fabsf(H.m[8]);
         bool tap_ok = true;
         if (__all(bound < 0x1p60f))
-            view_taps<true>(img, s.W, s.H, H, px, py, win, ss, sss, srs, tap_ok);
+            view_taps<true>(img, s.W, s.H, s.pitch, H, px, py, win, ss, sss, srs, tap_ok);
         else
-            view_taps<false>(img, s.W, s.H, H, px, py, win, ss, sss, srs, tap_ok);
+            view_taps<false>(img, s.W, s.H, s.pitch, H, px, py, win, ss, sss, srs, tap_ok);
         ok = ok & tap_ok;
     } else if (__any(cls == kInside)) {
-        if (cls == kInside) view_taps_inside(img, s.W, s.H, H, px, py, win, ss, sss, srs);
+        if (cls == kInside) view_taps_inside(img, s.H, s.pitch, H, px, py, win, ss, sss, srs);
     }
     if (!ok) return kInvalid;
     float mean = ss / win.sw;

@@ -254,9 +254,12 @@ __device__ __forceinline__ void view_taps(const uint32_t *__restrict__ img, int3
 // The texels are read through a buffer resource that spans the layer's texels, so a load can
 // never leave them (the hardware returns 0 past the end), at two address instructions per tap
 // (a 24-bit multiply-add and a shift) instead of three for a clamped 64-bit address.
+// cu, cv: the corner taps (0, 5, 30, 35) as classify_window computed them, with the same
+// expressions, so they are taken from there.
 __device__ __forceinline__ void view_taps_inside(const uint32_t *__restrict__ img, int32_t H,
                                                  int32_t pitch, const Mat3 &Hm, float px, float py,
-                                                 const Window &win, float &ss, float &sss,
+                                                 const Window &win, const float cu[4],
+                                                 const float cv[4], float &ss, float &sss,
                                                  float &srs) {
     const uint32_t stride = (uint32_t)pitch;
     // raw buffer, 32-bit data format; num_records in bytes
@@ -271,12 +274,20 @@ __device__ __forceinline__ void view_taps_inside(const uint32_t *__restrict__ im
 #pragma unroll
         for (int col = 0; col < 6; col++) {
             const int i = row * 6 + col;
-            const float X = px + (float)(col * 2 - 5);
-            const float qx = fmaf(Hm.m[0], X, rx);
-            const float qy = fmaf(Hm.m[3], X, ry);
-            const float qz = fmaf(Hm.m[6], X, rw);
-            const float rz = rcp_exact(qz);
-            const float u = qx * rz, v = qy * rz;
+            const int corner = (row == 0 || row == 5) && (col == 0 || col == 5) ? (col / 5) + 2 * (row / 5) : -1;
+            float u, v;
+            if (corner >= 0) {
+                u = cu[corner];
+                v = cv[corner];
+            } else {
+                const float X = px + (float)(col * 2 - 5);
+                const float qx = fmaf(Hm.m[0], X, rx);
+                const float qy = fmaf(Hm.m[3], X, ry);
+                const float qz = fmaf(Hm.m[6], X, rw);
+                const float rz = rcp_exact(qz);
+                u = qx * rz;
+                v = qy * rz;
+            }
             const uint32_t ix = (uint32_t)u, iy = (uint32_t)v;
             const float fx = __builtin_amdgcn_fractf(u), fy = __builtin_amdgcn_fractf(v);
             const uint32_t q = __builtin_amdgcn_raw_buffer_load_b32(texels, (__umul24(iy, stride) + ix) << 2, 0, 0);
@@ -302,7 +313,8 @@ __device__ __forceinline__ void view_taps_inside(const uint32_t *__restrict__ im
 //   kCareful  anything else (windows touching the border, huge |q.z|): per-tap checks.
 enum { kOutside = 0, kInside = 1, kCareful = 2 };
 
-__device__ __forceinline__ int classify_window(const PmSrc &s, const Mat3 &H, float px, float py) {
+__device__ __forceinline__ int classify_window(const PmSrc &s, const Mat3 &H, float px, float py,
+                                               float cu[4], float cv[4]) {
     const float wmax = (float)s.W - 1.0f, hmax = (float)s.H - 1.0f;
     const float ax = fabsf(px) + 5.0f, ay = fabsf(py) + 5.0f;
     // bounds on the magnitude of the terms of q.x, q.y, q.z over the window
@@ -321,6 +333,8 @@ __device__ __forceinline__ int classify_window(const PmSrc &s, const Mat3 &H, fl
         const float qz = fmaf(H.m[6], X, fmaf(H.m[7], Y, H.m[8]));
         const float rz = rcp_exact(qz);
         const float u = qx * rz, v = qy * rz;
+        cu[corner] = u;
+        cv[corner] = v;
         valid = valid & (qz > 1e-6f) & (u >= 0.0f) & (v >= 0.0f) & (u <= wmax) & (v <= hmax);
         zmin = fminf(zmin, qz);
         umin = fminf(umin, u);
@@ -347,7 +361,8 @@ __device__ __forceinline__ int classify_window(const PmSrc &s, const Mat3 &H, fl
 __device__ __forceinline__ float view_cost(const PmSrc &s, const Mat3 &H, float px, float py,
                                            const Window &win, const uint32_t *__restrict__ quad) {
     const uint32_t *__restrict__ img = quad + s.qoff;
-    const int cls = classify_window(s, H, px, py);
+    float cu[4], cv[4];
+    const int cls = classify_window(s, H, px, py, cu, cv);
     float ss = 0.0f, sss = 0.0f, srs = 0.0f;
     bool ok = cls != kOutside;
     if (__any(cls == kCareful)) {
@@ -361,7 +376,7 @@ __device__ __forceinline__ float view_cost(const PmSrc &s, const Mat3 &H, float 
             view_taps<false>(img, s.W, s.H, s.pitch, H, px, py, win, ss, sss, srs, tap_ok);
         ok = ok & tap_ok;
     } else if (__any(cls == kInside)) {
-        if (cls == kInside) view_taps_inside(img, s.H, s.pitch, H, px, py, win, ss, sss, srs);
+        if (cls == kInside) view_taps_inside(img, s.H, s.pitch, H, px, py, win, cu, cv, ss, sss, srs);
     }
     if (!ok) return kInvalid;
     float mean = ss / win.sw;

@@ -32,6 +32,8 @@ sys.path.insert(0, os.path.join(ROOT, "luvas.io-3d_amd"))
 # MI355X published peaks
 HBM_PEAK_GBS = 8000.0
 FP32_PEAK_TFLOPS = 157.3
+# the same ALUs one FP32 operation per lane per cycle (no packed FP32, which this build does not use)
+FP32_UNPACKED_TFLOPS = 78.6
 # Algorithmic FLOPs: 33 per NCC tap (homography 18, divide 3, bilinear 6, sums 6), 36 taps per
 # source view, 20 per view for the NCC finish; 12 per window tap for the bilateral window.
 FLOP_PER_TAP = 33
@@ -189,7 +191,8 @@ def main():
         roofline = {"kernel": "pm_propagate", "bound": "valu", "achieved": round(achieved, 3),
                     "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s", "frac": round(achieved / FP32_PEAK_TFLOPS, 4),
                     "traffic": None, "avg_launch_ms": round(prop_ms / prop_launches, 3),
-                    "flop_per_launch": flops}
+                    "flop_per_launch": flops, "peak_unpacked": FP32_UNPACKED_TFLOPS,
+                    "frac_unpacked": round(achieved / FP32_UNPACKED_TFLOPS, 4)}
     # HBM bytes per launch measured by rocprofv3 counters for this workload (profiles/traffic.json)
     measured = {}
     try:

@@ -567,7 +567,12 @@ static int32_t stage_priors(lr3d_stereo *st, const lr3d_solve_desc *descs, int32
     bool any = false;
     for (int32_t i = 0; i < n; i++) any |= descs[i].prior != nullptr;
     if (!any) return LR3D_OK;
-    if (!st->prior.ptr) HIP_TRY(st->prior.alloc(st->total_px));
+    if (!st->prior.ptr) {
+        HIP_TRY(st->prior.alloc(st->total_px));
+        // graphs captured so far hold the pools as they were (no prior buffer): capture again
+        for (auto &g : st->graphs) destroy_graph(g);
+        st->graphs.clear();
+    }
     for (int32_t i = 0; i < n; i++) {
         if (!descs[i].prior) continue;
         const int32_t ref = descs[i].ref;

@@ -280,6 +280,28 @@ def test_geometric_pass_matches_oracle(ctx, work, oracle_maps):
     assert agree >= 0.995 and exact >= 0.99
 
 
+def test_batched_prior_after_a_batch_without_one(ctx, work):
+    """A batched solve replays a HIP graph captured per batch layout; the prior buffer is
+    allocated by the first solve that passes a prior, after which graphs captured without it are
+    captured again.  Same layout without, with, then without a prior: each equals the oracle."""
+    say("case: batched solves of view 0 without, with and again without a prior (one batch layout)")
+    rng = np.random.default_rng(8)
+    truth = work.truth[0].reshape(-1)
+    prior = np.where(rng.random(truth.size) < 0.7, truth * (1 + 0.02 * rng.standard_normal(truth.size)),
+                     0).astype(np.float32)
+    st = make_stereo(ctx, work)
+    imgs = O.Images(work.images)
+    for with_prior in (False, True, False):
+        p = prior if with_prior else None
+        st.solve_batched([(0, opts(work, 0, prior=p, iterations=1))])
+        hyp, cost = st.hypotheses(0)
+        _, ohyp, ocost = O.solve(work.views, imgs, 0, work.sources[0], *work.ranges[0], prior=p,
+                                 iterations=1, topk=work.topk)
+        assert np.array_equal(hyp.view(np.uint32), ohyp.view(np.uint32)), with_prior
+        assert np.array_equal(cost.view(np.uint32), ocost.view(np.uint32)), with_prior
+    st.destroy()
+
+
 def test_prior_matches_oracle(ctx, work):
     say("case: solve with a phone-depth prior (noisy truth), view 0")
     rng = np.random.default_rng(7)

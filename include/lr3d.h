@@ -124,7 +124,9 @@ int32_t lr3d_stereo_read_hyp(lr3d_stereo *st, int32_t layer, float *hyp, float *
 /* Overwrite a layer's packed depth map (e.g. maps computed elsewhere, or restored state). */
 int32_t lr3d_stereo_write_depth(lr3d_stereo *st, int32_t layer, const uint32_t *packed);
 /* Attach a coarse level for from_coarse solves (coarse-to-fine, DS:77-82 CoarseLevel): the
- * coarse stereo must hold one layer per layer of `st`; its planes are read, not copied. */
+ * coarse stereo must hold one layer per layer of `st` and be another set of the same context;
+ * its planes are read, not copied.  NULL detaches.  Destroying the coarse level detaches it from
+ * every set using it (their from_coarse solves then fail with LR3D_ERR_STATE). */
 int32_t lr3d_stereo_set_coarse(lr3d_stereo *st, lr3d_stereo *coarse);
 
 /* Replace a layer's pose (world -> camera) for later solves, filters and adjoints (pose

@@ -255,6 +255,7 @@ struct lr3d_stereo {
     DevBuf<double> adj_partials;
     DevBuf<AdjGeom> adj_geom;
     lr3d_stereo *coarse = nullptr;
+    std::vector<lr3d_stereo *> fine_levels;  // stereos using this one as their coarse level
     std::vector<GraphEntry> graphs;
     // launch shapes (threads x, y) of init, propagate and pack/consistency; set by the autotuner
     TileShape tiles[3] = {pm_tile_for("init", 0, 0, 0), pm_tile_for("propagate", 0, 0, 0),
@@ -319,12 +320,27 @@ extern "C" int32_t lr3d_stereo_create(lr3d_ctx *ctx, const lr3d_view *views, int
     return LR3D_OK;
 }
 
+static void detach_coarse(lr3d_stereo *st) {
+    if (!st->coarse) return;
+    auto &users = st->coarse->fine_levels;
+    users.erase(std::remove(users.begin(), users.end(), st), users.end());
+    st->coarse = nullptr;
+}
+
 extern "C" int32_t lr3d_stereo_destroy(lr3d_stereo *st) {
     if (!st) return LR3D_OK;
     lr3d_ctx *ctx = st->ctx;
     std::lock_guard<std::mutex> lock(ctx->mu);
     (void)hipSetDevice(ctx->device);
     (void)hipStreamSynchronize(ctx->stream);
+    detach_coarse(st);
+    // stereos that read this one as their coarse level lose it (a later from_coarse solve fails
+    // with LR3D_ERR_STATE instead of reading freed memory); their graphs hold its planes' address
+    for (lr3d_stereo *fine : st->fine_levels) {
+        fine->coarse = nullptr;
+        for (auto &g : fine->graphs) destroy_graph(g);
+        fine->graphs.clear();
+    }
     delete st;
     return LR3D_OK;
 }
@@ -918,6 +934,7 @@ extern "C" int32_t lr3d_stereo_write_depth(lr3d_stereo *st, int32_t layer, const
 extern "C" int32_t lr3d_stereo_set_coarse(lr3d_stereo *st, lr3d_stereo *coarse) {
     if (!st) return fail(LR3D_ERR_INVALID, "lr3d_stereo_set_coarse: NULL argument");
     if (coarse) {
+        if (coarse == st) return fail(LR3D_ERR_INVALID, "lr3d_stereo_set_coarse: a set cannot be its own coarse level");
         if (coarse->ctx != st->ctx)
             return fail(LR3D_ERR_INVALID, "lr3d_stereo_set_coarse: levels belong to different contexts");
         if (coarse->views.size() != st->views.size())
@@ -925,7 +942,9 @@ extern "C" int32_t lr3d_stereo_set_coarse(lr3d_stereo *st, lr3d_stereo *coarse) 
                         (int32_t)coarse->views.size(), (int32_t)st->views.size());
     }
     std::lock_guard<std::mutex> lock(st->ctx->mu);
+    detach_coarse(st);
     st->coarse = coarse;
+    if (coarse) coarse->fine_levels.push_back(st);
     for (auto &g : st->graphs) destroy_graph(g);
     st->graphs.clear();
     return LR3D_OK;

@@ -1,8 +1,9 @@
 """Edge cases of the C ABI on the GPU: empty and degenerate calls (empty batches, views without
 sources, a zero-view TSDF pass, volumes nothing can see, empty meshes and clouds, an empty voxel
 list for the adjoint), rejected calls (duplicate or unknown layers, a coarse start without a
-coarse level), and ragged and extreme image sizes (1x1, 17x13, 16384x16, 16x16384: partial tiles
-and the widest rows) solved bit-exact to the oracle."""
+coarse level, a set as its own coarse level, a coarse level destroyed before its fine set), and
+ragged and extreme image sizes (1x1, 17x13, 16384x16, 16x16384: partial tiles and the widest rows)
+solved bit-exact to the oracle."""
 import numpy as np
 import pytest
 
@@ -50,6 +51,28 @@ def test_empty_and_degenerate_solves(ctx):
                                       from_coarse=True))
     # still usable after the rejected calls
     st.solve(0, o)
+    assert st.depth_map(0).any()
+    st.destroy()
+
+
+def test_coarse_level_lifetime(ctx):
+    import lr3d
+    from lr3d import StereoView, scale_intrinsics
+    say("case: a set cannot be its own coarse level; destroying the coarse level detaches it")
+    w = scenes.sphere_small().render()
+    coarse_views = [StereoView(scale_intrinsics(v.intrinsics, 0.5, 0.5), v.pose, 160, 120) for v in w.views]
+    st = lr3d.HipDenseStereo(ctx, w.views)
+    for i, im in enumerate(w.images):
+        st.set_image(i, im)
+    coarse = lr3d.HipDenseStereo(ctx, coarse_views)
+    with pytest.raises(lr3d.Lr3dError):
+        st.set_coarse(st)
+    st.set_coarse(coarse)
+    coarse.destroy()
+    o = lr3d.SolveOptions(sources=w.sources[0], min_depth=0.5, max_depth=3.0, iterations=1, from_coarse=True)
+    with pytest.raises(lr3d.Lr3dError):
+        st.solve_batched([(0, o)])
+    st.solve(0, lr3d.SolveOptions(sources=w.sources[0], min_depth=0.5, max_depth=3.0, iterations=1))
     assert st.depth_map(0).any()
     st.destroy()
 

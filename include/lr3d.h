@@ -85,6 +85,9 @@ typedef struct {
 const char *lr3d_last_error(void);
 /* Open device `device` and create its stream. */
 int32_t lr3d_ctx_create(int32_t device, lr3d_ctx **out);
+/* Stereo sets, volumes, meshes and clouds made on a context keep it alive: if any remain, the
+ * context is released by the destroy of the last of them.  The caller must not use `ctx` itself
+ * after this call. */
 int32_t lr3d_ctx_destroy(lr3d_ctx *ctx);
 /* Block until the context's stream is idle. */
 int32_t lr3d_ctx_synchronize(lr3d_ctx *ctx);

@@ -95,6 +95,8 @@ struct lr3d_ctx {
     std::vector<hipEvent_t> event_pool;
     std::vector<PendingEvent> pending;
     int64_t graph_launches = 0;
+    int64_t users = 0;      // live stereo sets, volumes, meshes and clouds made on this context
+    bool closing = false;   // lr3d_ctx_destroy was called while users remained
 
     hipEvent_t take_event() {
         if (!event_pool.empty()) {
@@ -131,6 +133,18 @@ struct lr3d_ctx {
         if (stream) (void)hipStreamDestroy(stream);
     }
 };
+
+// Object lifetimes: every stereo set, volume, mesh and cloud is a user of its context.  A context
+// destroyed while users remain is released by the destroy of its last user, so no object ever
+// outlives the stream and mutex it uses.  Creation counts a user with ctx->mu held; the destroy of
+// an object calls ctx_release with ctx->mu held through `lock`.
+static void ctx_release(lr3d_ctx *ctx, std::unique_lock<std::mutex> &lock) {
+    ctx->users--;
+    if (ctx->closing && ctx->users == 0) {
+        lock.unlock();
+        delete ctx;
+    }
+}
 
 // Records start/stop events around one launch when profiling is on.
 template <typename F>
@@ -183,6 +197,10 @@ extern "C" int32_t lr3d_ctx_destroy(lr3d_ctx *ctx) {
         std::lock_guard<std::mutex> lock(ctx->mu);
         (void)hipSetDevice(ctx->device);
         (void)hipStreamSynchronize(ctx->stream);
+        if (ctx->users > 0) {  // released by the last user's destroy
+            ctx->closing = true;
+            return LR3D_OK;
+        }
     }
     delete ctx;
     return LR3D_OK;
@@ -290,6 +308,7 @@ extern "C" int32_t lr3d_stereo_create(lr3d_ctx *ctx, const lr3d_view *views, int
             return fail(LR3D_ERR_INVALID, "lr3d_stereo_create: more than 2^32 pixels in one set");
     }
     std::lock_guard<std::mutex> lock(ctx->mu);
+    if (ctx->closing) return fail(LR3D_ERR_STATE, "lr3d_stereo_create: the context has been destroyed");
     HIP_TRY(hipSetDevice(ctx->device));
     lr3d_stereo *st = new lr3d_stereo();
     st->ctx = ctx;
@@ -316,6 +335,7 @@ extern "C" int32_t lr3d_stereo_create(lr3d_ctx *ctx, const lr3d_view *views, int
         return fail(e == hipErrorOutOfMemory ? LR3D_ERR_NOMEM : LR3D_ERR_DEVICE,
                     "lr3d_stereo_create: %s", hipGetErrorString(e));
     }
+    ctx->users++;
     *out = st;
     return LR3D_OK;
 }
@@ -330,7 +350,7 @@ static void detach_coarse(lr3d_stereo *st) {
 extern "C" int32_t lr3d_stereo_destroy(lr3d_stereo *st) {
     if (!st) return LR3D_OK;
     lr3d_ctx *ctx = st->ctx;
-    std::lock_guard<std::mutex> lock(ctx->mu);
+    std::unique_lock<std::mutex> lock(ctx->mu);
     (void)hipSetDevice(ctx->device);
     (void)hipStreamSynchronize(ctx->stream);
     detach_coarse(st);
@@ -342,6 +362,7 @@ extern "C" int32_t lr3d_stereo_destroy(lr3d_stereo *st) {
         fine->graphs.clear();
     }
     delete st;
+    ctx_release(ctx, lock);
     return LR3D_OK;
 }
 
@@ -1154,6 +1175,7 @@ extern "C" int32_t lr3d_tsdf_create(lr3d_ctx *ctx, const double origin[3], doubl
             if (coords[b * 4 + a] < 0 || coords[b * 4 + a] >= dims[a])
                 return fail(LR3D_ERR_INVALID, "lr3d_tsdf_create: brick %d lies outside the grid", b);
     std::lock_guard<std::mutex> lock(ctx->mu);
+    if (ctx->closing) return fail(LR3D_ERR_STATE, "lr3d_tsdf_create: the context has been destroyed");
     HIP_TRY(hipSetDevice(ctx->device));
     lr3d_tsdf *ts = new lr3d_tsdf();
     ts->ctx = ctx;
@@ -1188,6 +1210,7 @@ extern "C" int32_t lr3d_tsdf_create(lr3d_ctx *ctx, const double origin[3], doubl
         return fail(e == hipErrorOutOfMemory ? LR3D_ERR_NOMEM : LR3D_ERR_DEVICE, "lr3d_tsdf_create: %s",
                     hipGetErrorString(e));
     }
+    ctx->users++;
     *out = ts;
     return LR3D_OK;
 }
@@ -1195,10 +1218,11 @@ extern "C" int32_t lr3d_tsdf_create(lr3d_ctx *ctx, const double origin[3], doubl
 extern "C" int32_t lr3d_tsdf_destroy(lr3d_tsdf *ts) {
     if (!ts) return LR3D_OK;
     lr3d_ctx *ctx = ts->ctx;
-    std::lock_guard<std::mutex> lock(ctx->mu);
+    std::unique_lock<std::mutex> lock(ctx->mu);
     (void)hipSetDevice(ctx->device);
     (void)hipStreamSynchronize(ctx->stream);
     delete ts;
+    ctx_release(ctx, lock);
     return LR3D_OK;
 }
 
@@ -1639,6 +1663,7 @@ extern "C" int32_t lr3d_tsdf_extract_surface(lr3d_tsdf *ts, double min_weight, l
     if (rc) return rc;
     rc = mesh_normals(m.get(), scan);
     if (rc) return rc;
+    ctx->users++;  // a mesh made from a live volume keeps the context (held: ctx->mu)
     *out = m.release();
     return LR3D_OK;
 }
@@ -1685,6 +1710,7 @@ extern "C" int32_t lr3d_mesh_remove_small_components(lr3d_mesh *in, double fract
                                 in->normals.ptr, m->indices.ptr, m->positions.ptr, m->normals.ptr));
     int32_t rc = finish(ctx);
     if (rc) return rc;
+    ctx->users++;  // held: ctx->mu
     *out = m.release();
     return LR3D_OK;
 }
@@ -1901,6 +1927,7 @@ extern "C" int32_t lr3d_fuse_depth_maps(lr3d_stereo *st, const int32_t *layers, 
     rc = finish(ctx);
     if (rc) return rc;
     if (final_cell) *final_cell = cell;
+    ctx->users++;  // held: ctx->mu
     *out = c.release();
     return LR3D_OK;
 }
@@ -1927,10 +1954,11 @@ extern "C" int32_t lr3d_cloud_read(lr3d_cloud *c, float *positions, float *norma
 extern "C" int32_t lr3d_cloud_destroy(lr3d_cloud *c) {
     if (!c) return LR3D_OK;
     lr3d_ctx *ctx = c->ctx;
-    std::lock_guard<std::mutex> lock(ctx->mu);
+    std::unique_lock<std::mutex> lock(ctx->mu);
     (void)hipSetDevice(ctx->device);
     (void)hipStreamSynchronize(ctx->stream);
     delete c;
+    ctx_release(ctx, lock);
     return LR3D_OK;
 }
 
@@ -1958,10 +1986,11 @@ extern "C" int32_t lr3d_mesh_read(lr3d_mesh *m, float *positions, float *normals
 extern "C" int32_t lr3d_mesh_destroy(lr3d_mesh *m) {
     if (!m) return LR3D_OK;
     lr3d_ctx *ctx = m->ctx;
-    std::lock_guard<std::mutex> lock(ctx->mu);
+    std::unique_lock<std::mutex> lock(ctx->mu);
     (void)hipSetDevice(ctx->device);
     (void)hipStreamSynchronize(ctx->stream);
     delete m;
+    ctx_release(ctx, lock);
     return LR3D_OK;
 }
 

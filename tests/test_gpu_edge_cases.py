@@ -1,9 +1,9 @@
 """Edge cases of the C ABI on the GPU: empty and degenerate calls (empty batches, views without
 sources, a zero-view TSDF pass, volumes nothing can see, empty meshes and clouds, an empty voxel
 list for the adjoint), rejected calls (duplicate or unknown layers, a coarse start without a
-coarse level, a set as its own coarse level, a coarse level destroyed before its fine set), and
-ragged and extreme image sizes (1x1, 17x13, 16384x16, 16x16384: partial tiles and the widest rows)
-solved bit-exact to the oracle."""
+coarse level, a set as its own coarse level, a coarse level destroyed before its fine set), a
+context destroyed before its objects, and ragged and extreme image sizes (1x1, 17x13, 16384x16,
+16x16384: partial tiles and the widest rows) solved bit-exact to the oracle."""
 import numpy as np
 import pytest
 
@@ -75,6 +75,27 @@ def test_coarse_level_lifetime(ctx):
     st.solve(0, lr3d.SolveOptions(sources=w.sources[0], min_depth=0.5, max_depth=3.0, iterations=1))
     assert st.depth_map(0).any()
     st.destroy()
+
+
+def test_context_outlives_its_objects():
+    import lr3d
+    say("case: a context destroyed while a stereo set and a volume remain is released by the last destroy")
+    w = scenes.sphere_small().render()
+    c = lr3d.Context(0)
+    st = lr3d.HipDenseStereo(c, w.views)
+    for i, im in enumerate(w.images):
+        st.set_image(i, im)
+    grid, layout = lr3d.dense_grid(w.tsdf.origin, w.tsdf.voxel, w.tsdf.voxels)
+    vol = lr3d.HipTsdfVolume(c, grid, layout, w.tsdf.truncation_voxels)
+    c.destroy()  # two users remain: the context stays until the last one goes
+    st.solve(0, lr3d.SolveOptions(sources=w.sources[0], min_depth=0.5, max_depth=3.0, iterations=1))
+    v = w.views[0]
+    vol.integrate(st, 0, lr3d.IntegrationView(v.intrinsics, v.pose, 3.0, v.width, v.height))
+    m = vol.extract_surface(0.6)  # a mesh of a live volume is one more user
+    st.destroy()
+    vol.destroy()
+    assert len(m.read()["positions"]) > 0
+    m.destroy()  # the last user: the context is released here
 
 
 @pytest.mark.parametrize("sizes", [[(17, 13), (17, 13), (17, 13)], [(1, 1), (16, 16), (16, 16)],

@@ -283,22 +283,33 @@ def main():
         threads = min(16, os.cpu_count() or 1)
         O.set_threads(threads)
         imgs = O.Images([work.images[i] for i in range(n)])
-        cpu_iters = 1
+        # a workload small enough runs whole (every view, every iteration), repeated until ~10 s of
+        # CPU work; others run a bounded sample: init + 1 iteration per view until 10 s are spent
+        whole = mpx_hyp <= 300.0
+        cpu_iters = work.iterations if whole else 1
         budget_s = 10.0
-        log(f"cpu baseline: oracle, init + {cpu_iters} iteration per view until {budget_s:g}s, {threads} threads")
+        log(f"cpu baseline: oracle, init + {cpu_iters} iteration(s) per view for {budget_s:g}s, {threads} threads")
         t2 = time.perf_counter()
         done = []
-        for ref in range(n):
-            lo, hi = work.ranges[ref]
-            O.solve(work.views, imgs, ref, work.sources[ref], lo, hi, iterations=cpu_iters, topk=work.topk)
-            done.append(ref)
-            if time.perf_counter() - t2 >= budget_s:
+        passes = 0
+        while True:
+            for ref in range(n):
+                lo, hi = work.ranges[ref]
+                O.solve(work.views, imgs, ref, work.sources[ref], lo, hi, iterations=cpu_iters, topk=work.topk)
+                if passes == 0:
+                    done.append(ref)
+                if not whole and time.perf_counter() - t2 >= budget_s:
+                    break
+            passes += 1
+            if not whole or time.perf_counter() - t2 >= budget_s:
                 break
         cpu_s = time.perf_counter() - t2
-        sample = sum(work.views[r].width * work.views[r].height for r in done) / 1e6 * (1 + 14 * cpu_iters)
+        sample = passes * sum(work.views[r].width * work.views[r].height for r in done) / 1e6 * (1 + 14 * cpu_iters)
+        what = (f"the whole workload x {passes}" if whole
+                else f"views {done[0]}-{done[-1]} of {work.name}")
         cpu = {"value": round(sample / cpu_s, 3), "unit": "Mpx*hyp/s", "cores": threads, "kind": "port",
-               "sample": f"views {done[0]}-{done[-1]} of {work.name} (init + {cpu_iters} red-black iteration "
-                         f"each, same sources and top-k) = {sample:.2f} Mpx*hyp in {cpu_s:.1f}s on {cpu_model()}"}
+               "sample": f"{what} (init + {cpu_iters} red-black iteration(s) each, same sources and top-k) = "
+                         f"{sample:.2f} Mpx*hyp in {cpu_s:.1f}s on {cpu_model()}"}
 
     out = {
         "metric": "PatchMatch Mpx*hyp/s (batched PatchMatch + filter + TSDF step)",

@@ -28,7 +28,7 @@ from typing import List, Optional
 
 import numpy as np
 
-from .camera import Intrinsics, Pose, StereoView, look_at
+from .camera import Intrinsics, Pose, StereoView, look_at, scale_intrinsics
 
 LIB_PATH = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "lib",
                         "libscenegen.so")
@@ -345,10 +345,38 @@ def room_large(seed: int = 5, n: int = 200, iterations: int = 8) -> Workload:
                     tsdf=TsdfSpec((-2.048, 1.25 - 2.048, -2.048), 0.008, 512, 4.0))
 
 
+def table_scene(seed: int = 6) -> SceneSpec:
+    """A textured box (0.14 x 0.12 x 0.10 m) and a sphere (radius 5 cm) on a 1 m table top, at the
+    reference's texture frequencies (frontend/tools/synthetic-scene.ts:8-13 describes the same
+    layout)."""
+    return SceneSpec([Primitive(SG_BOX, (-0.07, 0.0, -0.05), (0.07, 0.12, 0.05)),
+                      Primitive(SG_SPHERE, (0.13, 0.05, 0.07), r=0.05),
+                      Primitive(SG_RECT_Y, (-0.5, 0.0, -0.5), (0.5, 0.0, 0.5))], seed=seed, freq=1.0)
+
+
+def reference_check(seed: int = 6) -> Workload:
+    """The reference's PatchMatch known-answer configuration (frontend/tools/verify/
+    stereo-check.ts:11-49): 5 views at 640x360 (f = 500, the 1280x720 f = 1000 camera scaled),
+    10 degrees apart on a 0.42 m circle at 0.2 m height looking at (0.03, 0.05, 0.02); view 2 is
+    solved from views 0, 1, 3, 4 over depths 0.1-2 m with 5 iterations, top-k by the reference
+    rule.  The other views take their ring neighbours."""
+    W, H = 640, 360
+    k = scale_intrinsics(Intrinsics(1000.0, 1000.0, 639.5, 359.5), W / 1280, H / 720)
+    views = []
+    for step in (-2, -1, 0, 1, 2):
+        a = step / 36 * 2 * math.pi
+        views.append(StereoView(k, look_at((0.42 * math.sin(a), 0.2, 0.42 * math.cos(a)), (0.03, 0.05, 0.02)),
+                                W, H))
+    sources = [[i for i in range(5) if i != r] for r in range(5)]
+    return Workload("reference-check-5x640x360", table_scene(seed), views, sources, [(0.1, 2.0)] * 5,
+                    iterations=5, topk=0, tsdf=TsdfSpec((-0.256, -0.05, -0.256), 0.004, 128, 4.0))
+
+
 WORKLOADS = {
     "sphere-small": sphere_small,
     "sphere-phone": sphere_phone,
     "room-batched": room_batched,
     "sphere-diff": sphere_diff,
     "room-large": room_large,
+    "reference-check": reference_check,
 }

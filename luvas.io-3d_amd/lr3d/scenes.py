@@ -372,6 +372,30 @@ def reference_check(seed: int = 6) -> Workload:
                     iterations=5, topk=0, tsdf=TsdfSpec((-0.256, -0.05, -0.256), 0.004, 128, 4.0))
 
 
+def table_rings(seed: int = 6, iterations: int = 6) -> Workload:
+    """The reference's verification capture around the table scene (frontend/tools/verify/
+    trajectory.ts:5-12): two rings of keyframes looking at (0.03, 0.05, 0.02), 36 at 0.2 m height
+    and 0.42 m radius and 24 at 0.36 m and 0.36 m, here at 640x360 (f = 500); each view takes its
+    8 nearest cameras as sources; depth 0.1-1.2 m; TSDF 256^3 x 2 mm over the table centre,
+    truncation 4 voxels."""
+    W, H = 640, 360
+    k = scale_intrinsics(Intrinsics(1000.0, 1000.0, 639.5, 359.5), W / 1280, H / 720)
+    views = []
+    for height, radius, count in ((0.2, 0.42, 36), (0.36, 0.36, 24)):
+        for i in range(count):
+            a = 2 * math.pi * i / count
+            views.append(StereoView(k, look_at((radius * math.sin(a), height, radius * math.cos(a)),
+                                               (0.03, 0.05, 0.02)), W, H))
+    centres = np.array([-(v.pose.R.T @ v.pose.t) for v in views])
+    sources = []
+    for r in range(len(views)):
+        d = np.linalg.norm(centres - centres[r], axis=1)
+        d[r] = np.inf
+        sources.append([int(j) for j in np.argsort(d, kind="stable")[:8]])
+    return Workload("table-rings-60x640x360", table_scene(seed), views, sources, [(0.1, 1.2)] * len(views),
+                    iterations=iterations, topk=0, tsdf=TsdfSpec((-0.256, -0.05, -0.256), 0.002, 256, 4.0))
+
+
 WORKLOADS = {
     "sphere-small": sphere_small,
     "sphere-phone": sphere_phone,
@@ -379,4 +403,5 @@ WORKLOADS = {
     "sphere-diff": sphere_diff,
     "room-large": room_large,
     "reference-check": reference_check,
+    "table-rings": table_rings,
 }

@@ -1,11 +1,12 @@
 """End to end on the reference's verification scene with true poses: the two-ring capture of
 frontend/tools/verify/trajectory.ts:5-12 (60 views, here at 640x360), batched PatchMatch, the
 consistency filter, one batched TSDF pass of the filtered maps (256^3 x 2 mm) and surface nets,
-scored like the reference's own checks (frontend/tools/verify/score.ts:208-296, the High
-profile's limits at :33-37): mesh vertex distance to the true surface (median < 1.5 mm, 90th
-percentile < 4 mm), completeness (> 85 % of true surface samples within 3 mm of a vertex) and
-the box's width, height and depth measured from the mesh like a caliper (each within 3 mm of
-140 x 120 x 100 mm).  The reference scores its whole pipeline (poses from its own tracking and
+and the fused dense cloud (2 mm cells), scored like the reference's own checks
+(frontend/tools/verify/score.ts:199-296, the High profile's limits at :33-37): dense points'
+distance to the true surface (median < 1.5 mm, 90th percentile < 5 mm), mesh vertex distance
+(median < 1.5 mm, 90th percentile < 4 mm), completeness (> 85 % of true surface samples within
+3 mm of a vertex) and the box's width, height and depth measured from the mesh like a caliper
+(each within 3 mm of 140 x 120 x 100 mm).  The reference scores its whole pipeline (poses from its own tracking and
 bundle adjustment); here the poses are exact, so this scores the hot path alone."""
 import numpy as np
 import pytest
@@ -72,7 +73,7 @@ def caliper(vertices, tol):
     return np.array([size[0], np.median(top[:, 1]) - BOX_MIN[1], size[1]])
 
 
-def test_table_scene_mesh_is_accurate_and_complete(ctx):
+def test_table_scene_cloud_and_mesh_are_accurate_and_complete(ctx):
     import lr3d
     from scipy.spatial import cKDTree
     w = scenes.table_rings().render()
@@ -94,7 +95,14 @@ def test_table_scene_mesh_is_accurate_and_complete(ctx):
     verts = mesh.read()["positions"].astype(np.float64)
     mesh.destroy()
     vol.destroy()
+    thumbs = [np.zeros((60, 80, 3), np.uint8) for _ in range(n)]
+    cloud, cell = lr3d.fuse_depth_maps(st, list(range(n)), thumbs, spec.voxel, 4_000_000, map=lr3d.MAP_FILTERED)
     st.destroy()
+    points = cloud["positions"].astype(np.float64)
+    cerr = np.abs(w.scene.signed_distance(points[::7])) * 1000
+    print(f"dense cloud: {len(points)} points in {cell * 1000:.1f} mm cells, error median "
+          f"{np.median(cerr):.2f} mm, p90 {np.percentile(cerr, 90):.2f} mm (limits 1.5, 5)", flush=True)
+    assert np.median(cerr) < 1.5 and np.percentile(cerr, 90) < 5.0
 
     err = np.abs(w.scene.signed_distance(verts)) * 1000
     med, p90 = float(np.median(err)), float(np.percentile(err, 90))

@@ -67,8 +67,10 @@ def reference(solved):
 def finite_or_report(out, tag):
     bad = ~np.isfinite(out["grad_depth"]) | ~np.isfinite(out["grad_normal"]).all(axis=1)
     if bad.any() or not np.isfinite(out["grad_pose"]).all():
-        os.makedirs("gpurun_out", exist_ok=True)
-        np.savez(f"gpurun_out/adjoint_{tag}.npz", **{k: v for k, v in out.items() if k != "loss"})
+        dump = os.environ.get("LR3D_TEST_DUMP")  # a directory to keep the outputs in, if set
+        if dump:
+            os.makedirs(dump, exist_ok=True)
+            np.savez(os.path.join(dump, f"adjoint_{tag}.npz"), **{k: v for k, v in out.items() if k != "loss"})
         px = [divmod(int(i), 320)[::-1] for i in np.flatnonzero(bad)[:8]]
         say(f"{tag}: {int(bad.sum())} pixels with non-finite gradients, first (x, y) {px}; "
             f"pose finite {bool(np.isfinite(out['grad_pose']).all())}")

@@ -8,7 +8,8 @@ iteration count and the depth range.  Two cases run the geometric pass over the 
 photometric maps (equal sizes, as the oracle's map layout needs); the last two take 8 and 7
 cameras with 7 and 6 sources, one with top-5 (the kernels' 8-slot selection network).  Planes and
 costs of every pixel must equal the oracle's bit for bit: the candidate early exit, the texel
-layout and the launch shapes change no result."""
+layout and the launch shapes change no result.  On the photometric cases the NCC adjoint's forward
+pass must reproduce the solved costs bit for bit, with finite gradients."""
 import numpy as np
 import pytest
 
@@ -85,6 +86,11 @@ def test_random_configuration_matches_oracle(ctx, base, case):
                           geometric=geometric)
     st.solve_batched([(ref, o)])
     hyp, cost = st.hypotheses(ref)
+    if not geometric:
+        # the NCC adjoint's forward pass recomputes the solved planes' costs bit for bit
+        out = st.ncc_backward(ref, o)
+        assert np.array_equal(out["cost"].view(np.uint32), cost.view(np.uint32))
+        assert all(np.isfinite(out[k]).all() for k in ("grad_depth", "grad_normal", "grad_pose"))
     st.destroy()
     _, ohyp, ocost = O.solve(views, imgs, ref, sources, lo, hi, geometric=geometric, iterations=iterations,
                              topk=topk, depths=None if depths is None else depths.copy())

@@ -6,7 +6,8 @@ so the pool order and the brick lookup differ from a dense cube), the voxel size
 and per view a prior (the true depth with holes, zeros and NaNs) or none and a confidence weight
 or none.  The views are integrated in one batched pass on the GPU and one view at a time by the
 oracle; weights and update counts must be equal and tsdf values within 1e-4 of the truncation
-distance (the TSDF bar; they have been bit-identical in every run)."""
+distance (the TSDF bar; they have been bit-identical in every run), and the GPU's surface nets over
+the sparse pool must equal the oracle's on the same volume."""
 import numpy as np
 import pytest
 
@@ -75,6 +76,9 @@ def test_random_sparse_pass_matches_oracle(ctx, base, case):
     vol.integrate_batched(st, list(range(n)), ivs)
     tsdf, weight = vol.read()
     count = vol.update_count()
+    m = vol.extract_surface(0.6)
+    mesh = m.read()
+    m.destroy()
     vol.destroy()
     st.destroy()
 
@@ -94,3 +98,9 @@ def test_random_sparse_pass_matches_oracle(ctx, base, case):
     assert count == updates
     assert np.array_equal(weight.view(np.uint32), oweight.view(np.uint32))
     assert diff <= 1e-4 * mu
+    # surface nets over the sparse pool (the brick lookup) equal the oracle's on the same volume
+    ref = O.extract_surface(grid.origin, grid.voxel, grid.dims, layout[1], tsdf, weight, 0.6)
+    say(f"case {case}: mesh {len(mesh['positions'])} vertices, {len(mesh['indices']) // 3} triangles")
+    for k in ("positions", "normals", "indices"):
+        a, r = np.asarray(mesh[k]), np.asarray(ref[k])
+        assert a.shape == r.shape and np.array_equal(a.view(np.uint32), r.view(np.uint32)), k

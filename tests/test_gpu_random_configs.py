@@ -9,7 +9,8 @@ photometric maps (equal sizes, as the oracle's map layout needs); the last two t
 cameras with 7 and 6 sources, one with top-5 (the kernels' 8-slot selection network).  Planes and
 costs of every pixel must equal the oracle's bit for bit: the candidate early exit, the texel
 layout and the launch shapes change no result.  On the photometric cases the NCC adjoint's forward
-pass must reproduce the solved costs bit for bit, with finite gradients."""
+pass must reproduce the solved costs bit for bit, with finite gradients.  Four coarse-to-fine draws
+start the fine solve from a coarse level of a random scale."""
 import numpy as np
 import pytest
 
@@ -98,3 +99,52 @@ def test_random_configuration_matches_oracle(ctx, base, case):
     costs = cost.view(np.uint32) == ocost.view(np.uint32)
     say(f"case {case}: planes bit-exact on {planes.mean() * 100:.3f} %, costs on {costs.mean() * 100:.3f} %")
     assert planes.all() and costs.all()
+
+
+@pytest.mark.parametrize("case", range(4))
+def test_random_coarse_to_fine_matches_oracle(ctx, base, case):
+    """Coarse-to-fine (PM:296-333) with a coarse level of a random scale (0.3-0.7 of each view's
+    size, so the fine-to-coarse pixel mapping is not a factor of 2): the coarse solve and the fine
+    solve started from it equal the oracle's coarse solve, init_coarse and propagation."""
+    import lr3d
+    from lr3d import StereoView, scale_intrinsics
+    views, images, ref, sources, topk, iterations, lo, hi, _ = draw(base, case)  # photometric draws
+    rng = np.random.default_rng(9900 + case)
+    scale = float(rng.uniform(0.3, 0.7))
+    cviews = [StereoView(scale_intrinsics(v.intrinsics, max(8, round(v.width * scale)) / v.width,
+                                          max(8, round(v.height * scale)) / v.height),
+                         v.pose, max(8, round(v.width * scale)), max(8, round(v.height * scale)))
+              for v in views]
+    cimages = [base.scene.render(v, depth=False)[0].reshape(v.height, v.width) for v in cviews]
+    say(f"coarse case {case}: {[(v.width, v.height) for v in views]} over {[(v.width, v.height) for v in cviews]}, "
+        f"ref {ref}, sources {sources}, topk {topk}, fine iterations {iterations}")
+    coarse = lr3d.HipDenseStereo(ctx, cviews)
+    for i, im in enumerate(cimages):
+        coarse.set_image(i, im)
+    fine = lr3d.HipDenseStereo(ctx, views)
+    for i, im in enumerate(images):
+        fine.set_image(i, im)
+    fine.set_coarse(coarse)
+    coarse.solve(ref, lr3d.SolveOptions(sources=sources, min_depth=lo, max_depth=hi, iterations=2, topk=topk,
+                                        pack=False))
+    fine.solve_batched([(ref, lr3d.SolveOptions(sources=sources, min_depth=lo, max_depth=hi,
+                                                iterations=iterations, topk=topk, from_coarse=True))])
+    hyp, cost = fine.hypotheses(ref)
+    chyp, _ = coarse.hypotheses(ref)
+    fine.destroy()
+    coarse.destroy()
+    _, ochyp, _ = O.solve(cviews, O.Images(cimages), ref, sources, lo, hi, iterations=2, topk=topk, pack=False)
+    assert np.array_equal(chyp.view(np.uint32), ochyp.view(np.uint32))
+    imgs = O.Images(images)
+    v, cv = views[ref], cviews[ref]
+    ck = scale_intrinsics(v.intrinsics, cv.width / v.width, cv.height / v.height)
+    p = O.make_params(views, ref, sources, lo, hi, topk=topk)
+    ohyp, ocost = O.init_coarse(p, imgs, [ck.fx, ck.fy, ck.cx, ck.cy], cv.width, cv.height, ochyp)
+    for it in range(iterations):
+        for phase in (0, 1):
+            p = O.make_params(views, ref, sources, lo, hi, iteration=it, phase=phase, step=0.5 ** it, topk=topk)
+            O.propagate(p, imgs, ohyp, ocost)
+    planes = np.all(hyp.view(np.uint32) == ohyp.view(np.uint32), axis=1)
+    say(f"coarse case {case}: planes bit-exact on {planes.mean() * 100:.3f} %")
+    assert planes.all()
+    assert np.array_equal(cost.view(np.uint32), ocost.view(np.uint32))

@@ -449,6 +449,9 @@ struct TopK {
 // that bound already reaches `bound`, the candidate cannot win and the remaining views are not
 // evaluated; the bound is returned instead of the mean, and the caller discards it as it would
 // have discarded the mean.  A wavefront stops once all of its lanes have stopped.
+// The views are taken last to first: the k smallest costs are summed in ascending order, so the
+// order of evaluation changes no result, and source lists put the nearest cameras (which tend
+// to match best) first, so leaving them to the end raises the bound sooner.
 template <int KMAX, bool GEOMETRIC>
 __device__ __forceinline__ float hypothesis_cost(const PmDesc &d, const PmPools &pools, float px,
                                                  float py, float rx, float ry, const Window &win,
@@ -464,9 +467,10 @@ __device__ __forceinline__ float hypothesis_cost(const PmDesc &d, const PmPools 
     TopK<KMAX> top;
     top.reset();
     const int32_t first_check = count - d.topk + 1;  // from here on fewer than k views are left
-    for (int32_t s = 0; s < count; s++) {
-        if (s >= first_check) {
-            const int32_t known = s - first_check + 1;  // k minus the views left
+    for (int32_t i = 0; i < count; i++) {
+        const int32_t s = count - 1 - i;  // last source first
+        if (i >= first_check) {
+            const int32_t known = i - first_check + 1;  // k minus the views left
             float total = 0.0f;
 #pragma unroll
             for (int j = 0; j < KMAX; j++)

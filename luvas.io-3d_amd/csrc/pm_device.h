@@ -450,9 +450,9 @@ struct TopK {
 // evaluated; the bound is returned instead of the mean, and the caller discards it as it would
 // have discarded the mean.  A wavefront stops once all of its lanes have stopped.
 // The views are taken last to first: the k smallest costs are summed in ascending order, so the
-// order of evaluation changes no result, and source lists are ranked best first (view-selection.ts
-// :46-48 sorts by the score of shared points), so leaving the best-matching views to the end
-// raises the bound sooner.
+// order of evaluation changes no result, and source lists are ranked best first
+// (mvs/view-selection.ts:46-48 sorts them by shared-point score), so leaving the best-matching
+// views to the end raises the bound sooner.
 template <int KMAX, bool GEOMETRIC>
 __device__ __forceinline__ float hypothesis_cost(const PmDesc &d, const PmPools &pools, float px,
                                                  float py, float rx, float ry, const Window &win,

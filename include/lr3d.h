@@ -35,13 +35,13 @@ extern "C" {
 #define LR3D_ERR_NOMEM (-3)   /* device allocation failed */
 #define LR3D_ERR_STATE (-4)   /* call not valid in the handle's current state */
 
-#define LR3D_MAX_SOURCES 8 /* DS:90 MAX_SOURCES */
+#define LR3D_MAX_SOURCES 8 /* DS:44 MAX_SOURCES */
 
 typedef struct lr3d_ctx lr3d_ctx;
 typedef struct lr3d_stereo lr3d_stereo;
 typedef struct lr3d_tsdf lr3d_tsdf;
 
-/* One camera of a stereo set: the reference's StereoView {intrinsics, pose} (DS:57-60) plus its
+/* One camera of a stereo set: the reference's StereoView {intrinsics, pose} (DS:11-14) plus its
  * own image size (the reference uses one size for all views; per-view sizes allow batches of
  * mixed resolution). */
 typedef struct {
@@ -51,10 +51,10 @@ typedef struct {
     double t[3];
 } lr3d_view;
 
-/* One PatchMatch solve: the reference's SolveOptions (DS:62-75) for reference view `ref`. */
+/* One PatchMatch solve: the reference's SolveOptions (DS:16-29) for reference view `ref`. */
 typedef struct {
     int32_t ref;
-    int32_t n_sources;                  /* sliced to LR3D_MAX_SOURCES like DS:232 */
+    int32_t n_sources;                  /* sliced to LR3D_MAX_SOURCES like DS:186 */
     int32_t sources[LR3D_MAX_SOURCES];
     double min_depth, max_depth;
     const float *prior;                 /* width*height depth guess (0 = none), or NULL */
@@ -99,41 +99,41 @@ int32_t lr3d_ctx_set_profiling(lr3d_ctx *ctx, int32_t enable);
  * returns the number of graph replays of the context instead. */
 int32_t lr3d_ctx_kernel_stats(lr3d_ctx *ctx, const char *name, int64_t *launches, double *total_ms);
 
-/* ---- PatchMatch stereo (replaces DenseStereo, DS:107-417) -------------------------------- */
+/* ---- PatchMatch stereo (replaces DenseStereo, DS:61-371) -------------------------------- */
 
-/* new DenseStereo(device, width, height, views) (DS:126-218).  flags: reserved, pass 0. */
+/* new DenseStereo(device, width, height, views) (DS:80-172).  flags: reserved, pass 0. */
 int32_t lr3d_stereo_create(lr3d_ctx *ctx, const lr3d_view *views, int32_t n_views, uint32_t flags,
                            lr3d_stereo **out);
-int32_t lr3d_stereo_destroy(lr3d_stereo *st);                    /* DS:403-416 */
-/* DenseStereo.setImage (DS:221-228): u8 gray, width x height of that layer. */
+int32_t lr3d_stereo_destroy(lr3d_stereo *st);                    /* DS:357-370 */
+/* DenseStereo.setImage (DS:175-182): u8 gray, width x height of that layer. */
 int32_t lr3d_stereo_set_image(lr3d_stereo *st, int32_t layer, const uint8_t *gray, int32_t width,
                               int32_t height);
-/* DenseStereo.solve (DS:231-285): init, `iterations` red-black rounds, pack.  One launch per
+/* DenseStereo.solve (DS:185-239): init, `iterations` red-black rounds, pack.  One launch per
  * kernel per phase, as the reference dispatches it (the "plain" path). */
 int32_t lr3d_stereo_solve(lr3d_stereo *st, const lr3d_solve_desc *desc);
 /* All solves of `descs` at once: every view's pixels share each launch, the iteration loop is
  * replayed from a captured HIP graph.  Results equal n calls of lr3d_stereo_solve. */
 int32_t lr3d_stereo_solve_batched(lr3d_stereo *st, const lr3d_solve_desc *descs, int32_t n);
-/* DenseStereo.filter (DS:296-309): consistency check of `ref` into its filtered map. */
+/* DenseStereo.filter (DS:250-263): consistency check of `ref` into its filtered map. */
 int32_t lr3d_stereo_filter(lr3d_stereo *st, int32_t ref, const int32_t *checks, int32_t n_checks,
                            int32_t needed);
-/* DenseStereo.depthMap (DS:355-358): packed u32 map of a layer (width*height words). */
+/* DenseStereo.depthMap (DS:309-312): packed u32 map of a layer (width*height words). */
 int32_t lr3d_stereo_read_depth(lr3d_stereo *st, int32_t layer, uint32_t *out);
-/* DenseStereo.filteredMap readback (DS:347-352) of the last filter of `layer`. */
+/* DenseStereo.filteredMap readback (DS:301-306) of the last filter of `layer`. */
 int32_t lr3d_stereo_read_filtered(lr3d_stereo *st, int32_t layer, uint32_t *out);
-/* DenseStereo.hypotheses (DS:288-290): planes (n.xyz, depth) and costs of a layer's last solve.
+/* DenseStereo.hypotheses (DS:242-244): planes (n.xyz, depth) and costs of a layer's last solve.
  * hyp: width*height*4 floats, cost: width*height floats (either may be NULL). */
 int32_t lr3d_stereo_read_hyp(lr3d_stereo *st, int32_t layer, float *hyp, float *cost);
 /* Overwrite a layer's packed depth map (e.g. maps computed elsewhere, or restored state). */
 int32_t lr3d_stereo_write_depth(lr3d_stereo *st, int32_t layer, const uint32_t *packed);
-/* Attach a coarse level for from_coarse solves (coarse-to-fine, DS:77-82 CoarseLevel): the
+/* Attach a coarse level for from_coarse solves (coarse-to-fine, DS:31-36 CoarseLevel): the
  * coarse stereo must hold one layer per layer of `st` and be another set of the same context;
  * its planes are read, not copied.  NULL detaches.  Destroying the coarse level detaches it from
  * every set using it (their from_coarse solves then fail with LR3D_ERR_STATE). */
 int32_t lr3d_stereo_set_coarse(lr3d_stereo *st, lr3d_stereo *coarse);
 
 /* Replace a layer's pose (world -> camera) for later solves, filters and adjoints (pose
- * refinement; the reference keeps poses in its StereoView list, DS:57-60). */
+ * refinement; the reference keeps poses in its StereoView list, DS:11-14). */
 int32_t lr3d_stereo_set_pose(lr3d_stereo *st, int32_t layer, const double R[9], const double t[3]);
 /* Overwrite a layer's planes (n.xyz, depth per pixel, width*height*4 floats). */
 int32_t lr3d_stereo_write_hyp(lr3d_stereo *st, int32_t layer, const float *hyp);

@@ -2,7 +2,7 @@
 //
 // Host semantics follow the reference's DenseStereo (frontend/src/app/vision/mvs/dense-stereo.ts,
 // DS:<line>) and TsdfVolume (frontend/src/app/vision/fusion/tsdf-volume.ts, TV:<line>): the same
-// parameter packing (DS:360-401, TV:244-253), seed derivation (DS:379), schedule (DS:231-285) and
+// parameter packing (DS:314-355, TV:244-253), seed derivation (DS:333), schedule (DS:185-239) and
 // culling (TV:220-234).  The reference slices work into row bands and awaits every submit so a
 // phone GPU stays responsive; bands do not change results, so here each dispatch covers whole
 // images and a call synchronises once, at its end.
@@ -407,7 +407,7 @@ TileShape lr3d::pm_tile_for(const char *kernel, int32_t W, int32_t H, int32_t n_
     return {64, 4};
 }
 
-// DS:360-401 for one solve, in double then rounded to f32 like Float32Array.set.
+// DS:314-355 for one solve, in double then rounded to f32 like Float32Array.set.
 // CAM:73-77 relativePose(ref, src) with math/mat3.ts multiply3 / apply3, in double:
 // R = R_s R_r^T, t = t_s - R t_r.
 static void relative_pose(const lr3d_view &rv, const lr3d_view &sv, double R[9], double t[3]) {
@@ -428,7 +428,7 @@ static int32_t build_desc(const lr3d_stereo *st, const lr3d_solve_desc &sd, uint
     int32_t rc = check_layer(st, sd.ref, what);
     if (rc) return rc;
     if (sd.n_sources < 0) return fail(LR3D_ERR_INVALID, "%s: n_sources < 0", what);
-    const int32_t n_src = std::min(sd.n_sources, (int32_t)kMaxSources);  // DS:232
+    const int32_t n_src = std::min(sd.n_sources, (int32_t)kMaxSources);  // DS:186
     if (!(sd.min_depth > 0.0) || !(sd.max_depth > sd.min_depth))
         return fail(LR3D_ERR_INVALID, "%s: depth range must satisfy 0 < min_depth < max_depth", what);
     if (sd.iterations < 0 || sd.iterations > 64)
@@ -453,12 +453,12 @@ static int32_t build_desc(const lr3d_stereo *st, const lr3d_solve_desc &sd, uint
     d.ky2 = -d.k[3] / d.k[1];
     d.inv_min = (float)(1.0 / sd.min_depth);
     d.inv_max = (float)(1.0 / sd.max_depth);
-    d.geo_weight = 0.2f;  // DS:92
+    d.geo_weight = 0.2f;  // DS:46
     d.flags = sd.geometric ? 1u : 0u;
     int32_t k = sd.topk > 0 ? std::min(sd.topk, n_src) : std::max(1, std::min(3, n_src - 1));  // PM:218
     d.topk = std::max(1, k);
     d.iterations = sd.iterations;
-    d.seed_base = (uint32_t)(sd.ref + 1) * 0x9e3779b1u + d.flags * 7u;  // DS:379
+    d.seed_base = (uint32_t)(sd.ref + 1) * 0x9e3779b1u + d.flags * 7u;  // DS:333
     d.off = st->off[sd.ref];
     d.pitch = quad_pitch(rv.width);
     d.qoff = st->qoff[sd.ref];
@@ -574,7 +574,7 @@ static PmPools pools_of(lr3d_stereo *st) {
     return p;
 }
 
-// The schedule of DS:231-285 for a batch: init, iterations x {phase 0, 1}, pack.
+// The schedule of DS:185-239 for a batch: init, iterations x {phase 0, 1}, pack.
 static hipError_t enqueue_schedule(lr3d_stereo *st, const Plans &plans, int32_t max_iterations,
                                    bool any_pack) {
     lr3d_ctx *ctx = st->ctx;
@@ -582,7 +582,7 @@ static hipError_t enqueue_schedule(lr3d_stereo *st, const Plans &plans, int32_t 
     hipError_t e = timed(ctx, "pm_init", [&] { return pm_launch_init(ctx->stream, plans.init, pools); });
     if (e != hipSuccess) return e;
     for (int32_t it = 0; it < max_iterations; it++) {
-        const float step = (float)pow(0.5, (double)it);  // DS:249
+        const float step = (float)pow(0.5, (double)it);  // DS:203
         for (uint32_t phase = 0; phase < 2; phase++) {
             e = timed(ctx, "pm_propagate", [&] {
                 return pm_launch_propagate(ctx->stream, plans.prop, pools, (uint32_t)it, phase, step);
@@ -640,7 +640,7 @@ static int32_t solve_impl(lr3d_stereo *st, const lr3d_solve_desc *descs, int32_t
     std::vector<int32_t> refs;
     std::vector<uint8_t> packs;
     for (int32_t i = 0; i < n; i++) {
-        if (std::min(descs[i].n_sources, (int32_t)kMaxSources) <= 0) continue;  // DS:233: no-op
+        if (std::min(descs[i].n_sources, (int32_t)kMaxSources) <= 0) continue;  // DS:187: no-op
         if (descs[i].from_coarse && !st->coarse)
             return fail(LR3D_ERR_STATE, "lr3d_stereo_solve: from_coarse without a coarse level");
         PmDesc d;
@@ -881,7 +881,7 @@ extern "C" int32_t lr3d_stereo_filter(lr3d_stereo *st, int32_t ref, const int32_
     lr3d_ctx *ctx = st->ctx;
     std::lock_guard<std::mutex> lock(ctx->mu);
     HIP_TRY(hipSetDevice(ctx->device));
-    // DS:296-309: the filter packs params with depth range 0.01..100 and step 0.
+    // DS:250-263: the filter packs params with depth range 0.01..100 and step 0.
     lr3d_solve_desc sd;
     memset(&sd, 0, sizeof(sd));
     sd.ref = ref;
@@ -1061,7 +1061,7 @@ static int32_t adjoint_impl(lr3d_stereo *st, const lr3d_solve_desc *descs, int32
                                    step_normal, apply);
     }));
     if (apply && plans.pack.blocks > 0) {
-        // refined planes -> packed depth maps of the descs that pack (DS:279-282)
+        // refined planes -> packed depth maps of the descs that pack (DS:233-236)
         HIP_TRY(timed(ctx, "pm_pack", [&] { return pm_launch_pack(ctx->stream, plans.pack, pools); }));
     }
     std::vector<double> partials((size_t)plan.blocks * kAdjPartial);

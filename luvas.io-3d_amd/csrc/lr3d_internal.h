@@ -25,7 +25,7 @@ namespace lr3d {
 constexpr int kMaxSources = 8;
 constexpr int kSamples = 36;
 
-// One source view of a reference view's solve (the WGSL `View`, PM:4-10, after DS:391-399).
+// One source view of a reference view's solve (the WGSL `View`, PM:4-10, after DS:345-353).
 struct PmSrc {
     float k[4];     // fx, fy, cx, cy
     float R[9];     // relative rotation ref -> src, row-major
@@ -46,7 +46,7 @@ struct PmDesc {
     uint32_t flags;            // bit 0: geometric pass
     int32_t topk;              // effective k (1..n_src)
     int32_t iterations;
-    uint32_t seed_base;        // (ref + 1) * 0x9e3779b1 + 7 * flags  (DS:379)
+    uint32_t seed_base;        // (ref + 1) * 0x9e3779b1 + 7 * flags  (DS:333)
     uint32_t off;              // pixel offset of the reference layer
     int32_t has_prior;
     uint32_t prior_off;

@@ -524,7 +524,7 @@ __device__ __forceinline__ float random_depth(uint32_t &rng, const PmDesc &d) {
 
 __device__ __forceinline__ uint32_t pixel_rng(const PmDesc &d, uint32_t iteration, uint32_t phase,
                                               uint32_t index) {
-    uint32_t seed = d.seed_base + iteration * 97u + phase * 13u;  // DS:379
+    uint32_t seed = d.seed_base + iteration * 97u + phase * 13u;  // DS:333
     return pcg(index ^ pcg(seed + iteration * 7919u + phase * 104729u));  // PM:251-253
 }
 

@@ -31,7 +31,7 @@ __global__ void __launch_bounds__(256) build_quad_kernel(const uint8_t *__restri
     quad[(size_t)y * pitch + x] = a | (b << 8) | (c << 16) | (e << 24);
 }
 
-// PM:255-289 (init) and PM:296-333 (init_coarse): iteration 0, phase 0 (DS:238-247)
+// PM:255-289 (init) and PM:296-333 (init_coarse): iteration 0, phase 0 (DS:192-201)
 template <int KMAX, bool GEOMETRIC>
 __global__ void __launch_bounds__(256) pm_init_kernel(PmPlan plan, PmPools pools) {
     PixelTask task = locate(plan, (int32_t)blockDim.x);

@@ -15,7 +15,7 @@ Scene choices that the workload descriptions leave open, fixed here:
     the room, 10-25 degrees down and up to 20 degrees off the radial direction, with a horizontal field of view
     of 70 degrees; resolutions cycle 640x480, 960x540, 1280x720, 1920x1080 along the loop, so
     neighbouring views differ in size; neighbour counts are drawn from 4-10 per view and sliced
-    to 8 (DS:232); depth 0.3-6 m; the 384^3 x 1 cm TSDF cube is centred on the room centre (the
+    to 8 (DS:186); depth 0.3-6 m; the 384^3 x 1 cm TSDF cube is centred on the room centre (the
     two x walls, 2 m from the centre, fall 8 cm outside it).
 """
 from __future__ import annotations

@@ -1,5 +1,5 @@
 """HipDenseStereo: the reference's DenseStereo (frontend/src/app/vision/mvs/dense-stereo.ts,
-DS:107-417) on the lr3d C ABI.
+DS:61-371) on the lr3d C ABI.
 
 Same members and argument meaning: ``set_image(layer, gray)``, ``solve(ref, options)``,
 ``filter(ref, sources, needed)``, ``filtered_map``, ``depth_map``, ``hypotheses``,
@@ -21,7 +21,7 @@ from .runtime import Context
 
 @dataclass
 class SolveOptions:
-    """DS:62-75 SolveOptions, plus the top-k override (0 = the reference rule)."""
+    """DS:16-29 SolveOptions, plus the top-k override (0 = the reference rule)."""
     sources: Sequence[int]
     min_depth: float
     max_depth: float
@@ -44,7 +44,7 @@ def _view_struct(v: StereoView) -> N.View:
 
 
 class HipDenseStereo:
-    """GPU PatchMatch for every keyframe (DS:102-107): one device pool per buffer kind holds every
+    """GPU PatchMatch for every keyframe (DS:56-61): one device pool per buffer kind holds every
     view, so the geometric pass and the consistency filter read any neighbour without copies."""
 
     def __init__(self, ctx: Context, views: Sequence[StereoView]):
@@ -58,7 +58,7 @@ class HipDenseStereo:
         self.handle = handle
         self._keep = []
 
-    # DS:126-133 exposes width/height of the (single-size) reference set.
+    # DS:80-87 exposes width/height of the (single-size) reference set.
     @property
     def width(self) -> int:
         return self.views[0].width
@@ -71,7 +71,7 @@ class HipDenseStereo:
         return self.views[layer].width, self.views[layer].height
 
     def set_image(self, layer: int, gray: np.ndarray) -> None:
-        """DS:221-228"""
+        """DS:175-182"""
         w, h = self.size(layer)
         g = np.ascontiguousarray(gray, dtype=np.uint8).reshape(h, w)
         N.check(self._lib.lr3d_stereo_set_image(self.handle, layer, N.ptr(g), w, h))
@@ -79,7 +79,7 @@ class HipDenseStereo:
     def _desc(self, ref: int, o: SolveOptions):
         d = N.SolveDesc()
         d.ref = ref
-        sources = list(o.sources)[: N.MAX_SOURCES]  # DS:232
+        sources = list(o.sources)[: N.MAX_SOURCES]  # DS:186
         d.n_sources = len(sources)
         for i, s in enumerate(sources):
             d.sources[i] = int(s)
@@ -97,9 +97,9 @@ class HipDenseStereo:
         return d, prior
 
     def solve(self, ref: int, options: SolveOptions) -> None:
-        """DS:231-285: one launch per kernel per red-black phase."""
+        """DS:185-239: one launch per kernel per red-black phase."""
         if len(options.sources) == 0:
-            return  # DS:233
+            return  # DS:187
         d, keep = self._desc(ref, options)
         N.check(self._lib.lr3d_stereo_solve(self.handle, C.byref(d)))
 
@@ -132,13 +132,13 @@ class HipDenseStereo:
         N.check(self._lib.lr3d_stereo_set_tile(self.handle, kernel.encode(), int(tx), int(ty)))
 
     def filter(self, ref: int, sources: Sequence[int], needed: int) -> None:
-        """DS:296-309: consistency check of `ref` into its filtered map (stays on the GPU)."""
+        """DS:250-263: consistency check of `ref` into its filtered map (stays on the GPU)."""
         checked = list(sources)[: N.MAX_SOURCES]
         arr = (C.c_int32 * max(1, len(checked)))(*checked)
         N.check(self._lib.lr3d_stereo_filter(self.handle, ref, arr, len(checked), int(needed)))
 
     def filtered_map(self, ref: int, sources: Sequence[int], needed: int) -> np.ndarray:
-        """DS:347-352"""
+        """DS:301-306"""
         self.filter(ref, sources, needed)
         return self.read_filtered(ref)
 
@@ -149,7 +149,7 @@ class HipDenseStereo:
         return out
 
     def depth_map(self, ref: int) -> np.ndarray:
-        """DS:355-358: packed u32 map of a layer."""
+        """DS:309-312: packed u32 map of a layer."""
         w, h = self.size(ref)
         out = np.empty(w * h, dtype=np.uint32)
         N.check(self._lib.lr3d_stereo_read_depth(self.handle, ref, N.ptr(out)))
@@ -161,7 +161,7 @@ class HipDenseStereo:
         N.check(self._lib.lr3d_stereo_write_depth(self.handle, ref, N.ptr(p)))
 
     def hypotheses(self, ref: int) -> tuple[np.ndarray, np.ndarray]:
-        """DS:288-290: planes (n.xyz, d) and costs of a layer's last solve."""
+        """DS:242-244: planes (n.xyz, d) and costs of a layer's last solve."""
         w, h = self.size(ref)
         hyp = np.empty((w * h, 4), dtype=np.float32)
         cost = np.empty(w * h, dtype=np.float32)
@@ -212,7 +212,7 @@ class HipDenseStereo:
         N.check(self._lib.lr3d_stereo_set_coarse(self.handle, coarse.handle if coarse else None))
 
     def destroy(self) -> None:
-        """DS:403-416"""
+        """DS:357-370"""
         if self.handle:
             self._lib.lr3d_stereo_destroy(self.handle)
             self.handle = None

@@ -33,6 +33,8 @@ export declare class HipDenseStereo {
     readonly width: number;
     readonly height: number;
     readonly hypotheses: HipDenseStereo;
+    /** The filtered map (DS:67), as TsdfVolume binds it: the map of the layer filtered last. */
+    readonly filtered: FilteredMap;
     readonly lastFiltered: number;
     setImage(layer: number, gray: Uint8Array): void;
     solve(ref: number, options: SolveOptions): Promise<void>;
@@ -49,6 +51,11 @@ export declare class HipDenseStereo {
     fuseDepthMaps(layers: number[], thumbs: Thumbnail[], cell: number, cellBudget: number, map?: number,
                   minSupport?: number): Promise<FusedCloud>;
     destroy(): void;
+}
+
+/** What `HipDenseStereo.filtered` returns; HipTsdfVolume reads the stereo level through it. */
+export declare class FilteredMap {
+    readonly stereo: HipDenseStereo;
 }
 
 /** A view's colour thumbnail (depth-fusion.ts: `Thumbnail`). */
@@ -95,7 +102,7 @@ export interface IntegrationView {
 
 export declare class HipTsdfVolume {
     constructor(ctx: Context, grid: BrickGrid, layout: { coords: Int32Array; lookup: Int32Array; count: number },
-                stereo: HipDenseStereo, width: number, height: number, truncationVoxels?: number);
+                depth: FilteredMap | HipDenseStereo, width: number, height: number, truncationVoxels?: number);
     readonly truncation: number;
     integrate(view: IntegrationView): Promise<void>;
     integrateLayers(layers: number[], views: IntegrationView[], map?: number): Promise<void>;

@@ -42,7 +42,7 @@ function cameraOf(view, width, height) {
 
 class HipDenseStereo {
     /**
-     * new DenseStereo(device, width, height, views, options) (DS:126-218).  `views` may carry
+     * new DenseStereo(device, width, height, views, options) (DS:80-172).  `views` may carry
      * their own width/height (mixed-resolution sets); options.coarse is the coarse level.
      */
     constructor(ctx, width, height, views, options) {
@@ -53,19 +53,23 @@ class HipDenseStereo {
         this.cameras = views.map((v) => cameraOf(v, width, height));
         this.handle = native.stereoCreate(ctx.handle, this.cameras);
         this.lastFiltered = -1;
+        this.filteredMapBinding = new FilteredMap(this);
         const coarse = options && options.coarse ? options.coarse : null;
         if (coarse) native.stereoSetCoarse(this.handle, coarse.hypotheses.handle);
     }
 
-    /** DS:221-228 */
+    /** DS:175-182 */
     setImage(layer, gray) {
         const c = this.cameras[layer];
         native.stereoSetImage(this.handle, layer, gray, c.width, c.height);
     }
 
-    static descOf(ref, options) {
+    /** SolveOptions of layer `ref` for the addon; `pixels` lets it check a prior's length. */
+    descOf(ref, options) {
+        const c = this.cameras[ref];
         return {
             ref,
+            pixels: c ? c.width * c.height : -1,
             sources: options.sources.slice(0, MAX_SOURCES),
             minDepth: options.minDepth,
             maxDepth: options.maxDepth,
@@ -78,38 +82,46 @@ class HipDenseStereo {
         };
     }
 
-    /** DS:231-285 */
+    /** DS:185-239 */
     async solve(ref, options) {
         if (options.sources.length === 0) return;
-        await native.stereoSolve(this.handle, HipDenseStereo.descOf(ref, options));
+        await native.stereoSolve(this.handle, this.descOf(ref, options));
     }
 
     /** Every (ref, options) pair in shared launches; the same result as solving them in order. */
     async solveBatched(solves) {
         const descs = solves.filter((s) => s[1].sources.length > 0)
-            .map((s) => HipDenseStereo.descOf(s[0], s[1]));
+            .map((s) => this.descOf(s[0], s[1]));
         if (descs.length) await native.stereoSolve(this.handle, descs);
     }
 
-    /** The planes of the last solve (DS:288-290): this level, usable as a coarse level. */
+    /** The planes of the last solve (DS:242-244): this level, usable as a coarse level. */
     get hypotheses() {
         return this;
     }
 
-    /** DS:296-309 */
+    /**
+     * The filtered map (DS:67 `filtered`): the map of the layer filtered last, as a binding that
+     * HipTsdfVolume accepts in place of the reference's GPUBuffer (tsdf-meshing.ts:71).
+     */
+    get filtered() {
+        return this.filteredMapBinding;
+    }
+
+    /** DS:250-263 */
     async filter(ref, sources, needed) {
         await native.stereoFilter(this.handle, ref, sources.slice(0, MAX_SOURCES), needed);
         this.lastFiltered = ref;
     }
 
-    /** DS:347-352 */
+    /** DS:301-306 */
     async filteredMap(ref, sources, needed) {
         await this.filter(ref, sources, needed);
         const c = this.cameras[ref];
         return native.stereoReadFiltered(this.handle, ref, c.width, c.height);
     }
 
-    /** DS:355-358 */
+    /** DS:309-312 */
     async depthMap(ref) {
         const c = this.cameras[ref];
         return native.stereoReadDepth(this.handle, ref, c.width, c.height);
@@ -128,7 +140,7 @@ class HipDenseStereo {
     async nccBackward(ref, options, weights) {
         const c = this.cameras[ref];
         const w = weights === undefined || weights === null ? null : Float32Array.from(weights);
-        return native.nccBackward(this.handle, HipDenseStereo.descOf(ref, options), w, c.width * c.height);
+        return native.nccBackward(this.handle, this.descOf(ref, options), w, c.width * c.height);
     }
 
     /**
@@ -136,7 +148,7 @@ class HipDenseStereo {
      * {gradPose (per solve, 8 x 12), lossBefore, lossAfter}.  Steps of 0 only evaluate.
      */
     async refineStep(solves, stepDepth, stepNormal) {
-        return native.nccRefineStep(this.handle, solves.map((s) => HipDenseStereo.descOf(s[0], s[1])),
+        return native.nccRefineStep(this.handle, solves.map((s) => this.descOf(s[0], s[1])),
             stepDepth, stepNormal);
     }
 
@@ -160,6 +172,16 @@ class HipDenseStereo {
     destroy() {
         if (this.handle) native.stereoDestroy(this.handle);
         this.handle = null;
+    }
+}
+
+/**
+ * What `stereo.filtered` hands to a volume: the reference binds the one `filtered` buffer, which
+ * holds the map of the layer filtered last (DS:250-263); integrate() reads that layer's map.
+ */
+class FilteredMap {
+    constructor(stereo) {
+        this.stereo = stereo;
     }
 }
 
@@ -208,15 +230,18 @@ class BrickGrid {
 
 class HipTsdfVolume {
     /**
-     * new TsdfVolume(device, grid, layout, depth, width, height) (TV:171-217); `depth` is the
-     * stereo level whose filtered map is integrated (the reference binds `stereo.filtered`).
-     * truncationVoxels: 4 in the reference (TV:179).
+     * new TsdfVolume(device, grid, layout, depth, width, height) (TV:171-217); `depth` is
+     * `stereo.filtered`, as the reference passes it (tsdf-meshing.ts:71), or the stereo level
+     * itself.  truncationVoxels: 4 in the reference (TV:179).
      */
-    constructor(ctx, grid, layout, stereo, width, height, truncationVoxels) {
+    constructor(ctx, grid, layout, depth, width, height, truncationVoxels) {
+        if (!(depth instanceof FilteredMap) && !(depth instanceof HipDenseStereo)) {
+            throw new TypeError('HipTsdfVolume: depth must be stereo.filtered or a HipDenseStereo');
+        }
         this.ctx = ctx;
         this.grid = grid;
         this.layout = layout;
-        this.stereo = stereo;
+        this.stereo = depth instanceof FilteredMap ? depth.stereo : depth;
         this.width = width;
         this.height = height;
         this.truncation = grid.voxel * (truncationVoxels || 4);
@@ -586,6 +611,6 @@ async function refinePyramid(stereo, coarse, solves, options) {
 
 module.exports = {
     Context, HipDenseStereo, HipTsdfVolume, BrickGrid, STEREO_METHODS, MAP_RAW, MAP_FILTERED,
-    MAX_SOURCES, BRICK, NccCostFunction, TsdfIntegrationFunction, native,
+    MAX_SOURCES, BRICK, NccCostFunction, TsdfIntegrationFunction, FilteredMap, native,
     refine, refinePyramid, halfResolution, worldPoseGradients, perturbPose, expSo3,
 };

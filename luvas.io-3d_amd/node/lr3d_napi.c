@@ -73,21 +73,33 @@ static int get_doubles(napi_env env, napi_value obj, const char *name, double *o
     return 1;
 }
 
-/* Copy of a typed array's bytes (NULL when the value is null/undefined). */
-static void *copy_typed(napi_env env, napi_value v, size_t elem, size_t *count) {
+/* Copy of a typed array of element type `want` (NULL when the value is null/undefined).  *bad is
+ * set when the value is anything else than such an array (another element type included), so the
+ * caller can throw instead of reading the wrong number of bytes. */
+static void *copy_typed(napi_env env, napi_value v, napi_typedarray_type want, size_t *count, int *bad) {
+    *bad = 0;
+    if (count) *count = 0;
     napi_valuetype t;
     napi_typeof(env, v, &t);
     if (t == napi_null || t == napi_undefined) return NULL;
     bool is_ta = false;
     napi_is_typedarray(env, v, &is_ta);
-    if (!is_ta) return NULL;
     napi_typedarray_type type;
     size_t length = 0, offset = 0;
     void *data = NULL;
     napi_value ab;
-    napi_get_typedarray_info(env, v, &type, &length, &data, &ab, &offset);
+    if (!is_ta || napi_get_typedarray_info(env, v, &type, &length, &data, &ab, &offset) != napi_ok ||
+        type != want) {
+        *bad = 1;
+        return NULL;
+    }
+    size_t elem = (want == napi_uint8_array) ? 1 : (want == napi_float64_array) ? 8 : 4;
     void *copy = malloc(length * elem + 1);
-    if (copy && length) memcpy(copy, data, length * elem);
+    if (!copy) {
+        *bad = 1;
+        return NULL;
+    }
+    if (length) memcpy(copy, data, length * elem);
     if (count) *count = length;
     return copy;
 }
@@ -389,8 +401,9 @@ static napi_value stereo_set_image(napi_env env, napi_callback_info info) {
     napi_get_value_int32(env, argv[3], &w);
     napi_get_value_int32(env, argv[4], &h);
     size_t count = 0;
-    uint8_t *gray = (uint8_t *)copy_typed(env, argv[2], 1, &count);
-    if (!gray || count < (size_t)w * (size_t)h) {
+    int bad = 0;
+    uint8_t *gray = (uint8_t *)copy_typed(env, argv[2], napi_uint8_array, &count, &bad);
+    if (!gray || count != (size_t)w * (size_t)h) {
         free(gray);
         napi_throw_type_error(env, NULL, "lr3d.node: setImage needs a Uint8Array of width*height bytes");
         return NULL;
@@ -401,7 +414,9 @@ static napi_value stereo_set_image(napi_env env, napi_callback_info info) {
     return undefined(env);
 }
 
-/* SolveOptions (DS:62-75) + ref -> lr3d_solve_desc; the prior is copied into *prior_out */
+/* SolveOptions (DS:16-29) + ref -> lr3d_solve_desc; the prior is copied into *prior_out.  Returns 1,
+ * 0 when ref/sources are missing, -1 when the prior is not a Float32Array of `pixels` (the
+ * reference layer's width*height, set by lr3d.js) elements. */
 static int read_solve(napi_env env, napi_value o, lr3d_solve_desc *d, float **prior_out) {
     memset(d, 0, sizeof(*d));
     d->ref = (int32_t)get_num(env, o, "ref", 0);
@@ -428,7 +443,11 @@ static int read_solve(napi_env env, napi_value o, lr3d_solve_desc *d, float **pr
     if (has) {
         napi_value p;
         napi_get_named_property(env, o, "prior", &p);
-        *prior_out = (float *)copy_typed(env, p, 4, NULL);
+        size_t count = 0;
+        int bad = 0;
+        *prior_out = (float *)copy_typed(env, p, napi_float32_array, &count, &bad);
+        if (bad) return -1;
+        if (*prior_out && (double)count != get_num(env, o, "pixels", -1)) return -1;
     }
     d->prior = *prior_out;
     return 1;
@@ -449,9 +468,11 @@ static napi_value stereo_solve(napi_env env, napi_callback_info info) {
     for (uint32_t i = 0; i < n; i++) {
         napi_value o = argv[1];
         if (is_array) napi_get_element(env, argv[1], i, &o);
-        if (!read_solve(env, o, &j->descs[i], &j->priors[i])) {
+        const int ok = read_solve(env, o, &j->descs[i], &j->priors[i]);
+        if (ok <= 0) {
             job_free(j);
-            napi_throw_type_error(env, NULL, "lr3d.node: solve options need ref and sources");
+            if (ok < 0) napi_throw_range_error(env, NULL, "lr3d.node: prior must be a Float32Array of width*height");
+            else napi_throw_type_error(env, NULL, "lr3d.node: solve options need ref and sources");
             return NULL;
         }
     }
@@ -521,7 +542,8 @@ static napi_value tsdf_create(napi_env env, napi_callback_info info) {
     }
     napi_get_value_double(env, argv[2], &voxel);
     size_t count = 0;
-    int32_t *coords = (int32_t *)copy_typed(env, argv[4], 4, &count);
+    int bad = 0;
+    int32_t *coords = (int32_t *)copy_typed(env, argv[4], napi_int32_array, &count, &bad);
     napi_get_value_double(env, argv[5], &trunc);
     if (!coords) {
         napi_throw_type_error(env, NULL, "lr3d.node: coords must be an Int32Array");
@@ -542,7 +564,8 @@ static napi_value tsdf_destroy(napi_env env, napi_callback_info info) {
     return undefined(env);
 }
 
-/* IntegrationView (TV:67-79) + size -> lr3d_integration_view (prior copied) */
+/* IntegrationView (TV:67-79) + size -> lr3d_integration_view (prior copied).  Returns 1, 0 when
+ * the pose is missing, -1 when the prior is not a Float32Array of width*height elements. */
 static int read_iview(napi_env env, napi_value o, lr3d_integration_view *v, float **prior) {
     memset(v, 0, sizeof(*v));
     v->width = (int32_t)get_num(env, o, "width", 0);
@@ -559,7 +582,10 @@ static int read_iview(napi_env env, napi_value o, lr3d_integration_view *v, floa
     if (has) {
         napi_value p;
         napi_get_named_property(env, o, "prior", &p);
-        *prior = (float *)copy_typed(env, p, 4, NULL);
+        size_t count = 0;
+        int bad = 0;
+        *prior = (float *)copy_typed(env, p, napi_float32_array, &count, &bad);
+        if (bad || (*prior && count != (size_t)v->width * (size_t)v->height)) return -1;
     }
     v->prior = *prior;
     return get_doubles(env, o, "R", v->R, 9) && get_doubles(env, o, "t", v->t, 3);
@@ -582,9 +608,11 @@ static napi_value tsdf_integrate(napi_env env, napi_callback_info info) {
         napi_get_element(env, argv[2], i, &e);
         napi_get_value_int32(env, e, &j->layers[i]);
         napi_get_element(env, argv[4], i, &o);
-        if (!read_iview(env, o, &j->iviews[i], &j->priors[i])) {
+        const int ok = read_iview(env, o, &j->iviews[i], &j->priors[i]);
+        if (ok <= 0) {
             job_free(j);
-            napi_throw_type_error(env, NULL, "lr3d.node: an integration view needs intrinsics, R[9], t[3]");
+            if (ok < 0) napi_throw_range_error(env, NULL, "lr3d.node: prior must be a Float32Array of width*height");
+            else napi_throw_type_error(env, NULL, "lr3d.node: an integration view needs intrinsics, R[9], t[3]");
             return NULL;
         }
     }
@@ -611,9 +639,11 @@ static napi_value ncc_backward(napi_env env, napi_callback_info info) {
     j->n = 1;
     j->descs = (lr3d_solve_desc *)calloc(1, sizeof(lr3d_solve_desc));
     j->priors = (float **)calloc(1, sizeof(float *));
-    if (!read_solve(env, argv[1], &j->descs[0], &j->priors[0])) {
+    const int ok = read_solve(env, argv[1], &j->descs[0], &j->priors[0]);
+    if (ok <= 0) {
         job_free(j);
-        napi_throw_type_error(env, NULL, "lr3d.node: solve options need ref and sources");
+        if (ok < 0) napi_throw_range_error(env, NULL, "lr3d.node: prior must be a Float32Array of width*height");
+        else napi_throw_type_error(env, NULL, "lr3d.node: solve options need ref and sources");
         return NULL;
     }
     int32_t px = 0;
@@ -622,7 +652,8 @@ static napi_value ncc_backward(napi_env env, napi_callback_info info) {
     napi_typeof(env, argv[2], &t);
     if (t == napi_object) {
         size_t count = 0;
-        j->weights = (float *)copy_typed(env, argv[2], 4, &count);
+        int bad = 0;
+        j->weights = (float *)copy_typed(env, argv[2], napi_float32_array, &count, &bad);
         if (!j->weights || count != (size_t)px) {
             job_free(j);
             napi_throw_range_error(env, NULL, "lr3d.node: weights must be a Float32Array of width*height");
@@ -652,9 +683,11 @@ static napi_value ncc_refine_step(napi_env env, napi_callback_info info) {
     for (uint32_t i = 0; i < n; i++) {
         napi_value o;
         napi_get_element(env, argv[1], i, &o);
-        if (!read_solve(env, o, &j->descs[i], &j->priors[i])) {
+        const int ok = read_solve(env, o, &j->descs[i], &j->priors[i]);
+        if (ok <= 0) {
             job_free(j);
-            napi_throw_type_error(env, NULL, "lr3d.node: solve options need ref and sources");
+            if (ok < 0) napi_throw_range_error(env, NULL, "lr3d.node: prior must be a Float32Array of width*height");
+            else napi_throw_type_error(env, NULL, "lr3d.node: solve options need ref and sources");
             return NULL;
         }
     }
@@ -713,16 +746,24 @@ static napi_value tsdf_backward(napi_env env, napi_callback_info info) {
         napi_get_element(env, argv[2], i, &e);
         napi_get_value_int32(env, e, &j->layers[i]);
         napi_get_element(env, argv[4], i, &o);
-        if (!read_iview(env, o, &j->iviews[i], &j->priors[i])) {
+        const int ok = read_iview(env, o, &j->iviews[i], &j->priors[i]);
+        if (ok <= 0) {
             job_free(j);
-            napi_throw_type_error(env, NULL, "lr3d.node: an integration view needs intrinsics, R[9], t[3]");
+            if (ok < 0) napi_throw_range_error(env, NULL, "lr3d.node: prior must be a Float32Array of width*height");
+            else napi_throw_type_error(env, NULL, "lr3d.node: an integration view needs intrinsics, R[9], t[3]");
             return NULL;
         }
         px += (size_t)j->iviews[i].width * j->iviews[i].height;
     }
     size_t nv = 0, nt = 0;
-    j->voxels = (uint32_t *)copy_typed(env, argv[5], 4, &nv);
-    j->targets = (float *)copy_typed(env, argv[6], 4, &nt);
+    int bad_v = 0, bad_t = 0;
+    j->voxels = (uint32_t *)copy_typed(env, argv[5], napi_uint32_array, &nv, &bad_v);
+    j->targets = (float *)copy_typed(env, argv[6], napi_float32_array, &nt, &bad_t);
+    if (bad_v || bad_t) {
+        job_free(j);
+        napi_throw_type_error(env, NULL, "lr3d.node: voxels must be a Uint32Array and targets a Float32Array");
+        return NULL;
+    }
     if (nv != nt) {
         job_free(j);
         napi_throw_range_error(env, NULL, "lr3d.node: voxels and targets differ in length");
@@ -776,8 +817,10 @@ static napi_value stereo_fuse_depth_maps(napi_env env, napi_callback_info info) 
         j->thumbs[i].width = (int32_t)get_num(env, t, "width", 0);
         j->thumbs[i].height = (int32_t)get_num(env, t, "height", 0);
         size_t count = 0;
-        if (napi_get_named_property(env, t, "rgb", &rgb) == napi_ok) j->rgbs[i] = (uint8_t *)copy_typed(env, rgb, 1, &count);
-        if (!j->rgbs[i] || count < (size_t)j->thumbs[i].width * j->thumbs[i].height * 3u) {
+        int bad = 0;
+        if (napi_get_named_property(env, t, "rgb", &rgb) == napi_ok)
+            j->rgbs[i] = (uint8_t *)copy_typed(env, rgb, napi_uint8_array, &count, &bad);
+        if (!j->rgbs[i] || count != (size_t)j->thumbs[i].width * j->thumbs[i].height * 3u) {
             job_free(j);
             napi_throw_type_error(env, NULL, "lr3d.node: a thumbnail needs width, height and rgb (width*height*3 bytes)");
             return NULL;

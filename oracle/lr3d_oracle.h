@@ -46,7 +46,7 @@ typedef struct {
 } orc_view;
 
 /* One source view in a PatchMatch dispatch: the `View` struct (PM:4-10) after the host packed
- * it (DS:391-399): intrinsics, relative rotation ref->src (rows), translation, layer. */
+ * it (DS:345-353): intrinsics, relative rotation ref->src (rows), translation, layer. */
 typedef struct {
     float k[4];
     float R[9];
@@ -55,7 +55,7 @@ typedef struct {
     int32_t width, height; /* source image size (equal to the reference size in the reference) */
 } orc_src;
 
-/* The `Params` uniform (PM:12-26) as DenseStereo.params() fills it (DS:360-401). */
+/* The `Params` uniform (PM:12-26) as DenseStereo.params() fills it (DS:314-355). */
 typedef struct {
     int32_t width, height, ref;
     float k[4];
@@ -75,7 +75,7 @@ typedef struct {
     const int32_t *heights;
 } orc_images;
 
-/* ---- RNG (PM:56-65, PM:251-253, DS:378-380) ---- */
+/* ---- RNG (PM:56-65, PM:251-253, DS:332-334) ---- */
 uint32_t orc_pcg(uint32_t v);
 uint32_t orc_host_seed(int32_t ref, uint32_t iteration, uint32_t phase, uint32_t flags);
 uint32_t orc_pixel_rng(uint32_t seed, uint32_t iteration, uint32_t phase, uint32_t index);
@@ -90,7 +90,7 @@ uint32_t orc_unpack_cost(uint32_t packed);
 uint16_t orc_f32_to_f16(float v);
 float orc_f16_to_f32(uint16_t h);
 
-/* ---- host parameter packing (DS:360-401, CAM:73-77) ---- */
+/* ---- host parameter packing (DS:314-355, CAM:73-77) ---- */
 void orc_make_params(const orc_view *views, int32_t n_views, int32_t ref, const int32_t *sources,
                      int32_t n_src, double min_depth, double max_depth, uint32_t iteration,
                      uint32_t phase, double step, uint32_t flags, uint32_t needed, int32_t topk,
@@ -128,14 +128,14 @@ void orc_init_coarse(const orc_params *p, const orc_images *img, const float coa
                      int32_t coarse_w, int32_t coarse_h, const float *coarse_hyp, float *hyp,
                      float *cost);
 
-/* DenseStereo.solve (DS:231-285): init, `iterations` x {phase 0, 1}, pack into depths[ref].
- * geometric: flags bit 0.  Returns 0, or -1 when there are no sources (a no-op, DS:233). */
+/* DenseStereo.solve (DS:185-239): init, `iterations` x {phase 0, 1}, pack into depths[ref].
+ * geometric: flags bit 0.  Returns 0, or -1 when there are no sources (a no-op, DS:187). */
 int32_t orc_solve(const orc_view *views, int32_t n_views, const orc_images *img, int32_t ref,
                   const int32_t *sources, int32_t n_src, double min_depth, double max_depth,
                   const float *prior, int32_t geometric, int32_t iterations, int32_t topk,
                   int32_t pack, uint32_t *depths, float *hyp, float *cost);
 
-/* DenseStereo.filter (DS:296-309) */
+/* DenseStereo.filter (DS:250-263) */
 void orc_filter(const orc_view *views, int32_t n_views, int32_t ref, const int32_t *checks,
                 int32_t n_checks, int32_t needed, const uint32_t *depths, uint32_t *filtered);
 

@@ -257,7 +257,7 @@ def solve(views, images: Images, ref: int, sources: Sequence[int], min_depth: fl
           max_depth: float, prior: Optional[np.ndarray] = None, geometric: bool = False,
           iterations: int = 5, topk: int = 0, pack: bool = True,
           depths: Optional[np.ndarray] = None):
-    """DS:231-285.  depths: all layers' packed maps (layer-major, equal layer sizes)."""
+    """DS:185-239.  depths: all layers' packed maps (layer-major, equal layer sizes)."""
     arr = views_array(views)
     W, H = views[ref].width, views[ref].height
     if depths is None:

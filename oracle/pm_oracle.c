@@ -50,7 +50,7 @@ uint32_t orc_pcg(uint32_t v) {
     return (word >> 22u) ^ word;
 }
 
-/* DS:378-380: Math.imul(ref + 1, 0x9e3779b1) + iteration*97 + phase*13 + flags*7, then >>> 0. */
+/* DS:332-334: Math.imul(ref + 1, 0x9e3779b1) + iteration*97 + phase*13 + flags*7, then >>> 0. */
 uint32_t orc_host_seed(int32_t ref, uint32_t iteration, uint32_t phase, uint32_t flags) {
     return (uint32_t)(ref + 1) * 0x9e3779b1u + iteration * 97u + phase * 13u + flags * 7u;
 }
@@ -233,7 +233,7 @@ static void relative_pose(const orc_view *a, const orc_view *b, double R[9], dou
     }
 }
 
-/* DS:360-401 */
+/* DS:314-355 */
 void orc_make_params(const orc_view *views, int32_t n_views, int32_t ref, const int32_t *sources,
                      int32_t n_src, double min_depth, double max_depth, uint32_t iteration,
                      uint32_t phase, double step, uint32_t flags, uint32_t needed, int32_t topk,
@@ -251,13 +251,13 @@ void orc_make_params(const orc_view *views, int32_t n_views, int32_t ref, const 
     out->inv_min = (float)(1.0 / min_depth);
     out->inv_max = (float)(1.0 / max_depth);
     out->step = (float)step;
-    out->geo_weight = 0.2f; /* DS:92 GEOMETRIC_WEIGHT */
+    out->geo_weight = 0.2f; /* DS:46 GEOMETRIC_WEIGHT */
     out->iteration = iteration;
     out->phase = phase;
     out->seed = orc_host_seed(ref, iteration, phase, flags);
     out->flags = flags;
     out->needed = needed;
-    if (n_src > ORC_MAX_SOURCES) n_src = ORC_MAX_SOURCES; /* DS:232 */
+    if (n_src > ORC_MAX_SOURCES) n_src = ORC_MAX_SOURCES; /* DS:186 */
     out->n_src = n_src;
     out->topk = topk;
     for (int s = 0; s < n_src; s++) {
@@ -340,7 +340,7 @@ void orc_homography(const orc_params *p, int32_t s, const float n[3], float rho,
 }
 
 /* Bilinear sample with clamp-to-edge at texel coordinates (u, v) in [0, w-1] x [0, h-1]
- * (the reference samples (uv + 0.5) / size with a linear, clamp-to-edge sampler, DS:187). */
+ * (the reference samples (uv + 0.5) / size with a linear, clamp-to-edge sampler, DS:141). */
 static inline float bilinear(const uint8_t *im, int32_t w, int32_t h, float u, float v) {
     float x0 = floorf(u), y0 = floorf(v);
     float fx = u - x0, fy = v - y0;
@@ -753,7 +753,7 @@ void orc_consistency(const orc_params *p, const uint32_t *depths, uint32_t *filt
 /* Host schedule                                                                              */
 /* ------------------------------------------------------------------------------------------ */
 
-/* DS:231-285 */
+/* DS:185-239 */
 int32_t orc_solve(const orc_view *views, int32_t n_views, const orc_images *img, int32_t ref,
                   const int32_t *sources, int32_t n_src, double min_depth, double max_depth,
                   const float *prior, int32_t geometric, int32_t iterations, int32_t topk,
@@ -782,7 +782,7 @@ int32_t orc_solve(const orc_view *views, int32_t n_views, const orc_images *img,
     return 0;
 }
 
-/* DS:296-309 */
+/* DS:250-263 */
 void orc_filter(const orc_view *views, int32_t n_views, int32_t ref, const int32_t *checks,
                 int32_t n_checks, int32_t needed, const uint32_t *depths, uint32_t *filtered) {
     orc_params p;

@@ -24,7 +24,8 @@ async function main() {
     const grid = new lr3d.BrickGrid(job.origin, job.voxel, job.extent);
     grid.mark(job.centre[0], job.centre[1], job.centre[2], job.radius);
     const layout = grid.build();
-    const volume = new lr3d.HipTsdfVolume(ctx, grid, layout, run.stereo, job.width, job.height, job.truncation);
+    // constructed as the reference does (tsdf-meshing.ts:71): the volume binds `stereo.filtered`
+    const volume = new lr3d.HipTsdfVolume(ctx, grid, layout, run.stereo.filtered, job.width, job.height, job.truncation);
     for (let layer = 0; layer < job.views.length; layer++) {
         fs.writeFileSync(path.join(dir, `depth${layer}.u32`), Buffer.from((await run.stereo.depthMap(layer)).buffer));
         await run.stereo.filter(layer, job.sources[layer], job.needed);

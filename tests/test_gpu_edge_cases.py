@@ -39,7 +39,7 @@ def test_empty_and_degenerate_solves(ctx):
     for i, im in enumerate(w.images):
         st.set_image(i, im)
     st.solve_batched([])
-    st.solve(0, lr3d.SolveOptions(sources=[], min_depth=0.5, max_depth=3.0, iterations=2))  # DS:233 no-op
+    st.solve(0, lr3d.SolveOptions(sources=[], min_depth=0.5, max_depth=3.0, iterations=2))  # DS:187 no-op
     assert not st.depth_map(0).any()
     o = lr3d.SolveOptions(sources=w.sources[0], min_depth=0.5, max_depth=3.0, iterations=1)
     with pytest.raises(lr3d.Lr3dError):

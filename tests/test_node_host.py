@@ -181,3 +181,47 @@ def test_node_pose_math_matches_python():
     p = perturb(views[1].pose, w, tau)
     assert np.allclose(np.reshape(res["pose"]["R"], (3, 3)), p.R, rtol=0, atol=1e-14)
     assert np.allclose(res["pose"]["t"], p.t, rtol=0, atol=1e-14)
+
+
+def test_addon_rejects_wrong_typed_arrays():
+    """Typed arrays are checked for their element type and length before anything is copied: a
+    short prior, a prior of another element type, voxels of the wrong type and an image of the
+    wrong type throw before any device work (no handle is needed to reach the checks)."""
+    code = r"""
+const m = require('./luvas.io-3d_amd/node/lr3d.js');
+const n = m.native;
+const out = {};
+function tryCall(name, f) {
+    try { f(); out[name] = 'no error'; } catch (e) { out[name] = e.constructor.name + ': ' + e.message; }
+}
+const desc = (prior) => ({ ref: 0, pixels: 12, sources: [1], minDepth: 0.5, maxDepth: 3, prior, iterations: 1 });
+tryCall('short', () => n.stereoSolve(null, desc(new Float32Array(11))));
+tryCall('u8', () => n.stereoSolve(null, desc(new Uint8Array(12))));
+tryCall('f64', () => n.stereoSolve(null, desc(new Float64Array(12))));
+tryCall('batched', () => n.stereoSolve(null, [desc(null), desc(new Float32Array(13))]));
+tryCall('nopixels', () => n.stereoSolve(null, { ref: 0, sources: [1], minDepth: 0.5, maxDepth: 3,
+    prior: new Float32Array(12), iterations: 1 }));
+const iview = (prior) => ({ width: 4, height: 3, fx: 1, fy: 1, cx: 1, cy: 1, R: [1, 0, 0, 0, 1, 0, 0, 0, 1],
+    t: [0, 0, 0], maxDepth: 3, prior, confidence: 0 });
+tryCall('iview_short', () => n.tsdfIntegrate(null, null, [0], 0, [iview(new Float32Array(5))]));
+tryCall('iview_u8', () => n.tsdfIntegrate(null, null, [0], 0, [iview(new Uint8Array(12))]));
+tryCall('voxels_f32', () => n.tsdfBackward(null, null, [0], 0, [iview(null)], new Float32Array(3), new Float32Array(3)));
+tryCall('image_f32', () => n.stereoSetImage(null, 0, new Float32Array(12), 4, 3));
+tryCall('image_short', () => n.stereoSetImage(null, 0, new Uint8Array(11), 4, 3));
+tryCall('coords_u32', () => n.tsdfCreate(null, [0, 0, 0], 0.01, [1, 1, 1], new Uint32Array(4), 4));
+tryCall('volume_depth', () => new m.HipTsdfVolume(null, { voxel: 0.01 }, { coords: new Int32Array(4) }, {}, 4, 3));
+// a well-formed prior reaches the C ABI, which rejects the missing handle asynchronously
+const p = n.stereoSolve(null, desc(new Float32Array(12)));
+p.then(() => { out.valid = 'resolved'; }, (e) => { out.valid = 'rejected: ' + e.message; })
+ .then(() => console.log(JSON.stringify(out)));
+"""
+    r = node(code)
+    assert r.returncode == 0, r.stderr
+    out = json.loads(r.stdout.strip().splitlines()[-1])
+    for k in ("short", "u8", "f64", "batched", "nopixels", "iview_short", "iview_u8"):
+        assert out[k].startswith("RangeError") and "Float32Array of width*height" in out[k], (k, out[k])
+    assert out["voxels_f32"].startswith("TypeError") and "Uint32Array" in out["voxels_f32"]
+    assert out["image_f32"].startswith("TypeError") and out["image_short"].startswith("TypeError")
+    assert out["coords_u32"].startswith("TypeError") and "Int32Array" in out["coords_u32"]
+    assert out["volume_depth"].startswith("TypeError") and "stereo.filtered" in out["volume_depth"]
+    assert out["valid"].startswith("rejected") and "NULL" in out["valid"]

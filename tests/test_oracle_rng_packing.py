@@ -1,4 +1,4 @@
-"""Pins of the oracle's integer and format code: the RNG (PM:56-65, PM:251-253, DS:378-380) and
+"""Pins of the oracle's integer and format code: the RNG (PM:56-65, PM:251-253, DS:332-334) and
 the packed depth format (PK:3-33, depth-maps.ts:44-85)."""
 import json
 import os
@@ -46,7 +46,7 @@ def _js_imul(a, b):
 @pytest.mark.parametrize("ref", [0, 1, 7, 31, 199, 70000])
 @pytest.mark.parametrize("it,phase,flags", [(0, 0, 0), (3, 1, 0), (7, 1, 1), (12, 0, 1)])
 def test_host_seed_matches_javascript_semantics(ref, it, phase, flags):
-    # DS:379-380: Math.imul(ref + 1, 0x9e3779b1) + it*97 + phase*13 + flags*7, then >>> 0
+    # DS:333-334: Math.imul(ref + 1, 0x9e3779b1) + it*97 + phase*13 + flags*7, then >>> 0
     js = (_js_imul(ref + 1, 0x9E3779B1) + it * 97 + phase * 13 + flags * 7) % (1 << 32)
     assert O.host_seed(ref, it, phase, flags) == js
 

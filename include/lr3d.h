@@ -111,8 +111,11 @@ int32_t lr3d_stereo_set_image(lr3d_stereo *st, int32_t layer, const uint8_t *gra
 /* DenseStereo.solve (DS:185-239): init, `iterations` red-black rounds, pack.  One launch per
  * kernel per phase, as the reference dispatches it (the "plain" path). */
 int32_t lr3d_stereo_solve(lr3d_stereo *st, const lr3d_solve_desc *desc);
-/* All solves of `descs` at once: every view's pixels share each launch, the iteration loop is
- * replayed from a captured HIP graph.  Results equal n calls of lr3d_stereo_solve. */
+/* All solves of `descs` at once, the launch sequence replayed from a captured HIP graph.  Results
+ * equal n calls of lr3d_stereo_solve in array order.  Photometric solves share every launch; a
+ * geometric pass that reads (as a source) the packed map of an earlier solve of the batch starts a
+ * new group of launches after that solve's pack, so it sees the refined map, as the reference's
+ * layer-by-layer geometric pass does (…/pipeline/reconstructor.ts:96-108). */
 int32_t lr3d_stereo_solve_batched(lr3d_stereo *st, const lr3d_solve_desc *descs, int32_t n);
 /* DenseStereo.filter (DS:250-263): consistency check of `ref` into its filtered map. */
 int32_t lr3d_stereo_filter(lr3d_stereo *st, int32_t ref, const int32_t *checks, int32_t n_checks,

@@ -503,63 +503,127 @@ static int32_t build_desc(const lr3d_stereo *st, const lr3d_solve_desc &sd, uint
     return LR3D_OK;
 }
 
-// Device copy of descriptors and the three block tables (init, propagate, pack).
+// Device copy of descriptors and the three block tables (init, propagate, pack) of one launch
+// group.
 struct Plans {
     PmPlan init, prop, pack;
+    int32_t max_iterations = 0;
+    bool any_pack = false;
 };
 
+// Launch plans of consecutive groups of `descs` (group g = descs[groups[g], groups[g + 1])), all
+// in the set's plan buffer: the descriptors, then each group's three block tables.
 // tiles: launch shapes of init, propagate, pack (the stereo's own when NULL).
-static int32_t upload_plans(lr3d_stereo *st, const std::vector<PmDesc> &descs,
-                            const std::vector<uint8_t> &pack, Plans &plans, std::vector<int64_t> &key,
-                            const TileShape *tiles = nullptr) {
+static int32_t upload_groups(lr3d_stereo *st, const std::vector<PmDesc> &descs, const std::vector<uint8_t> &pack,
+                             const std::vector<int32_t> &groups, std::vector<Plans> &plans,
+                             std::vector<int64_t> &key, const TileShape *tiles = nullptr) {
     lr3d_ctx *ctx = st->ctx;
     const int32_t n = (int32_t)descs.size();
+    const int32_t n_groups = (int32_t)groups.size() - 1;
     if (!tiles) tiles = st->tiles;
     const TileShape ti = tiles[0], tp = tiles[1], tk = tiles[2];
-    std::vector<uint32_t> bi(n + 1), bp(n + 1), bk(n + 1);
-    int32_t kmax = 1, geometric = 0;
-    key.clear();
-    key.push_back(n);
-    bi[0] = bp[0] = bk[0] = 0;
-    for (int32_t v = 0; v < n; v++) {
-        const PmDesc &d = descs[v];
-        const uint64_t rows_i = (d.H + ti.ty - 1) / ti.ty, cols_i = (d.W + ti.tx - 1) / ti.tx;
-        const uint64_t rows_p = (d.H + tp.ty - 1) / tp.ty, cols_p = (d.W + 2 * tp.tx - 1) / (2 * tp.tx);
-        const uint64_t rows_k = (d.H + tk.ty - 1) / tk.ty, cols_k = (d.W + tk.tx - 1) / tk.tx;
-        bi[v + 1] = bi[v] + (uint32_t)(rows_i * cols_i);
-        bp[v + 1] = bp[v] + (d.iterations > 0 ? (uint32_t)(rows_p * cols_p) : 0u);
-        bk[v + 1] = bk[v] + (pack[v] ? (uint32_t)(rows_k * cols_k) : 0u);
-        kmax = std::max(kmax, d.topk);
-        geometric |= (int32_t)(d.flags & 1u);
-        key.push_back(d.W);
-        key.push_back(d.H);
-        key.push_back(d.iterations);
-        key.push_back(pack[v]);
-    }
     const size_t desc_bytes = sizeof(PmDesc) * n;
-    const size_t table_bytes = sizeof(uint32_t) * (n + 1);
-    const size_t total = desc_bytes + 3 * table_bytes;
-    HIP_TRY(st->plan_buf.reserve(total));
+    // per group: 3 tables of (group size + 1) entries
+    const size_t table_words = 3 * ((size_t)n + (size_t)n_groups);
+    const size_t total = desc_bytes + 4 * table_words;
     std::vector<uint8_t> host(total);
     memcpy(host.data(), descs.data(), desc_bytes);
-    memcpy(host.data() + desc_bytes, bi.data(), table_bytes);
-    memcpy(host.data() + desc_bytes + table_bytes, bp.data(), table_bytes);
-    memcpy(host.data() + desc_bytes + 2 * table_bytes, bk.data(), table_bytes);
+    uint32_t *tab = (uint32_t *)(host.data() + desc_bytes);
+    key.clear();
+    key.push_back(n);
+    key.push_back(n_groups);
+    plans.assign(n_groups, Plans());
+    std::vector<size_t> table_at(n_groups);
+    size_t w = 0;
+    for (int32_t g = 0; g < n_groups; g++) {
+        const int32_t b = groups[g], m = groups[g + 1] - b;
+        key.push_back(b);
+        uint32_t *bi = tab + w, *bp = bi + (m + 1), *bk = bp + (m + 1);
+        table_at[g] = w;
+        w += 3 * (size_t)(m + 1);
+        int32_t kmax = 1, geometric = 0;
+        bi[0] = bp[0] = bk[0] = 0;
+        Plans &pl = plans[g];
+        for (int32_t v = 0; v < m; v++) {
+            const PmDesc &d = descs[b + v];
+            const uint64_t rows_i = (d.H + ti.ty - 1) / ti.ty, cols_i = (d.W + ti.tx - 1) / ti.tx;
+            const uint64_t rows_p = (d.H + tp.ty - 1) / tp.ty, cols_p = (d.W + 2 * tp.tx - 1) / (2 * tp.tx);
+            const uint64_t rows_k = (d.H + tk.ty - 1) / tk.ty, cols_k = (d.W + tk.tx - 1) / tk.tx;
+            bi[v + 1] = bi[v] + (uint32_t)(rows_i * cols_i);
+            bp[v + 1] = bp[v] + (d.iterations > 0 ? (uint32_t)(rows_p * cols_p) : 0u);
+            bk[v + 1] = bk[v] + (pack[b + v] ? (uint32_t)(rows_k * cols_k) : 0u);
+            kmax = std::max(kmax, d.topk);
+            geometric |= (int32_t)(d.flags & 1u);
+            pl.max_iterations = std::max(pl.max_iterations, d.iterations);
+            pl.any_pack |= pack[b + v] != 0;
+            key.push_back(d.W);
+            key.push_back(d.H);
+            key.push_back(d.iterations);
+            key.push_back(pack[b + v]);
+        }
+        pl.init = {nullptr, nullptr, m, bi[m], ti.tx, ti.ty, kmax, geometric};
+        pl.prop = {nullptr, nullptr, m, bp[m], tp.tx, tp.ty, kmax, geometric};
+        pl.pack = {nullptr, nullptr, m, bk[m], tk.tx, tk.ty, kmax, geometric};
+        key.push_back(kmax);
+        key.push_back(geometric);
+    }
+    HIP_TRY(st->plan_buf.reserve(total));
     HIP_TRY(hipMemcpyAsync(st->plan_buf.ptr, host.data(), total, hipMemcpyHostToDevice, ctx->stream));
     HIP_TRY(hipStreamSynchronize(ctx->stream));  // host staging goes out of scope
     const PmDesc *dd = (const PmDesc *)st->plan_buf.ptr;
     const uint32_t *tables = (const uint32_t *)(st->plan_buf.ptr + desc_bytes);
-    plans.init = {dd, tables, n, bi[n], ti.tx, ti.ty, kmax, geometric};
-    plans.prop = {dd, tables + (n + 1), n, bp[n], tp.tx, tp.ty, kmax, geometric};
-    plans.pack = {dd, tables + 2 * (n + 1), n, bk[n], tk.tx, tk.ty, kmax, geometric};
-    key.push_back(kmax);
-    key.push_back(geometric);
+    for (int32_t g = 0; g < n_groups; g++) {
+        const int32_t m = groups[g + 1] - groups[g];
+        const uint32_t *t0 = tables + table_at[g];
+        plans[g].init.descs = plans[g].prop.descs = plans[g].pack.descs = dd + groups[g];
+        plans[g].init.block_start = t0;
+        plans[g].prop.block_start = t0 + (m + 1);
+        plans[g].pack.block_start = t0 + 2 * (m + 1);
+    }
     key.push_back((int64_t)(uintptr_t)st->plan_buf.ptr);
     for (int i = 0; i < 3; i++) {
         key.push_back(tiles[i].tx);
         key.push_back(tiles[i].ty);
     }
     return LR3D_OK;
+}
+
+// One launch group over all of `descs`.
+static int32_t upload_plans(lr3d_stereo *st, const std::vector<PmDesc> &descs,
+                            const std::vector<uint8_t> &pack, Plans &plans, std::vector<int64_t> &key,
+                            const TileShape *tiles = nullptr) {
+    std::vector<Plans> groups;
+    int32_t rc = upload_groups(st, descs, pack, {0, (int32_t)descs.size()}, groups, key, tiles);
+    if (rc) return rc;
+    plans = groups[0];
+    return LR3D_OK;
+}
+
+// The geometric pass runs layer after layer, each layer reading the packed maps its predecessors
+// have just written (…/pipeline/reconstructor.ts:96-108, Gauss-Seidel over the views).  A batch is
+// cut into groups that run one after another, each ending with its pack: a solve starts a new
+// group when it is a geometric pass whose sources include the layer of an earlier solve of the
+// current group that packs.  Every other read sees the same map as in a sequence of single
+// solves: photometric passes read no packed map; a geometric pass reads its own layer's map (no
+// other solve writes it) and its sources' maps, which a later solve of its group only overwrites
+// at the group's end.  Photometric batches stay one group.
+static std::vector<int32_t> gauss_seidel_groups(const std::vector<PmDesc> &dv, const std::vector<uint8_t> &packs) {
+    std::vector<int32_t> groups(1, 0);
+    std::vector<int32_t> written;  // layers packed by the current group
+    for (int32_t i = 0; i < (int32_t)dv.size(); i++) {
+        const PmDesc &d = dv[i];
+        bool reads_group = false;
+        if (d.flags & 1u)
+            for (int32_t s = 0; s < d.n_src && !reads_group; s++)
+                reads_group = std::find(written.begin(), written.end(), d.src[s].layer) != written.end();
+        if (reads_group) {
+            groups.push_back(i);
+            written.clear();
+        }
+        if (packs[i]) written.push_back(d.ref);
+    }
+    groups.push_back((int32_t)dv.size());
+    return groups;
 }
 
 static PmPools pools_of(lr3d_stereo *st) {
@@ -574,14 +638,13 @@ static PmPools pools_of(lr3d_stereo *st) {
     return p;
 }
 
-// The schedule of DS:185-239 for a batch: init, iterations x {phase 0, 1}, pack.
-static hipError_t enqueue_schedule(lr3d_stereo *st, const Plans &plans, int32_t max_iterations,
-                                   bool any_pack) {
+// The schedule of DS:185-239 for one launch group: init, iterations x {phase 0, 1}, pack.
+static hipError_t enqueue_schedule(lr3d_stereo *st, const Plans &plans) {
     lr3d_ctx *ctx = st->ctx;
     const PmPools pools = pools_of(st);
     hipError_t e = timed(ctx, "pm_init", [&] { return pm_launch_init(ctx->stream, plans.init, pools); });
     if (e != hipSuccess) return e;
-    for (int32_t it = 0; it < max_iterations; it++) {
+    for (int32_t it = 0; it < plans.max_iterations; it++) {
         const float step = (float)pow(0.5, (double)it);  // DS:203
         for (uint32_t phase = 0; phase < 2; phase++) {
             e = timed(ctx, "pm_propagate", [&] {
@@ -590,8 +653,16 @@ static hipError_t enqueue_schedule(lr3d_stereo *st, const Plans &plans, int32_t 
             if (e != hipSuccess) return e;
         }
     }
-    if (any_pack) {
+    if (plans.any_pack) {
         e = timed(ctx, "pm_pack", [&] { return pm_launch_pack(ctx->stream, plans.pack, pools); });
+        if (e != hipSuccess) return e;
+    }
+    return hipSuccess;
+}
+
+static hipError_t enqueue_groups(lr3d_stereo *st, const std::vector<Plans> &groups) {
+    for (const Plans &g : groups) {
+        hipError_t e = enqueue_schedule(st, g);
         if (e != hipSuccess) return e;
     }
     return hipSuccess;
@@ -635,8 +706,6 @@ static int32_t solve_impl(lr3d_stereo *st, const lr3d_solve_desc *descs, int32_t
     if (rc) return rc;
     std::vector<PmDesc> dv;
     dv.reserve(n);
-    int32_t max_it = 0;
-    bool any_pack = false;
     std::vector<int32_t> refs;
     std::vector<uint8_t> packs;
     for (int32_t i = 0; i < n; i++) {
@@ -648,14 +717,12 @@ static int32_t solve_impl(lr3d_stereo *st, const lr3d_solve_desc *descs, int32_t
         if (rc) return rc;
         dv.push_back(d);
         packs.push_back(descs[i].pack ? 1 : 0);
-        max_it = std::max(max_it, d.iterations);
-        any_pack |= descs[i].pack != 0;
         refs.push_back(descs[i].ref);
     }
     if (dv.empty()) return finish(ctx);
-    Plans plans;
+    std::vector<Plans> plans;
     std::vector<int64_t> key;
-    rc = upload_plans(st, dv, packs, plans, key);
+    rc = upload_groups(st, dv, packs, gauss_seidel_groups(dv, packs), plans, key);
     if (rc) return rc;
     hipError_t e = hipSuccess;
     // HIP cannot time event records captured into a graph (hipEventElapsedTime reports an
@@ -669,7 +736,7 @@ static int32_t solve_impl(lr3d_stereo *st, const lr3d_solve_desc *descs, int32_t
             GraphEntry g;
             g.key = key;
             HIP_TRY(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
-            e = enqueue_schedule(st, plans, max_it, any_pack);
+            e = enqueue_groups(st, plans);
             hipGraph_t graph = nullptr;
             hipError_t e2 = hipStreamEndCapture(ctx->stream, &graph);
             g.graph = graph;
@@ -689,7 +756,7 @@ static int32_t solve_impl(lr3d_stereo *st, const lr3d_solve_desc *descs, int32_t
         HIP_TRY(hipGraphLaunch(entry->exec, ctx->stream));
         ctx->graph_launches++;
     } else {
-        e = enqueue_schedule(st, plans, max_it, any_pack);
+        e = enqueue_groups(st, plans);
         if (e != hipSuccess)
             return fail(LR3D_ERR_DEVICE, "lr3d_stereo_solve: launch failed: %s", hipGetErrorString(e));
     }

@@ -104,7 +104,8 @@ class HipDenseStereo:
         N.check(self._lib.lr3d_stereo_solve(self.handle, C.byref(d)))
 
     def solve_batched(self, solves: Sequence[tuple[int, SolveOptions]]) -> None:
-        """All (ref, options) pairs in shared launches; equals the solves one by one."""
+        """All (ref, options) pairs in shared launches; equals the solves one by one, in order
+        (a geometric pass reading an earlier solve's map runs after that solve's pack)."""
         descs, keep = [], []
         for ref, o in solves:
             d, k = self._desc(ref, o)

@@ -88,7 +88,10 @@ class HipDenseStereo {
         await native.stereoSolve(this.handle, this.descOf(ref, options));
     }
 
-    /** Every (ref, options) pair in shared launches; the same result as solving them in order. */
+    /**
+     * Every (ref, options) pair in shared launches; the same result as solving them in order (a
+     * geometric pass that reads an earlier solve's map runs after that solve's pack).
+     */
     async solveBatched(solves) {
         const descs = solves.filter((s) => s[1].sources.length > 0)
             .map((s) => this.descOf(s[0], s[1]));

@@ -53,6 +53,7 @@ typedef struct {
     float t[3];
     int32_t layer;
     int32_t width, height; /* source image size (equal to the reference size in the reference) */
+    int64_t map_off;       /* pixel offset of the layer's packed map in the `depths` pool */
 } orc_src;
 
 /* The `Params` uniform (PM:12-26) as DenseStereo.params() fills it (DS:314-355). */
@@ -64,6 +65,7 @@ typedef struct {
     uint32_t needed;                          /* band.z (consistency filter) */
     int32_t n_src;
     int32_t topk; /* 0 = reference rule max(1, min(3, n-1)) (PM:218) */
+    int64_t ref_off; /* pixel offset of the reference layer's packed map in `depths` */
     orc_src src[ORC_MAX_SOURCES];
 } orc_params;
 
@@ -106,8 +108,20 @@ void orc_homography(const orc_params *p, int32_t s, const float n[3], float rho,
 float orc_view_cost(const orc_params *p, const orc_images *img, int32_t s, const float H[9],
                     int32_t x, int32_t y, const float *values, const float *weights,
                     const float *stats);
-/* Aggregated cost of a plane hypothesis at pixel (x, y) (PM:192-233).  `depths` (all layers'
- * packed maps, layer-major) is read only when flags & 1. */
+/* Packed depth maps of all layers live in one pool, `depths`: layer l's map starts at pixel offset
+ * orc_map_offset(views, l) = sum of width*height over the layers before it (the reference keeps
+ * one W*H*4-byte slice per layer, DS:97-98, which is this layout when all sizes are equal). */
+int64_t orc_map_offset(const orc_view *views, int32_t layer);
+
+/* Forward-backward reprojection of reference point x (camera coordinates, seen at pixel (px, py))
+ * through source s's packed map (PM:166-190): out = (pixel distance, depth in the reference,
+ * 0, 1), or all -1 when the point is behind the source, projects outside its image, or meets no
+ * usable depth there (depth <= 0 or cost code > 7). */
+void orc_reproject(const orc_params *p, const uint32_t *depths, int32_t s, float px, float py,
+                   const float x[3], float out[4]);
+
+/* Aggregated cost of a plane hypothesis at pixel (x, y) (PM:192-233).  `depths` (the map pool)
+ * is read only when flags & 1. */
 float orc_hypothesis_cost(const orc_params *p, const orc_images *img, const uint32_t *depths,
                           int32_t x, int32_t y, const float *values, const float *weights,
                           const float *stats, const float n[3], float d);

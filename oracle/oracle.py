@@ -26,7 +26,8 @@ class OrcView(C.Structure):
 
 class OrcSrc(C.Structure):
     _fields_ = [("k", C.c_float * 4), ("R", C.c_float * 9), ("t", C.c_float * 3),
-                ("layer", C.c_int32), ("width", C.c_int32), ("height", C.c_int32)]
+                ("layer", C.c_int32), ("width", C.c_int32), ("height", C.c_int32),
+                ("map_off", C.c_int64)]
 
 
 class OrcParams(C.Structure):
@@ -36,7 +37,7 @@ class OrcParams(C.Structure):
                 ("geo_weight", C.c_float),
                 ("iteration", C.c_uint32), ("phase", C.c_uint32), ("seed", C.c_uint32),
                 ("flags", C.c_uint32), ("needed", C.c_uint32), ("n_src", C.c_int32),
-                ("topk", C.c_int32), ("src", OrcSrc * MAX_SOURCES)]
+                ("topk", C.c_int32), ("ref_off", C.c_int64), ("src", OrcSrc * MAX_SOURCES)]
 
 
 class OrcImages(C.Structure):
@@ -89,6 +90,7 @@ def load():
         "orc_propagate": (None, [P(OrcParams), P(OrcImages), V, V, V]),
         "orc_pack": (None, [P(OrcParams), V, V, V]),
         "orc_consistency": (None, [P(OrcParams), V, V]),
+        "orc_reproject": (None, [P(OrcParams), V, i32, f32, f32, P(f32), V]),
         "orc_init_coarse": (None, [P(OrcParams), P(OrcImages), P(f32), i32, i32, V, V, V]),
         "orc_solve": (i32, [P(OrcView), i32, P(OrcImages), i32, P(i32), i32, dbl, dbl, V, i32, i32,
                             i32, i32, V, V, V]),
@@ -253,15 +255,33 @@ def pack(p, hyp, cost) -> np.ndarray:
     return out
 
 
+def map_offsets(views) -> np.ndarray:
+    """Pixel offset of every layer's packed map in the map pool, plus the pool size at the end:
+    layers one after another, each width*height (orc_map_offset)."""
+    return np.concatenate([[0], np.cumsum([v.width * v.height for v in views])]).astype(np.int64)
+
+
+def map_pool(views) -> np.ndarray:
+    """An empty map pool for `views` (all maps 0)."""
+    return np.zeros(int(map_offsets(views)[-1]), dtype=np.uint32)
+
+
+def layer_map(views, depths: np.ndarray, layer: int) -> np.ndarray:
+    """Layer `layer`'s packed map in the pool `depths` (a view, not a copy)."""
+    off = map_offsets(views)
+    return depths[off[layer]:off[layer + 1]]
+
+
 def solve(views, images: Images, ref: int, sources: Sequence[int], min_depth: float,
           max_depth: float, prior: Optional[np.ndarray] = None, geometric: bool = False,
           iterations: int = 5, topk: int = 0, pack: bool = True,
           depths: Optional[np.ndarray] = None):
-    """DS:185-239.  depths: all layers' packed maps (layer-major, equal layer sizes)."""
+    """DS:185-239.  depths: the map pool of all layers (map_pool; layers may differ in size)."""
     arr = views_array(views)
     W, H = views[ref].width, views[ref].height
     if depths is None:
-        depths = np.zeros(len(views) * W * H, dtype=np.uint32)
+        depths = map_pool(views)
+    assert depths.size == map_offsets(views)[-1], "depths must be the map pool of all views"
     hyp = np.zeros((W * H, 4), dtype=np.float32)
     cost = np.zeros(W * H, dtype=np.float32)
     src = (C.c_int32 * max(1, len(sources)))(*sources)
@@ -274,11 +294,21 @@ def solve(views, images: Images, ref: int, sources: Sequence[int], min_depth: fl
 
 
 def filter_map(views, ref, checks, needed, depths) -> np.ndarray:
+    """DS:250-263 over the map pool `depths`."""
+    assert depths.size == map_offsets(views)[-1], "depths must be the map pool of all views"
     arr = views_array(views)
     W, H = views[ref].width, views[ref].height
     out = np.zeros(W * H, dtype=np.uint32)
     src = (C.c_int32 * max(1, len(checks)))(*checks)
     load().orc_filter(arr, len(views), ref, src, len(checks), needed, _p(depths), _p(out))
+    return out
+
+
+def reproject(p, depths, s: int, px: float, py: float, x) -> np.ndarray:
+    """PM:166-190: (distance, depth, 0, 1) or all -1."""
+    out = np.empty(4, dtype=np.float32)
+    xx = (C.c_float * 3)(*[float(v) for v in x])
+    load().orc_reproject(C.byref(p), _p(depths), s, px, py, xx, _p(out))
     return out
 
 

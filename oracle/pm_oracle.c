@@ -260,6 +260,7 @@ void orc_make_params(const orc_view *views, int32_t n_views, int32_t ref, const 
     if (n_src > ORC_MAX_SOURCES) n_src = ORC_MAX_SOURCES; /* DS:186 */
     out->n_src = n_src;
     out->topk = topk;
+    out->ref_off = orc_map_offset(views, ref);
     for (int s = 0; s < n_src; s++) {
         const orc_view *sv = &views[sources[s]];
         double R[9], t[3];
@@ -274,7 +275,14 @@ void orc_make_params(const orc_view *views, int32_t n_views, int32_t ref, const 
         o->layer = sources[s];
         o->width = sv->width;
         o->height = sv->height;
+        o->map_off = orc_map_offset(views, sources[s]);
     }
+}
+
+int64_t orc_map_offset(const orc_view *views, int32_t layer) {
+    int64_t off = 0;
+    for (int32_t l = 0; l < layer; l++) off += (int64_t)views[l].width * (int64_t)views[l].height;
+    return off;
 }
 
 /* ------------------------------------------------------------------------------------------ */
@@ -399,8 +407,7 @@ static void reproject(const orc_params *p, const uint32_t *depths, int32_t s, fl
     int32_t pxs = round_even((v->k[0] * xs[0]) / xs[2] + v->k[2]);
     int32_t pys = round_even((v->k[1] * xs[1]) / xs[2] + v->k[3]);
     if (pxs < 0 || pys < 0 || pxs >= v->width || pys >= v->height) return;
-    size_t plane = (size_t)v->width * (size_t)v->height;
-    uint32_t packed = depths[(size_t)v->layer * plane + (size_t)pys * v->width + pxs];
+    uint32_t packed = depths[(size_t)v->map_off + (size_t)pys * v->width + pxs];
     float ds = orc_unpack_depth(packed);
     if (!(ds > 0.0f) || orc_unpack_cost(packed) > FILTER_COST) return;
     float back_s[3] = {(((float)pxs - v->k[2]) / v->k[0]) * ds, (((float)pys - v->k[3]) / v->k[1]) * ds,
@@ -416,6 +423,11 @@ static void reproject(const orc_params *p, const uint32_t *depths, int32_t s, fl
     out[1] = back[2];
     out[2] = 0.0f;
     out[3] = 1.0f;
+}
+
+void orc_reproject(const orc_params *p, const uint32_t *depths, int32_t s, float px, float py,
+                   const float x[3], float out[4]) {
+    reproject(p, depths, s, px, py, x, out);
 }
 
 static inline int32_t topk_of(const orc_params *p) {
@@ -511,7 +523,7 @@ void orc_init(const orc_params *p, const orc_images *img, const float *prior,
             random_normal(&rng, ray, n);
             float d = random_depth(&rng, p);
             if ((p->flags & 1u) != 0u) {
-                uint32_t packed = depths[(size_t)p->ref * (size_t)W * Hh + index];
+                uint32_t packed = depths[(size_t)p->ref_off + index];
                 if (orc_unpack_depth(packed) > 0.0f) {
                     d = orc_unpack_depth(packed);
                     orc_unpack_normal(packed, n);
@@ -712,12 +724,11 @@ void orc_pack(const orc_params *p, const float *hyp, const float *cost, uint32_t
 /* PM:436-469 */
 void orc_consistency(const orc_params *p, const uint32_t *depths, uint32_t *filtered) {
     int32_t W = p->width, Hh = p->height;
-    size_t plane = (size_t)W * Hh;
 #pragma omp parallel for schedule(static) num_threads(orc_get_threads())
     for (int32_t y = 0; y < Hh; y++) {
         for (int32_t x = 0; x < W; x++) {
             size_t index = (size_t)y * W + x;
-            uint32_t packed = depths[(size_t)p->ref * plane + index];
+            uint32_t packed = depths[(size_t)p->ref_off + index];
             float d = orc_unpack_depth(packed);
             if (!(d > 0.0f) || orc_unpack_cost(packed) > FILTER_COST) {
                 filtered[index] = 0u;
@@ -776,8 +787,7 @@ int32_t orc_solve(const orc_view *views, int32_t n_views, const orc_images *img,
     if (pack) {
         orc_make_params(views, n_views, ref, sources, n_src, min_depth, max_depth, 0, 0, 0.0, flags,
                         0, topk, &p);
-        size_t plane = (size_t)p.width * p.height;
-        orc_pack(&p, hyp, cost, depths + (size_t)ref * plane);
+        orc_pack(&p, hyp, cost, depths + p.ref_off);
     }
     return 0;
 }

@@ -5,7 +5,7 @@ size per view (24-200 x 16-160, mixed within a case, so row pitches, partial til
 views larger or smaller than the reference all occur), a reference view, its sources (1 to 7,
 in a random order), the top-k (the reference's rule or any value from 1 to the source count), the
 iteration count and the depth range.  Two cases run the geometric pass over the oracle's
-photometric maps (equal sizes, as the oracle's map layout needs); the last two take 8 and 7
+photometric maps (mixed sizes: source maps of other sizes than the reference's); the last two take 8 and 7
 cameras with 7 and 6 sources, one with top-5 (the kernels' 8-slot selection network).  Planes and
 costs of every pixel must equal the oracle's bit for bit: the candidate early exit, the texel
 layout and the launch shapes change no result.  On the photometric cases the NCC adjoint's forward
@@ -38,11 +38,7 @@ def draw(base, case):
     n = int(rng.integers(3, 7)) if case < 10 else 8 - (case - 10)
     cams = sorted(rng.choice(len(base.views), n, replace=False).tolist())
     geometric = case in GEOMETRIC_CASES
-    if geometric:
-        W, H = int(rng.integers(24, 201)), int(rng.integers(16, 161))
-        sizes = [(W, H)] * n
-    else:
-        sizes = [(int(rng.integers(24, 201)), int(rng.integers(16, 161))) for _ in range(n)]
+    sizes = [(int(rng.integers(24, 201)), int(rng.integers(16, 161))) for _ in range(n)]
     views, images = [], []
     for c, (W, H) in zip(cams, sizes):
         v = base.views[c]
@@ -76,13 +72,12 @@ def test_random_configuration_matches_oracle(ctx, base, case):
         st.set_image(i, im)
     depths = None
     if geometric:
-        n = views[0].width * views[0].height
-        depths = np.zeros(len(views) * n, dtype=np.uint32)
+        depths = O.map_pool(views)
         for r in range(len(views)):
             srcs = [i for i in range(len(views)) if i != r]
             O.solve(views, imgs, r, srcs, lo, hi, iterations=1, depths=depths)
         for r in range(len(views)):
-            st.write_depth(r, depths[r * n:(r + 1) * n])
+            st.write_depth(r, O.layer_map(views, depths, r))
     o = lr3d.SolveOptions(sources=sources, min_depth=lo, max_depth=hi, iterations=iterations, topk=topk,
                           geometric=geometric)
     st.solve_batched([(ref, o)])

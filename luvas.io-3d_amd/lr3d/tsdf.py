@@ -169,10 +169,11 @@ class HipTsdfVolume:
         N.check(self._lib.lr3d_tsdf_set_depth_culling(self.handle, int(bool(on))))
 
     def backward(self, stereo, layers: Sequence[int], views: Sequence[IntegrationView], voxels, targets,
-                 map: int = N.MAP_RAW) -> dict:
+                 map: int = N.MAP_RAW, grad_depth: bool = True) -> dict:
         """Replay of integrate_batched into a fresh volume at the listed voxels (pool indices) and the
         adjoint of L = sum (tsdf - target)^2: dict(tsdf, grad_depth (list per view, H x W), grad_pose
-        (n, 12): dL/dR row-major and dL/dt per view, loss)."""
+        (n, 12): dL/dR row-major and dL/dt per view, loss).  grad_depth=False leaves the per-pixel
+        depth gradients on the device (not copied to the host; "grad_depth" is then None)."""
         structs, keep = [], []
         for v in views:
             s, k = _iv_struct(v)
@@ -184,13 +185,15 @@ class HipTsdfVolume:
         tgt = np.ascontiguousarray(targets, dtype=np.float32).reshape(-1)
         sizes = [v.width * v.height for v in views]
         tsdf = np.empty(vox.size, dtype=np.float32)
-        gd = np.empty(sum(sizes), dtype=np.float32)
+        gd = np.empty(sum(sizes), dtype=np.float32) if grad_depth else None
         gp = np.empty((len(views), 12), dtype=np.float64)
         loss = np.zeros(1, dtype=np.float64)
         N.check(self._lib.lr3d_tsdf_backward(self.handle, stereo.handle, ls, map, arr, len(structs), N.ptr(vox),
-                                             N.ptr(tgt), vox.size, N.ptr(tsdf), N.ptr(gd), N.ptr(gp), N.ptr(loss)))
-        parts = np.split(gd, np.cumsum(sizes)[:-1])
-        return {"tsdf": tsdf, "grad_depth": [p.reshape(v.height, v.width) for p, v in zip(parts, views)],
+                                             N.ptr(tgt), vox.size, N.ptr(tsdf), N.float_ptr(gd), N.ptr(gp),
+                                             N.ptr(loss)))
+        parts = np.split(gd, np.cumsum(sizes)[:-1]) if grad_depth else None
+        return {"tsdf": tsdf,
+                "grad_depth": [p.reshape(v.height, v.width) for p, v in zip(parts, views)] if grad_depth else None,
                 "grad_pose": gp, "loss": float(loss[0])}
 
     def write(self, tsdf: np.ndarray, weight: np.ndarray) -> None:

@@ -40,8 +40,7 @@ def check_view(ctx, w, batch, ref):
     v = w.views[ref]
     exact = (np.all(hyp.view(np.uint32) == ohyp.view(np.uint32), axis=1) &
              (cost.view(np.uint32) == ocost.view(np.uint32))).mean()
-    n = v.width * v.height
-    agree = depth_agreement(packed, opacked[ref * n:(ref + 1) * n])  # the oracle packs at layer ref of its size
+    agree = depth_agreement(packed, O.layer_map(w.views, opacked, ref))
     say(f"view {ref} ({v.width}x{v.height}, {len(w.sources[ref][:8])} sources): planes and costs bit-exact on "
         f"{exact * 100:.4f} % of pixels, depth agreement {agree * 100:.4f} % (oracle {time.perf_counter() - t0:.1f} s)")
     assert exact >= 0.999 and agree >= 0.995

@@ -101,7 +101,11 @@ int32_t lr3d_ctx_kernel_stats(lr3d_ctx *ctx, const char *name, int64_t *launches
 
 /* ---- PatchMatch stereo (replaces DenseStereo, DS:61-371) -------------------------------- */
 
-/* new DenseStereo(device, width, height, views) (DS:80-172).  flags: reserved, pass 0. */
+/* Flags of lr3d_stereo_create. */
+#define LR3D_STEREO_PRIOR_POOL 1u /* allocate the prior pool now (else on the first solve with a
+                                     prior); lr3d_stereo_solve_batched_noalloc needs it for priors */
+
+/* new DenseStereo(device, width, height, views) (DS:80-172).  flags: 0 or LR3D_STEREO_PRIOR_POOL. */
 int32_t lr3d_stereo_create(lr3d_ctx *ctx, const lr3d_view *views, int32_t n_views, uint32_t flags,
                            lr3d_stereo **out);
 int32_t lr3d_stereo_destroy(lr3d_stereo *st);                    /* DS:357-370 */
@@ -117,6 +121,18 @@ int32_t lr3d_stereo_solve(lr3d_stereo *st, const lr3d_solve_desc *desc);
  * new group of launches after that solve's pack, so it sees the refined map, as the reference's
  * layer-by-layer geometric pass does (…/pipeline/reconstructor.ts:96-108). */
 int32_t lr3d_stereo_solve_batched(lr3d_stereo *st, const lr3d_solve_desc *descs, int32_t n);
+/* No counterpart in the reference: lr3d_stereo_solve_batched for callers that cannot allocate
+ * device memory (code under their own HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc).  Same solves, launches and results (equal to n calls of lr3d_stereo_solve in array
+ * order), but the launch plan lives in a fixed per-GPU scratch array of 1 MiB instead of a buffer
+ * the set allocates, the launches are issued eagerly (no graph is captured or instantiated), and
+ * kernel timing (lr3d_ctx_set_profiling) does not cover them.  At most LR3D_NOALLOC_MAX_SOLVES
+ * solves per call (LR3D_ERR_INVALID beyond); a solve with a prior needs a set created with
+ * LR3D_STEREO_PRIOR_POOL (LR3D_ERR_STATE otherwise).  Every call on one GPU reuses that GPU's
+ * scratch array: calls that overlap on one GPU (sets of different contexts used from several
+ * threads) race on it; calls on one context are serialised. */
+#define LR3D_NOALLOC_MAX_SOLVES 1024
+int32_t lr3d_stereo_solve_batched_noalloc(lr3d_stereo *st, const lr3d_solve_desc *descs, int32_t n);
 /* DenseStereo.filter (DS:250-263): consistency check of `ref` into its filtered map. */
 int32_t lr3d_stereo_filter(lr3d_stereo *st, int32_t ref, const int32_t *checks, int32_t n_checks,
                            int32_t needed);

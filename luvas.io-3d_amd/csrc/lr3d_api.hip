@@ -74,6 +74,35 @@ struct DevBuf {
     ~DevBuf() { release(); }
 };
 
+// ---- plan scratch of the allocation-free batched solve ------------------------------------------------
+//
+// lr3d_stereo_solve_batched_noalloc keeps its launch plan (solve descriptors and block tables) in
+// this fixed array instead of a buffer it allocates: every GPU the library runs on holds one copy
+// until the process ends, and every call on that GPU reuses it (calls that overlap on one GPU race
+// on it).  In host code the array's name stands for a host-side stand-in, not device memory: host
+// code only ever uses the address that scratch_on_device() resolved with the device current.
+constexpr size_t kNoAllocScratchBytes = (size_t)1 << 20;
+static __device__ __align__(256) uint8_t scratch[kNoAllocScratchBytes];
+
+static std::mutex g_scratch_mu;
+static std::map<int32_t, uint8_t *> g_scratch_ptr;  // device -> its copy of `scratch`
+
+// Device address of `scratch` on `device`, which must be current; resolved once per device.
+static hipError_t scratch_on_device(int32_t device, uint8_t **out) {
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    auto it = g_scratch_ptr.find(device);
+    if (it != g_scratch_ptr.end()) {
+        *out = it->second;
+        return hipSuccess;
+    }
+    void *p = nullptr;
+    hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess) return e;
+    g_scratch_ptr[device] = (uint8_t *)p;
+    *out = (uint8_t *)p;
+    return hipSuccess;
+}
+
 // ---- context ------------------------------------------------------------------------------------------
 
 struct KernelStats {
@@ -97,6 +126,7 @@ struct lr3d_ctx {
     int64_t graph_launches = 0;
     int64_t users = 0;      // live stereo sets, volumes, meshes and clouds made on this context
     bool closing = false;   // lr3d_ctx_destroy was called while users remained
+    uint8_t *scratch = nullptr;  // this device's copy of `scratch` (device address)
 
     hipEvent_t take_event() {
         if (!event_pool.empty()) {
@@ -180,8 +210,13 @@ extern "C" int32_t lr3d_ctx_create(int32_t device, lr3d_ctx **out) {
     if (device < 0 || device >= count)
         return fail(LR3D_ERR_INVALID, "lr3d_ctx_create: device %d out of range [0, %d)", device, count);
     HIP_TRY(hipSetDevice(device));
+    uint8_t *plan_scratch = nullptr;
+    e = scratch_on_device(device, &plan_scratch);
+    if (e != hipSuccess)
+        return fail(LR3D_ERR_DEVICE, "lr3d_ctx_create: hipGetSymbolAddress(scratch): %s", hipGetErrorString(e));
     lr3d_ctx *ctx = new lr3d_ctx();
     ctx->device = device;
+    ctx->scratch = plan_scratch;
     e = hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking);
     if (e != hipSuccess) {
         delete ctx;
@@ -294,7 +329,8 @@ extern "C" int32_t lr3d_stereo_create(lr3d_ctx *ctx, const lr3d_view *views, int
     if (!ctx || !views || !out) return fail(LR3D_ERR_INVALID, "lr3d_stereo_create: NULL argument");
     *out = nullptr;
     if (n_views <= 0) return fail(LR3D_ERR_INVALID, "lr3d_stereo_create: n_views must be > 0");
-    (void)flags;
+    if (flags & ~(uint32_t)LR3D_STEREO_PRIOR_POOL)
+        return fail(LR3D_ERR_INVALID, "lr3d_stereo_create: unknown flags 0x%x", flags);
     size_t total = 0, total_q = 0;
     std::vector<uint32_t> off(n_views), qoff(n_views);
     for (int32_t i = 0; i < n_views; i++) {
@@ -324,6 +360,7 @@ extern "C" int32_t lr3d_stereo_create(lr3d_ctx *ctx, const lr3d_view *views, int
     if (e == hipSuccess) e = st->filt.alloc(total);
     if (e == hipSuccess) e = st->hyp.alloc(total);
     if (e == hipSuccess) e = st->cost.alloc(total);
+    if (e == hipSuccess && (flags & LR3D_STEREO_PRIOR_POOL)) e = st->prior.alloc(total);
     if (e == hipSuccess) e = hipMemsetAsync(st->quad.ptr, 0, total_q * 4, ctx->stream);
     if (e == hipSuccess) e = hipMemsetAsync(st->depth.ptr, 0, total * 4, ctx->stream);
     if (e == hipSuccess) e = hipMemsetAsync(st->filt.ptr, 0, total * 4, ctx->stream);
@@ -512,11 +549,13 @@ struct Plans {
 };
 
 // Launch plans of consecutive groups of `descs` (group g = descs[groups[g], groups[g + 1])), all
-// in the set's plan buffer: the descriptors, then each group's three block tables.
+// in one device buffer: the descriptors, then each group's three block tables.  The buffer is the
+// set's plan buffer (grown as needed), or `dest` (dest_bytes long) when given.
 // tiles: launch shapes of init, propagate, pack (the stereo's own when NULL).
 static int32_t upload_groups(lr3d_stereo *st, const std::vector<PmDesc> &descs, const std::vector<uint8_t> &pack,
                              const std::vector<int32_t> &groups, std::vector<Plans> &plans,
-                             std::vector<int64_t> &key, const TileShape *tiles = nullptr) {
+                             std::vector<int64_t> &key, const TileShape *tiles = nullptr,
+                             uint8_t *dest = nullptr, size_t dest_bytes = 0) {
     lr3d_ctx *ctx = st->ctx;
     const int32_t n = (int32_t)descs.size();
     const int32_t n_groups = (int32_t)groups.size() - 1;
@@ -567,11 +606,17 @@ static int32_t upload_groups(lr3d_stereo *st, const std::vector<PmDesc> &descs, 
         key.push_back(kmax);
         key.push_back(geometric);
     }
-    HIP_TRY(st->plan_buf.reserve(total));
-    HIP_TRY(hipMemcpyAsync(st->plan_buf.ptr, host.data(), total, hipMemcpyHostToDevice, ctx->stream));
+    uint8_t *buf = dest;
+    if (!buf) {
+        HIP_TRY(st->plan_buf.reserve(total));
+        buf = st->plan_buf.ptr;
+    } else if (total > dest_bytes) {
+        return fail(LR3D_ERR_INVALID, "launch plan of %zu bytes exceeds the %zu-byte plan scratch", total, dest_bytes);
+    }
+    HIP_TRY(hipMemcpyAsync(buf, host.data(), total, hipMemcpyHostToDevice, ctx->stream));
     HIP_TRY(hipStreamSynchronize(ctx->stream));  // host staging goes out of scope
-    const PmDesc *dd = (const PmDesc *)st->plan_buf.ptr;
-    const uint32_t *tables = (const uint32_t *)(st->plan_buf.ptr + desc_bytes);
+    const PmDesc *dd = (const PmDesc *)buf;
+    const uint32_t *tables = (const uint32_t *)(buf + desc_bytes);
     for (int32_t g = 0; g < n_groups; g++) {
         const int32_t m = groups[g + 1] - groups[g];
         const uint32_t *t0 = tables + table_at[g];
@@ -580,7 +625,7 @@ static int32_t upload_groups(lr3d_stereo *st, const std::vector<PmDesc> &descs, 
         plans[g].prop.block_start = t0 + (m + 1);
         plans[g].pack.block_start = t0 + 2 * (m + 1);
     }
-    key.push_back((int64_t)(uintptr_t)st->plan_buf.ptr);
+    key.push_back((int64_t)(uintptr_t)buf);
     for (int i = 0; i < 3; i++) {
         key.push_back(tiles[i].tx);
         key.push_back(tiles[i].ty);
@@ -639,42 +684,52 @@ static PmPools pools_of(lr3d_stereo *st) {
 }
 
 // The schedule of DS:185-239 for one launch group: init, iterations x {phase 0, 1}, pack.
-static hipError_t enqueue_schedule(lr3d_stereo *st, const Plans &plans) {
+// with_timing: event pairs around the launches when profiling is on (events are created on
+// demand, so the allocation-free entry point launches without them).
+static hipError_t enqueue_schedule(lr3d_stereo *st, const Plans &plans, bool with_timing) {
     lr3d_ctx *ctx = st->ctx;
     const PmPools pools = pools_of(st);
-    hipError_t e = timed(ctx, "pm_init", [&] { return pm_launch_init(ctx->stream, plans.init, pools); });
+    auto launch = [&](const char *name, auto &&f) -> hipError_t {
+        if (with_timing) return timed(ctx, name, f);
+        (void)hipGetLastError();
+        return f();
+    };
+    hipError_t e = launch("pm_init", [&] { return pm_launch_init(ctx->stream, plans.init, pools); });
     if (e != hipSuccess) return e;
     for (int32_t it = 0; it < plans.max_iterations; it++) {
         const float step = (float)pow(0.5, (double)it);  // DS:203
         for (uint32_t phase = 0; phase < 2; phase++) {
-            e = timed(ctx, "pm_propagate", [&] {
+            e = launch("pm_propagate", [&] {
                 return pm_launch_propagate(ctx->stream, plans.prop, pools, (uint32_t)it, phase, step);
             });
             if (e != hipSuccess) return e;
         }
     }
     if (plans.any_pack) {
-        e = timed(ctx, "pm_pack", [&] { return pm_launch_pack(ctx->stream, plans.pack, pools); });
+        e = launch("pm_pack", [&] { return pm_launch_pack(ctx->stream, plans.pack, pools); });
         if (e != hipSuccess) return e;
     }
     return hipSuccess;
 }
 
-static hipError_t enqueue_groups(lr3d_stereo *st, const std::vector<Plans> &groups) {
+static hipError_t enqueue_groups(lr3d_stereo *st, const std::vector<Plans> &groups, bool with_timing = true) {
     for (const Plans &g : groups) {
-        hipError_t e = enqueue_schedule(st, g);
+        hipError_t e = enqueue_schedule(st, g, with_timing);
         if (e != hipSuccess) return e;
     }
     return hipSuccess;
 }
 
 static int32_t stage_priors(lr3d_stereo *st, const lr3d_solve_desc *descs, int32_t n,
-                            std::vector<uint32_t> &prior_off) {
+                            std::vector<uint32_t> &prior_off, bool may_allocate = true) {
     lr3d_ctx *ctx = st->ctx;
     prior_off.assign(n, 0);
     bool any = false;
     for (int32_t i = 0; i < n; i++) any |= descs[i].prior != nullptr;
     if (!any) return LR3D_OK;
+    if (!st->prior.ptr && !may_allocate)
+        return fail(LR3D_ERR_STATE, "lr3d_stereo_solve_batched_noalloc: a solve passes a prior but the set has "
+                                    "no prior pool (create it with LR3D_STEREO_PRIOR_POOL)");
     if (!st->prior.ptr) {
         HIP_TRY(st->prior.alloc(st->total_px));
         // graphs captured so far hold the pools as they were (no prior buffer): capture again
@@ -692,17 +747,22 @@ static int32_t stage_priors(lr3d_stereo *st, const lr3d_solve_desc *descs, int32
     return LR3D_OK;
 }
 
-static int32_t solve_impl(lr3d_stereo *st, const lr3d_solve_desc *descs, int32_t n, bool batched) {
+// batched: shared launches replayed from a captured graph; noalloc: shared launches with the plan in
+// the device's `scratch` array, launched eagerly, no device memory allocated.
+static int32_t solve_impl(lr3d_stereo *st, const lr3d_solve_desc *descs, int32_t n, bool batched,
+                          bool noalloc = false) {
     lr3d_ctx *ctx = st->ctx;
+    const char *what = noalloc ? "lr3d_stereo_solve_batched_noalloc"
+                               : batched ? "lr3d_stereo_solve_batched" : "lr3d_stereo_solve";
     for (int32_t i = 0; i < n; i++) {
-        int32_t rc = check_layer(st, descs[i].ref, batched ? "lr3d_stereo_solve_batched" : "lr3d_stereo_solve");
+        int32_t rc = check_layer(st, descs[i].ref, what);
         if (rc) return rc;
         for (int32_t j = 0; j < i; j++)
             if (descs[j].ref == descs[i].ref)
-                return fail(LR3D_ERR_INVALID, "lr3d_stereo_solve_batched: layer %d appears twice", descs[i].ref);
+                return fail(LR3D_ERR_INVALID, "%s: layer %d appears twice", what, descs[i].ref);
     }
     std::vector<uint32_t> prior_off;
-    int32_t rc = stage_priors(st, descs, n, prior_off);
+    int32_t rc = stage_priors(st, descs, n, prior_off, !noalloc);
     if (rc) return rc;
     std::vector<PmDesc> dv;
     dv.reserve(n);
@@ -722,12 +782,13 @@ static int32_t solve_impl(lr3d_stereo *st, const lr3d_solve_desc *descs, int32_t
     if (dv.empty()) return finish(ctx);
     std::vector<Plans> plans;
     std::vector<int64_t> key;
-    rc = upload_groups(st, dv, packs, gauss_seidel_groups(dv, packs), plans, key);
+    rc = upload_groups(st, dv, packs, gauss_seidel_groups(dv, packs), plans, key, nullptr,
+                       noalloc ? ctx->scratch : nullptr, noalloc ? kNoAllocScratchBytes : 0);
     if (rc) return rc;
     hipError_t e = hipSuccess;
     // HIP cannot time event records captured into a graph (hipEventElapsedTime reports an
     // invalid handle), so profiled calls launch eagerly with an event pair around each kernel.
-    const bool use_graph = batched && !ctx->profiling;
+    const bool use_graph = batched && !noalloc && !ctx->profiling;
     GraphEntry *entry = nullptr;
     if (use_graph) {
         for (auto &g : st->graphs)
@@ -756,9 +817,8 @@ static int32_t solve_impl(lr3d_stereo *st, const lr3d_solve_desc *descs, int32_t
         HIP_TRY(hipGraphLaunch(entry->exec, ctx->stream));
         ctx->graph_launches++;
     } else {
-        e = enqueue_groups(st, plans);
-        if (e != hipSuccess)
-            return fail(LR3D_ERR_DEVICE, "lr3d_stereo_solve: launch failed: %s", hipGetErrorString(e));
+        e = enqueue_groups(st, plans, !noalloc);
+        if (e != hipSuccess) return fail(LR3D_ERR_DEVICE, "%s: launch failed: %s", what, hipGetErrorString(e));
     }
     rc = finish(ctx);
     if (rc) return rc;
@@ -779,6 +839,16 @@ extern "C" int32_t lr3d_stereo_solve_batched(lr3d_stereo *st, const lr3d_solve_d
     std::lock_guard<std::mutex> lock(st->ctx->mu);
     HIP_TRY(hipSetDevice(st->ctx->device));
     return solve_impl(st, descs, n, true);
+}
+
+extern "C" int32_t lr3d_stereo_solve_batched_noalloc(lr3d_stereo *st, const lr3d_solve_desc *descs, int32_t n) {
+    if (!st || (!descs && n > 0)) return fail(LR3D_ERR_INVALID, "lr3d_stereo_solve_batched_noalloc: NULL argument");
+    if (n < 0 || n > LR3D_NOALLOC_MAX_SOLVES)
+        return fail(LR3D_ERR_INVALID, "lr3d_stereo_solve_batched_noalloc: n = %d, must be in [0, %d]", n,
+                    LR3D_NOALLOC_MAX_SOLVES);
+    std::lock_guard<std::mutex> lock(st->ctx->mu);
+    HIP_TRY(hipSetDevice(st->ctx->device));
+    return solve_impl(st, descs, n, true, true);
 }
 
 // ---- launch-shape autotuner (no counterpart in the reference) ----------------------------------------

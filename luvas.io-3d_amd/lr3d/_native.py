@@ -18,6 +18,8 @@ LR3D_ERR_DEVICE = -2
 LR3D_ERR_NOMEM = -3
 LR3D_ERR_STATE = -4
 MAX_SOURCES = 8
+STEREO_PRIOR_POOL = 1
+NOALLOC_MAX_SOLVES = 1024
 MAP_RAW = 0
 MAP_FILTERED = 1
 
@@ -26,7 +28,7 @@ EXPORTED = (
     "lr3d_last_error", "lr3d_ctx_create", "lr3d_ctx_destroy", "lr3d_ctx_synchronize",
     "lr3d_ctx_set_profiling", "lr3d_ctx_kernel_stats",
     "lr3d_stereo_create", "lr3d_stereo_destroy", "lr3d_stereo_set_image", "lr3d_stereo_solve",
-    "lr3d_stereo_solve_batched", "lr3d_stereo_filter", "lr3d_stereo_read_depth",
+    "lr3d_stereo_solve_batched", "lr3d_stereo_solve_batched_noalloc", "lr3d_stereo_filter", "lr3d_stereo_read_depth",
     "lr3d_stereo_read_filtered", "lr3d_stereo_read_hyp", "lr3d_stereo_write_depth",
     "lr3d_stereo_set_coarse", "lr3d_stereo_set_pose", "lr3d_stereo_write_hyp",
     "lr3d_ncc_backward", "lr3d_ncc_refine_step",
@@ -96,6 +98,7 @@ def load() -> C.CDLL:
         "lr3d_stereo_set_image": (i32, [P, i32, C.c_void_p, i32, i32]),
         "lr3d_stereo_solve": (i32, [P, C.POINTER(SolveDesc)]),
         "lr3d_stereo_solve_batched": (i32, [P, C.POINTER(SolveDesc), i32]),
+        "lr3d_stereo_solve_batched_noalloc": (i32, [P, C.POINTER(SolveDesc), i32]),
         "lr3d_stereo_filter": (i32, [P, i32, C.POINTER(i32), i32, i32]),
         "lr3d_stereo_read_depth": (i32, [P, i32, C.c_void_p]),
         "lr3d_stereo_read_filtered": (i32, [P, i32, C.c_void_p]),

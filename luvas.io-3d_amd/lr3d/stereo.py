@@ -47,14 +47,16 @@ class HipDenseStereo:
     """GPU PatchMatch for every keyframe (DS:56-61): one device pool per buffer kind holds every
     view, so the geometric pass and the consistency filter read any neighbour without copies."""
 
-    def __init__(self, ctx: Context, views: Sequence[StereoView]):
+    def __init__(self, ctx: Context, views: Sequence[StereoView], prior_pool: bool = False):
+        """prior_pool: allocate the prior pool now (needed by solve_batched_noalloc for priors)."""
         self._lib = N.load()
         self.ctx = ctx
         # own copies: set_pose replaces poses here, never in the caller's objects
         self.views = [StereoView(v.intrinsics, v.pose, v.width, v.height) for v in views]
         arr = (N.View * len(self.views))(*[_view_struct(v) for v in self.views])
         handle = C.c_void_p()
-        N.check(self._lib.lr3d_stereo_create(ctx.handle, arr, len(self.views), 0, C.byref(handle)))
+        flags = N.STEREO_PRIOR_POOL if prior_pool else 0
+        N.check(self._lib.lr3d_stereo_create(ctx.handle, arr, len(self.views), flags, C.byref(handle)))
         self.handle = handle
         self._keep = []
 
@@ -113,6 +115,17 @@ class HipDenseStereo:
             keep.append(k)
         arr = (N.SolveDesc * len(descs))(*descs)
         N.check(self._lib.lr3d_stereo_solve_batched(self.handle, arr, len(descs)))
+
+    def solve_batched_noalloc(self, solves: Sequence[tuple[int, SolveOptions]]) -> None:
+        """solve_batched without device allocation: the launch plan in the library's fixed
+        per-GPU scratch array, eager launches (at most NOALLOC_MAX_SOLVES solves)."""
+        descs, keep = [], []
+        for ref, o in solves:
+            d, k = self._desc(ref, o)
+            descs.append(d)
+            keep.append(k)
+        arr = (N.SolveDesc * max(1, len(descs)))(*descs)
+        N.check(self._lib.lr3d_stereo_solve_batched_noalloc(self.handle, arr, len(descs)))
 
     def autotune(self, solves: Sequence[tuple[int, SolveOptions]], repeats: int = 3) -> dict:
         """Times the candidate launch shapes on this batch and keeps the fastest; returns them."""

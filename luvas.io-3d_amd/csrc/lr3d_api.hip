@@ -459,6 +459,27 @@ static void relative_pose(const lr3d_view &rv, const lr3d_view &sv, double R[9],
     }
 }
 
+// The plane-independent parts of the homography H = Ks (R + t n^T / rho) Kr^-1 (PM:116-128) of
+// reference view rv and source view sv, in double, rounded to f32: G = Ks R Kr^-1 and e = Ks t, so
+// that H = G + (e / rho) (Kr^-T n)^T.  R, t: the relative pose ref -> src (CAM:73-77).
+static void homography_parts(const lr3d_view &rv, const lr3d_view &sv, const double R[9], const double t[3],
+                             float G[9], float e[3]) {
+    double A[9];  // Ks R
+    for (int j = 0; j < 3; j++) {
+        A[j] = sv.fx * R[j] + sv.cx * R[6 + j];
+        A[3 + j] = sv.fy * R[3 + j] + sv.cy * R[6 + j];
+        A[6 + j] = R[6 + j];
+    }
+    for (int i = 0; i < 3; i++) {  // (Ks R) Kr^-1
+        G[i * 3 + 0] = (float)(A[i * 3 + 0] / rv.fx);
+        G[i * 3 + 1] = (float)(A[i * 3 + 1] / rv.fy);
+        G[i * 3 + 2] = (float)((A[i * 3 + 2] - A[i * 3 + 0] * rv.cx / rv.fx) - A[i * 3 + 1] * rv.cy / rv.fy);
+    }
+    e[0] = (float)(sv.fx * t[0] + sv.cx * t[2]);
+    e[1] = (float)(sv.fy * t[1] + sv.cy * t[2]);
+    e[2] = (float)t[2];
+}
+
 static int32_t build_desc(const lr3d_stereo *st, const lr3d_solve_desc &sd, uint32_t prior_off,
                           bool require_images, PmDesc &d) {
     const char *what = "lr3d_stereo_solve";
@@ -530,6 +551,7 @@ static int32_t build_desc(const lr3d_stereo *st, const lr3d_solve_desc &sd, uint
         o.k[3] = (float)sv.cy;
         for (int i = 0; i < 9; i++) o.R[i] = (float)R[i];
         for (int i = 0; i < 3; i++) o.t[i] = (float)t[i];
+        homography_parts(rv, sv, R, t, o.G, o.e);
         o.W = sv.width;
         o.H = sv.height;
         o.off = st->off[layer];

@@ -35,6 +35,11 @@ struct PmSrc {
     int32_t layer;
     int32_t pitch;  // texels per row of the layer in the quad pool
     uint32_t qoff;  // texel offset of the layer in the quad pool
+    // The plane homography H = Ks (R + t n^T / rho) Kr^-1 (PM:116-128) split into its parts that do
+    // not depend on the plane, formed in double on the host and rounded: G = Ks R Kr^-1, e = Ks t;
+    // then H = G + (e / rho) m^T with m = Kr^-T n per hypothesis.
+    float G[9];
+    float e[3];
 };
 
 // One reference view's solve (the WGSL `Params`, PM:12-26, minus the per-launch state).

@@ -213,11 +213,12 @@ __global__ void __launch_bounds__(64) ncc_backward_kernel(PmPlan plan, PmPools p
         wvar = win.var;
         rs = ref_stats(lds, lane);
         const float rho = dot3(h.x, h.y, h.z, rx * h.w, ry * h.w, 1.0f * h.w);
+        const PlaneTerms plane = plane_terms(d, h.x, h.y, h.z, rho);
         // pass 1: view costs, exactly as the forward pass, and the k smallest by (cost, index)
 #pragma unroll 1
         for (int s = 0; s < d.n_src; s++) {
             Mat3 H;
-            homography(d, d.src[s], h.x, h.y, h.z, rho, H);
+            homography(d.src[s], plane, H);
             const float c = view_cost(d.src[s], H, px, py, win, pools.quad);
 #pragma unroll
             for (int q = 0; q < kMaxSources; q++)
@@ -336,6 +337,7 @@ __global__ void __launch_bounds__(64) ncc_backward_kernel(PmPlan plan, PmPools p
         win.sw = wsw;
         win.mean = wmean;
         win.var = wvar;
+        win.isw = 1.0f / wsw;
         const float c =
             hypothesis_cost<kMaxSources, false>(d, pools, px, py, rx, ry, win, mx, my, mz, nd, cost);
         if (c < cost) {

@@ -10,6 +10,12 @@
 
 #include <math.h>
 
+// One 4-byte record of a structured buffer: the buffer load with the index register scaled by the
+// descriptor's stride (the LLVM intrinsic behind it; clang has no builtin for it).
+extern "C" __device__ uint32_t struct_buffer_load_u32(__amdgpu_buffer_rsrc_t rsrc, int32_t vindex, int32_t voffset,
+                                           int32_t soffset, int32_t aux)
+    __asm("llvm.amdgcn.struct.ptr.buffer.load.i32");
+
 namespace lr3d {
 namespace {
 
@@ -126,6 +132,7 @@ struct Window {
     float v[kSamples];
     float w[kSamples];
     float sw, mean, var;
+    float isw;  // 1 / sw
 };
 
 __device__ __forceinline__ float ref_pixel(const uint32_t *__restrict__ img, int32_t W, int32_t H,
@@ -157,35 +164,40 @@ __device__ __forceinline__ void prepare_window(const uint32_t *__restrict__ img,
     win.sw = sw;
     win.mean = mean;
     win.var = fmaxf(srr / sw - mean * mean, 0.0f);
+    win.isw = 1.0f / sw;
 }
 
 // ---- plane homography (PM:112-128) -------------------------------------------------------------
+//
+// H = Ks (R + t n^T / rho) Kr^-1 = G + (e / rho) m^T with G = Ks R Kr^-1 and e = Ks t per source
+// (PmSrc, formed on the host) and m = Kr^-T n per hypothesis: the plane's part is formed once per
+// hypothesis (PlaneTerms), each source view then costs 3 products and 9 FMAs.
 
 struct Mat3 {
     float m[9];
 };
 
-__device__ __forceinline__ void homography(const PmDesc &d, const PmSrc &s, float nx, float ny,
-                                           float nz, float rho, Mat3 &H) {
-    const float n[3] = {nx, ny, nz};
-    const float t[3] = {s.t[0] / rho, s.t[1] / rho, s.t[2] / rho};
-    float M[9];
-#pragma unroll
-    for (int i = 0; i < 3; i++)
-#pragma unroll
-        for (int j = 0; j < 3; j++) M[i * 3 + j] = s.R[i * 3 + j] + t[i] * n[j];
-    float A[9];
-#pragma unroll
-    for (int j = 0; j < 3; j++) {
-        A[j] = s.k[0] * M[j] + s.k[2] * M[6 + j];
-        A[3 + j] = s.k[1] * M[3 + j] + s.k[3] * M[6 + j];
-        A[6 + j] = M[6 + j];
-    }
+struct PlaneTerms {
+    float irho;        // 1 / rho
+    float m0, m1, m2;  // Kr^-T n
+};
+
+__device__ __forceinline__ PlaneTerms plane_terms(const PmDesc &d, float nx, float ny, float nz, float rho) {
+    PlaneTerms p;
+    p.irho = 1.0f / rho;
+    p.m0 = nx * d.ikx;
+    p.m1 = ny * d.iky;
+    p.m2 = (nx * d.kx2 + ny * d.ky2) + nz;
+    return p;
+}
+
+__device__ __forceinline__ void homography(const PmSrc &s, const PlaneTerms &p, Mat3 &H) {
+    const float m[3] = {p.m0, p.m1, p.m2};
 #pragma unroll
     for (int i = 0; i < 3; i++) {
-        H.m[i * 3 + 0] = A[i * 3 + 0] * d.ikx;
-        H.m[i * 3 + 1] = A[i * 3 + 1] * d.iky;
-        H.m[i * 3 + 2] = (A[i * 3 + 0] * d.kx2 + A[i * 3 + 1] * d.ky2) + A[i * 3 + 2];
+        const float f = s.e[i] * p.irho;
+#pragma unroll
+        for (int j = 0; j < 3; j++) H.m[i * 3 + j] = fmaf(f, m[j], s.G[i * 3 + j]);
     }
 }
 
@@ -250,10 +262,14 @@ __device__ __forceinline__ void view_taps(const uint32_t *__restrict__ img, int3
     }
 }
 
+__device__ __forceinline__ uint32_t texel_load(__amdgpu_buffer_rsrc_t texels, int32_t index) {
+    return struct_buffer_load_u32(texels, index, 0, 0, 0);
+}
+
 // Taps of a window known to lie inside the source image (see view_cost): no per-tap checks.
-// The texels are read through a buffer resource that spans the layer's texels, so a load can
-// never leave them (the hardware returns 0 past the end), at two address instructions per tap
-// (a 24-bit multiply-add and a shift) instead of three for a clamped 64-bit address.
+// The texels are read through a structured buffer resource over the layer's texels, so a load can
+// never leave them (the hardware returns 0 past the last record), at one address instruction per
+// tap (a 24-bit multiply-add giving the texel index; the hardware scales it by the 4-byte record).
 // cu, cv: the corner taps (0, 5, 30, 35) as classify_window computed them, with the same
 // expressions, so they are taken from there.
 __device__ __forceinline__ void view_taps_inside(const uint32_t *__restrict__ img, int32_t H,
@@ -262,9 +278,9 @@ __device__ __forceinline__ void view_taps_inside(const uint32_t *__restrict__ im
                                                  const float cv[4], float &ss, float &sss,
                                                  float &srs) {
     const uint32_t stride = (uint32_t)pitch;
-    // raw buffer, 32-bit data format; num_records in bytes
+    // structured buffer of 4-byte records (the hardware scales the index), one record per texel
     const __amdgpu_buffer_rsrc_t texels = __builtin_amdgcn_make_buffer_rsrc(
-        (void *)img, (short)0, (int)((uint32_t)pitch * (uint32_t)H * 4u), 0x00020000);
+        (void *)img, (short)4, (int)((uint32_t)pitch * (uint32_t)H), 0x00020000);
 #pragma unroll
     for (int row = 0; row < 6; row++) {
         const float Y = py + (float)(row * 2 - 5);
@@ -290,7 +306,7 @@ __device__ __forceinline__ void view_taps_inside(const uint32_t *__restrict__ im
             }
             const uint32_t ix = (uint32_t)u, iy = (uint32_t)v;
             const float fx = __builtin_amdgcn_fractf(u), fy = __builtin_amdgcn_fractf(v);
-            const uint32_t q = __builtin_amdgcn_raw_buffer_load_b32(texels, (__umul24(iy, stride) + ix) << 2, 0, 0);
+            const uint32_t q = texel_load(texels, (int32_t)(__umul24(iy, stride) + ix));
             const float a = (float)(q & 0xffu), b = (float)((q >> 8) & 0xffu);
             const float c = (float)((q >> 16) & 0xffu), e = (float)(q >> 24);
             const float top = fmaf(fx, b - a, a);
@@ -310,7 +326,7 @@ __device__ __forceinline__ void view_taps_inside(const uint32_t *__restrict__ im
 //   kInside   every corner lies inside the image by more than a rounding margin: q.z is affine in
 //             (X, Y) and positive on the window, so the warped window is the convex hull of the
 //             warped corners, and every tap is inside as computed;
-//   kCareful  anything else (windows touching the border, huge |q.z|): per-tap checks.
+//   kCareful  anything else (windows touching the border, huge terms of q): per-tap checks.
 enum { kOutside = 0, kInside = 1, kCareful = 2 };
 
 __device__ __forceinline__ int classify_window(const PmSrc &s, const Mat3 &H, float px, float py,
@@ -321,8 +337,10 @@ __device__ __forceinline__ int classify_window(const PmSrc &s, const Mat3 &H, fl
     const float bx = fabsf(H.m[0]) * ax + fabsf(H.m[1]) * ay + fabsf(H.m[2]);
     const float by = fabsf(H.m[3]) * ax + fabsf(H.m[4]) * ay + fabsf(H.m[5]);
     const float bz = fabsf(H.m[6]) * ax + fabsf(H.m[7]) * ay + fabsf(H.m[8]);
-    if (!(bz < 0x1p60f)) return kCareful;  // outside the range where rcp_exact == 1/q.z
-    bool valid = true;
+    // Past this, rcp_exact may differ from 1 / q.z; below it every q component is finite, and so is
+    // every u, v of a corner with q.z > 1e-6 (no NaN), so a corner is invalid exactly when the
+    // minimum or maximum over the corners fails a test.
+    if (!(bz < 0x1p60f) || !(bx < 0x1p60f) || !(by < 0x1p60f)) return kCareful;
     float zmin = INFINITY, umin = INFINITY, umax = -INFINITY, vmin = INFINITY, vmax = -INFINITY;
 #pragma unroll
     for (int corner = 0; corner < 4; corner++) {
@@ -335,14 +353,15 @@ __device__ __forceinline__ int classify_window(const PmSrc &s, const Mat3 &H, fl
         const float u = qx * rz, v = qy * rz;
         cu[corner] = u;
         cv[corner] = v;
-        valid = valid & (qz > 1e-6f) & (u >= 0.0f) & (v >= 0.0f) & (u <= wmax) & (v <= hmax);
         zmin = fminf(zmin, qz);
         umin = fminf(umin, u);
         umax = fmaxf(umax, u);
         vmin = fminf(vmin, v);
         vmax = fmaxf(vmax, v);
     }
-    if (!valid) return kOutside;
+    // (u and v of a corner with q.z <= 1e-6 may be NaN; zmin decides those windows first)
+    if (!(zmin > 1e-6f) || !(umin >= 0.0f) || !(vmin >= 0.0f) || !(umax <= wmax) || !(vmax <= hmax))
+        return kOutside;
     // Each computed q component is within 2^-20 of its term bound of the exact value (two FMAs
     // round at most 2 ulp of the larger term), so exact q.z >= zmin - ez on the window.  A
     // computed u = q.x / q.z then lies within eu of the exact ratio; require corners inside
@@ -350,7 +369,8 @@ __device__ __forceinline__ int classify_window(const PmSrc &s, const Mat3 &H, fl
     const float ez = bz * 0x1p-20f;
     const float zlow = zmin - 2.0f * ez;
     if (!(zlow > 2e-6f)) return kCareful;
-    const float ilow = 1.0f / zlow;
+    // an upper bound of 1 / zlow: v_rcp_f32 is within 1 ulp, the factor covers it and its rounding
+    const float ilow = __builtin_amdgcn_rcpf(zlow) * 1.000001f;
     const float eu = (bx * 0x1p-20f + (wmax + 1.0f) * ez) * ilow + (wmax + 1.0f) * 0x1p-20f;
     const float ev = (by * 0x1p-20f + (hmax + 1.0f) * ez) * ilow + (hmax + 1.0f) * 0x1p-20f;
     const bool inside = umin >= 4.0f * eu && umax <= wmax - 4.0f * eu && vmin >= 4.0f * ev &&
@@ -379,12 +399,14 @@ __device__ __forceinline__ float view_cost(const PmSrc &s, const Mat3 &H, float 
         if (cls == kInside) view_taps_inside(img, s.H, s.pitch, H, px, py, win, cu, cv, ss, sss, srs);
     }
     if (!ok) return kInvalid;
-    float mean = ss / win.sw;
-    float var_s = fmaxf(sss / win.sw - mean * mean, 0.0f);
+    // PM:151-158 with the window's sums scaled by 1 / sw and the correlation by 1 / den (IEEE
+    // reciprocals; den >= kDegenerateDen lies in the range where rcp_exact is the IEEE reciprocal)
+    float mean = ss * win.isw;
+    float var_s = fmaxf(sss * win.isw - mean * mean, 0.0f);
     float den = sqrtf(win.var * var_s);
     if (den < kDegenerateDen) return 1.0f;
-    float cov = srs / win.sw - win.mean * mean;
-    return clampf(1.0f - cov / den, 0.0f, 2.0f);
+    float cov = srs * win.isw - win.mean * mean;
+    return clampf(1.0f - cov * rcp_exact(den), 0.0f, 2.0f);
 }
 
 // PM:161-190: forward-backward reprojection through a source view's packed depth map.
@@ -447,8 +469,8 @@ struct TopK {
 // the final mean from below: every view cost is >= 0, the i-th smallest of a set can only fall
 // when members are lowered to 0, and rounded addition and division by k are monotone.  When
 // that bound already reaches `bound`, the candidate cannot win and the remaining views are not
-// evaluated; the bound is returned instead of the mean, and the caller discards it as it would
-// have discarded the mean.  A wavefront stops once all of its lanes have stopped.
+// evaluated; `bound` itself is returned instead of the mean, and the caller discards it as it
+// would have discarded the mean.  A wavefront stops once all of its lanes have stopped.
 // The views are taken last to first: the k smallest costs are summed in ascending order, so the
 // order of evaluation changes no result, and source lists are ranked best first
 // (mvs/view-selection.ts:46-48 sorts them by shared-point score), so leaving the best-matching
@@ -465,6 +487,7 @@ __device__ __forceinline__ float hypothesis_cost(const PmDesc &d, const PmPools 
     float rho = dot3(nx, ny, nz, X0, X1, X2);
     const int32_t count = d.n_src;
     if (count == 0) return kInvalid;
+    const PlaneTerms plane = plane_terms(d, nx, ny, nz, rho);
     TopK<KMAX> top;
     top.reset();
     const int32_t first_check = count - d.topk + 1;  // from here on fewer than k views are left
@@ -476,12 +499,13 @@ __device__ __forceinline__ float hypothesis_cost(const PmDesc &d, const PmPools 
 #pragma unroll
             for (int j = 0; j < KMAX; j++)
                 if (j < known) total = total + top.s[j];
-            const float low = total / (float)d.topk;
-            if (low >= bound) return low;
+            // total >= k * bound exactly (the FMA rounds the exact difference once, keeping its
+            // sign), so total / k, and the final mean above it, cannot fall below `bound`
+            if (fmaf(-(float)d.topk, bound, total) >= 0.0f) return bound;
         }
         const PmSrc &src = d.src[s];
         Mat3 H;
-        homography(d, src, nx, ny, nz, rho, H);
+        homography(src, plane, H);
         float c = view_cost(src, H, px, py, win, pools.quad);
         if (GEOMETRIC && (d.flags & 1u) != 0u) {
             float r0 = 0.0f, r1 = 0.0f;

@@ -54,6 +54,10 @@ typedef struct {
     int32_t layer;
     int32_t width, height; /* source image size (equal to the reference size in the reference) */
     int64_t map_off;       /* pixel offset of the layer's packed map in the `depths` pool */
+    /* plane-independent parts of the homography (PM:116-128), formed in double and rounded:
+     * G = Ks R Kr^-1, e = Ks t, so that H = G + (e / rho) (Kr^-T n)^T */
+    float G[9];
+    float e[3];
 } orc_src;
 
 /* The `Params` uniform (PM:12-26) as DenseStereo.params() fills it (DS:314-355). */
@@ -102,7 +106,8 @@ void orc_make_params(const orc_view *views, int32_t n_views, int32_t ref, const 
 /* Bilateral window of pixel (x, y) (PM:90-110): values[36], weights[36], stats[3] = sum, mean, var */
 void orc_window(const orc_params *p, const orc_images *img, int32_t x, int32_t y, float *values,
                 float *weights, float *stats);
-/* Plane homography of source s (PM:116-128), row-major 3x3 */
+/* Plane homography of source s (PM:116-128), row-major 3x3: H = G + (e / rho) m^T with
+ * m = Kr^-T n, in the fixed order of orc_homography */
 void orc_homography(const orc_params *p, int32_t s, const float n[3], float rho, float H[9]);
 /* Bilateral NCC cost of one source view (PM:130-159) */
 float orc_view_cost(const orc_params *p, const orc_images *img, int32_t s, const float H[9],

@@ -27,7 +27,7 @@ class OrcView(C.Structure):
 class OrcSrc(C.Structure):
     _fields_ = [("k", C.c_float * 4), ("R", C.c_float * 9), ("t", C.c_float * 3),
                 ("layer", C.c_int32), ("width", C.c_int32), ("height", C.c_int32),
-                ("map_off", C.c_int64)]
+                ("map_off", C.c_int64), ("G", C.c_float * 9), ("e", C.c_float * 3)]
 
 
 class OrcParams(C.Structure):

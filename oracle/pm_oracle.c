@@ -276,6 +276,21 @@ void orc_make_params(const orc_view *views, int32_t n_views, int32_t ref, const 
         o->width = sv->width;
         o->height = sv->height;
         o->map_off = orc_map_offset(views, sources[s]);
+        /* H = Ks (R + t n^T / rho) Kr^-1 = G + (e / rho) (Kr^-T n)^T: G = (Ks R) Kr^-1, e = Ks t */
+        double A[9];
+        for (int j = 0; j < 3; j++) {
+            A[j] = sv->fx * R[j] + sv->cx * R[6 + j];
+            A[3 + j] = sv->fy * R[3 + j] + sv->cy * R[6 + j];
+            A[6 + j] = R[6 + j];
+        }
+        for (int i = 0; i < 3; i++) {
+            o->G[i * 3 + 0] = (float)(A[i * 3 + 0] / rv->fx);
+            o->G[i * 3 + 1] = (float)(A[i * 3 + 1] / rv->fy);
+            o->G[i * 3 + 2] = (float)((A[i * 3 + 2] - A[i * 3 + 0] * rv->cx / rv->fx) - A[i * 3 + 1] * rv->cy / rv->fy);
+        }
+        o->e[0] = (float)(sv->fx * t[0] + sv->cx * t[2]);
+        o->e[1] = (float)(sv->fy * t[1] + sv->cy * t[2]);
+        o->e[2] = (float)t[2];
     }
 }
 
@@ -325,25 +340,17 @@ void orc_window(const orc_params *p, const orc_images *img, int32_t x, int32_t y
     stats[2] = fmaxf(srr / sw - mean * mean, 0.0f);
 }
 
-/* PM:112-128.  H = Ks (R + (t/rho) n^T) Kr^-1, evaluated in the fixed order below. */
+/* PM:112-128.  H = Ks (R + (t/rho) n^T) Kr^-1 = G + (e / rho) m^T with m = Kr^-T n =
+ * (nx / fx, ny / fy, nz - nx cx / fx - ny cy / fy), evaluated in the fixed order below. */
 void orc_homography(const orc_params *p, int32_t s, const float n[3], float rho, float H[9]) {
     const orc_src *v = &p->src[s];
-    float t[3] = {v->t[0] / rho, v->t[1] / rho, v->t[2] / rho};
-    float M[9];
-    for (int i = 0; i < 3; i++)
-        for (int j = 0; j < 3; j++) M[i * 3 + j] = v->R[i * 3 + j] + t[i] * n[j];
-    float A[9];
-    for (int j = 0; j < 3; j++) {
-        A[j] = v->k[0] * M[j] + v->k[2] * M[6 + j];
-        A[3 + j] = v->k[1] * M[3 + j] + v->k[3] * M[6 + j];
-        A[6 + j] = M[6 + j];
-    }
     float ikx = 1.0f / p->k[0], iky = 1.0f / p->k[1];
     float kx2 = -p->k[2] / p->k[0], ky2 = -p->k[3] / p->k[1];
+    float irho = 1.0f / rho;
+    float m[3] = {n[0] * ikx, n[1] * iky, (n[0] * kx2 + n[1] * ky2) + n[2]};
     for (int i = 0; i < 3; i++) {
-        H[i * 3 + 0] = A[i * 3 + 0] * ikx;
-        H[i * 3 + 1] = A[i * 3 + 1] * iky;
-        H[i * 3 + 2] = (A[i * 3 + 0] * kx2 + A[i * 3 + 1] * ky2) + A[i * 3 + 2];
+        float f = v->e[i] * irho;
+        for (int j = 0; j < 3; j++) H[i * 3 + j] = fmaf(f, m[j], v->G[i * 3 + j]);
     }
 }
 
@@ -386,13 +393,15 @@ float orc_view_cost(const orc_params *p, const orc_images *img, int32_t s, const
         sss = fmaf(wv, val, sss);
         srs = fmaf(wv, values[i], srs);
     }
-    float sw = stats[0];
-    float mean = ss / sw;
-    float var_s = fmaxf(sss / sw - mean * mean, 0.0f);
+    /* PM:151-158 with the sums scaled by 1 / sw and the correlation by 1 / den (IEEE reciprocals;
+     * a product with a correctly rounded reciprocal is within WGSL's 2.5 ulp for division) */
+    float isw = 1.0f / stats[0];
+    float mean = ss * isw;
+    float var_s = fmaxf(sss * isw - mean * mean, 0.0f);
     float den = sqrtf(stats[2] * var_s);
     if (den < DEGENERATE_DEN) return 1.0f;
-    float cov = srs / sw - stats[1] * mean;
-    return clampf(1.0f - cov / den, 0.0f, 2.0f);
+    float cov = srs * isw - stats[1] * mean;
+    return clampf(1.0f - cov * (1.0f / den), 0.0f, 2.0f);
 }
 
 /* PM:161-190: forward-backward reprojection through source s's packed depth map.

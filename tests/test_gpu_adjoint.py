@@ -49,6 +49,33 @@ def seeded_pixels(cost, n=1024, seed=12):
     return np.sort(rng.choice(ok, size=min(n, ok.size), replace=False))
 
 
+def selection_gap(r, n_src):
+    """Gap between the largest selected view cost and the smallest unselected one of an FP64
+    result (infinite when every view is selected).  Where two views nearly tie, the FP32 forward
+    costs may select the other one, and the gradient belongs to another branch of the top-k."""
+    costs = np.array(r.view_cost[:n_src])
+    sel = np.array(r.selected[:n_src], dtype=bool)
+    if sel.all() or not sel.any():
+        return np.inf
+    return float(costs[~sel].min() - costs[sel].max())
+
+
+def clamp_gap(r, n_src):
+    """Smallest distance of a selected view's FP64 cost to the clamps at 0 and 2 (PM:158).  Near
+    them the FP32 cost of a highly correlated window (cancellation in the variance) can be clamped,
+    carrying no gradient, where the FP64 one is not."""
+    costs = np.array(r.view_cost[:n_src])
+    sel = np.array(r.selected[:n_src], dtype=bool)
+    return float(np.min(np.minimum(costs[sel], 2.0 - costs[sel]))) if sel.any() else np.inf
+
+
+def well_conditioned(r, cost, n_src):
+    """FP32 and FP64 costs agree, no tap within 1e-4 px of a texel edge, no near-tie in the top-k,
+    no selected view within 1e-3 of a clamp."""
+    return (abs(r.cost - cost) <= 1e-4 and r.min_kink >= 1e-4 and selection_gap(r, n_src) >= 1e-4
+            and clamp_gap(r, n_src) >= 1e-3)
+
+
 @pytest.fixture(scope="module")
 def reference(solved):
     """FP64 cost and gradient of the seeded pixels; the well-conditioned ones."""
@@ -60,7 +87,8 @@ def reference(solved):
         y, x = divmod(int(i), 320)
         h = hyp[i].astype(np.float64)
         res[int(i)] = O.cost_grad64(p, imgs, 0, x, y, h[:3], h[3])
-    clean = [i for i, r in res.items() if abs(r.cost - cost[i]) <= 1e-4 and r.min_kink >= 1e-4]
+    n_src = min(len(w.sources[0]), 8)
+    clean = [i for i, r in res.items() if well_conditioned(r, cost[i], n_src)]
     return imgs, res, np.array(clean)
 
 

@@ -127,6 +127,7 @@ struct lr3d_ctx {
     int64_t users = 0;      // live stereo sets, volumes, meshes and clouds made on this context
     bool closing = false;   // lr3d_ctx_destroy was called while users remained
     uint8_t *scratch = nullptr;  // this device's copy of `scratch` (device address)
+    DevBuf<float> wtab;          // bilateral weight table of the PatchMatch window
 
     hipEvent_t take_event() {
         if (!event_pool.empty()) {
@@ -199,6 +200,20 @@ static int32_t finish(lr3d_ctx *ctx) {
     return LR3D_OK;
 }
 
+// PM:90-110 bilateral weights of every (offset class, |dc|): the f32 argument -(dc dc) kColorK -
+// |o|^2 kSpaceK as the window writes it, exp in double, rounded to f32.
+static void window_weight_table(std::vector<float> &out) {
+    out.resize((size_t)kWindowClasses * 256);
+    for (int r = 0; r < kWindowClasses; r++) {
+        const float ds = (float)kWindowClassDist[r];
+        for (int a = 0; a < 256; a++) {
+            const float dc = (float)a;
+            const float arg = -(dc * dc) * kColorK - ds * kSpaceK;
+            out[(size_t)r * 256 + a] = (float)exp((double)arg);
+        }
+    }
+}
+
 extern "C" int32_t lr3d_ctx_create(int32_t device, lr3d_ctx **out) {
     if (!out) return fail(LR3D_ERR_INVALID, "lr3d_ctx_create: out is NULL");
     *out = nullptr;
@@ -221,6 +236,16 @@ extern "C" int32_t lr3d_ctx_create(int32_t device, lr3d_ctx **out) {
     if (e != hipSuccess) {
         delete ctx;
         return fail(LR3D_ERR_DEVICE, "lr3d_ctx_create: hipStreamCreate: %s", hipGetErrorString(e));
+    }
+    std::vector<float> table;
+    window_weight_table(table);
+    e = ctx->wtab.alloc(table.size());
+    if (e == hipSuccess)
+        e = hipMemcpy(ctx->wtab.ptr, table.data(), table.size() * sizeof(float), hipMemcpyHostToDevice);
+    if (e != hipSuccess) {
+        delete ctx;
+        return fail(e == hipErrorOutOfMemory ? LR3D_ERR_NOMEM : LR3D_ERR_DEVICE,
+                    "lr3d_ctx_create: weight table: %s", hipGetErrorString(e));
     }
     *out = ctx;
     return LR3D_OK;
@@ -702,6 +727,7 @@ static PmPools pools_of(lr3d_stereo *st) {
     p.filt = st->filt.ptr;
     p.prior = st->prior.ptr;
     p.coarse_hyp = st->coarse ? st->coarse->hyp.ptr : nullptr;
+    p.wtab = st->ctx->wtab.ptr;
     return p;
 }
 

@@ -25,6 +25,22 @@ namespace lr3d {
 constexpr int kMaxSources = 8;
 constexpr int kSamples = 36;
 
+// Bilateral window weights (PM:90-110) in u8 units: w = exp(-(dc^2) kColorK - |o|^2 kSpaceK) with
+// kColorK = 1 / (2 (0.06 * 255)^2) (PM:39) and kSpaceK = 1 / (2 * 5^2) (PM:40).  dc is an integer
+// in [-255, 255] and the squared tap offset |o|^2 one of six values, so every weight is an entry of
+// a kWindowClasses x 256 table (row = offset class, column = |dc|), formed once on the host: the
+// argument in f32 as written, exp in double, rounded to f32.
+constexpr float kColorK = (float)(1.0 / (2.0 * (0.06 * 255.0) * (0.06 * 255.0)));
+constexpr float kSpaceK = 0.02f;
+constexpr int kWindowClasses = 6;
+// squared offsets |o|^2 = ox^2 + oy^2 with |ox|, |oy| in {1, 3, 5}, in table-row order
+constexpr int kWindowClassDist[kWindowClasses] = {2, 10, 18, 26, 34, 50};
+// row of the class of tap offset (ox, oy)
+__host__ __device__ constexpr int window_class(int ox, int oy) {
+    const int d = ox * ox + oy * oy;
+    return d == 2 ? 0 : d == 10 ? 1 : d == 18 ? 2 : d == 26 ? 3 : d == 34 ? 4 : 5;
+}
+
 // One source view of a reference view's solve (the WGSL `View`, PM:4-10, after DS:345-353).
 struct PmSrc {
     float k[4];     // fx, fy, cx, cy
@@ -75,6 +91,7 @@ struct PmPools {
     uint32_t *filt;
     const float *prior;
     const float4 *coarse_hyp;
+    const float *wtab;  // bilateral weight table, kWindowClasses x 256
 };
 
 // Launch plan: view v owns logical blocks [block_start[v], block_start[v+1]).

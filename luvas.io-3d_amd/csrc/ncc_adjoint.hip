@@ -202,7 +202,7 @@ __global__ void __launch_bounds__(64) ncc_backward_kernel(PmPlan plan, PmPools p
     for (int s = 0; s < kMaxSources; s++) costs[s] = INFINITY;
     if (valid) {
         Window win;
-        prepare_window(pools.quad + d.qoff, d.W, d.H, d.pitch, x, y, win);
+        prepare_window(pools.quad + d.qoff, d.W, d.H, d.pitch, x, y, pools.wtab, win);
 #pragma unroll
         for (int i = 0; i < kSamples; i++) {
             lds.v[i][lane] = win.v[i];

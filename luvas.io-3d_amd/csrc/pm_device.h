@@ -24,9 +24,7 @@ constexpr float kGeometricCap = 3.0f;   // PM:42
 constexpr uint32_t kFilterCost = 7u;    // PM:43
 constexpr float kFilterPixels = 1.0f;   // PM:44
 constexpr float kFilterDepth = 0.01f;   // PM:45
-// PM:39-40 and PM:154 in u8 units: 1 / (2 (0.06 * 255)^2), 1 / (2 * 5^2), 1e-5 * 255^2.
-constexpr float kColorK = (float)(1.0 / (2.0 * (0.06 * 255.0) * (0.06 * 255.0)));
-constexpr float kSpaceK = 0.02f;
+// PM:154 in u8 units: 1e-5 * 255^2 (the window's colour and space constants are in lr3d_internal.h).
 constexpr float kDegenerateDen = (float)(1e-5 * 255.0 * 255.0);
 
 // ---- RNG (PM:56-65, PM:251-253) --------------------------------------------------------------
@@ -142,8 +140,11 @@ __device__ __forceinline__ float ref_pixel(const uint32_t *__restrict__ img, int
     return (float)(img[__umul24((uint32_t)y, (uint32_t)pitch) + (uint32_t)x] & 0xffu);
 }
 
+// wtab: the weight table (kWindowClasses x 256, see lr3d_internal.h): weight of tap i =
+// wtab[class(i) * 256 + |dc|], the value exp(-(dc^2) kColorK - |o|^2 kSpaceK) rounded to f32.
 __device__ __forceinline__ void prepare_window(const uint32_t *__restrict__ img, int32_t W, int32_t H,
-                                               int32_t pitch, int32_t x, int32_t y, Window &win) {
+                                               int32_t pitch, int32_t x, int32_t y,
+                                               const float *__restrict__ wtab, Window &win) {
     float center = ref_pixel(img, W, H, pitch, x, y);
     float sw = 0.0f, sr = 0.0f, srr = 0.0f;
 #pragma unroll
@@ -151,8 +152,7 @@ __device__ __forceinline__ void prepare_window(const uint32_t *__restrict__ img,
         const int ox = (i % 6) * 2 - 5, oy = (i / 6) * 2 - 5;
         float v = ref_pixel(img, W, H, pitch, x + ox, y + oy);
         float dc = v - center;
-        const float ds = (float)(ox * ox + oy * oy);
-        float w = (float)exp((double)(-(dc * dc) * kColorK - ds * kSpaceK));
+        float w = wtab[window_class(ox, oy) * 256 + (int32_t)fabsf(dc)];
         win.v[i] = v;
         win.w[i] = w;
         float wv = w * v;

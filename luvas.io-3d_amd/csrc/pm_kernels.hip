@@ -42,7 +42,7 @@ __global__ void __launch_bounds__(256) pm_init_kernel(PmPlan plan, PmPools pools
     const float px = (float)x, py = (float)y;
     uint32_t rng = pixel_rng(d, 0u, 0u, index);
     Window win;
-    prepare_window(pools.quad + d.qoff, d.W, d.H, d.pitch, x, y, win);
+    prepare_window(pools.quad + d.qoff, d.W, d.H, d.pitch, x, y, pools.wtab, win);
     const float rx = (px - d.k[2]) / d.k[0];
     const float ry = (py - d.k[3]) / d.k[1];
     float nx, ny, nz;
@@ -151,7 +151,7 @@ pm_propagate_kernel(PmPlan plan, PmPools pools,
     const float px = (float)x, py = (float)y;
     uint32_t rng = pixel_rng(d, iteration, phase, index);
     Window win;
-    prepare_window(pools.quad + d.qoff, W, H, d.pitch, x, y, win);
+    prepare_window(pools.quad + d.qoff, W, H, d.pitch, x, y, pools.wtab, win);
     const float rx = (px - d.k[2]) / d.k[0];
     const float ry = (py - d.k[3]) / d.k[1];
     const float *__restrict__ cost = pools.cost + d.off;

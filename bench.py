@@ -199,6 +199,7 @@ def main():
     vol.set_counting(True)
     vol.integrate_batched(st, list(range(n)), iviews, map=lr3d.MAP_FILTERED)
     updates = vol.update_count()
+    counts = vol.pass_counts()
     vol.set_counting(False)
 
     # accuracy of the last step against the analytic depth (filtered maps, as fused)
@@ -249,15 +250,21 @@ def main():
         t_avg = tsdf_ms / tsdf_launches / 1e3
         map_bytes = sum(v.width * v.height * 4 for v in work.views)
         bytes_alg = layout[2] * 512 * 16 + map_bytes
+        # bytes of the bricks the pass actually touches: every brick with a kept view is read
+        # (8 B per voxel), changed voxels are written (8 B), plus the maps
+        bytes_touched = counts["bricks_read"] * 512 * 8 + counts["voxels_written"] * 8 + map_bytes
+        traffic = measured.get("tsdf_integrate")
         tsdf.update({"avg_launch_ms": round(t_avg * 1e3, 3),
                      "voxel_upd_per_s": updates / t_avg,
-                     # the bytes a one-view-per-pass schedule moves for the same updates
-                     # (16 B per voxel update), per second of this pass
-                     "plain_schedule_gbs": round(updates * 16 / t_avg / 1e9, 1),
+                     "pass_counts": counts,
+                     "pairs_per_update": round(counts["pairs"] / max(updates, 1), 3),
                      "roofline": {"kernel": "tsdf_integrate", "bound": "hbm",
                                   "achieved": round(bytes_alg / t_avg / 1e9, 1), "peak": HBM_PEAK_GBS,
                                   "unit": "GB/s", "frac": round(bytes_alg / t_avg / 1e9 / HBM_PEAK_GBS, 4),
-                                  "bytes_per_launch": bytes_alg, "traffic": measured.get("tsdf_integrate")}})
+                                  "bytes_per_launch": bytes_alg, "traffic": traffic,
+                                  "frac_touched": round(bytes_touched / t_avg / 1e9 / HBM_PEAK_GBS, 4),
+                                  "bytes_touched": bytes_touched,
+                                  "frac_traffic": round(traffic / t_avg / 1e9 / HBM_PEAK_GBS, 4) if traffic else None}})
 
     # The NCC adjoint on the same batch (outside the step): every view's aggregated cost, dL/dplane
     # and relative-pose gradients in one launch (lr3d_ncc_refine_step with zero steps), timed with

@@ -280,6 +280,10 @@ int32_t lr3d_tsdf_read(lr3d_tsdf *ts, float *tsdf, float *weight);
 /* Voxel updates ((voxel, view) pairs written) of the integrate calls since the last reset. */
 int32_t lr3d_tsdf_set_counting(lr3d_tsdf *ts, int32_t enable);
 int32_t lr3d_tsdf_update_count(lr3d_tsdf *ts, int64_t *out);
+/* The same counting passes' totals: out[0] voxel updates, out[1] (voxel, view) pairs evaluated
+ * (512 per brick and view that the culling keeps), out[2] bricks read (at least one view kept),
+ * out[3] voxels written (changed by the pass).  No counterpart in the reference (measurement). */
+int32_t lr3d_tsdf_pass_counts(lr3d_tsdf *ts, int64_t out[4]);
 
 #ifdef __cplusplus
 }

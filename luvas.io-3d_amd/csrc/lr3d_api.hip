@@ -10,10 +10,12 @@
 
 #include <math.h>
 #include <stdarg.h>
+#include <stdlib.h>
 #include <stdio.h>
 #include <string.h>
 
 #include <algorithm>
+#include <atomic>
 #include <map>
 #include <memory>
 #include <mutex>
@@ -41,9 +43,31 @@ static int32_t fail(int32_t code, const char *fmt, ...) {
     return code;
 }
 
+// Fault injection, for tests of the error paths: with LR3D_FAULT_INJECT=k (k >= 1) in the
+// environment, the k-th HIP runtime call the library checks (counted over the process from 1)
+// reports a device failure, once, whatever its own status; the device is synchronised first, so
+// no work of the failing call is still in flight when it returns.  Unset or 0: no injection.
+static int64_t fault_inject_at() {
+    static const int64_t k = [] {
+        const char *v = getenv("LR3D_FAULT_INJECT");
+        return v ? (int64_t)atoll(v) : (int64_t)0;
+    }();
+    return k;
+}
+static std::atomic<int64_t> g_checked_calls{0};
+static bool fault_injected() {
+    const int64_t k = fault_inject_at();
+    if (k <= 0 || g_checked_calls.fetch_add(1) + 1 != k) return false;
+    (void)hipDeviceSynchronize();
+    return true;
+}
+
 #define HIP_TRY(expr)                                                                              \
     do {                                                                                           \
         hipError_t e_ = (expr);                                                                    \
+        if (fault_injected())                                                                      \
+            return fail(LR3D_ERR_DEVICE, "%s: injected fault (LR3D_FAULT_INJECT=%lld) (%s:%d)", #expr, \
+                        (long long)fault_inject_at(), __FILE__, __LINE__);                          \
         if (e_ != hipSuccess)                                                                      \
             return fail(e_ == hipErrorOutOfMemory ? LR3D_ERR_NOMEM : LR3D_ERR_DEVICE, "%s: %s (%s:%d)", \
                         #expr, hipGetErrorString(e_), __FILE__, __LINE__);                         \
@@ -219,6 +243,9 @@ extern "C" int32_t lr3d_ctx_create(int32_t device, lr3d_ctx **out) {
     *out = nullptr;
     int count = 0;
     hipError_t e = hipGetDeviceCount(&count);
+    if (fault_injected())
+        return fail(LR3D_ERR_DEVICE, "lr3d_ctx_create: hipGetDeviceCount: injected fault (LR3D_FAULT_INJECT=%lld)",
+                    (long long)fault_inject_at());
     if (e != hipSuccess || count == 0)
         return fail(LR3D_ERR_DEVICE, "lr3d_ctx_create: no HIP device (%s)",
                     e != hipSuccess ? hipGetErrorString(e) : "0 devices");
@@ -1379,7 +1406,7 @@ extern "C" int32_t lr3d_tsdf_create(lr3d_ctx *ctx, const double origin[3], doubl
     hipError_t e = ts->coords.alloc(n_bricks);
     if (e == hipSuccess) e = ts->tsdf.alloc(voxels);
     if (e == hipSuccess) e = ts->weight.alloc(voxels);
-    if (e == hipSuccess) e = ts->counter.alloc(1);
+    if (e == hipSuccess) e = ts->counter.alloc(4);
     if (e == hipSuccess)
         e = hipMemcpyAsync(ts->coords.ptr, coords, (size_t)n_bricks * 16, hipMemcpyHostToDevice, ctx->stream);
     if (e == hipSuccess) {
@@ -1388,7 +1415,7 @@ extern "C" int32_t lr3d_tsdf_create(lr3d_ctx *ctx, const double origin[3], doubl
         e = hipGetLastError();
     }
     if (e == hipSuccess) e = hipMemsetAsync(ts->weight.ptr, 0, voxels * 4, ctx->stream);
-    if (e == hipSuccess) e = hipMemsetAsync(ts->counter.ptr, 0, 8, ctx->stream);
+    if (e == hipSuccess) e = hipMemsetAsync(ts->counter.ptr, 0, 32, ctx->stream);
     if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
     if (e != hipSuccess) {
         delete ts;
@@ -2196,7 +2223,7 @@ extern "C" int32_t lr3d_tsdf_set_counting(lr3d_tsdf *ts, int32_t enable) {
     std::lock_guard<std::mutex> lock(ctx->mu);
     HIP_TRY(hipSetDevice(ctx->device));
     ts->counting = enable != 0;
-    HIP_TRY(hipMemsetAsync(ts->counter.ptr, 0, 8, ctx->stream));
+    HIP_TRY(hipMemsetAsync(ts->counter.ptr, 0, 32, ctx->stream));
     return finish(ctx);
 }
 
@@ -2210,5 +2237,18 @@ extern "C" int32_t lr3d_tsdf_update_count(lr3d_tsdf *ts, int64_t *out) {
     int32_t rc = finish(ctx);
     if (rc) return rc;
     *out = (int64_t)v;
+    return LR3D_OK;
+}
+
+extern "C" int32_t lr3d_tsdf_pass_counts(lr3d_tsdf *ts, int64_t out[4]) {
+    if (!ts || !out) return fail(LR3D_ERR_INVALID, "lr3d_tsdf_pass_counts: NULL argument");
+    lr3d_ctx *ctx = ts->ctx;
+    std::lock_guard<std::mutex> lock(ctx->mu);
+    HIP_TRY(hipSetDevice(ctx->device));
+    unsigned long long v[4] = {0, 0, 0, 0};
+    HIP_TRY(hipMemcpyAsync(v, ts->counter.ptr, 32, hipMemcpyDeviceToHost, ctx->stream));
+    int32_t rc = finish(ctx);
+    if (rc) return rc;
+    for (int i = 0; i < 4; i++) out[i] = (int64_t)v[i];
     return LR3D_OK;
 }

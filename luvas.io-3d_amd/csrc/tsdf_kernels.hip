@@ -311,6 +311,7 @@ __global__ void __launch_bounds__(256) integrate_kernel(TsdfGridDev g, const Tsd
     };
     int32_t v = next_view();
     if (v < 0) return;  // no view reaches this brick: nothing read, nothing written
+    uint32_t views_applied = 0;  // visible views of the brick (wave-uniform)
 
     float ts[8], ws[8];
 #pragma unroll
@@ -328,6 +329,7 @@ __global__ void __launch_bounds__(256) integrate_kernel(TsdfGridDev g, const Tsd
             vs[n] = next_view();
             if (vs[n] < 0) break;
         }
+        views_applied += (uint32_t)n;
         ViewGather grp[kViewGroup];
 #pragma unroll
         for (int k = 0; k < kViewGroup; k++)
@@ -345,9 +347,19 @@ __global__ void __launch_bounds__(256) integrate_kernel(TsdfGridDev g, const Tsd
         }
     }
     if (COUNT) {
-        // wavefront sum, one atomic per wavefront
-        for (int offset = 32; offset > 0; offset >>= 1) updates += __shfl_down(updates, offset, 64);
-        if (lane == 0 && updates) atomicAdd(counter, (unsigned long long)updates);
+        // wavefront sums, one atomic each: voxel updates, (voxel, view) pairs evaluated (512 per
+        // visible view of the brick), bricks read, voxels written
+        uint32_t written = (uint32_t)__builtin_popcount(changed);
+        for (int offset = 32; offset > 0; offset >>= 1) {
+            updates += __shfl_down(updates, offset, 64);
+            written += __shfl_down(written, offset, 64);
+        }
+        if (lane == 0) {
+            if (updates) atomicAdd(counter, (unsigned long long)updates);
+            atomicAdd(counter + 1, 512ull * views_applied);
+            atomicAdd(counter + 2, 1ull);
+            if (written) atomicAdd(counter + 3, (unsigned long long)written);
+        }
     }
 }
 

@@ -34,7 +34,7 @@ EXPORTED = (
     "lr3d_ncc_backward", "lr3d_ncc_refine_step",
     "lr3d_tsdf_create", "lr3d_tsdf_destroy", "lr3d_tsdf_integrate", "lr3d_tsdf_integrate_host",
     "lr3d_tsdf_integrate_batched", "lr3d_tsdf_read", "lr3d_tsdf_set_counting",
-    "lr3d_tsdf_update_count", "lr3d_tsdf_backward", "lr3d_stereo_autotune", "lr3d_stereo_get_tile",
+    "lr3d_tsdf_update_count", "lr3d_tsdf_pass_counts", "lr3d_tsdf_backward", "lr3d_stereo_autotune", "lr3d_stereo_get_tile",
     "lr3d_stereo_set_tile", "lr3d_tsdf_autotune", "lr3d_tsdf_get_bricks_per_block",
     "lr3d_tsdf_set_bricks_per_block", "lr3d_tsdf_set_depth_culling", "lr3d_tsdf_write", "lr3d_tsdf_extract_surface",
     "lr3d_mesh_remove_small_components", "lr3d_mesh_size", "lr3d_mesh_read", "lr3d_mesh_destroy",
@@ -141,6 +141,7 @@ def load() -> C.CDLL:
         "lr3d_tsdf_read": (i32, [P, C.c_void_p, C.c_void_p]),
         "lr3d_tsdf_set_counting": (i32, [P, i32]),
         "lr3d_tsdf_update_count": (i32, [P, C.POINTER(i64)]),
+        "lr3d_tsdf_pass_counts": (i32, [P, C.POINTER(i64)]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)

@@ -226,6 +226,13 @@ class HipTsdfVolume:
         N.check(self._lib.lr3d_tsdf_update_count(self.handle, C.byref(v)))
         return v.value
 
+    def pass_counts(self) -> dict:
+        """Totals of the counting passes: updates, (voxel, view) pairs evaluated, bricks read,
+        voxels written."""
+        v = (C.c_int64 * 4)()
+        N.check(self._lib.lr3d_tsdf_pass_counts(self.handle, v))
+        return {"updates": v[0], "pairs": v[1], "bricks_read": v[2], "voxels_written": v[3]}
+
     def destroy(self) -> None:
         if self.handle:
             self._lib.lr3d_tsdf_destroy(self.handle)

@@ -54,3 +54,49 @@ def test_context_without_device_fails_loudly():
         lr3d.Context(0)
     assert err.value.code == N.LR3D_ERR_DEVICE
     assert "no HIP device" in str(err.value)
+
+
+def _run_with_fault(k, code):
+    """Runs `code` in a fresh interpreter with LR3D_FAULT_INJECT=k (the variable is read once per
+    process, so each setting needs its own process)."""
+    import subprocess
+    import sys
+    env = dict(os.environ, LR3D_FAULT_INJECT=str(k))
+    pkg = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "luvas.io-3d_amd")
+    env["PYTHONPATH"] = pkg + os.pathsep + env.get("PYTHONPATH", "")
+    out = subprocess.run([sys.executable, "-c", code], env=env, capture_output=True, text=True, timeout=120)
+    assert out.returncode == 0, out.stderr
+    return out.stdout
+
+
+def test_injected_fault_is_reported_as_device_error():
+    # The first checked HIP call of a process is lr3d_ctx_create's device count, with or without a
+    # device: the injected failure surfaces as LR3D_ERR_DEVICE with its own message, and only once.
+    code = (
+        "import ctypes as C\n"
+        "from lr3d import _native as N\n"
+        "lib = N.load(); h = C.c_void_p()\n"
+        "print(lib.lr3d_ctx_create(0, C.byref(h)), lib.lr3d_last_error().decode())\n"
+        "rc = lib.lr3d_ctx_create(0, C.byref(h))\n"
+        "print(rc, lib.lr3d_last_error().decode() if rc else '')\n"
+        "if rc == 0: lib.lr3d_ctx_destroy(h)\n")
+    first, second = _run_with_fault(1, code).splitlines()
+    rc, msg = first.split(" ", 1)
+    assert int(rc) == N.LR3D_ERR_DEVICE
+    assert "injected fault (LR3D_FAULT_INJECT=1)" in msg and "hipGetDeviceCount" in msg
+    rc, msg = second.split(" ", 1) if " " in second else (second, "")
+    assert "injected" not in msg
+    assert int(rc) in (N.LR3D_OK, N.LR3D_ERR_DEVICE)
+
+
+def test_fault_injection_off_by_default():
+    code = (
+        "import ctypes as C\n"
+        "from lr3d import _native as N\n"
+        "lib = N.load(); h = C.c_void_p()\n"
+        "rc = lib.lr3d_ctx_create(0, C.byref(h))\n"
+        "print(rc, lib.lr3d_last_error().decode() if rc else '')\n"
+        "if rc == 0: lib.lr3d_ctx_destroy(h)\n")
+    for k in (0, -3):
+        out = _run_with_fault(k, code)
+        assert "injected" not in out

@@ -196,6 +196,35 @@ def test_tsdf_matches_oracle(ctx, work, oracle_maps):
     assert count == oupdates
 
 
+def test_tsdf_pass_counts(ctx, work, oracle_maps):
+    """The counting pass's totals of one batched pass into a fresh volume: updates equal the
+    oracle's, written voxels equal the voxels with weight > 0, and the evaluated pairs and bricks
+    read bound them (512 pairs per kept (brick, view); a brick is read when a view is kept)."""
+    say("case: TSDF pass counts, 8 oracle maps in one batched pass")
+    from lr3d import HipTsdfVolume
+    depths = oracle_maps[0]
+    grid, layout, views = _tsdf_setup(work)
+    st = make_stereo(ctx, work)
+    n = 320 * 240
+    for i in range(len(views)):
+        st.write_depth(i, depths[i * n:(i + 1) * n])
+    vol = HipTsdfVolume(ctx, grid, layout, work.tsdf.truncation_voxels)
+    vol.set_counting(True)
+    vol.integrate_batched(st, list(range(len(views))), views)
+    counts = vol.pass_counts()
+    tsdf, weight = vol.read()
+    vol.destroy()
+    st.destroy()
+    _, _, oupdates = _oracle_tsdf(work, grid, layout, depths, len(views))
+    touched = (weight.reshape(-1, 512) > 0).any(axis=1).sum()
+    say(f"tsdf counts: {counts}, voxels with weight {(weight > 0).sum()}, bricks touched {touched}")
+    assert counts["updates"] == oupdates
+    assert counts["voxels_written"] == int((weight > 0).sum())
+    assert counts["bricks_read"] >= touched and counts["bricks_read"] <= layout[2]
+    assert counts["pairs"] >= counts["updates"] and counts["pairs"] % 512 == 0
+    assert counts["pairs"] <= 512 * counts["bricks_read"] * len(views)
+
+
 def test_tsdf_batched_equals_sequential(ctx, work, oracle_maps):
     say("case: TSDF batched (8 views, one pass) vs sequential")
     from lr3d import HipTsdfVolume

@@ -14,7 +14,8 @@ from typing import Optional, Sequence
 import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(HERE, "liblr3d_oracle.so")
+# LR3D_ORACLE_LIB selects another build of the same sources (the sanitizer build, for one).
+LIB_PATH = os.environ.get("LR3D_ORACLE_LIB") or os.path.join(HERE, "liblr3d_oracle.so")
 MAX_SOURCES = 8
 
 

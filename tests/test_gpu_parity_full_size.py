@@ -1,6 +1,7 @@
 """GPU parity at full size: one 1280x720 view of the phone configuration (32 views, 8 sources,
-top-3, 8 iterations) and one 1920x1080 view of the mixed-resolution room batch (64 views, 4-10
-sources), each solved on the GPU inside a batch and by the CPU oracle on the same seeded inputs.
+top-3, 8 iterations), one 1920x1080 view of the mixed-resolution room batch (64 views, 4-10
+sources) and one view of the 200-view room (1280x720, 4-8 sources, 8 iterations), each solved on
+the GPU inside a batch and by the CPU oracle on the same seeded inputs.
 
 Bars as in test_gpu_parity.py: planes and costs bit for bit on >= 99.9 % of pixels, packed depths
 within 1e-3 relative on >= 99.5 % of valid pixels."""
@@ -57,5 +58,13 @@ def test_room_full_hd_view_matches_oracle(ctx):
     ref = 3  # 1920x1080
     assert (w.views[ref].width, w.views[ref].height) == (1920, 1080)
     say("case: room batch 64 views mixed resolution, batched solve of all views; view 3 (1920x1080) "
+        "against the oracle")
+    check_view(ctx, w, list(range(len(w.views))), ref)
+
+
+def test_room_large_view_matches_oracle(ctx):
+    w = scenes.room_large().render()
+    ref = 117
+    say("case: room 200 x 1280x720 (the bench workload), batched solve of all 200 views; view 117 "
         "against the oracle")
     check_view(ctx, w, list(range(len(w.views))), ref)

@@ -159,7 +159,8 @@ export declare const BRICK: 8;
 /** Options of refine / refinePyramid (defaults as in the Python host's lr3d.refine.refine). */
 export interface RefineOptions {
     steps?: number;
-    coarseSteps?: number;
+    /** refinePyramid: steps on each coarse copy (one count, or one per copy, finest first) */
+    coarseSteps?: number | number[];
     stepDepth?: number;
     stepNormal?: number;
     stepRot?: number;
@@ -173,7 +174,11 @@ export declare function refine(stereo: HipDenseStereo, solves: Array<[number, So
                                options?: RefineOptions): Promise<number[]>;
 /** Half-resolution copy of a stereo set (frames box-filtered 2 x 2) for refinePyramid. */
 export declare function halfResolution(stereo: HipDenseStereo, images: Uint8Array[]): HipDenseStereo;
-/** Refinement coarse to fine: coarseSteps on `coarse`, then steps on `stereo`. */
-export declare function refinePyramid(stereo: HipDenseStereo, coarse: HipDenseStereo,
+/** `levels` (default 3) coarse copies of a stereo set, each half the resolution of the one before,
+ * finest first, for refinePyramid. */
+export declare function pyramidLevels(stereo: HipDenseStereo, images: Uint8Array[], levels?: number): HipDenseStereo[];
+/** Refinement coarse to fine: coarseSteps on each coarse copy (coarsest first), then steps on
+ * `stereo`.  Histories: one per coarse copy (coarsest first) when `coarse` is an array. */
+export declare function refinePyramid(stereo: HipDenseStereo, coarse: HipDenseStereo | HipDenseStereo[],
                                       solves: Array<[number, SolveOptions]>, options?: RefineOptions):
-    Promise<{ coarse: number[]; fine: number[] }>;
+    Promise<{ coarse: number[] | number[][]; fine: number[] }>;

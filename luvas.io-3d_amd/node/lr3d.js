@@ -530,7 +530,7 @@ const sum = (a) => a.reduce((x, y) => x + y, 0);
  */
 async function refine(stereo, solves, options) {
     const o = Object.assign({ steps: 50, stepDepth: 2e-4, stepNormal: 2e-3, stepRot: 1e-3, stepTrans: 1e-3,
-        fixed: [0], momentum: 0.7 }, options || {});
+        fixed: [0], momentum: 0.9 }, options || {});
     let rot = o.stepRot, trans = o.stepTrans;
     let r = await stereo.refineStep(solves, o.stepDepth, o.stepNormal);
     const history = [sum(r.lossBefore)];
@@ -573,47 +573,86 @@ async function refine(stereo, solves, options) {
     return history;
 }
 
-/**
- * A half-resolution copy of a stereo set for the coarse stage of refinePyramid: frames box-filtered
- * 2 x 2 (rounded), intrinsics scaled by 0.5 with pixel centres aligned (camera.ts:28-36).
- */
-function halfResolution(stereo, images) {
-    const views = stereo.views.map((v, i) => {
-        const c = stereo.cameras[i];
-        const w = Math.floor(c.width / 2), h = Math.floor(c.height / 2);
-        const sx = w / c.width, sy = h / c.height, k = v.intrinsics;
-        return { intrinsics: { fx: k.fx * sx, fy: k.fy * sy, cx: (k.cx + 0.5) * sx - 0.5, cy: (k.cy + 0.5) * sy - 0.5 },
-                 pose: v.pose, width: w, height: h };
-    });
-    const coarse = new HipDenseStereo(stereo.ctx, views[0].width, views[0].height, views);
-    images.forEach((im, i) => {
-        const W = stereo.cameras[i].width, w = views[i].width, h = views[i].height;
-        const out = new Uint8Array(w * h);
+/** Every view's frame box-filtered 2 x 2 (rounded), intrinsics scaled by 0.5 with pixel centres
+ * aligned (camera.ts:28-36), the same poses. */
+function halfFrames(views, sizes, images) {
+    const outViews = [], frames = [];
+    views.forEach((v, i) => {
+        const W = sizes[i].width, H = sizes[i].height;
+        const w = Math.floor(W / 2), h = Math.floor(H / 2);
+        const sx = w / W, sy = h / H, k = v.intrinsics;
+        outViews.push({ intrinsics: { fx: k.fx * sx, fy: k.fy * sy, cx: (k.cx + 0.5) * sx - 0.5, cy: (k.cy + 0.5) * sy - 0.5 },
+                        pose: v.pose, width: w, height: h });
+        const im = images[i], out = new Uint8Array(w * h);
         for (let y = 0; y < h; y++)
             for (let x = 0; x < w; x++) {
                 const a = (2 * y) * W + 2 * x;
                 out[y * w + x] = (im[a] + im[a + W] + im[a + 1] + im[a + W + 1] + 2) >> 2;
             }
-        coarse.setImage(i, out);
+        frames.push(out);
     });
-    return coarse;
+    return { views: outViews, frames };
 }
 
-/** Coarse to fine: options.coarseSteps steps of refine on `coarse`, then options.steps on `stereo`. */
+function stereoOf(ctx, views, frames) {
+    const st = new HipDenseStereo(ctx, views[0].width, views[0].height, views);
+    frames.forEach((f, i) => st.setImage(i, f));
+    return st;
+}
+
+/**
+ * A half-resolution copy of a stereo set for the coarse stage of refinePyramid: frames box-filtered
+ * 2 x 2 (rounded), intrinsics scaled by 0.5 with pixel centres aligned (camera.ts:28-36).
+ */
+function halfResolution(stereo, images) {
+    const h = halfFrames(stereo.views, stereo.cameras, images);
+    return stereoOf(stereo.ctx, h.views, h.frames);
+}
+
+/** `levels` coarse copies of a stereo set, each half the resolution of the one before (finest first). */
+function pyramidLevels(stereo, images, levels) {
+    let views = stereo.views, sizes = stereo.cameras, frames = images;
+    const out = [];
+    for (let l = 0; l < (levels === undefined ? 3 : levels); l++) {
+        const h = halfFrames(views, sizes, frames);
+        views = h.views;
+        sizes = h.views;
+        frames = h.frames;
+        out.push(stereoOf(stereo.ctx, h.views, h.frames));
+    }
+    return out;
+}
+
+/**
+ * Coarse to fine, as the Python host's lr3d.refine.refine_pyramid: `coarse` is one coarse copy or an
+ * array of them (finest first), options.coarseSteps a count or one per copy; from the coarsest copy
+ * up each level starts from the poses the level below reached, then options.steps steps run on
+ * `stereo`.  Returns { coarse: histories (coarsest first; one history for a single copy), fine }.
+ */
 async function refinePyramid(stereo, coarse, solves, options) {
-    const o = Object.assign({ coarseSteps: 25, steps: 25, fixed: [0] }, options || {});
-    for (let layer = 0; layer < stereo.views.length; layer++) coarse.setPose(layer, stereo.views[layer].pose);
-    await coarse.solveBatched(solves);
-    const hc = await refine(coarse, solves, Object.assign({}, o, { steps: o.coarseSteps }));
+    const o = Object.assign({ coarseSteps: [50, 50, 50], steps: 25, fixed: [0] }, options || {});
+    const single = !Array.isArray(coarse);
+    const levels = single ? [coarse] : coarse;
+    const counts = Array.isArray(o.coarseSteps) ? o.coarseSteps : levels.map(() => o.coarseSteps);
+    if (counts.length !== levels.length) throw new RangeError('refinePyramid: one coarseSteps entry per coarse level');
+    let poses = stereo.views.map((v) => v.pose);
+    const histories = [];
+    for (let l = levels.length - 1; l >= 0; l--) {
+        const level = levels[l];
+        for (let layer = 0; layer < stereo.views.length; layer++) level.setPose(layer, poses[layer]);
+        await level.solveBatched(solves);
+        histories.push(await refine(level, solves, Object.assign({}, o, { steps: counts[l] })));
+        poses = level.views.map((v) => v.pose);
+    }
     for (let layer = 0; layer < stereo.views.length; layer++)
-        if (!o.fixed.includes(layer)) stereo.setPose(layer, coarse.views[layer].pose);
+        if (!o.fixed.includes(layer)) stereo.setPose(layer, poses[layer]);
     await stereo.solveBatched(solves);
     const hf = await refine(stereo, solves, o);
-    return { coarse: hc, fine: hf };
+    return { coarse: single ? histories[0] : histories, fine: hf };
 }
 
 module.exports = {
     Context, HipDenseStereo, HipTsdfVolume, BrickGrid, STEREO_METHODS, MAP_RAW, MAP_FILTERED,
     MAX_SOURCES, BRICK, NccCostFunction, TsdfIntegrationFunction, FilteredMap, native,
-    refine, refinePyramid, halfResolution, worldPoseGradients, perturbPose, expSo3,
+    refine, refinePyramid, halfResolution, pyramidLevels, worldPoseGradients, perturbPose, expSo3,
 };

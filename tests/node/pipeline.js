@@ -69,6 +69,12 @@ async function main() {
         maxDepth: job.ranges[i][1], prior: null, geometric: false, iterations: job.iterations }]);
     const history = await lr3d.refine(run.stereo, solves, { steps: 5 });
     fs.writeFileSync(path.join(dir, 'refine.json'), JSON.stringify(history));
+    // the coarse-to-fine driver over two coarse copies (half and quarter resolution)
+    const grays = await Promise.all(job.views.map((v, i) => input.gray(i)));
+    const levels = lr3d.pyramidLevels(run.stereo, grays, 2);
+    const pyramid = await lr3d.refinePyramid(run.stereo, levels, solves, { coarseSteps: [2, 2], steps: 2 });
+    fs.writeFileSync(path.join(dir, 'refine_pyramid.json'), JSON.stringify(pyramid));
+    levels.forEach((l) => l.destroy());
     volume.destroy();
     run.destroy();
     ctx.destroy();

@@ -147,6 +147,10 @@ def test_node_pipeline_matches_python_host(tmp_path, ctx):
     hist = json.loads((tmp_path / "refine.json").read_text())
     assert len(hist) == 6 and np.isfinite(hist).all()
     assert all(b <= a for a, b in zip(hist, hist[1:])) and hist[-1] < hist[0]
+    pyr = json.loads((tmp_path / "refine_pyramid.json").read_text())
+    assert len(pyr["coarse"]) == 2
+    for h in pyr["coarse"] + [pyr["fine"]]:
+        assert len(h) == 3 and np.isfinite(h).all() and all(b <= a for a, b in zip(h, h[1:]))
     vol.destroy()
     st.destroy()
 

@@ -26,6 +26,9 @@ constexpr float kFilterPixels = 1.0f;   // PM:44
 constexpr float kFilterDepth = 0.01f;   // PM:45
 // PM:154 in u8 units: 1e-5 * 255^2 (the window's colour and space constants are in lr3d_internal.h).
 constexpr float kDegenerateDen = (float)(1e-5 * 255.0 * 255.0);
+// The smallest float x with sqrtf(x) >= kDegenerateDen (IEEE square root is monotone, so
+// sqrtf(x) < kDegenerateDen exactly when x < kDegenerateSq).
+constexpr float kDegenerateSq = 0x1.b0f90ep-2f;
 
 // ---- RNG (PM:56-65, PM:251-253) --------------------------------------------------------------
 
@@ -218,6 +221,17 @@ __device__ __forceinline__ float rcp_exact(float z) {
     return fmaf(fmaf(-z, r, 1.0f), r, r);
 }
 
+// IEEE sqrtf(x) for normal x >= 2^-96: v_sqrt_f32 is within 1 ulp, and the neighbour whose square
+// brackets x is selected with two FMA residuals (the compiler's correctly rounded expansion,
+// without its rescaling of tiny inputs and its zero/inf/NaN fix-ups, which this range never needs).
+__device__ __forceinline__ float sqrt_normal(float x) {
+    const float s = __builtin_amdgcn_sqrtf(x);
+    const float below = __int_as_float(__float_as_int(s) - 1);
+    const float above = __int_as_float(__float_as_int(s) + 1);
+    const float r = fmaf(-below, s, x) <= 0.0f ? below : s;
+    return fmaf(-above, s, x) > 0.0f ? above : r;
+}
+
 template <bool FAST>
 __device__ __forceinline__ void view_taps(const uint32_t *__restrict__ img, int32_t W, int32_t H,
                                           int32_t pitch, const Mat3 &Hm, float px, float py,
@@ -333,10 +347,11 @@ __device__ __forceinline__ int classify_window(const PmSrc &s, const Mat3 &H, fl
                                                float cu[4], float cv[4]) {
     const float wmax = (float)s.W - 1.0f, hmax = (float)s.H - 1.0f;
     const float ax = fabsf(px) + 5.0f, ay = fabsf(py) + 5.0f;
-    // bounds on the magnitude of the terms of q.x, q.y, q.z over the window
-    const float bx = fabsf(H.m[0]) * ax + fabsf(H.m[1]) * ay + fabsf(H.m[2]);
-    const float by = fabsf(H.m[3]) * ax + fabsf(H.m[4]) * ay + fabsf(H.m[5]);
-    const float bz = fabsf(H.m[6]) * ax + fabsf(H.m[7]) * ay + fabsf(H.m[8]);
+    // bounds on the magnitude of the terms of q.x, q.y, q.z over the window (each within a few
+    // units of 2^-24 of its exact value, far inside the 2^-20 slack the margins below carry)
+    const float bx = fmaf(fabsf(H.m[0]), ax, fmaf(fabsf(H.m[1]), ay, fabsf(H.m[2])));
+    const float by = fmaf(fabsf(H.m[3]), ax, fmaf(fabsf(H.m[4]), ay, fabsf(H.m[5])));
+    const float bz = fmaf(fabsf(H.m[6]), ax, fmaf(fabsf(H.m[7]), ay, fabsf(H.m[8])));
     // Past this, rcp_exact may differ from 1 / q.z; below it every q component is finite, and so is
     // every u, v of a corner with q.z > 1e-6 (no NaN), so a corner is invalid exactly when the
     // minimum or maximum over the corners fails a test.
@@ -367,14 +382,16 @@ __device__ __forceinline__ int classify_window(const PmSrc &s, const Mat3 &H, fl
     // computed u = q.x / q.z then lies within eu of the exact ratio; require corners inside
     // by 4 eu so that exact and computed interior taps stay inside.
     const float ez = bz * 0x1p-20f;
-    const float zlow = zmin - 2.0f * ez;
+    const float zlow = fmaf(-2.0f, ez, zmin);
     if (!(zlow > 2e-6f)) return kCareful;
     // an upper bound of 1 / zlow: v_rcp_f32 is within 1 ulp, the factor covers it and its rounding
     const float ilow = __builtin_amdgcn_rcpf(zlow) * 1.000001f;
-    const float eu = (bx * 0x1p-20f + (wmax + 1.0f) * ez) * ilow + (wmax + 1.0f) * 0x1p-20f;
-    const float ev = (by * 0x1p-20f + (hmax + 1.0f) * ez) * ilow + (hmax + 1.0f) * 0x1p-20f;
-    const bool inside = umin >= 4.0f * eu && umax <= wmax - 4.0f * eu && vmin >= 4.0f * ev &&
-                        vmax <= hmax - 4.0f * ev;
+    // 4 eu and 4 ev, eu = (bx 2^-20 + W ez) / zlow + W 2^-20 (W = wmax + 1), likewise ev
+    const float ez4 = bz * 0x1p-18f;
+    const float w1 = wmax + 1.0f, h1 = hmax + 1.0f;
+    const float eu4 = fmaf(fmaf(bx, 0x1p-18f, w1 * ez4), ilow, w1 * 0x1p-18f);
+    const float ev4 = fmaf(fmaf(by, 0x1p-18f, h1 * ez4), ilow, h1 * 0x1p-18f);
+    const bool inside = umin >= eu4 && umax <= wmax - eu4 && vmin >= ev4 && vmax <= hmax - ev4;
     return inside ? kInside : kCareful;
 }
 
@@ -403,8 +420,11 @@ __device__ __forceinline__ float view_cost(const PmSrc &s, const Mat3 &H, float 
     // reciprocals; den >= kDegenerateDen lies in the range where rcp_exact is the IEEE reciprocal)
     float mean = ss * win.isw;
     float var_s = fmaxf(sss * win.isw - mean * mean, 0.0f);
-    float den = sqrtf(win.var * var_s);
-    if (den < kDegenerateDen) return 1.0f;
+    // den = sqrtf(var_r * var_s) and the test den < kDegenerateDen, with the test taken on the
+    // square (finite and >= 0 here) so that the square root only sees x >= kDegenerateSq
+    const float x = win.var * var_s;
+    if (!(x >= kDegenerateSq)) return 1.0f;
+    const float den = sqrt_normal(x);
     float cov = srs * win.isw - win.mean * mean;
     return clampf(1.0f - cov * rcp_exact(den), 0.0f, 2.0f);
 }

@@ -3,6 +3,7 @@
 (PM:192-233) — by special cases with known answers and by an FP64 brute force in the reference's
 own units (r8unorm values, sigma_colour 0.06, variance floor 1e-5)."""
 import math
+import os
 
 import numpy as np
 import pytest
@@ -239,3 +240,21 @@ def test_hypothesis_validity_rules():
     assert O.hypothesis_cost(p, imgs, 30, 24, win, facing, 3.5) == 2.0   # beyond max depth
     assert O.hypothesis_cost(p, imgs, 30, 24, win, [0.0, 0.0, 1.0], 1.0) == 2.0  # faces away
     assert O.hypothesis_cost(p, imgs, 30, 24, win, facing, 1.0) < 2.0
+
+
+def test_degenerate_threshold_on_the_square_is_exact():
+    """The kernels test den < 1e-5 * 255^2 (den = sqrtf(var_r * var_s)) as var_r * var_s < T before
+    taking the square root; T must be the smallest float whose IEEE square root reaches the floor,
+    so that the two tests agree on every float."""
+    import re
+    path = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))),
+                        "luvas.io-3d_amd", "csrc", "pm_device.h")
+    text = open(path).read()
+    T = np.float32(float.fromhex(re.search(r"kDegenerateSq = (0x[0-9a-fp.+-]+)f;", text).group(1)))
+    floor = np.float32(1e-5 * 255.0 * 255.0)
+    below = np.nextafter(T, np.float32(0))
+    assert np.sqrt(T) >= floor and np.sqrt(below) < floor
+    # and on a band of floats around it, the two tests agree
+    bits = T.view(np.uint32) + np.arange(-4096, 4096, dtype=np.int64)
+    xs = bits.astype(np.uint32).view(np.float32)
+    assert np.array_equal(np.sqrt(xs) < floor, xs < T)

@@ -347,7 +347,7 @@ struct lr3d_stereo {
     std::vector<lr3d_view> views;
     std::vector<uint32_t> off;  // pixel offset per layer
     size_t total_px = 0;
-    std::vector<uint32_t> qoff;  // texel offset per layer in the quad pool (rows quad_pitch(W) apart)
+    std::vector<uint32_t> qoff;  // texel offset per layer in the quad pool (quad_texels per layer)
     size_t total_q = 0;
     std::vector<uint8_t> has_image;
     std::vector<uint8_t> solved;
@@ -391,7 +391,7 @@ extern "C" int32_t lr3d_stereo_create(lr3d_ctx *ctx, const lr3d_view *views, int
         off[i] = (uint32_t)total;
         qoff[i] = (uint32_t)total_q;
         total += (size_t)views[i].width * (size_t)views[i].height;
-        total_q += (size_t)quad_pitch(views[i].width) * (size_t)views[i].height;
+        total_q += quad_texels(views[i].width, views[i].height);
         if (total_q >= (1ull << 32))
             return fail(LR3D_ERR_INVALID, "lr3d_stereo_create: more than 2^32 pixels in one set");
     }
@@ -478,7 +478,7 @@ extern "C" int32_t lr3d_stereo_set_image(lr3d_stereo *st, int32_t layer, const u
     HIP_TRY(st->staging.reserve(px));
     HIP_TRY(hipMemcpyAsync(st->staging.ptr, gray, px, hipMemcpyHostToDevice, ctx->stream));
     HIP_TRY(timed(ctx, "pm_build_quad", [&] {
-        return pm_build_quad(ctx->stream, st->staging.ptr, width, height, quad_pitch(width),
+        return pm_build_quad(ctx->stream, st->staging.ptr, width, height, quad_rows(height),
                              st->quad.ptr + st->qoff[layer]);
     }));
     st->has_image[layer] = 1;
@@ -570,7 +570,7 @@ static int32_t build_desc(const lr3d_stereo *st, const lr3d_solve_desc &sd, uint
     d.iterations = sd.iterations;
     d.seed_base = (uint32_t)(sd.ref + 1) * 0x9e3779b1u + d.flags * 7u;  // DS:333
     d.off = st->off[sd.ref];
-    d.pitch = quad_pitch(rv.width);
+    d.qrows = quad_rows(rv.height);
     d.qoff = st->qoff[sd.ref];
     d.has_prior = sd.prior != nullptr;
     d.prior_off = prior_off;
@@ -608,7 +608,7 @@ static int32_t build_desc(const lr3d_stereo *st, const lr3d_solve_desc &sd, uint
         o.H = sv.height;
         o.off = st->off[layer];
         o.layer = layer;
-        o.pitch = quad_pitch(sv.width);
+        o.qrows = quad_rows(sv.height);
         o.qoff = st->qoff[layer];
     }
     return LR3D_OK;

@@ -3,10 +3,12 @@
 // Device data layout (all views of a stereo set share one pool per kind, each layer at its own
 // pixel offset `off[layer]` = sum of the previous layers' width*height):
 //   quad   u32 per pixel: the 2x2 block I(x,y) | I(x+1,y)<<8 | I(x,y+1)<<16 | I(x+1,y+1)<<24,
-//          clamped to the image, so one 32-bit gather feeds one bilinear tap.  Rows are `pitch`
-//          texels apart, an odd number of 128-byte lines (quad_pitch), and layers sit at their
-//          own offsets `qoff[layer]`: a gather spans ~8 rows, and with a pitch of an even number
-//          of lines (1280 px = 40 lines) all of them would fall in the same L1 tag bank;
+//          clamped to the image, so one 32-bit gather feeds one bilinear tap.  A layer (at its
+//          own offset `qoff[layer]`, a whole number of 128-byte lines) is stored in column blocks
+//          of 16 texels: block b holds columns 16b..16b+15 of every row, row after row (64 bytes
+//          per row, `qrows` rows per block), so one 128-byte line holds a 16 x 2 tile and the
+//          rows of the ~16 x 8 source patch that one tap of a wavefront gathers sit in
+//          consecutive lines (quad_index);
 //   hyp    float4 per pixel (normal xyz, depth), cost f32 per pixel;
 //   depth  packed u32 per pixel (f16 depth | 6+6-bit octahedral normal | 4-bit cost);
 //   filt   packed u32 per pixel (consistency-filtered depth);
@@ -49,7 +51,7 @@ struct PmSrc {
     int32_t W, H;
     uint32_t off;   // pixel offset of the source layer in the per-pixel pools
     int32_t layer;
-    int32_t pitch;  // texels per row of the layer in the quad pool
+    int32_t qrows;  // rows per column block of the layer in the quad pool (quad_rows)
     uint32_t qoff;  // texel offset of the layer in the quad pool
     // The plane homography H = Ks (R + t n^T / rho) Kr^-1 (PM:116-128) split into its parts that do
     // not depend on the plane, formed in double on the host and rounded: G = Ks R Kr^-1, e = Ks t;
@@ -76,7 +78,7 @@ struct PmDesc {
     uint32_t coarse_off;       // pixel offset of this layer in the coarse stereo's hyp pool
     float ck[4];               // coarse intrinsics (scaleIntrinsics, CAM:28-36)
     int32_t tiles_x;           // tiles per row for the launch's tile shape (set per launch plan)
-    int32_t pitch;             // quad-pool row pitch and offset of the reference layer
+    int32_t qrows;             // quad-pool block rows and offset of the reference layer
     uint32_t qoff;
     int32_t pad_;
     PmSrc src[kMaxSources];
@@ -152,13 +154,17 @@ struct TsdfGridDev {
 };
 
 // Kernel launchers (stream-ordered; no synchronisation).
-hipError_t pm_build_quad(hipStream_t s, const uint8_t *gray, int32_t W, int32_t H, int32_t pitch, uint32_t *quad);
+hipError_t pm_build_quad(hipStream_t s, const uint8_t *gray, int32_t W, int32_t H, int32_t qrows, uint32_t *quad);
 
-// Row pitch (texels) of a W-wide layer in the quad pool: whole 128-byte lines, an odd number.
-inline int32_t quad_pitch(int32_t W) {
-    const int32_t lines = (W + 31) / 32;
-    return ((lines & 1) ? lines : lines + 1) * 32;
+// Rows per column block of an H-tall layer in the quad pool: H rounded up so that a block is an
+// odd number of 128-byte lines (qrows / 2 odd), so that the blocks a gather straddles start in
+// different L1 tag banks.
+__host__ __device__ inline int32_t quad_rows(int32_t H) {
+    const int32_t lines = (H + 1) / 2;
+    return ((lines & 1) ? lines : lines + 1) * 2;
 }
+// Texels of a W x H layer in the quad pool (whole column blocks; a multiple of 32 texels).
+__host__ __device__ inline size_t quad_texels(int32_t W, int32_t H) { return (size_t)((W + 15) / 16) * 16 * (size_t)quad_rows(H); }
 hipError_t pm_launch_init(hipStream_t s, const PmPlan &plan, const PmPools &pools);
 hipError_t pm_launch_propagate(hipStream_t s, const PmPlan &plan, const PmPools &pools,
                                uint32_t iteration, uint32_t phase, float step);

@@ -91,7 +91,7 @@ struct Tap {
     double u, v, rz, val, gu, gv;
 };
 
-__device__ __forceinline__ Tap source_tap(const uint32_t *__restrict__ img, int32_t pitch, double wmax, double hmax,
+__device__ __forceinline__ Tap source_tap(const uint32_t *__restrict__ img, int32_t qrows, double wmax, double hmax,
                                           const Mat3d &H, double X, double Y) {
     const double qx = fma(H.m[0], X, fma(H.m[1], Y, H.m[2]));
     const double qy = fma(H.m[3], X, fma(H.m[4], Y, H.m[5]));
@@ -102,7 +102,7 @@ __device__ __forceinline__ Tap source_tap(const uint32_t *__restrict__ img, int3
     t.v = fmin(fmax(qy * t.rz, 0.0), hmax);
     const double x0 = floor(t.u), y0 = floor(t.v);
     const double fx = t.u - x0, fy = t.v - y0;
-    const uint32_t q = img[__umul24((uint32_t)y0, (uint32_t)pitch) + (uint32_t)x0];
+    const uint32_t q = img[quad_index((uint32_t)x0, (uint32_t)y0, (uint32_t)qrows)];
     const double a = (double)(q & 0xffu), b = (double)((q >> 8) & 0xffu);
     const double c = (double)((q >> 16) & 0xffu), e = (double)(q >> 24);
     const double top = fma(fx, b - a, a), bot = fma(fx, e - c, c);
@@ -124,7 +124,7 @@ __device__ __forceinline__ void view_grad_h(const PmSrc &s, const Mat3d &H, doub
 #pragma unroll 1
     for (int i = 0; i < kSamples; i++) {
         const double X = px + (double)((i % 6) * 2 - 5), Y = py + (double)((i / 6) * 2 - 5);
-        const Tap t = source_tap(img, s.pitch, wmax, hmax, H, X, Y);
+        const Tap t = source_tap(img, s.qrows, wmax, hmax, H, X, Y);
         const double wv = (double)win.w[i][lane] * t.val;
         ss += wv;
         sss = fma(wv, t.val, sss);
@@ -142,7 +142,7 @@ __device__ __forceinline__ void view_grad_h(const PmSrc &s, const Mat3d &H, doub
 #pragma unroll 1
     for (int i = 0; i < kSamples; i++) {
         const double X = px + (double)((i % 6) * 2 - 5), Y = py + (double)((i / 6) * 2 - 5);
-        const Tap t = source_tap(img, s.pitch, wmax, hmax, H, X, Y);
+        const Tap t = source_tap(img, s.qrows, wmax, hmax, H, X, Y);
         const double dval = (double)win.w[i][lane] * inv_sw *
                             (B * (t.val - mean) - ((double)win.v[i][lane] - rs.mean) * A);
         const double gx = dval * t.gu * t.rz, gy = dval * t.gv * t.rz;
@@ -202,7 +202,7 @@ __global__ void __launch_bounds__(64) ncc_backward_kernel(PmPlan plan, PmPools p
     for (int s = 0; s < kMaxSources; s++) costs[s] = INFINITY;
     if (valid) {
         Window win;
-        prepare_window(pools.quad + d.qoff, d.W, d.H, d.pitch, x, y, pools.wtab, win);
+        prepare_window(pools.quad + d.qoff, d.W, d.H, d.qrows, x, y, pools.wtab, win);
 #pragma unroll
         for (int i = 0; i < kSamples; i++) {
             lds.v[i][lane] = win.v[i];

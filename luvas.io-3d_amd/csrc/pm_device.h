@@ -136,24 +136,30 @@ struct Window {
     float isw;  // 1 / sw
 };
 
+// Texel (x, y) of a layer in the quad pool: column block x / 16, row y within it, column x % 16
+// (see lr3d_internal.h).
+__device__ __forceinline__ uint32_t quad_index(uint32_t x, uint32_t y, uint32_t qrows) {
+    return ((__umul24(x >> 4, qrows) + y) << 4) | (x & 15u);
+}
+
 __device__ __forceinline__ float ref_pixel(const uint32_t *__restrict__ img, int32_t W, int32_t H,
-                                           int32_t pitch, int32_t x, int32_t y) {
+                                           int32_t qrows, int32_t x, int32_t y) {
     x = min(max(x, 0), W - 1);
     y = min(max(y, 0), H - 1);
-    return (float)(img[__umul24((uint32_t)y, (uint32_t)pitch) + (uint32_t)x] & 0xffu);
+    return (float)(img[quad_index((uint32_t)x, (uint32_t)y, (uint32_t)qrows)] & 0xffu);
 }
 
 // wtab: the weight table (kWindowClasses x 256, see lr3d_internal.h): weight of tap i =
 // wtab[class(i) * 256 + |dc|], the value exp(-(dc^2) kColorK - |o|^2 kSpaceK) rounded to f32.
 __device__ __forceinline__ void prepare_window(const uint32_t *__restrict__ img, int32_t W, int32_t H,
-                                               int32_t pitch, int32_t x, int32_t y,
+                                               int32_t qrows, int32_t x, int32_t y,
                                                const float *__restrict__ wtab, Window &win) {
-    float center = ref_pixel(img, W, H, pitch, x, y);
+    float center = ref_pixel(img, W, H, qrows, x, y);
     float sw = 0.0f, sr = 0.0f, srr = 0.0f;
 #pragma unroll
     for (int i = 0; i < kSamples; i++) {
         const int ox = (i % 6) * 2 - 5, oy = (i / 6) * 2 - 5;
-        float v = ref_pixel(img, W, H, pitch, x + ox, y + oy);
+        float v = ref_pixel(img, W, H, qrows, x + ox, y + oy);
         float dc = v - center;
         float w = wtab[window_class(ox, oy) * 256 + (int32_t)fabsf(dc)];
         win.v[i] = v;
@@ -234,11 +240,10 @@ __device__ __forceinline__ float sqrt_normal(float x) {
 
 template <bool FAST>
 __device__ __forceinline__ void view_taps(const uint32_t *__restrict__ img, int32_t W, int32_t H,
-                                          int32_t pitch, const Mat3 &Hm, float px, float py,
+                                          int32_t qrows, const Mat3 &Hm, float px, float py,
                                           const Window &win, float &ss, float &sss, float &srs,
                                           bool &ok) {
     const float wmax = (float)W - 1.0f, hmax = (float)H - 1.0f;
-    const uint32_t stride = (uint32_t)pitch;
 #pragma unroll
     for (int row = 0; row < 6; row++) {
         const float Y = py + (float)(row * 2 - 5);
@@ -262,7 +267,7 @@ __device__ __forceinline__ void view_taps(const uint32_t *__restrict__ img, int3
             const int32_t ix = (int32_t)uc;
             const int32_t iy = (int32_t)vc;
             const float fx = __builtin_amdgcn_fractf(u), fy = __builtin_amdgcn_fractf(v);
-            const uint32_t q = img[__umul24((uint32_t)iy, stride) + (uint32_t)ix];
+            const uint32_t q = img[quad_index((uint32_t)ix, (uint32_t)iy, (uint32_t)qrows)];
             const float a = (float)(q & 0xffu), b = (float)((q >> 8) & 0xffu);
             const float c = (float)((q >> 16) & 0xffu), e = (float)(q >> 24);
             const float top = fmaf(fx, b - a, a);
@@ -276,25 +281,29 @@ __device__ __forceinline__ void view_taps(const uint32_t *__restrict__ img, int3
     }
 }
 
-__device__ __forceinline__ uint32_t texel_load(__amdgpu_buffer_rsrc_t texels, int32_t index) {
-    return struct_buffer_load_u32(texels, index, 0, 0, 0);
+// Texel (x, y) of a layer through a structured buffer resource of 4-byte records over it: record
+// 16 ((x / 16) (qrows - 1) + y) + x = quad_index(x, y, qrows), formed with a shift, one 24-bit
+// multiply-add and one shift-add into a single address register (an index and a byte offset in
+// two registers would cost the texture addresser about twice the cycles per load, measured).  The
+// hardware range-checks the record index against the layer's texel count, so a load can never
+// leave the layer.
+__device__ __forceinline__ uint32_t texel_load(__amdgpu_buffer_rsrc_t texels, uint32_t x, uint32_t y,
+                                               uint32_t qrows_m1) {
+    return struct_buffer_load_u32(texels, (int32_t)(((__umul24(x >> 4, qrows_m1) + y) << 4) + x), 0, 0, 0);
 }
 
 // Taps of a window known to lie inside the source image (see view_cost): no per-tap checks.
-// The texels are read through a structured buffer resource over the layer's texels, so a load can
-// never leave them (the hardware returns 0 past the last record), at one address instruction per
-// tap (a 24-bit multiply-add giving the texel index; the hardware scales it by the 4-byte record).
-// cu, cv: the corner taps (0, 5, 30, 35) as classify_window computed them, with the same
-// expressions, so they are taken from there.
-__device__ __forceinline__ void view_taps_inside(const uint32_t *__restrict__ img, int32_t H,
-                                                 int32_t pitch, const Mat3 &Hm, float px, float py,
+// Texels are read through texel_load (three address instructions per tap beyond the two
+// conversions).  cu, cv: the corner taps (0, 5, 30, 35) as classify_window computed them, with
+// the same expressions, so they are taken from there.
+__device__ __forceinline__ void view_taps_inside(const uint32_t *__restrict__ img, int32_t W, int32_t H,
+                                                 int32_t qrows, const Mat3 &Hm, float px, float py,
                                                  const Window &win, const float cu[4],
                                                  const float cv[4], float &ss, float &sss,
                                                  float &srs) {
-    const uint32_t stride = (uint32_t)pitch;
-    // structured buffer of 4-byte records (the hardware scales the index), one record per texel
     const __amdgpu_buffer_rsrc_t texels = __builtin_amdgcn_make_buffer_rsrc(
-        (void *)img, (short)4, (int)((uint32_t)pitch * (uint32_t)H), 0x00020000);
+        (void *)img, (short)4, (int)quad_texels(W, H), 0x00020000);
+    const uint32_t qrows_m1 = (uint32_t)(qrows - 1);
 #pragma unroll
     for (int row = 0; row < 6; row++) {
         const float Y = py + (float)(row * 2 - 5);
@@ -320,7 +329,7 @@ __device__ __forceinline__ void view_taps_inside(const uint32_t *__restrict__ im
             }
             const uint32_t ix = (uint32_t)u, iy = (uint32_t)v;
             const float fx = __builtin_amdgcn_fractf(u), fy = __builtin_amdgcn_fractf(v);
-            const uint32_t q = texel_load(texels, (int32_t)(__umul24(iy, stride) + ix));
+            const uint32_t q = texel_load(texels, ix, iy, qrows_m1);
             const float a = (float)(q & 0xffu), b = (float)((q >> 8) & 0xffu);
             const float c = (float)((q >> 16) & 0xffu), e = (float)(q >> 24);
             const float top = fmaf(fx, b - a, a);
@@ -402,18 +411,18 @@ __device__ __forceinline__ float view_cost(const PmSrc &s, const Mat3 &H, float 
     const int cls = classify_window(s, H, px, py, cu, cv);
     float ss = 0.0f, sss = 0.0f, srs = 0.0f;
     bool ok = cls != kOutside;
+    // the checked loop is exact for every class; lanes already known outside stay invalid
     if (__any(cls == kCareful)) {
-        // the checked loop is exact for every class; lanes already known outside stay invalid
         const float bound = fabsf(H.m[6]) * (fabsf(px) + 5.0f) + fabsf(H.m[7]) * (fabsf(py) + 5.0f) +
                             fabsf(H.m[8]);
         bool tap_ok = true;
         if (__all(bound < 0x1p60f))
-            view_taps<true>(img, s.W, s.H, s.pitch, H, px, py, win, ss, sss, srs, tap_ok);
+            view_taps<true>(img, s.W, s.H, s.qrows, H, px, py, win, ss, sss, srs, tap_ok);
         else
-            view_taps<false>(img, s.W, s.H, s.pitch, H, px, py, win, ss, sss, srs, tap_ok);
+            view_taps<false>(img, s.W, s.H, s.qrows, H, px, py, win, ss, sss, srs, tap_ok);
         ok = ok & tap_ok;
     } else if (__any(cls == kInside)) {
-        if (cls == kInside) view_taps_inside(img, s.H, s.pitch, H, px, py, win, cu, cv, ss, sss, srs);
+        if (cls == kInside) view_taps_inside(img, s.W, s.H, s.qrows, H, px, py, win, cu, cv, ss, sss, srs);
     }
     if (!ok) return kInvalid;
     // PM:151-158 with the window's sums scaled by 1 / sw and the correlation by 1 / den (IEEE

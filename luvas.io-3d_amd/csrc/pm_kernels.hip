@@ -21,14 +21,14 @@ namespace {
 // ---- kernels -------------------------------------------------------------------------------------
 
 __global__ void __launch_bounds__(256) build_quad_kernel(const uint8_t *__restrict__ gray, int32_t W,
-                                                         int32_t H, int32_t pitch, uint32_t *__restrict__ quad) {
+                                                         int32_t H, int32_t qrows, uint32_t *__restrict__ quad) {
     int32_t x = blockIdx.x * blockDim.x + threadIdx.x;
     int32_t y = blockIdx.y * blockDim.y + threadIdx.y;
     if (x >= W || y >= H) return;
     int32_t x1 = min(x + 1, W - 1), y1 = min(y + 1, H - 1);
     uint32_t a = gray[(size_t)y * W + x], b = gray[(size_t)y * W + x1];
     uint32_t c = gray[(size_t)y1 * W + x], e = gray[(size_t)y1 * W + x1];
-    quad[(size_t)y * pitch + x] = a | (b << 8) | (c << 16) | (e << 24);
+    quad[quad_index((uint32_t)x, (uint32_t)y, (uint32_t)qrows)] = a | (b << 8) | (c << 16) | (e << 24);
 }
 
 // PM:255-289 (init) and PM:296-333 (init_coarse): iteration 0, phase 0 (DS:192-201)
@@ -42,7 +42,7 @@ __global__ void __launch_bounds__(256) pm_init_kernel(PmPlan plan, PmPools pools
     const float px = (float)x, py = (float)y;
     uint32_t rng = pixel_rng(d, 0u, 0u, index);
     Window win;
-    prepare_window(pools.quad + d.qoff, d.W, d.H, d.pitch, x, y, pools.wtab, win);
+    prepare_window(pools.quad + d.qoff, d.W, d.H, d.qrows, x, y, pools.wtab, win);
     const float rx = (px - d.k[2]) / d.k[0];
     const float ry = (py - d.k[3]) / d.k[1];
     float nx, ny, nz;
@@ -151,7 +151,7 @@ pm_propagate_kernel(PmPlan plan, PmPools pools,
     const float px = (float)x, py = (float)y;
     uint32_t rng = pixel_rng(d, iteration, phase, index);
     Window win;
-    prepare_window(pools.quad + d.qoff, W, H, d.pitch, x, y, pools.wtab, win);
+    prepare_window(pools.quad + d.qoff, W, H, d.qrows, x, y, pools.wtab, win);
     const float rx = (px - d.k[2]) / d.k[0];
     const float ry = (py - d.k[3]) / d.k[1];
     const float *__restrict__ cost = pools.cost + d.off;
@@ -288,11 +288,11 @@ __global__ void __launch_bounds__(256) pm_consistency_kernel(PmPlan plan, PmPool
 
 // ---- launchers ------------------------------------------------------------------------------------
 
-hipError_t pm_build_quad(hipStream_t s, const uint8_t *gray, int32_t W, int32_t H, int32_t pitch, uint32_t *quad) {
-    if (pitch < W) return hipErrorInvalidValue;
+hipError_t pm_build_quad(hipStream_t s, const uint8_t *gray, int32_t W, int32_t H, int32_t qrows, uint32_t *quad) {
+    if (qrows < H) return hipErrorInvalidValue;
     dim3 block(64, 4);
     dim3 grid((W + 63) / 64, (H + 3) / 4);
-    hipLaunchKernelGGL(build_quad_kernel, grid, block, 0, s, gray, W, H, pitch, quad);
+    hipLaunchKernelGGL(build_quad_kernel, grid, block, 0, s, gray, W, H, qrows, quad);
     return hipGetLastError();
 }
 

@@ -24,12 +24,19 @@ export interface SolveOptions {
 export declare class Context {
     constructor(device?: number);
     readonly handle: unknown;
+    synchronize(): void;
+    /** Kernel timing with events around every launch (statistics reset when turned on). */
+    setProfiling(on: boolean): void;
+    kernelStats(name: string): { launches: number; ms: number };
     destroy(): void;
 }
 
+/** Launch shapes [threads x, threads y] per PatchMatch kernel. */
+export interface TileShapes { init: [number, number]; propagate: [number, number]; pack: [number, number]; }
+
 export declare class HipDenseStereo {
     constructor(ctx: Context, width: number, height: number, views: StereoView[],
-                options?: { coarse?: HipDenseStereo });
+                options?: { coarse?: HipDenseStereo; priorPool?: boolean });
     readonly width: number;
     readonly height: number;
     readonly hypotheses: HipDenseStereo;
@@ -39,6 +46,15 @@ export declare class HipDenseStereo {
     setImage(layer: number, gray: Uint8Array): void;
     solve(ref: number, options: SolveOptions): Promise<void>;
     solveBatched(solves: Array<[number, SolveOptions]>): Promise<void>;
+    /** solveBatched without device allocation (fixed per-GPU plan scratch; at most 1024 solves). */
+    solveBatchedNoalloc(solves: Array<[number, SolveOptions]>): Promise<void>;
+    /** Times the launch-shape candidates on these solves and keeps the fastest (results unchanged). */
+    autotune(solves: Array<[number, SolveOptions]>, repeats?: number): Promise<TileShapes>;
+    tiles(): TileShapes;
+    setTile(kernel: 'init' | 'propagate' | 'pack', tx: number, ty: number): void;
+    writeDepth(layer: number, packed: Uint32Array): void;
+    /** n.xyz, depth per pixel (4 * width * height floats). */
+    writeHyp(layer: number, planes: Float32Array): void;
     filter(ref: number, sources: number[], needed: number): Promise<void>;
     filteredMap(ref: number, sources: number[], needed: number): Promise<Uint32Array>;
     depthMap(ref: number): Promise<Uint32Array>;
@@ -106,6 +122,16 @@ export declare class HipTsdfVolume {
     readonly truncation: number;
     integrate(view: IntegrationView): Promise<void>;
     integrateLayers(layers: number[], views: IntegrationView[], map?: number): Promise<void>;
+    /** A packed map from host memory (width * height words). */
+    integrateHost(packed: Uint32Array, view: IntegrationView): Promise<void>;
+    /** Times bricks per block (1, 2, 4) on a pass of these layers and keeps the fastest. */
+    autotune(layers: number[], views: IntegrationView[], map?: number, repeats?: number): Promise<number>;
+    bricksPerBlock: number;
+    setDepthCulling(on: boolean): void;
+    setCounting(on: boolean): void;
+    updateCount(): number;
+    passCounts(): { updates: number; pairs: number; bricksRead: number; voxelsWritten: number };
+    write(tsdf: Float32Array, weight: Float32Array): void;
     read(): Promise<{ tsdf: Float32Array; weight: Float32Array }>;
     backward(layers: number[], views: IntegrationView[], voxels: ArrayLike<number>, targets: ArrayLike<number>,
              map?: number): Promise<TsdfGradients>;

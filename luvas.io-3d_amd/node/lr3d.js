@@ -24,6 +24,18 @@ class Context {
     constructor(device) {
         this.handle = native.ctxCreate(device === undefined ? 0 : device);
     }
+    /** Waits for the context's queued work. */
+    synchronize() {
+        native.ctxSynchronize(this.handle);
+    }
+    /** Kernel timing with events around every launch (statistics reset when turned on). */
+    setProfiling(on) {
+        native.ctxSetProfiling(this.handle, !!on);
+    }
+    /** {launches, ms} of kernel `name` since profiling was turned on. */
+    kernelStats(name) {
+        return native.ctxKernelStats(this.handle, name);
+    }
     destroy() {
         if (this.handle) native.ctxDestroy(this.handle);
         this.handle = null;
@@ -43,7 +55,8 @@ function cameraOf(view, width, height) {
 class HipDenseStereo {
     /**
      * new DenseStereo(device, width, height, views, options) (DS:80-172).  `views` may carry
-     * their own width/height (mixed-resolution sets); options.coarse is the coarse level.
+     * their own width/height (mixed-resolution sets); options.coarse is the coarse level;
+     * options.priorPool allocates the prior pool now (solveBatchedNoalloc needs it for priors).
      */
     constructor(ctx, width, height, views, options) {
         this.ctx = ctx;
@@ -51,7 +64,7 @@ class HipDenseStereo {
         this.height = height;
         this.views = views.slice();  // own copy: setPose replaces entries here, not in the caller's array
         this.cameras = views.map((v) => cameraOf(v, width, height));
-        this.handle = native.stereoCreate(ctx.handle, this.cameras);
+        this.handle = native.stereoCreate(ctx.handle, this.cameras, options && options.priorPool ? 1 : 0);
         this.lastFiltered = -1;
         this.filteredMapBinding = new FilteredMap(this);
         const coarse = options && options.coarse ? options.coarse : null;
@@ -96,6 +109,51 @@ class HipDenseStereo {
         const descs = solves.filter((s) => s[1].sources.length > 0)
             .map((s) => this.descOf(s[0], s[1]));
         if (descs.length) await native.stereoSolve(this.handle, descs);
+    }
+
+    /**
+     * solveBatched for callers that cannot allocate device memory: the launch plan goes to a fixed
+     * per-GPU scratch array (at most 1024 solves), launches are eager, priors need a set created
+     * with options.priorPool.
+     */
+    async solveBatchedNoalloc(solves) {
+        const descs = solves.filter((s) => s[1].sources.length > 0)
+            .map((s) => this.descOf(s[0], s[1]));
+        if (descs.length) await native.stereoSolveNoalloc(this.handle, descs);
+    }
+
+    /**
+     * Times the launch-shape candidates on these solves and keeps the fastest per kernel; the set's
+     * planes, costs and maps are restored afterwards.  Returns the chosen shapes.
+     */
+    async autotune(solves, repeats) {
+        const descs = solves.filter((s) => s[1].sources.length > 0)
+            .map((s) => this.descOf(s[0], s[1]));
+        if (descs.length) await native.stereoAutotune(this.handle, descs, repeats === undefined ? 3 : repeats);
+        return this.tiles();
+    }
+
+    /** Launch shapes [threads x, threads y] of 'init', 'propagate' and 'pack'. */
+    tiles() {
+        const out = {};
+        for (const k of ['init', 'propagate', 'pack']) out[k] = native.stereoGetTile(this.handle, k);
+        return out;
+    }
+
+    setTile(kernel, tx, ty) {
+        native.stereoSetTile(this.handle, kernel, tx, ty);
+    }
+
+    /** Overwrites a layer's packed depth map (width*height words). */
+    writeDepth(layer, packed) {
+        const c = this.cameras[layer];
+        native.stereoWriteDepth(this.handle, layer, packed, c.width * c.height);
+    }
+
+    /** Overwrites a layer's planes (n.xyz, depth per pixel: 4*width*height floats). */
+    writeHyp(layer, planes) {
+        const c = this.cameras[layer];
+        native.stereoWriteHyp(this.handle, layer, planes, c.width * c.height);
     }
 
     /** The planes of the last solve (DS:242-244): this level, usable as a coarse level. */
@@ -272,6 +330,50 @@ class HipTsdfVolume {
     async integrateLayers(layers, views, map) {
         await native.tsdfIntegrate(this.handle, this.stereo.handle, layers, map === undefined ? MAP_RAW : map,
             views.map((v) => HipTsdfVolume.viewOf(v, this.width, this.height)));
+    }
+
+    /** A packed map from host memory (view.width * view.height words) in one pass. */
+    async integrateHost(packed, view) {
+        await native.tsdfIntegrateHost(this.handle, packed, HipTsdfVolume.viewOf(view, this.width, this.height));
+    }
+
+    /** Times bricks per block (1, 2, 4) on a pass of these layers and keeps the fastest. */
+    async autotune(layers, views, map, repeats) {
+        await native.tsdfAutotune(this.handle, this.stereo.handle, layers, map === undefined ? MAP_RAW : map,
+            views.map((v) => HipTsdfVolume.viewOf(v, this.width, this.height)), repeats === undefined ? 3 : repeats);
+        return this.bricksPerBlock;
+    }
+
+    get bricksPerBlock() {
+        return native.tsdfGetBricksPerBlock(this.handle);
+    }
+
+    set bricksPerBlock(n) {
+        native.tsdfSetBricksPerBlock(this.handle, n);
+    }
+
+    /** Culling of (brick, view) pairs against the views' depth tiles (on by default; no result changes). */
+    setDepthCulling(on) {
+        native.tsdfSetDepthCulling(this.handle, !!on);
+    }
+
+    /** Counting of the passes that follow (off by default): updateCount(), passCounts(). */
+    setCounting(on) {
+        native.tsdfSetCounting(this.handle, !!on);
+    }
+
+    updateCount() {
+        return native.tsdfUpdateCount(this.handle);
+    }
+
+    /** {updates, pairs, bricksRead, voxelsWritten} of the counted passes. */
+    passCounts() {
+        return native.tsdfPassCounts(this.handle);
+    }
+
+    /** Overwrites the tsdf and weight pools (layout.count * 512 floats each). */
+    write(tsdf, weight) {
+        native.tsdfWrite(this.handle, tsdf, weight, this.layout.count);
     }
 
     /** TV:270-277 */

@@ -1,11 +1,13 @@
 /*
  * lr3d.node — thin N-API binding of the lr3d C ABI (include/lr3d.h) for the TypeScript host.
  *
- * It binds exactly the calls the reference's stage objects make on the GPU:
- * DenseStereo (frontend/src/app/vision/mvs/dense-stereo.ts) and TsdfVolume
- * (frontend/src/app/vision/fusion/tsdf-volume.ts).  Handles are N-API externals; long calls
- * (solve, filter, integrate, reads) run as async work on the libuv pool and return Promises, the
- * way the reference awaits queue submissions.  Inputs are copied before the call returns.
+ * It binds the calls the reference's stage objects make on the GPU — DenseStereo
+ * (frontend/src/app/vision/mvs/dense-stereo.ts) and TsdfVolume
+ * (frontend/src/app/vision/fusion/tsdf-volume.ts) — and the rest of the C ABI (adjoints, meshing,
+ * fusion, launch-shape tuning, counting and timing).  Handles are N-API externals; long calls
+ * (solves, filter, integration, tuning, reads) run as async work on the libuv pool and return
+ * Promises, the way the reference awaits queue submissions.  Inputs are copied before the call
+ * returns.
  */
 #define NAPI_VERSION 6
 #include <node_api.h>
@@ -118,7 +120,8 @@ static int read_view(napi_env env, napi_value o, lr3d_view *v) {
 /* ---- async work -------------------------------------------------------------------------------- */
 
 enum { OP_SOLVE, OP_SOLVE_BATCHED, OP_FILTER, OP_INTEGRATE, OP_READ_DEPTH, OP_READ_FILTERED,
-       OP_READ_HYP, OP_READ_TSDF, OP_NCC_BACKWARD, OP_NCC_REFINE, OP_TSDF_BACKWARD, OP_MESH, OP_FUSE };
+       OP_READ_HYP, OP_READ_TSDF, OP_NCC_BACKWARD, OP_NCC_REFINE, OP_TSDF_BACKWARD, OP_MESH, OP_FUSE,
+       OP_SOLVE_NOALLOC, OP_STEREO_AUTOTUNE, OP_TSDF_AUTOTUNE, OP_INTEGRATE_HOST };
 
 typedef struct {
     int op;
@@ -154,6 +157,9 @@ typedef struct {
     double cell, final_cell;
     int64_t cell_budget;
     int32_t min_support;
+    /* tuning, host-map integration */
+    int32_t repeats;
+    uint32_t *packed;
 } job_t;
 
 static void job_free(job_t *j) {
@@ -172,6 +178,7 @@ static void job_free(job_t *j) {
     free(j->weights);
     free(j->targets);
     free(j->voxels);
+    free(j->packed);
     if (j->rgbs) {
         for (int32_t i = 0; i < j->n; i++) free(j->rgbs[i]);
         free(j->rgbs);
@@ -186,6 +193,12 @@ static void job_execute(napi_env env, void *data) {
     switch (j->op) {
     case OP_SOLVE: j->status = lr3d_stereo_solve(j->st, &j->descs[0]); break;
     case OP_SOLVE_BATCHED: j->status = lr3d_stereo_solve_batched(j->st, j->descs, j->n); break;
+    case OP_SOLVE_NOALLOC: j->status = lr3d_stereo_solve_batched_noalloc(j->st, j->descs, j->n); break;
+    case OP_STEREO_AUTOTUNE: j->status = lr3d_stereo_autotune(j->st, j->descs, j->n, j->repeats); break;
+    case OP_TSDF_AUTOTUNE:
+        j->status = lr3d_tsdf_autotune(j->ts, j->st, j->layers, j->map, j->iviews, j->n, j->repeats);
+        break;
+    case OP_INTEGRATE_HOST: j->status = lr3d_tsdf_integrate_host(j->ts, j->packed, &j->iviews[0]); break;
     case OP_FILTER: j->status = lr3d_stereo_filter(j->st, j->layer, j->checks, j->n_checks, j->needed); break;
     case OP_INTEGRATE:
         j->status = lr3d_tsdf_integrate_batched(j->ts, j->st, j->layers, j->map, j->iviews, j->n);
@@ -363,9 +376,12 @@ static napi_value ctx_destroy(napi_env env, napi_callback_info info) {
     return undefined(env);
 }
 
+/* stereoCreate(ctx, views[], flags?) (flags: LR3D_STEREO_PRIOR_POOL = 1) */
 static napi_value stereo_create(napi_env env, napi_callback_info info) {
-    ARGS(2);
+    ARGS(3);
     lr3d_ctx *ctx = (lr3d_ctx *)get_external(env, argv[0]);
+    uint32_t flags = 0;
+    if (argc > 2) napi_get_value_uint32(env, argv[2], &flags);
     uint32_t n = 0;
     NAPI_CALL(env, napi_get_array_length(env, argv[1], &n));
     lr3d_view *views = (lr3d_view *)calloc(n ? n : 1, sizeof(lr3d_view));
@@ -379,7 +395,7 @@ static napi_value stereo_create(napi_env env, napi_callback_info info) {
         }
     }
     lr3d_stereo *st = NULL;
-    int32_t rc = lr3d_stereo_create(ctx, views, (int32_t)n, 0, &st);
+    int32_t rc = lr3d_stereo_create(ctx, views, (int32_t)n, flags, &st);
     free(views);
     if (rc != LR3D_OK) return throw_status(env, rc);
     napi_value out;
@@ -453,15 +469,16 @@ static int read_solve(napi_env env, napi_value o, lr3d_solve_desc *d, float **pr
     return 1;
 }
 
-static napi_value stereo_solve(napi_env env, napi_callback_info info) {
-    ARGS(2);
-    job_t *j = new_job(OP_SOLVE_BATCHED);
+/* Solve descriptors of argv[1] (one object or an array) into a new job of `op` (OP_SOLVE for a
+ * single object given to a batched op).  NULL after throwing. */
+static job_t *solve_job(napi_env env, napi_value *argv, int op) {
+    job_t *j = new_job(op);
     j->st = (lr3d_stereo *)get_external(env, argv[0]);
     bool is_array = false;
     napi_is_array(env, argv[1], &is_array);
     uint32_t n = 1;
     if (is_array) napi_get_array_length(env, argv[1], &n);
-    else j->op = OP_SOLVE;
+    else if (op == OP_SOLVE_BATCHED) j->op = OP_SOLVE;
     j->n = (int32_t)n;
     j->descs = (lr3d_solve_desc *)calloc(n ? n : 1, sizeof(lr3d_solve_desc));
     j->priors = (float **)calloc(n ? n : 1, sizeof(float *));
@@ -476,6 +493,29 @@ static napi_value stereo_solve(napi_env env, napi_callback_info info) {
             return NULL;
         }
     }
+    return j;
+}
+
+static napi_value stereo_solve(napi_env env, napi_callback_info info) {
+    ARGS(2);
+    job_t *j = solve_job(env, argv, OP_SOLVE_BATCHED);
+    return j ? queue(env, j) : NULL;
+}
+
+/* stereoSolveNoalloc(stereo, solves[]): lr3d_stereo_solve_batched_noalloc */
+static napi_value stereo_solve_noalloc(napi_env env, napi_callback_info info) {
+    ARGS(2);
+    job_t *j = solve_job(env, argv, OP_SOLVE_NOALLOC);
+    return j ? queue(env, j) : NULL;
+}
+
+/* stereoAutotune(stereo, solves[], repeats) */
+static napi_value stereo_autotune(napi_env env, napi_callback_info info) {
+    ARGS(3);
+    job_t *j = solve_job(env, argv, OP_STEREO_AUTOTUNE);
+    if (!j) return NULL;
+    j->repeats = 3;
+    if (argc > 2) napi_get_value_int32(env, argv[2], &j->repeats);
     return queue(env, j);
 }
 
@@ -591,9 +631,9 @@ static int read_iview(napi_env env, napi_value o, lr3d_integration_view *v, floa
     return get_doubles(env, o, "R", v->R, 9) && get_doubles(env, o, "t", v->t, 3);
 }
 
-static napi_value tsdf_integrate(napi_env env, napi_callback_info info) {
-    ARGS(5);
-    job_t *j = new_job(OP_INTEGRATE);
+/* (volume, stereo, layers[], map, views[]) of argv into a new job of `op`.  NULL after throwing. */
+static job_t *integrate_job(napi_env env, napi_value *argv, int op) {
+    job_t *j = new_job(op);
     j->ts = (lr3d_tsdf *)get_external(env, argv[0]);
     j->st = (lr3d_stereo *)get_external(env, argv[1]);
     uint32_t n = 0;
@@ -616,7 +656,213 @@ static napi_value tsdf_integrate(napi_env env, napi_callback_info info) {
             return NULL;
         }
     }
+    return j;
+}
+
+static napi_value tsdf_integrate(napi_env env, napi_callback_info info) {
+    ARGS(5);
+    job_t *j = integrate_job(env, argv, OP_INTEGRATE);
+    return j ? queue(env, j) : NULL;
+}
+
+/* tsdfAutotune(volume, stereo, layers[], map, views[], repeats) */
+static napi_value tsdf_autotune(napi_env env, napi_callback_info info) {
+    ARGS(6);
+    job_t *j = integrate_job(env, argv, OP_TSDF_AUTOTUNE);
+    if (!j) return NULL;
+    j->repeats = 3;
+    if (argc > 5) napi_get_value_int32(env, argv[5], &j->repeats);
     return queue(env, j);
+}
+
+/* tsdfIntegrateHost(volume, packed: Uint32Array (width*height), view) */
+static napi_value tsdf_integrate_host(napi_env env, napi_callback_info info) {
+    ARGS(3);
+    job_t *j = new_job(OP_INTEGRATE_HOST);
+    j->ts = (lr3d_tsdf *)get_external(env, argv[0]);
+    j->n = 1;
+    j->iviews = (lr3d_integration_view *)calloc(1, sizeof(lr3d_integration_view));
+    j->priors = (float **)calloc(1, sizeof(float *));
+    const int ok = read_iview(env, argv[2], &j->iviews[0], &j->priors[0]);
+    size_t count = 0;
+    int bad = 0;
+    j->packed = (uint32_t *)copy_typed(env, argv[1], napi_uint32_array, &count, &bad);
+    if (ok <= 0 || bad || !j->packed || count != (size_t)j->iviews[0].width * (size_t)j->iviews[0].height) {
+        job_free(j);
+        napi_throw_range_error(env, NULL, "lr3d.node: tsdfIntegrateHost needs a Uint32Array of width*height and a view");
+        return NULL;
+    }
+    return queue(env, j);
+}
+
+/* ---- synchronous calls (no device work worth a worker thread, or plain uploads) ---------------- */
+
+static napi_value status_or_undefined(napi_env env, int32_t rc) {
+    return rc == LR3D_OK ? undefined(env) : throw_status(env, rc);
+}
+
+static napi_value ctx_synchronize(napi_env env, napi_callback_info info) {
+    ARGS(1);
+    return status_or_undefined(env, lr3d_ctx_synchronize((lr3d_ctx *)get_external(env, argv[0])));
+}
+
+static napi_value ctx_set_profiling(napi_env env, napi_callback_info info) {
+    ARGS(2);
+    bool on = false;
+    napi_get_value_bool(env, argv[1], &on);
+    return status_or_undefined(env, lr3d_ctx_set_profiling((lr3d_ctx *)get_external(env, argv[0]), on ? 1 : 0));
+}
+
+/* ctxKernelStats(ctx, name) -> { launches, ms } */
+static napi_value ctx_kernel_stats(napi_env env, napi_callback_info info) {
+    ARGS(2);
+    char name[128];
+    size_t len = 0;
+    NAPI_CALL(env, napi_get_value_string_utf8(env, argv[1], name, sizeof(name), &len));
+    int64_t launches = 0;
+    double ms = 0;
+    const int32_t rc = lr3d_ctx_kernel_stats((lr3d_ctx *)get_external(env, argv[0]), name, &launches, &ms);
+    if (rc != LR3D_OK) return throw_status(env, rc);
+    napi_value out, a, b;
+    napi_create_object(env, &out);
+    napi_create_int64(env, launches, &a);
+    napi_create_double(env, ms, &b);
+    napi_set_named_property(env, out, "launches", a);
+    napi_set_named_property(env, out, "ms", b);
+    return out;
+}
+
+/* stereoGetTile(stereo, kernel) -> [tx, ty]; stereoSetTile(stereo, kernel, tx, ty) */
+static napi_value stereo_get_tile(napi_env env, napi_callback_info info) {
+    ARGS(2);
+    char kernel[32];
+    size_t len = 0;
+    NAPI_CALL(env, napi_get_value_string_utf8(env, argv[1], kernel, sizeof(kernel), &len));
+    int32_t tx = 0, ty = 0;
+    const int32_t rc = lr3d_stereo_get_tile((lr3d_stereo *)get_external(env, argv[0]), kernel, &tx, &ty);
+    if (rc != LR3D_OK) return throw_status(env, rc);
+    napi_value out, a, b;
+    napi_create_array_with_length(env, 2, &out);
+    napi_create_int32(env, tx, &a);
+    napi_create_int32(env, ty, &b);
+    napi_set_element(env, out, 0, a);
+    napi_set_element(env, out, 1, b);
+    return out;
+}
+
+static napi_value stereo_set_tile(napi_env env, napi_callback_info info) {
+    ARGS(4);
+    char kernel[32];
+    size_t len = 0;
+    NAPI_CALL(env, napi_get_value_string_utf8(env, argv[1], kernel, sizeof(kernel), &len));
+    int32_t tx = 0, ty = 0;
+    napi_get_value_int32(env, argv[2], &tx);
+    napi_get_value_int32(env, argv[3], &ty);
+    return status_or_undefined(env, lr3d_stereo_set_tile((lr3d_stereo *)get_external(env, argv[0]), kernel, tx, ty));
+}
+
+/* stereoWriteDepth(stereo, layer, packed: Uint32Array, pixels); stereoWriteHyp(stereo, layer,
+ * planes: Float32Array, pixels) (4 floats per pixel) */
+static napi_value stereo_write(napi_env env, napi_callback_info info, int hyp) {
+    ARGS(4);
+    int32_t layer = 0;
+    double pixels = 0;
+    napi_get_value_int32(env, argv[1], &layer);
+    napi_get_value_double(env, argv[3], &pixels);
+    size_t count = 0;
+    int bad = 0;
+    void *data = copy_typed(env, argv[2], hyp ? napi_float32_array : napi_uint32_array, &count, &bad);
+    if (bad || !data || (double)count != pixels * (hyp ? 4.0 : 1.0)) {
+        free(data);
+        napi_throw_range_error(env, NULL, hyp ? "lr3d.node: planes must be a Float32Array of 4*width*height"
+                                              : "lr3d.node: packed must be a Uint32Array of width*height");
+        return NULL;
+    }
+    lr3d_stereo *st = (lr3d_stereo *)get_external(env, argv[0]);
+    const int32_t rc = hyp ? lr3d_stereo_write_hyp(st, layer, (const float *)data)
+                           : lr3d_stereo_write_depth(st, layer, (const uint32_t *)data);
+    free(data);
+    return status_or_undefined(env, rc);
+}
+static napi_value stereo_write_depth(napi_env env, napi_callback_info info) { return stereo_write(env, info, 0); }
+static napi_value stereo_write_hyp(napi_env env, napi_callback_info info) { return stereo_write(env, info, 1); }
+
+/* tsdfGetBricksPerBlock(volume) -> n; tsdfSetBricksPerBlock(volume, n) */
+static napi_value tsdf_get_bricks_per_block(napi_env env, napi_callback_info info) {
+    ARGS(1);
+    int32_t n = 0;
+    const int32_t rc = lr3d_tsdf_get_bricks_per_block((lr3d_tsdf *)get_external(env, argv[0]), &n);
+    if (rc != LR3D_OK) return throw_status(env, rc);
+    napi_value out;
+    napi_create_int32(env, n, &out);
+    return out;
+}
+
+static napi_value tsdf_set_bricks_per_block(napi_env env, napi_callback_info info) {
+    ARGS(2);
+    int32_t n = 0;
+    napi_get_value_int32(env, argv[1], &n);
+    return status_or_undefined(env, lr3d_tsdf_set_bricks_per_block((lr3d_tsdf *)get_external(env, argv[0]), n));
+}
+
+static napi_value tsdf_set_flag(napi_env env, napi_callback_info info, int which) {
+    ARGS(2);
+    bool on = false;
+    napi_get_value_bool(env, argv[1], &on);
+    lr3d_tsdf *ts = (lr3d_tsdf *)get_external(env, argv[0]);
+    return status_or_undefined(env, which ? lr3d_tsdf_set_depth_culling(ts, on ? 1 : 0)
+                                          : lr3d_tsdf_set_counting(ts, on ? 1 : 0));
+}
+static napi_value tsdf_set_counting(napi_env env, napi_callback_info info) { return tsdf_set_flag(env, info, 0); }
+static napi_value tsdf_set_depth_culling(napi_env env, napi_callback_info info) { return tsdf_set_flag(env, info, 1); }
+
+/* tsdfUpdateCount(volume) -> number; tsdfPassCounts(volume) -> { updates, pairs, bricksRead,
+ * voxelsWritten } (the counting passes' totals) */
+static napi_value tsdf_update_count(napi_env env, napi_callback_info info) {
+    ARGS(1);
+    int64_t n = 0;
+    const int32_t rc = lr3d_tsdf_update_count((lr3d_tsdf *)get_external(env, argv[0]), &n);
+    if (rc != LR3D_OK) return throw_status(env, rc);
+    napi_value out;
+    napi_create_int64(env, n, &out);
+    return out;
+}
+
+static napi_value tsdf_pass_counts(napi_env env, napi_callback_info info) {
+    ARGS(1);
+    int64_t c[4] = {0, 0, 0, 0};
+    const int32_t rc = lr3d_tsdf_pass_counts((lr3d_tsdf *)get_external(env, argv[0]), c);
+    if (rc != LR3D_OK) return throw_status(env, rc);
+    static const char *const names[4] = {"updates", "pairs", "bricksRead", "voxelsWritten"};
+    napi_value out;
+    napi_create_object(env, &out);
+    for (int i = 0; i < 4; i++) {
+        napi_value v;
+        napi_create_int64(env, c[i], &v);
+        napi_set_named_property(env, out, names[i], v);
+    }
+    return out;
+}
+
+/* tsdfWrite(volume, tsdf: Float32Array, weight: Float32Array, bricks) (bricks*512 floats each) */
+static napi_value tsdf_write(napi_env env, napi_callback_info info) {
+    ARGS(4);
+    double bricks = 0;
+    napi_get_value_double(env, argv[3], &bricks);
+    size_t na = 0, nb = 0;
+    int bad_a = 0, bad_b = 0;
+    float *a = (float *)copy_typed(env, argv[1], napi_float32_array, &na, &bad_a);
+    float *b = (float *)copy_typed(env, argv[2], napi_float32_array, &nb, &bad_b);
+    if (bad_a || bad_b || !a || !b || (double)na != bricks * 512.0 || (double)nb != bricks * 512.0) {
+        free(a);
+        free(b);
+        napi_throw_range_error(env, NULL, "lr3d.node: tsdf and weight must be Float32Arrays of bricks*512");
+        return NULL;
+    }
+    const int32_t rc = lr3d_tsdf_write((lr3d_tsdf *)get_external(env, argv[0]), a, b);
+    free(a);
+    free(b);
+    return status_or_undefined(env, rc);
 }
 
 static napi_value tsdf_read(napi_env env, napi_callback_info info) {
@@ -859,6 +1105,15 @@ static napi_value init(napi_env env, napi_value exports) {
         {"nccBackward", ncc_backward},       {"nccRefineStep", ncc_refine_step},
         {"stereoSetPose", stereo_set_pose},  {"tsdfBackward", tsdf_backward},
         {"tsdfExtractSurface", tsdf_extract_surface}, {"stereoFuseDepthMaps", stereo_fuse_depth_maps},
+        {"stereoSolveNoalloc", stereo_solve_noalloc}, {"stereoAutotune", stereo_autotune},
+        {"stereoGetTile", stereo_get_tile},  {"stereoSetTile", stereo_set_tile},
+        {"stereoWriteDepth", stereo_write_depth}, {"stereoWriteHyp", stereo_write_hyp},
+        {"tsdfAutotune", tsdf_autotune},     {"tsdfIntegrateHost", tsdf_integrate_host},
+        {"tsdfGetBricksPerBlock", tsdf_get_bricks_per_block}, {"tsdfSetBricksPerBlock", tsdf_set_bricks_per_block},
+        {"tsdfSetCounting", tsdf_set_counting}, {"tsdfSetDepthCulling", tsdf_set_depth_culling},
+        {"tsdfUpdateCount", tsdf_update_count}, {"tsdfPassCounts", tsdf_pass_counts},
+        {"tsdfWrite", tsdf_write},           {"ctxSynchronize", ctx_synchronize},
+        {"ctxSetProfiling", ctx_set_profiling}, {"ctxKernelStats", ctx_kernel_stats},
     };
     for (size_t i = 0; i < sizeof(fns) / sizeof(fns[0]); i++) {
         napi_value f;

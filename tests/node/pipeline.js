@@ -64,9 +64,49 @@ async function main() {
     fs.writeFileSync(path.join(dir, 'tsdf.f32'), Buffer.from(tsdf.buffer));
     fs.writeFileSync(path.join(dir, 'weight.f32'), Buffer.from(weight.buffer));
     fs.writeFileSync(path.join(dir, 'coords.i32'), Buffer.from(layout.coords.buffer));
-    // gradient refinement of planes and poses through the Node host (last: it moves the poses)
     const solves = job.views.map((v, i) => [i, { sources: job.sources[i], minDepth: job.ranges[i][0],
         maxDepth: job.ranges[i][1], prior: null, geometric: false, iterations: job.iterations }]);
+    // the rest of the C ABI through the Node host: allocation-free solves, launch shapes, timing,
+    // counting, host-map integration and volume upload
+    const extras = {};
+    const before = [];
+    for (let layer = 0; layer < job.views.length; layer++) before.push(await run.stereo.depthMap(layer));
+    await run.stereo.solveBatchedNoalloc(solves);
+    let same = true;
+    for (let layer = 0; layer < job.views.length; layer++) {
+        const after = await run.stereo.depthMap(layer);
+        for (let i = 0; i < after.length; i++) if (after[i] !== before[layer][i]) { same = false; break; }
+    }
+    extras.noallocSame = same;
+    ctx.setProfiling(true);
+    await run.stereo.solve(0, solves[0][1]);
+    extras.propagateLaunches = ctx.kernelStats('pm_propagate').launches;
+    ctx.setProfiling(false);
+    ctx.synchronize();
+    extras.tiles = run.stereo.tiles();
+    run.stereo.setTile('propagate', 16, 4);
+    extras.tilesSet = run.stereo.tiles().propagate;
+    run.stereo.setTile('propagate', extras.tiles.propagate[0], extras.tiles.propagate[1]);
+    run.stereo.writeDepth(1, before[1]);
+    const v2 = new lr3d.HipTsdfVolume(ctx, grid, layout, run.stereo.filtered, job.width, job.height, job.truncation);
+    v2.setCounting(true);
+    await v2.integrateLayers(layers, ivs, lr3d.MAP_FILTERED);
+    extras.passCounts = v2.passCounts();
+    extras.updateCount = v2.updateCount();
+    extras.bricksPerBlock = await v2.autotune(layers, ivs, lr3d.MAP_FILTERED, 1);
+    v2.setDepthCulling(false);
+    v2.write(tsdf, weight);
+    const back = await v2.read();
+    extras.writeRoundTrip = back.tsdf.every((x, i) => Object.is(x, tsdf[i])) && back.weight.every((x, i) => Object.is(x, weight[i]));
+    const v3 = new lr3d.HipTsdfVolume(ctx, grid, layout, run.stereo, job.width, job.height, job.truncation);
+    const v4 = new lr3d.HipTsdfVolume(ctx, grid, layout, run.stereo, job.width, job.height, job.truncation);
+    await v3.integrateHost(before[0], ivs[0]);
+    await v4.integrateLayers([0], [ivs[0]], lr3d.MAP_RAW);
+    const r3 = await v3.read(), r4 = await v4.read();
+    extras.hostSame = r3.tsdf.every((x, i) => Object.is(x, r4.tsdf[i])) && r3.weight.every((x, i) => Object.is(x, r4.weight[i]));
+    [v2, v3, v4].forEach((v) => v.destroy());
+    fs.writeFileSync(path.join(dir, 'extras.json'), JSON.stringify(extras));
+    // gradient refinement of planes and poses through the Node host (last: it moves the poses)
     const history = await lr3d.refine(run.stereo, solves, { steps: 5 });
     fs.writeFileSync(path.join(dir, 'refine.json'), JSON.stringify(history));
     // the coarse-to-fine driver over two coarse copies (half and quarter resolution)

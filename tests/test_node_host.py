@@ -18,7 +18,13 @@ pytestmark = pytest.mark.skipif(NODE is None, reason="node is not installed")
 EXPORTS = ["ctxCreate", "ctxDestroy", "stereoCreate", "stereoDestroy", "stereoSetImage", "stereoSolve",
            "stereoFilter", "stereoReadDepth", "stereoReadFiltered", "stereoReadHyp", "stereoSetCoarse",
            "tsdfCreate", "tsdfDestroy", "tsdfIntegrate", "tsdfRead", "lastError", "nccBackward",
-           "nccRefineStep", "stereoSetPose", "tsdfBackward", "tsdfExtractSurface", "stereoFuseDepthMaps"]
+           "nccRefineStep", "stereoSetPose", "tsdfBackward", "tsdfExtractSurface", "stereoFuseDepthMaps",
+           "stereoSolveNoalloc", "stereoAutotune", "stereoGetTile", "stereoSetTile", "stereoWriteDepth",
+           "stereoWriteHyp", "tsdfAutotune", "tsdfIntegrateHost", "tsdfGetBricksPerBlock",
+           "tsdfSetBricksPerBlock", "tsdfSetCounting", "tsdfSetDepthCulling", "tsdfUpdateCount",
+           "tsdfPassCounts", "tsdfWrite", "ctxSynchronize", "ctxSetProfiling", "ctxKernelStats"]
+# C-ABI entry points the addon reaches through another one
+THROUGH = {"lr3d_tsdf_integrate": "HipTsdfVolume.integrate passes its one view to lr3d_tsdf_integrate_batched"}
 
 
 def node(code, timeout=60):
@@ -33,6 +39,14 @@ def test_addon_exports_the_binding():
     lines = r.stdout.strip().split("\n")
     assert sorted(json.loads(lines[0])) == sorted(EXPORTS)
     assert lines[1] == "hip-single-scale,hip-coarse-to-fine"
+
+
+def test_addon_binds_every_entry_point_of_the_header():
+    import re
+    header = re.sub(r"/\*.*?\*/", "", open(os.path.join(ROOT, "include", "lr3d.h")).read(), flags=re.S)
+    declared = set(re.findall(r"\b(lr3d_[a-z0-9_]+)\s*\(", header))
+    called = set(re.findall(r"\b(lr3d_[a-z0-9_]+)\(", open(os.path.join(ROOT, "luvas.io-3d_amd", "node", "lr3d_napi.c")).read()))
+    assert sorted(declared - called - set(THROUGH)) == []
 
 
 def test_brickgrid_matches_python_host():
@@ -147,6 +161,14 @@ def test_node_pipeline_matches_python_host(tmp_path, ctx):
     hist = json.loads((tmp_path / "refine.json").read_text())
     assert len(hist) == 6 and np.isfinite(hist).all()
     assert all(b <= a for a, b in zip(hist, hist[1:])) and hist[-1] < hist[0]
+    # the rest of the C ABI through the Node host
+    ex = json.loads((tmp_path / "extras.json").read_text())
+    assert ex["noallocSame"] and ex["writeRoundTrip"] and ex["hostSame"]
+    assert ex["propagateLaunches"] > 0
+    assert ex["tilesSet"] == [16, 4] and set(ex["tiles"]) == {"init", "propagate", "pack"}
+    pc = ex["passCounts"]
+    assert pc["updates"] == ex["updateCount"] > 0 and pc["pairs"] >= pc["updates"] and pc["voxelsWritten"] > 0
+    assert ex["bricksPerBlock"] in (1, 2, 4)
     pyr = json.loads((tmp_path / "refine_pyramid.json").read_text())
     assert len(pyr["coarse"]) == 2
     for h in pyr["coarse"] + [pyr["fine"]]:

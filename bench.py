@@ -231,7 +231,8 @@ def main():
                     "traffic": None, "avg_launch_ms": round(prop_ms / prop_launches, 3),
                     "flop_per_launch": flops, "peak_unpacked": FP32_UNPACKED_TFLOPS,
                     "frac_unpacked": round(achieved / FP32_UNPACKED_TFLOPS, 4)}
-    # HBM bytes per launch measured by rocprofv3 counters for this workload (profiles/traffic.json)
+    # bytes per launch between L2 and memory (FETCH_SIZE + WRITE_SIZE, copy-calibrated) measured by
+    # rocprofv3 counters for this workload (profiles/traffic.json); an upper bound of HBM traffic
     measured = {}
     try:
         measured = json.load(open(os.path.join(ROOT, "profiles", "traffic.json"))).get(work.name, {})

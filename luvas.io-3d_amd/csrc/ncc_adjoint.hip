@@ -91,13 +91,22 @@ struct Tap {
     double u, v, rz, val, gu, gv;
 };
 
+// 1 / z to within an ulp or two of double precision: v_rcp_f64 and two Newton steps (the passes are
+// compared with the FP64 reference to 1e-4, far above that; the IEEE quotient's scaling and
+// special-case sequence is not needed for the normal, positive q.z of a differentiable view).
+__device__ __forceinline__ double rcp64(double z) {
+    double r = __builtin_amdgcn_rcp(z);
+    r = fma(fma(-z, r, 1.0), r, r);
+    return fma(fma(-z, r, 1.0), r, r);
+}
+
 __device__ __forceinline__ Tap source_tap(const uint32_t *__restrict__ img, int32_t qrows, double wmax, double hmax,
                                           const Mat3d &H, double X, double Y) {
     const double qx = fma(H.m[0], X, fma(H.m[1], Y, H.m[2]));
     const double qy = fma(H.m[3], X, fma(H.m[4], Y, H.m[5]));
     const double qz = fma(H.m[6], X, fma(H.m[7], Y, H.m[8]));
     Tap t;
-    t.rz = 1.0 / qz;
+    t.rz = rcp64(qz);
     t.u = fmin(fmax(qx * t.rz, 0.0), wmax);
     t.v = fmin(fmax(qy * t.rz, 0.0), hmax);
     const double x0 = floor(t.u), y0 = floor(t.v);

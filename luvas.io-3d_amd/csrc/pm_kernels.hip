@@ -132,12 +132,8 @@ __device__ __forceinline__ void region_offset(int region, int i, int &ox, int &o
 }
 
 // PM:349-423: one red-black phase.
-#ifndef LR3D_PM_WAVES_PER_EU
-#define LR3D_PM_WAVES_PER_EU 1
-#endif
 template <int KMAX, bool GEOMETRIC>
-__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(LR3D_PM_WAVES_PER_EU, 8)))
-pm_propagate_kernel(PmPlan plan, PmPools pools,
+__global__ void __launch_bounds__(256) pm_propagate_kernel(PmPlan plan, PmPools pools,
                                                            uint32_t iteration, uint32_t phase,
                                                            float step) {
     PixelTask task = locate(plan, 2 * (int32_t)blockDim.x);

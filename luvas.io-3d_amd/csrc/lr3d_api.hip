@@ -15,6 +15,7 @@
 #include <string.h>
 
 #include <algorithm>
+#include <cmath>
 #include <atomic>
 #include <map>
 #include <memory>
@@ -152,6 +153,7 @@ struct lr3d_ctx {
     bool closing = false;   // lr3d_ctx_destroy was called while users remained
     uint8_t *scratch = nullptr;  // this device's copy of `scratch` (device address)
     DevBuf<float> wtab;          // bilateral weight table of the PatchMatch window
+    DevBuf<float4> ntab;         // decoded normal of every packed octahedral code (TSDF pass)
 
     hipEvent_t take_event() {
         if (!event_pool.empty()) {
@@ -269,10 +271,13 @@ extern "C" int32_t lr3d_ctx_create(int32_t device, lr3d_ctx **out) {
     e = ctx->wtab.alloc(table.size());
     if (e == hipSuccess)
         e = hipMemcpy(ctx->wtab.ptr, table.data(), table.size() * sizeof(float), hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = ctx->ntab.alloc(4096);
+    if (e == hipSuccess) e = tsdf_launch_normal_table(ctx->stream, ctx->ntab.ptr);
+    if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
     if (e != hipSuccess) {
         delete ctx;
         return fail(e == hipErrorOutOfMemory ? LR3D_ERR_NOMEM : LR3D_ERR_DEVICE,
-                    "lr3d_ctx_create: weight table: %s", hipGetErrorString(e));
+                    "lr3d_ctx_create: weight and normal tables: %s", hipGetErrorString(e));
     }
     *out = ctx;
     return LR3D_OK;
@@ -1401,6 +1406,7 @@ extern "C" int32_t lr3d_tsdf_create(lr3d_ctx *ctx, const double origin[3], doubl
     ts->grid.voxel_d = voxel;
     ts->grid.mu_d = voxel * truncation_voxels;  // TV:179
     ts->grid.mu = (float)ts->grid.mu_d;
+    ts->grid.inv_mu = 1.0f / ts->grid.mu;
     ts->grid.n_bricks = n_bricks;
     const size_t voxels = (size_t)n_bricks * 512u;
     hipError_t e = ts->coords.alloc(n_bricks);
@@ -1525,6 +1531,30 @@ static const float *pass_priors(const lr3d_tsdf *ts, const std::vector<TsdfView>
     return nullptr;
 }
 
+// Whether a pass lies in the ranges where integrate_kernel's correctly rounded short forms equal
+// the IEEE divisions and square roots: every view's camera coordinates of the grid's voxel centres
+// below 2^40 in magnitude (bounded over the box of all the grid's voxel centres, with a margin far
+// above the FP32 rounding of cam = R world + t), |fx|, |fy| below 2^40, and mu in [2^-40, 2^40].
+static bool integrate_fast_ok(const lr3d_tsdf *ts, const std::vector<TsdfView> &tv) {
+    const double mu = ts->grid.mu;
+    if (!(mu >= 0x1p-40 && mu <= 0x1p40)) return false;
+    double m[3];
+    for (int a = 0; a < 3; a++) {
+        const double lo = (double)ts->grid.origin[a] + 0.5 * (double)ts->grid.voxel;
+        const double hi = (double)ts->grid.origin[a] + ((double)ts->dims[a] * 8.0 - 0.5) * (double)ts->grid.voxel;
+        m[a] = std::max(std::fabs(lo), std::fabs(hi));
+    }
+    for (const TsdfView &v : tv) {
+        if (!(std::fabs((double)v.k[0]) < 0x1p40 && std::fabs((double)v.k[1]) < 0x1p40)) return false;
+        for (int i = 0; i < 3; i++) {
+            const double b = std::fabs((double)v.R[3 * i]) * m[0] + std::fabs((double)v.R[3 * i + 1]) * m[1] +
+                             std::fabs((double)v.R[3 * i + 2]) * m[2] + std::fabs((double)v.t[i]);
+            if (!(b * 1.001 < 0x1p40)) return false;
+        }
+    }
+    return true;
+}
+
 static int32_t integrate_impl(lr3d_tsdf *ts, const uint32_t *maps, std::vector<TsdfView> &tv,
                               const lr3d_integration_view *views) {
     lr3d_ctx *ctx = ts->ctx;
@@ -1535,7 +1565,8 @@ static int32_t integrate_impl(lr3d_tsdf *ts, const uint32_t *maps, std::vector<T
     if (rc) return rc;
     HIP_TRY(timed(ctx, "tsdf_integrate", [&] {
         return tsdf_launch_integrate(ctx->stream, ts->grid, ts->views.ptr, n, ts->vis.ptr, words,
-                                     ts->coords.ptr, maps, pass_priors(ts, tv), ts->tsdf.ptr, ts->weight.ptr,
+                                     ts->coords.ptr, maps, pass_priors(ts, tv), ctx->ntab.ptr,
+                                     integrate_fast_ok(ts, tv), ts->tsdf.ptr, ts->weight.ptr,
                                      ts->counting ? ts->counter.ptr : nullptr, ts->bricks_per_block);
     }));
     return finish(ctx);
@@ -1724,8 +1755,8 @@ extern "C" int32_t lr3d_tsdf_autotune(lr3d_tsdf *ts, lr3d_stereo *st, const int3
             HIP_TRY(hipMemcpyAsync(ts->weight.ptr, s_weight.ptr, voxels * 4, hipMemcpyDeviceToDevice, ctx->stream));
             HIP_TRY(hipEventRecord(ev.a, ctx->stream));
             HIP_TRY(tsdf_launch_integrate(ctx->stream, ts->grid, ts->views.ptr, n, ts->vis.ptr, words, ts->coords.ptr,
-                                          maps, pass_priors(ts, tv), ts->tsdf.ptr, ts->weight.ptr, nullptr,
-                                          candidates[c]));
+                                          maps, pass_priors(ts, tv), ctx->ntab.ptr, integrate_fast_ok(ts, tv),
+                                          ts->tsdf.ptr, ts->weight.ptr, nullptr, candidates[c]));
             HIP_TRY(hipEventRecord(ev.b, ctx->stream));
             HIP_TRY(hipEventSynchronize(ev.b));
             float ms = 0.0f;

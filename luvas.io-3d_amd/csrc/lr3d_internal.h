@@ -147,6 +147,7 @@ struct TsdfGridDev {
     float origin[3];
     float voxel;
     float mu;
+    float inv_mu;  // RN(1 / mu), formed on the host
     int32_t n_bricks;
     double origin_d[3];
     double voxel_d;
@@ -187,11 +188,15 @@ hipError_t tsdf_launch_depth_tiles(hipStream_t s, const TsdfView *views, int32_t
 hipError_t tsdf_launch_visibility(hipStream_t s, const TsdfGridDev &g, const int4 *coords,
                                   const TsdfView *views, int32_t n_views, uint32_t *vis,
                                   int32_t words, const float *tiles);
+// ntab: the decoded normal of each packed octahedral code (tsdf_launch_normal_table); fast: the
+// pass lies in the ranges of the correctly rounded short forms (see integrate_kernel).
 hipError_t tsdf_launch_integrate(hipStream_t s, const TsdfGridDev &g, const TsdfView *views,
                                  int32_t n_views, const uint32_t *vis, int32_t words,
                                  const int4 *coords, const uint32_t *maps, const float *priors,
-                                 float *tsdf, float *weight, unsigned long long *counter,
-                                 int32_t bricks_per_block);
+                                 const float4 *ntab, bool fast, float *tsdf, float *weight,
+                                 unsigned long long *counter, int32_t bricks_per_block);
+// The 4096-entry normal table of tsdf_launch_integrate.
+hipError_t tsdf_launch_normal_table(hipStream_t s, float4 *tab);
 // TSDF adjoint over listed voxels (pool indices); n_voxels > 0.  ckpt holds
 // ceil(n_views / 16) * n_voxels float2; grad_depth (per pixel of each view, at pixel_off[v]) and
 // grad_pose (n_views * 12) are accumulated (zero them first); loss_partials: one per wavefront.

@@ -7,6 +7,7 @@
 #pragma once
 
 #include "lr3d_internal.h"
+#include "fp_exact.h"
 
 #include <math.h>
 
@@ -218,25 +219,9 @@ __device__ __forceinline__ void homography(const PmSrc &s, const PlaneTerms &p, 
 // the first such tap, with the same result).  Sample indices are clamped into the image so that
 // invalid taps still read inside the source image.
 //
-// 1/q.z: v_rcp_f32 followed by one Newton step with FMA reproduces the IEEE quotient 1.0f / q.z
-// bit for bit for every |q.z| in [2^-60, 2^61) (checked exhaustively on gfx950).  Valid taps
-// have q.z > 1e-6 > 2^-60; a view whose window could reach |q.z| >= 2^60 takes the IEEE
-// division for the whole wavefront.
-__device__ __forceinline__ float rcp_exact(float z) {
-    const float r = __builtin_amdgcn_rcpf(z);
-    return fmaf(fmaf(-z, r, 1.0f), r, r);
-}
-
-// IEEE sqrtf(x) for normal x >= 2^-96: v_sqrt_f32 is within 1 ulp, and the neighbour whose square
-// brackets x is selected with two FMA residuals (the compiler's correctly rounded expansion,
-// without its rescaling of tiny inputs and its zero/inf/NaN fix-ups, which this range never needs).
-__device__ __forceinline__ float sqrt_normal(float x) {
-    const float s = __builtin_amdgcn_sqrtf(x);
-    const float below = __int_as_float(__float_as_int(s) - 1);
-    const float above = __int_as_float(__float_as_int(s) + 1);
-    const float r = fmaf(-below, s, x) <= 0.0f ? below : s;
-    return fmaf(-above, s, x) > 0.0f ? above : r;
-}
+// 1/q.z: rcp_exact (fp_exact.h) is the IEEE quotient 1.0f / q.z for every |q.z| in
+// [2^-60, 2^61).  Valid taps have q.z > 1e-6 > 2^-60; a view whose window could reach
+// |q.z| >= 2^60 takes the IEEE division for the whole wavefront.
 
 template <bool FAST>
 __device__ __forceinline__ void view_taps(const uint32_t *__restrict__ img, int32_t W, int32_t H,

@@ -13,6 +13,7 @@
 // and stores the voxels that changed: per voxel the update sequence is exactly the reference's
 // one-view-at-a-time sequence, so one pass over n views equals n single-view passes.
 #include "lr3d_internal.h"
+#include "fp_exact.h"
 
 #include <math.h>
 
@@ -193,11 +194,28 @@ struct ViewGather {
     float prior[8];  // the prior at the pixel (views with a prior), else 0
 };
 
+// Correctly rounded forms of the pass's divisions and square roots (fp_exact.h) are used when the
+// host finds the pass inside their ranges (FAST; integrate_fast_ok): every view's camera
+// coordinates of the grid's voxel centres below 2^40 in magnitude, |fx|, |fy| below 2^40, and mu in
+// [2^-40, 2^40].  Then, with the same bits as the IEEE forms:
+//   - the projection divides k*cam by zc in [0.01, 2^40) with one shared reciprocal.  A quotient
+//     below 2^-70 in magnitude (where a residual could underflow and the last bit differ) is then
+//     added to the principal point: either |c| >= 2^-40 and both sums round to c, or both sums are
+//     below 0.5 and round to pixel 0;
+//   - sdf / mu: sdf = d - cam.z with d > 0 and cam.z > 0.01 is 0 or at least 2^-30 in magnitude
+//     (both are multiples of 2^-30 when d >= 2^-7, and |sdf| > 0.002 otherwise), so quotients below
+//     1 stay normal; quotients from 1 up, and the NaN an overflowing intermediate gives, all clamp to
+//     1 in fminf(1, .);
+//   - 1 / sqrt(|cam|^2) has |cam|^2 in [1e-4, 3 * 2^80];
+//   - the packed normal's decode (two divisions by 63, a square root and a division) is looked up
+//     in a 4096-entry table formed with the same IEEE arithmetic (normal_table_kernel).
+// The running mean's division keeps the IEEE form: uploaded volumes may hold any values.
+
 // TV:110-121 up to the map lookup.  cam = R world + t, row i = fmaf(R_i2, z, fmaf(R_i1, y, fmaf(R_i0, x, t_i))).
 // Every lane loads (a skipped voxel reads pixel 0 and is marked idx = -1), so the loads of a view
 // are straight-line code and the waits before their use count only the loads issued after them.
 // PRIORS: the pass has views with a prior; the prior at the pixel is gathered for those views.
-template <bool PRIORS>
+template <bool PRIORS, bool FAST>
 __device__ __forceinline__ void gather_view(const TsdfView &V, const uint32_t *__restrict__ maps,
                                             const float *__restrict__ priors, float wx, float wy, const float wz[8],
                                             ViewGather &out) {
@@ -212,8 +230,17 @@ __device__ __forceinline__ void gather_view(const TsdfView &V, const uint32_t *_
         const float cam2 = fmaf(V.R[8], wz[z], c2);
         const bool front = cam2 > 0.01f;
         const float zc = front ? cam2 : 1.0f;
-        const int32_t px = (int32_t)rintf((V.k[0] * cam0) / zc + V.k[2]);
-        const int32_t py = (int32_t)rintf((V.k[1] * cam1) / zc + V.k[3]);
+        float qu, qv;
+        if (FAST) {
+            const float y = rcp_exact(zc);
+            qu = div_rn(V.k[0] * cam0, zc, y);
+            qv = div_rn(V.k[1] * cam1, zc, y);
+        } else {
+            qu = (V.k[0] * cam0) / zc;
+            qv = (V.k[1] * cam1) / zc;
+        }
+        const int32_t px = (int32_t)rintf(qu + V.k[2]);
+        const int32_t py = (int32_t)rintf(qv + V.k[3]);
         const bool in = front && px >= 0 && py >= 0 && px < V.W && py < V.H;
         const uint32_t i = in ? (uint32_t)py * (uint32_t)V.W + (uint32_t)px : 0u;
         out.idx[z] = in ? (int32_t)i : -1;
@@ -224,8 +251,9 @@ __device__ __forceinline__ void gather_view(const TsdfView &V, const uint32_t *_
 }
 
 // TV:122-153 from the map lookup on: the update of a lane's 8 voxels by one view.
-template <bool COUNT>
-__device__ __forceinline__ void apply_view(const TsdfView &V, const ViewGather &gv, float mu, float wx, float wy,
+template <bool COUNT, bool FAST>
+__device__ __forceinline__ void apply_view(const TsdfView &V, const ViewGather &gv, float mu, float inv_mu,
+                                           const float4 *__restrict__ ntab, float wx, float wy,
                                            const float wz[8], float ts[8], float ws[8], uint32_t &changed,
                                            uint32_t &updates) {
     const float c0 = fmaf(V.R[1], wy, fmaf(V.R[0], wx, V.t[0]));
@@ -247,9 +275,18 @@ __device__ __forceinline__ void apply_view(const TsdfView &V, const ViewGather &
         // The weight only matters for an update, so it is evaluated after the skips.
         float w;
         if (from_map) {
-            float nx, ny, nz;
-            unpack_normal(packed, nx, ny, nz);
-            const float inv = 1.0f / sqrtf(cam0 * cam0 + cam1 * cam1 + cam2 * cam2);
+            float nx, ny, nz, inv;
+            const float ss = cam0 * cam0 + cam1 * cam1 + cam2 * cam2;
+            if (FAST) {
+                const float4 n = ntab[(packed >> 16u) & 4095u];
+                nx = n.x;
+                ny = n.y;
+                nz = n.z;
+                inv = rcp_exact(sqrt_normal(ss));
+            } else {
+                unpack_normal(packed, nx, ny, nz);
+                inv = 1.0f / sqrtf(ss);
+            }
             w = fmaxf(0.2f, fabsf(nx * (cam0 * inv) + ny * (cam1 * inv) + nz * (cam2 * inv)));
             if (V.confidence > 0.0f) {
                 const float cc = (float)(packed >> 28u) / 8.0f;
@@ -259,7 +296,7 @@ __device__ __forceinline__ void apply_view(const TsdfView &V, const ViewGather &
         } else {
             w = kPriorWeight;
         }
-        const float observed = fminf(1.0f, sdf / mu);
+        const float observed = fminf(1.0f, FAST ? div_rn(sdf, mu, inv_mu) : sdf / mu);
         const float w0 = ws[z];
         ts[z] = (ts[z] * w0 + observed * w) / (w0 + w);
         ws[z] = fminf(w0 + w, kMaxWeight);
@@ -268,17 +305,28 @@ __device__ __forceinline__ void apply_view(const TsdfView &V, const ViewGather &
     }
 }
 
+// The decoded normal of every packed octahedral code: entry i = unpack_normal of the code with
+// x = i % 64 and y = i / 64 (bits 16-27 of a packed sample), in the same IEEE arithmetic.
+__global__ void __launch_bounds__(256) normal_table_kernel(float4 *__restrict__ tab) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= 4096u) return;
+    float nx, ny, nz;
+    unpack_normal(i << 16u, nx, ny, nz);
+    tab[i] = make_float4(nx, ny, nz, 0.0f);
+}
+
 // TV:103-154, one wavefront per brick, all visible views of the pass in order.  Visible views are
 // taken kViewGroup at a time: the group's depth samples are all gathered before the first view's
 // updates, so the later views' gathers are in flight while the earlier ones are applied (a brick's
 // chain of views is not one gather latency per view); the updates are the same sequence as
 // applying the views one at a time.  Groups of 2 measured fastest (4 and 6 hold more registers).
-template <bool COUNT, bool PRIORS>
+template <bool COUNT, bool PRIORS, bool FAST>
 __global__ void __launch_bounds__(256) integrate_kernel(TsdfGridDev g, const TsdfView *__restrict__ views,
                                                         int32_t n_views, const uint32_t *__restrict__ vis,
                                                         int32_t words, const int4 *__restrict__ coords,
                                                         const uint32_t *__restrict__ maps,
                                                         const float *__restrict__ priors,
+                                                        const float4 *__restrict__ ntab,
                                                         float *__restrict__ tsdf, float *__restrict__ weight,
                                                         unsigned long long *__restrict__ counter) {
     // one brick per wavefront: the brick, its visibility words and the views are wave-uniform
@@ -333,10 +381,12 @@ __global__ void __launch_bounds__(256) integrate_kernel(TsdfGridDev g, const Tsd
         ViewGather grp[kViewGroup];
 #pragma unroll
         for (int k = 0; k < kViewGroup; k++)
-            if (k < n) gather_view<PRIORS>(views[vs[k]], maps, priors, wx, wy, wz, grp[k]);
+            if (k < n) gather_view<PRIORS, FAST>(views[vs[k]], maps, priors, wx, wy, wz, grp[k]);
 #pragma unroll
         for (int k = 0; k < kViewGroup; k++)
-            if (k < n) apply_view<COUNT>(views[vs[k]], grp[k], mu, wx, wy, wz, ts, ws, changed, updates);
+            if (k < n)
+                apply_view<COUNT, FAST>(views[vs[k]], grp[k], mu, g.inv_mu, ntab, wx, wy, wz, ts, ws, changed,
+                                        updates);
         v = n == kViewGroup ? next_view() : -1;
     }
 #pragma unroll
@@ -591,27 +641,46 @@ hipError_t tsdf_launch_visibility(hipStream_t s, const TsdfGridDev &g, const int
     return hipGetLastError();
 }
 
+hipError_t tsdf_launch_normal_table(hipStream_t s, float4 *tab) {
+    hipLaunchKernelGGL(normal_table_kernel, dim3(16), dim3(256), 0, s, tab);
+    return hipGetLastError();
+}
+
+template <bool FAST>
+static void launch_integrate(hipStream_t s, dim3 grid, dim3 block, const TsdfGridDev &g, const TsdfView *views,
+                             int32_t n_views, const uint32_t *vis, int32_t words, const int4 *coords,
+                             const uint32_t *maps, const float *priors, const float4 *ntab, float *tsdf,
+                             float *weight, unsigned long long *counter) {
+    // priors != nullptr: some view of the pass has a prior (the caller passes nullptr otherwise)
+    if (counter && priors)
+        hipLaunchKernelGGL((integrate_kernel<true, true, FAST>), grid, block, 0, s, g, views, n_views, vis, words,
+                           coords, maps, priors, ntab, tsdf, weight, counter);
+    else if (counter)
+        hipLaunchKernelGGL((integrate_kernel<true, false, FAST>), grid, block, 0, s, g, views, n_views, vis, words,
+                           coords, maps, priors, ntab, tsdf, weight, counter);
+    else if (priors)
+        hipLaunchKernelGGL((integrate_kernel<false, true, FAST>), grid, block, 0, s, g, views, n_views, vis, words,
+                           coords, maps, priors, ntab, tsdf, weight, counter);
+    else
+        hipLaunchKernelGGL((integrate_kernel<false, false, FAST>), grid, block, 0, s, g, views, n_views, vis, words,
+                           coords, maps, priors, ntab, tsdf, weight, counter);
+}
+
 hipError_t tsdf_launch_integrate(hipStream_t s, const TsdfGridDev &g, const TsdfView *views,
                                  int32_t n_views, const uint32_t *vis, int32_t words,
                                  const int4 *coords, const uint32_t *maps, const float *priors,
-                                 float *tsdf, float *weight, unsigned long long *counter,
-                                 int32_t bricks_per_block) {
+                                 const float4 *ntab, bool fast, float *tsdf, float *weight,
+                                 unsigned long long *counter, int32_t bricks_per_block) {
     if (bricks_per_block != 1 && bricks_per_block != 2 && bricks_per_block != 4) return hipErrorInvalidValue;
+    if (fast && !ntab) return hipErrorInvalidValue;
     const uint32_t bpb = (uint32_t)bricks_per_block;
     dim3 block(64u * bpb), grid(((uint32_t)g.n_bricks + bpb - 1u) / bpb);
-    // priors != nullptr: some view of the pass has a prior (the caller passes nullptr otherwise)
-    if (counter && priors)
-        hipLaunchKernelGGL((integrate_kernel<true, true>), grid, block, 0, s, g, views, n_views, vis, words,
-                           coords, maps, priors, tsdf, weight, counter);
-    else if (counter)
-        hipLaunchKernelGGL((integrate_kernel<true, false>), grid, block, 0, s, g, views, n_views, vis, words,
-                           coords, maps, priors, tsdf, weight, counter);
-    else if (priors)
-        hipLaunchKernelGGL((integrate_kernel<false, true>), grid, block, 0, s, g, views, n_views, vis, words,
-                           coords, maps, priors, tsdf, weight, counter);
+    if (fast)
+        launch_integrate<true>(s, grid, block, g, views, n_views, vis, words, coords, maps, priors, ntab, tsdf,
+                               weight, counter);
     else
-        hipLaunchKernelGGL((integrate_kernel<false, false>), grid, block, 0, s, g, views, n_views, vis, words,
-                           coords, maps, priors, tsdf, weight, counter);
+        launch_integrate<false>(s, grid, block, g, views, n_views, vis, words, coords, maps, priors, ntab, tsdf,
+                                weight, counter);
     return hipGetLastError();
 }
 

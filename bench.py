@@ -77,7 +77,7 @@ def main():
     ap.add_argument("--no-profile", action="store_true", help="time without kernel events")
     ap.add_argument("--no-autotune", action="store_true", help="keep the default launch shapes")
     ap.add_argument("--tiles", default=None,
-                    help="pin launch shapes instead of tuning, e.g. init=16x8,propagate=8x8,pack=64x4,bricks=4")
+                    help="pin launch shapes instead of tuning, e.g. init=16x8,propagate=8x8,pack=64x4,bricks=4,chunk=64")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU only; --gpus must be 1",
@@ -146,11 +146,14 @@ def main():
             k, v = item.split("=")
             if k == "bricks":
                 vol.bricks_per_block = int(v)
+            elif k == "chunk":
+                vol.xcd_chunk = int(v)
             else:
                 tx, ty = (int(x) for x in v.split("x"))
                 st.set_tile(k, tx, ty)
         tiles = {k: list(v) for k, v in st.tiles().items()}
         tiles["tsdf_bricks_per_block"] = vol.bricks_per_block
+        tiles["tsdf_xcd_chunk"] = vol.xcd_chunk
         log(f"pinned launch shapes: {tiles}")
     elif not args.no_autotune:
         t_tune = time.perf_counter()
@@ -161,6 +164,7 @@ def main():
         bricks = vol.autotune(st, list(range(n)), iviews, map=lr3d.MAP_FILTERED, repeats=3)
         tiles = {k: list(v) for k, v in tiles.items()}
         tiles["tsdf_bricks_per_block"] = bricks
+        tiles["tsdf_xcd_chunk"] = vol.xcd_chunk
         log(f"autotune ({time.perf_counter() - t_tune:.1f}s): {tiles}")
 
     for w in range(args.warmup):
@@ -292,6 +296,7 @@ def main():
         for k, (tx, ty) in st.tiles().items():  # same launch shapes as the batched path
             st2.set_tile(k, tx, ty)
         vol2.bricks_per_block = vol.bricks_per_block
+        vol2.xcd_chunk = vol.xcd_chunk
         batched_adjoint = last["adjoint"]
         ctx.synchronize()
         t1 = time.perf_counter()

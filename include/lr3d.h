@@ -223,12 +223,19 @@ int32_t lr3d_tsdf_backward(lr3d_tsdf *ts, lr3d_stereo *st, const int32_t *layers
                            const float *targets, int64_t n_voxels, float *tsdf_out, float *grad_depth,
                            double *grad_pose, double *loss);
 
-/* Times integrate passes of these views with 1, 2 and 4 bricks (wavefronts) per block and keeps
- * the fastest for later passes; the volume is restored.  No counterpart in the reference. */
+/* Times integrate passes of these views with 1, 2 and 4 bricks (wavefronts) per block, each with
+ * blocks dealt to the GPU's 8 XCDs one at a time or in chunks of 16, 32, 64, 128 and one grid row of
+ * consecutive blocks, and keeps the fastest pair for later passes; the volume is restored.  No
+ * counterpart in the reference. */
 int32_t lr3d_tsdf_autotune(lr3d_tsdf *ts, lr3d_stereo *st, const int32_t *layers, int32_t map,
                            const lr3d_integration_view *views, int32_t n, int32_t repeats);
 int32_t lr3d_tsdf_get_bricks_per_block(lr3d_tsdf *ts, int32_t *bricks);
 int32_t lr3d_tsdf_set_bricks_per_block(lr3d_tsdf *ts, int32_t bricks);
+/* Consecutive blocks of the integration pass given to one XCD (neighbouring bricks then share the
+ * depth-map lines of their footprints in that XCD's L2): 1 (round robin, the default) to 4096.
+ * Results do not depend on it.  No counterpart in the reference. */
+int32_t lr3d_tsdf_get_xcd_chunk(lr3d_tsdf *ts, int32_t *chunk);
+int32_t lr3d_tsdf_set_xcd_chunk(lr3d_tsdf *ts, int32_t chunk);
 
 /* Depth culling (on by default): before a pass, every view's map is reduced to the largest usable
  * depth per 16x16 and 128x128 pixels, and a (brick, view) pair that the reference's culling keeps

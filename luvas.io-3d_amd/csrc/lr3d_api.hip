@@ -1361,6 +1361,7 @@ struct lr3d_tsdf {
     DevBuf<unsigned long long> counter;
     bool counting = false;
     int32_t bricks_per_block = 4;  // integrate_kernel: wavefronts (bricks) per block; set by the autotuner
+    int32_t xcd_chunk = 1;         // integrate_kernel: consecutive blocks per XCD (1: round robin); autotuned
     std::vector<int32_t> coords_host;  // brick grid coordinates (4 per brick), for the mesher's lookup
     DevBuf<int32_t> lookup;            // grid index -> pool brick or -1 (built on first use)
     // adjoint scratch
@@ -1567,7 +1568,8 @@ static int32_t integrate_impl(lr3d_tsdf *ts, const uint32_t *maps, std::vector<T
         return tsdf_launch_integrate(ctx->stream, ts->grid, ts->views.ptr, n, ts->vis.ptr, words,
                                      ts->coords.ptr, maps, pass_priors(ts, tv), ctx->ntab.ptr,
                                      integrate_fast_ok(ts, tv), ts->tsdf.ptr, ts->weight.ptr,
-                                     ts->counting ? ts->counter.ptr : nullptr, ts->bricks_per_block);
+                                     ts->counting ? ts->counter.ptr : nullptr, ts->bricks_per_block,
+                                     ts->xcd_chunk);
     }));
     return finish(ctx);
 }
@@ -1721,6 +1723,22 @@ extern "C" int32_t lr3d_tsdf_set_bricks_per_block(lr3d_tsdf *ts, int32_t bricks)
     return LR3D_OK;
 }
 
+extern "C" int32_t lr3d_tsdf_get_xcd_chunk(lr3d_tsdf *ts, int32_t *out) {
+    if (!ts || !out) return fail(LR3D_ERR_INVALID, "lr3d_tsdf_get_xcd_chunk: NULL argument");
+    std::lock_guard<std::mutex> lock(ts->ctx->mu);
+    *out = ts->xcd_chunk;
+    return LR3D_OK;
+}
+
+extern "C" int32_t lr3d_tsdf_set_xcd_chunk(lr3d_tsdf *ts, int32_t chunk) {
+    if (!ts) return fail(LR3D_ERR_INVALID, "lr3d_tsdf_set_xcd_chunk: NULL argument");
+    if (chunk < 1 || chunk > kMaxXcdChunk)
+        return fail(LR3D_ERR_INVALID, "lr3d_tsdf_set_xcd_chunk: %d is outside [1, %d]", chunk, kMaxXcdChunk);
+    std::lock_guard<std::mutex> lock(ts->ctx->mu);
+    ts->xcd_chunk = chunk;
+    return LR3D_OK;
+}
+
 extern "C" int32_t lr3d_tsdf_autotune(lr3d_tsdf *ts, lr3d_stereo *st, const int32_t *layers, int32_t map,
                                       const lr3d_integration_view *views, int32_t n, int32_t repeats) {
     if (!ts || !layers || !views || n <= 0) return fail(LR3D_ERR_INVALID, "lr3d_tsdf_autotune: NULL argument or n <= 0");
@@ -1745,30 +1763,43 @@ extern "C" int32_t lr3d_tsdf_autotune(lr3d_tsdf *ts, lr3d_stereo *st, const int3
     EventPair ev;
     HIP_TRY(hipEventCreate(&ev.a));
     HIP_TRY(hipEventCreate(&ev.b));
-    const int32_t candidates[3] = {1, 2, 4};
+    // candidates: bricks (wavefronts) per block x chunks of consecutive blocks per XCD (round robin,
+    // a few sizes, and the length of a row of the grid's bricks)
+    std::vector<int32_t> chunks = {1, 16, 32, 64, 128};
+    if (ts->dims[0] > 1 && ts->dims[0] <= kMaxXcdChunk &&
+        std::find(chunks.begin(), chunks.end(), ts->dims[0]) == chunks.end())
+        chunks.push_back(ts->dims[0]);
+    const bool fast = integrate_fast_ok(ts, tv);
+    const float *priors = pass_priors(ts, tv);
     float best_ms = 0.0f;
-    int32_t best = ts->bricks_per_block;
-    for (int c = 0; c < 3; c++) {
-        float total = 0.0f;
-        for (int32_t k = 0; k < repeats; k++) {
-            HIP_TRY(hipMemcpyAsync(ts->tsdf.ptr, s_tsdf.ptr, voxels * 4, hipMemcpyDeviceToDevice, ctx->stream));
-            HIP_TRY(hipMemcpyAsync(ts->weight.ptr, s_weight.ptr, voxels * 4, hipMemcpyDeviceToDevice, ctx->stream));
-            HIP_TRY(hipEventRecord(ev.a, ctx->stream));
-            HIP_TRY(tsdf_launch_integrate(ctx->stream, ts->grid, ts->views.ptr, n, ts->vis.ptr, words, ts->coords.ptr,
-                                          maps, pass_priors(ts, tv), ctx->ntab.ptr, integrate_fast_ok(ts, tv),
-                                          ts->tsdf.ptr, ts->weight.ptr, nullptr, candidates[c]));
-            HIP_TRY(hipEventRecord(ev.b, ctx->stream));
-            HIP_TRY(hipEventSynchronize(ev.b));
-            float ms = 0.0f;
-            HIP_TRY(hipEventElapsedTime(&ms, ev.a, ev.b));
-            total += ms;
-        }
-        if (c == 0 || total < best_ms) {
-            best_ms = total;
-            best = candidates[c];
+    int32_t best_bricks = ts->bricks_per_block, best_chunk = ts->xcd_chunk;
+    bool first = true;
+    for (const int32_t bricks : {1, 2, 4}) {
+        for (const int32_t chunk : chunks) {
+            float total = 0.0f;
+            for (int32_t k = 0; k < repeats; k++) {
+                HIP_TRY(hipMemcpyAsync(ts->tsdf.ptr, s_tsdf.ptr, voxels * 4, hipMemcpyDeviceToDevice, ctx->stream));
+                HIP_TRY(hipMemcpyAsync(ts->weight.ptr, s_weight.ptr, voxels * 4, hipMemcpyDeviceToDevice, ctx->stream));
+                HIP_TRY(hipEventRecord(ev.a, ctx->stream));
+                HIP_TRY(tsdf_launch_integrate(ctx->stream, ts->grid, ts->views.ptr, n, ts->vis.ptr, words,
+                                              ts->coords.ptr, maps, priors, ctx->ntab.ptr, fast, ts->tsdf.ptr,
+                                              ts->weight.ptr, nullptr, bricks, chunk));
+                HIP_TRY(hipEventRecord(ev.b, ctx->stream));
+                HIP_TRY(hipEventSynchronize(ev.b));
+                float ms = 0.0f;
+                HIP_TRY(hipEventElapsedTime(&ms, ev.a, ev.b));
+                total += ms;
+            }
+            if (first || total < best_ms) {
+                best_ms = total;
+                best_bricks = bricks;
+                best_chunk = chunk;
+                first = false;
+            }
         }
     }
-    ts->bricks_per_block = best;
+    ts->bricks_per_block = best_bricks;
+    ts->xcd_chunk = best_chunk;
     HIP_TRY(hipMemcpyAsync(ts->tsdf.ptr, s_tsdf.ptr, voxels * 4, hipMemcpyDeviceToDevice, ctx->stream));
     HIP_TRY(hipMemcpyAsync(ts->weight.ptr, s_weight.ptr, voxels * 4, hipMemcpyDeviceToDevice, ctx->stream));
     return finish(ctx);

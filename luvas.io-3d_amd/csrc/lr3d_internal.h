@@ -157,6 +157,15 @@ struct TsdfGridDev {
 // Kernel launchers (stream-ordered; no synchronisation).
 hipError_t pm_build_quad(hipStream_t s, const uint8_t *gray, int32_t W, int32_t H, int32_t qrows, uint32_t *quad);
 
+// Hardware block i of an n-block launch runs on XCD i % 8: the logical block it takes, such that
+// each XCD gets a contiguous range of logical blocks (neighbouring blocks then share that XCD's L2).
+__device__ __forceinline__ uint32_t xcd_remap(uint32_t i, uint32_t n) {
+    constexpr uint32_t kXcds = 8;
+    uint32_t xcd = i % kXcds, local = i / kXcds;
+    uint32_t per = n / kXcds, rem = n % kXcds;
+    return xcd * per + (xcd < rem ? xcd : rem) + local;
+}
+
 // Rows per column block of an H-tall layer in the quad pool: H rounded up so that a block is an
 // odd number of 128-byte lines (qrows / 2 odd), so that the blocks a gather straddles start in
 // different L1 tag banks.
@@ -194,7 +203,9 @@ hipError_t tsdf_launch_integrate(hipStream_t s, const TsdfGridDev &g, const Tsdf
                                  int32_t n_views, const uint32_t *vis, int32_t words,
                                  const int4 *coords, const uint32_t *maps, const float *priors,
                                  const float4 *ntab, bool fast, float *tsdf, float *weight,
-                                 unsigned long long *counter, int32_t bricks_per_block);
+                                 unsigned long long *counter, int32_t bricks_per_block, int32_t chunk);
+// Largest chunk of consecutive blocks dealt to one XCD by tsdf_launch_integrate.
+constexpr int32_t kMaxXcdChunk = 4096;
 // The 4096-entry normal table of tsdf_launch_integrate.
 hipError_t tsdf_launch_normal_table(hipStream_t s, float4 *tab);
 // TSDF adjoint over listed voxels (pool indices); n_voxels > 0.  ckpt holds

@@ -109,14 +109,6 @@ __device__ __forceinline__ uint32_t pack_sample(float depth, float nx, float ny,
 
 // ---- work mapping ---------------------------------------------------------------------------
 
-// Hardware block i runs on XCD i % 8; give each XCD a contiguous range of logical blocks.
-__device__ __forceinline__ uint32_t xcd_remap(uint32_t i, uint32_t n) {
-    constexpr uint32_t kXcds = 8;
-    uint32_t xcd = i % kXcds, local = i / kXcds;
-    uint32_t per = n / kXcds, rem = n % kXcds;
-    return xcd * per + (xcd < rem ? xcd : rem) + local;
-}
-
 // View owning logical block b (uniform: block_start is read through the scalar cache).
 __device__ __forceinline__ int32_t find_view(const uint32_t *__restrict__ block_start, int32_t n,
                                              uint32_t b) {

@@ -328,10 +328,20 @@ __global__ void __launch_bounds__(256) integrate_kernel(TsdfGridDev g, const Tsd
                                                         const float *__restrict__ priors,
                                                         const float4 *__restrict__ ntab,
                                                         float *__restrict__ tsdf, float *__restrict__ weight,
-                                                        unsigned long long *__restrict__ counter) {
+                                                        unsigned long long *__restrict__ counter, uint32_t chunk) {
     // one brick per wavefront: the brick, its visibility words and the views are wave-uniform
-    // (scalar loads, and the prior branch below is decided per view, not per lane)
-    const uint32_t brick = __builtin_amdgcn_readfirstlane(blockIdx.x * (blockDim.x / 64u) + (threadIdx.x >> 6));
+    // (scalar loads, and the prior branch below is decided per view, not per lane).  Hardware
+    // block i runs on XCD i % 8; blocks are dealt to the XCDs in chunks of `chunk` consecutive
+    // blocks, so neighbouring bricks run on one XCD and share the depth-map lines of their
+    // footprints in its L2, while every XCD still takes work from the whole pool (chunk 1: plain
+    // round robin).  The blocks past the last whole round keep their index.
+    uint32_t block = blockIdx.x;
+    const uint32_t per_round = 8u * chunk;
+    if (chunk > 1u && block < gridDim.x / per_round * per_round) {
+        const uint32_t round = block / per_round, r = block % per_round;
+        block = round * per_round + (r % 8u) * chunk + r / 8u;
+    }
+    const uint32_t brick = __builtin_amdgcn_readfirstlane(block * (blockDim.x / 64u) + (threadIdx.x >> 6));
     const uint32_t lane = threadIdx.x & 63u;
     if (brick >= (uint32_t)g.n_bricks) return;
     const int4 c = coords[brick];
@@ -650,37 +660,38 @@ template <bool FAST>
 static void launch_integrate(hipStream_t s, dim3 grid, dim3 block, const TsdfGridDev &g, const TsdfView *views,
                              int32_t n_views, const uint32_t *vis, int32_t words, const int4 *coords,
                              const uint32_t *maps, const float *priors, const float4 *ntab, float *tsdf,
-                             float *weight, unsigned long long *counter) {
+                             float *weight, unsigned long long *counter, uint32_t chunk) {
     // priors != nullptr: some view of the pass has a prior (the caller passes nullptr otherwise)
     if (counter && priors)
         hipLaunchKernelGGL((integrate_kernel<true, true, FAST>), grid, block, 0, s, g, views, n_views, vis, words,
-                           coords, maps, priors, ntab, tsdf, weight, counter);
+                           coords, maps, priors, ntab, tsdf, weight, counter, chunk);
     else if (counter)
         hipLaunchKernelGGL((integrate_kernel<true, false, FAST>), grid, block, 0, s, g, views, n_views, vis, words,
-                           coords, maps, priors, ntab, tsdf, weight, counter);
+                           coords, maps, priors, ntab, tsdf, weight, counter, chunk);
     else if (priors)
         hipLaunchKernelGGL((integrate_kernel<false, true, FAST>), grid, block, 0, s, g, views, n_views, vis, words,
-                           coords, maps, priors, ntab, tsdf, weight, counter);
+                           coords, maps, priors, ntab, tsdf, weight, counter, chunk);
     else
         hipLaunchKernelGGL((integrate_kernel<false, false, FAST>), grid, block, 0, s, g, views, n_views, vis, words,
-                           coords, maps, priors, ntab, tsdf, weight, counter);
+                           coords, maps, priors, ntab, tsdf, weight, counter, chunk);
 }
 
 hipError_t tsdf_launch_integrate(hipStream_t s, const TsdfGridDev &g, const TsdfView *views,
                                  int32_t n_views, const uint32_t *vis, int32_t words,
                                  const int4 *coords, const uint32_t *maps, const float *priors,
                                  const float4 *ntab, bool fast, float *tsdf, float *weight,
-                                 unsigned long long *counter, int32_t bricks_per_block) {
+                                 unsigned long long *counter, int32_t bricks_per_block, int32_t chunk) {
     if (bricks_per_block != 1 && bricks_per_block != 2 && bricks_per_block != 4) return hipErrorInvalidValue;
+    if (chunk < 1 || chunk > kMaxXcdChunk) return hipErrorInvalidValue;
     if (fast && !ntab) return hipErrorInvalidValue;
     const uint32_t bpb = (uint32_t)bricks_per_block;
     dim3 block(64u * bpb), grid(((uint32_t)g.n_bricks + bpb - 1u) / bpb);
     if (fast)
         launch_integrate<true>(s, grid, block, g, views, n_views, vis, words, coords, maps, priors, ntab, tsdf,
-                               weight, counter);
+                               weight, counter, (uint32_t)chunk);
     else
         launch_integrate<false>(s, grid, block, g, views, n_views, vis, words, coords, maps, priors, ntab, tsdf,
-                                weight, counter);
+                                weight, counter, (uint32_t)chunk);
     return hipGetLastError();
 }
 

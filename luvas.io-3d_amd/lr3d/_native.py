@@ -36,7 +36,8 @@ EXPORTED = (
     "lr3d_tsdf_integrate_batched", "lr3d_tsdf_read", "lr3d_tsdf_set_counting",
     "lr3d_tsdf_update_count", "lr3d_tsdf_pass_counts", "lr3d_tsdf_backward", "lr3d_stereo_autotune", "lr3d_stereo_get_tile",
     "lr3d_stereo_set_tile", "lr3d_tsdf_autotune", "lr3d_tsdf_get_bricks_per_block",
-    "lr3d_tsdf_set_bricks_per_block", "lr3d_tsdf_set_depth_culling", "lr3d_tsdf_write", "lr3d_tsdf_extract_surface",
+    "lr3d_tsdf_set_bricks_per_block", "lr3d_tsdf_get_xcd_chunk", "lr3d_tsdf_set_xcd_chunk",
+    "lr3d_tsdf_set_depth_culling", "lr3d_tsdf_write", "lr3d_tsdf_extract_surface",
     "lr3d_mesh_remove_small_components", "lr3d_mesh_size", "lr3d_mesh_read", "lr3d_mesh_destroy",
     "lr3d_fuse_depth_maps", "lr3d_cloud_size", "lr3d_cloud_read", "lr3d_cloud_destroy",
 )
@@ -120,6 +121,8 @@ def load() -> C.CDLL:
         "lr3d_tsdf_autotune": (i32, [P, P, C.c_void_p, i32, C.c_void_p, i32, i32]),
         "lr3d_tsdf_get_bricks_per_block": (i32, [P, C.POINTER(i32)]),
         "lr3d_tsdf_set_bricks_per_block": (i32, [P, i32]),
+        "lr3d_tsdf_get_xcd_chunk": (i32, [P, C.POINTER(i32)]),
+        "lr3d_tsdf_set_xcd_chunk": (i32, [P, i32]),
         "lr3d_tsdf_set_depth_culling": (i32, [P, i32]),
         "lr3d_tsdf_write": (i32, [P, C.c_void_p, C.c_void_p]),
         "lr3d_tsdf_extract_surface": (i32, [P, dbl, C.POINTER(P)]),

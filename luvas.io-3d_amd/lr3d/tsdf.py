@@ -143,7 +143,9 @@ class HipTsdfVolume:
 
     def autotune(self, stereo, layers: Sequence[int], views: Sequence[IntegrationView], map: int = N.MAP_RAW,
                  repeats: int = 3) -> int:
-        """Times integrate passes with 1, 2 and 4 bricks per block, keeps the fastest (returned)."""
+        """Times integrate passes with 1, 2 and 4 bricks per block, each with several chunks of
+        consecutive blocks per XCD, and keeps the fastest pair; returns the bricks per block (the
+        chunk is `xcd_chunk`)."""
         structs, keep = [], []
         for v in views:
             s, k = _iv_struct(v)
@@ -163,6 +165,17 @@ class HipTsdfVolume:
     @bricks_per_block.setter
     def bricks_per_block(self, bricks: int) -> None:
         N.check(self._lib.lr3d_tsdf_set_bricks_per_block(self.handle, int(bricks)))
+
+    @property
+    def xcd_chunk(self) -> int:
+        """Consecutive blocks of the integration pass given to one XCD (1: round robin)."""
+        v = C.c_int32()
+        N.check(self._lib.lr3d_tsdf_get_xcd_chunk(self.handle, C.byref(v)))
+        return v.value
+
+    @xcd_chunk.setter
+    def xcd_chunk(self, chunk: int) -> None:
+        N.check(self._lib.lr3d_tsdf_set_xcd_chunk(self.handle, int(chunk)))
 
     def set_depth_culling(self, on: bool) -> None:
         """Drop (brick, view) pairs that cannot update a voxel (on by default; results unchanged)."""

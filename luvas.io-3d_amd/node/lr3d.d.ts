@@ -124,9 +124,12 @@ export declare class HipTsdfVolume {
     integrateLayers(layers: number[], views: IntegrationView[], map?: number): Promise<void>;
     /** A packed map from host memory (width * height words). */
     integrateHost(packed: Uint32Array, view: IntegrationView): Promise<void>;
-    /** Times bricks per block (1, 2, 4) on a pass of these layers and keeps the fastest. */
+    /** Times bricks per block (1, 2, 4) and chunks of consecutive blocks per XCD on a pass of these
+     * layers and keeps the fastest pair; resolves to the bricks per block. */
     autotune(layers: number[], views: IntegrationView[], map?: number, repeats?: number): Promise<number>;
     bricksPerBlock: number;
+    /** Consecutive blocks of the integration pass given to one XCD (1: round robin). */
+    xcdChunk: number;
     setDepthCulling(on: boolean): void;
     setCounting(on: boolean): void;
     updateCount(): number;

@@ -337,7 +337,8 @@ class HipTsdfVolume {
         await native.tsdfIntegrateHost(this.handle, packed, HipTsdfVolume.viewOf(view, this.width, this.height));
     }
 
-    /** Times bricks per block (1, 2, 4) on a pass of these layers and keeps the fastest. */
+    /** Times bricks per block (1, 2, 4) and chunks of consecutive blocks per XCD on a pass of these
+     * layers and keeps the fastest pair; resolves to the bricks per block (the chunk is xcdChunk). */
     async autotune(layers, views, map, repeats) {
         await native.tsdfAutotune(this.handle, this.stereo.handle, layers, map === undefined ? MAP_RAW : map,
             views.map((v) => HipTsdfVolume.viewOf(v, this.width, this.height)), repeats === undefined ? 3 : repeats);
@@ -350,6 +351,15 @@ class HipTsdfVolume {
 
     set bricksPerBlock(n) {
         native.tsdfSetBricksPerBlock(this.handle, n);
+    }
+
+    /** Consecutive blocks of the integration pass given to one XCD (1: round robin). */
+    get xcdChunk() {
+        return native.tsdfGetXcdChunk(this.handle);
+    }
+
+    set xcdChunk(n) {
+        native.tsdfSetXcdChunk(this.handle, n);
     }
 
     /** Culling of (brick, view) pairs against the views' depth tiles (on by default; no result changes). */

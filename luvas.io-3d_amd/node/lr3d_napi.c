@@ -805,6 +805,24 @@ static napi_value tsdf_set_bricks_per_block(napi_env env, napi_callback_info inf
     return status_or_undefined(env, lr3d_tsdf_set_bricks_per_block((lr3d_tsdf *)get_external(env, argv[0]), n));
 }
 
+/* tsdfGetXcdChunk(volume) -> n; tsdfSetXcdChunk(volume, n) */
+static napi_value tsdf_get_xcd_chunk(napi_env env, napi_callback_info info) {
+    ARGS(1);
+    int32_t n = 0;
+    const int32_t rc = lr3d_tsdf_get_xcd_chunk((lr3d_tsdf *)get_external(env, argv[0]), &n);
+    if (rc != LR3D_OK) return throw_status(env, rc);
+    napi_value out;
+    napi_create_int32(env, n, &out);
+    return out;
+}
+
+static napi_value tsdf_set_xcd_chunk(napi_env env, napi_callback_info info) {
+    ARGS(2);
+    int32_t n = 0;
+    napi_get_value_int32(env, argv[1], &n);
+    return status_or_undefined(env, lr3d_tsdf_set_xcd_chunk((lr3d_tsdf *)get_external(env, argv[0]), n));
+}
+
 static napi_value tsdf_set_flag(napi_env env, napi_callback_info info, int which) {
     ARGS(2);
     bool on = false;
@@ -1110,6 +1128,7 @@ static napi_value init(napi_env env, napi_value exports) {
         {"stereoWriteDepth", stereo_write_depth}, {"stereoWriteHyp", stereo_write_hyp},
         {"tsdfAutotune", tsdf_autotune},     {"tsdfIntegrateHost", tsdf_integrate_host},
         {"tsdfGetBricksPerBlock", tsdf_get_bricks_per_block}, {"tsdfSetBricksPerBlock", tsdf_set_bricks_per_block},
+        {"tsdfGetXcdChunk", tsdf_get_xcd_chunk}, {"tsdfSetXcdChunk", tsdf_set_xcd_chunk},
         {"tsdfSetCounting", tsdf_set_counting}, {"tsdfSetDepthCulling", tsdf_set_depth_culling},
         {"tsdfUpdateCount", tsdf_update_count}, {"tsdfPassCounts", tsdf_pass_counts},
         {"tsdfWrite", tsdf_write},           {"ctxSynchronize", ctx_synchronize},

@@ -94,6 +94,7 @@ async function main() {
     extras.passCounts = v2.passCounts();
     extras.updateCount = v2.updateCount();
     extras.bricksPerBlock = await v2.autotune(layers, ivs, lr3d.MAP_FILTERED, 1);
+    extras.xcdChunk = v2.xcdChunk;
     v2.setDepthCulling(false);
     v2.write(tsdf, weight);
     const back = await v2.read();

@@ -70,7 +70,7 @@ def test_autotune_keeps_the_set_and_picks_a_candidate(ctx, work):
 
 def test_tsdf_autotune_and_every_block_size(ctx, work):
     import lr3d
-    say("case: TSDF 128^3 x 8 mm, bricks per block 1 / 2 / 4 and autotune")
+    say("case: TSDF 128^3 x 8 mm, bricks per block 1 / 2 / 4, XCD chunks 16 / 64 / 4096 and autotune")
     st, solves = _stereo(ctx, work)
     st.solve_batched(solves)
     spec = work.tsdf
@@ -84,15 +84,27 @@ def test_tsdf_autotune_and_every_block_size(ctx, work):
         vol.integrate_batched(st, list(range(8)), views)
         results.append(vol.read())
         vol.destroy()
+    for chunk in (16, 64, 4096):  # the last is beyond the launch's blocks: every block keeps its index
+        vol = lr3d.HipTsdfVolume(ctx, grid, layout, spec.truncation_voxels)
+        vol.bricks_per_block = 2
+        vol.xcd_chunk = chunk
+        assert vol.xcd_chunk == chunk
+        vol.integrate_batched(st, list(range(8)), views)
+        results.append(vol.read())
+        vol.destroy()
     for t, w in results[1:]:
         assert np.array_equal(t.view(np.uint32), results[0][0].view(np.uint32))
         assert np.array_equal(w.view(np.uint32), results[0][1].view(np.uint32))
     vol = lr3d.HipTsdfVolume(ctx, grid, layout, spec.truncation_voxels)
+    for bad in (0, 4097):
+        with pytest.raises(lr3d.Lr3dError):
+            vol.xcd_chunk = bad
     vol.integrate_batched(st, list(range(4)), views[:4])
     before = vol.read()
     chosen = vol.autotune(st, list(range(8)), views, repeats=2)
-    say(f"tsdf autotune: {chosen} bricks per block")
+    say(f"tsdf autotune: {chosen} bricks per block, chunks of {vol.xcd_chunk} blocks per XCD")
     assert chosen in (1, 2, 4)
+    assert vol.xcd_chunk in (1, 16, 32, 64, 128, grid.dims[0])
     after = vol.read()
     assert np.array_equal(before[0].view(np.uint32), after[0].view(np.uint32))
     assert np.array_equal(before[1].view(np.uint32), after[1].view(np.uint32))

@@ -21,7 +21,7 @@ EXPORTS = ["ctxCreate", "ctxDestroy", "stereoCreate", "stereoDestroy", "stereoSe
            "nccRefineStep", "stereoSetPose", "tsdfBackward", "tsdfExtractSurface", "stereoFuseDepthMaps",
            "stereoSolveNoalloc", "stereoAutotune", "stereoGetTile", "stereoSetTile", "stereoWriteDepth",
            "stereoWriteHyp", "tsdfAutotune", "tsdfIntegrateHost", "tsdfGetBricksPerBlock",
-           "tsdfSetBricksPerBlock", "tsdfSetCounting", "tsdfSetDepthCulling", "tsdfUpdateCount",
+           "tsdfSetBricksPerBlock", "tsdfGetXcdChunk", "tsdfSetXcdChunk", "tsdfSetCounting", "tsdfSetDepthCulling", "tsdfUpdateCount",
            "tsdfPassCounts", "tsdfWrite", "ctxSynchronize", "ctxSetProfiling", "ctxKernelStats"]
 # C-ABI entry points the addon reaches through another one
 THROUGH = {"lr3d_tsdf_integrate": "HipTsdfVolume.integrate passes its one view to lr3d_tsdf_integrate_batched"}
@@ -169,6 +169,7 @@ def test_node_pipeline_matches_python_host(tmp_path, ctx):
     pc = ex["passCounts"]
     assert pc["updates"] == ex["updateCount"] > 0 and pc["pairs"] >= pc["updates"] and pc["voxelsWritten"] > 0
     assert ex["bricksPerBlock"] in (1, 2, 4)
+    assert 1 <= ex["xcdChunk"] <= 4096
     pyr = json.loads((tmp_path / "refine_pyramid.json").read_text())
     assert len(pyr["coarse"]) == 2
     for h in pyr["coarse"] + [pyr["fine"]]:

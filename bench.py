@@ -311,6 +311,45 @@ def main():
         st2.destroy()
         vol2.destroy()
 
+    # TSDF integration in the reference's schedule, one map per pass (TV:103-154 per keyframe): the
+    # same filtered maps into a fresh volume, view after view.  A counting run gives what each pass
+    # touches (every brick with a kept view is read, 8 B per voxel; changed voxels are written, 8 B;
+    # the view's map, 4 B per pixel); a second run times each integrate kernel with HIP events.
+    per_view = None
+    if not args.no_plain:
+        counts_pv = None
+        pv_ms = pv_launches = 0
+        for timed in (False, True):
+            vol3 = lr3d.HipTsdfVolume(ctx, grid, layout, spec.truncation_voxels)
+            vol3.bricks_per_block = vol.bricks_per_block
+            vol3.xcd_chunk = vol.xcd_chunk
+            vol3.set_counting(not timed)
+            ctx.synchronize()
+            ctx.set_profiling(timed)
+            for r in range(n):
+                vol3.integrate(st, r, iviews[r], map=lr3d.MAP_FILTERED)
+            ctx.synchronize()
+            if timed:
+                pv_launches, pv_ms = ctx.kernel_stats("tsdf_integrate")
+            else:
+                counts_pv = vol3.pass_counts()
+            ctx.set_profiling(False)
+            vol3.destroy()
+        if pv_launches:
+            map_bytes = sum(v.width * v.height * 4 for v in work.views)
+            touched = counts_pv["bricks_read"] * 512 * 8 + counts_pv["voxels_written"] * 8 + map_bytes
+            pv_s = pv_ms / 1e3
+            per_view = {"passes": pv_launches, "avg_launch_ms": round(pv_ms / pv_launches, 4),
+                        "total_ms": round(pv_ms, 3), "pass_counts": counts_pv,
+                        "bytes_touched": touched, "achieved_gbs": round(touched / pv_s / 1e9, 1),
+                        "frac_touched": round(touched / pv_s / 1e9 / HBM_PEAK_GBS, 4),
+                        "voxel_upd_per_s": counts_pv["updates"] / pv_s,
+                        "batched_pass_speedup": round(pv_ms / (tsdf["avg_launch_ms"]), 3)
+                        if tsdf.get("avg_launch_ms") else None}
+            log(f"per-view TSDF passes: {pv_launches} x {per_view['avg_launch_ms']} ms, "
+                f"{per_view['achieved_gbs']} GB/s on the bytes they touch")
+        tsdf["per_view"] = per_view
+
     cpu = None
     if not args.no_cpu_baseline:
         sys.path.insert(0, os.path.join(ROOT, "oracle"))

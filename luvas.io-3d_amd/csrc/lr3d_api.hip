@@ -1353,6 +1353,7 @@ struct lr3d_tsdf {
     DevBuf<int4> coords;
     DevBuf<float> tsdf, weight;
     DevBuf<uint32_t> vis;
+    DevBuf<uint32_t> brick_list, brick_ctrs;  // bricks with a kept view, and the pass's counters
     DevBuf<float> tiles;               // depth tiles of the views of the last pass
     bool depth_culling = true;         // drop (brick, view) pairs that cannot update a voxel
     DevBuf<TsdfView> views;
@@ -1414,6 +1415,8 @@ extern "C" int32_t lr3d_tsdf_create(lr3d_ctx *ctx, const double origin[3], doubl
     if (e == hipSuccess) e = ts->tsdf.alloc(voxels);
     if (e == hipSuccess) e = ts->weight.alloc(voxels);
     if (e == hipSuccess) e = ts->counter.alloc(4);
+    if (e == hipSuccess) e = ts->brick_list.alloc(n_bricks);
+    if (e == hipSuccess) e = ts->brick_ctrs.alloc(kTsdfCounters);
     if (e == hipSuccess)
         e = hipMemcpyAsync(ts->coords.ptr, coords, (size_t)n_bricks * 16, hipMemcpyHostToDevice, ctx->stream);
     if (e == hipSuccess) {
@@ -1519,7 +1522,8 @@ static int32_t stage_pass(lr3d_tsdf *ts, std::vector<TsdfView> &tv, const lr3d_i
     HIP_TRY(ts->vis.reserve((size_t)ts->grid.n_bricks * words));
     HIP_TRY(timed(ctx, "tsdf_visibility", [&] {
         return tsdf_launch_visibility(ctx->stream, ts->grid, ts->coords.ptr, ts->views.ptr, n, ts->vis.ptr, words,
-                                      cull ? ts->tiles.ptr : nullptr);
+                                      cull ? ts->tiles.ptr : nullptr, ts->brick_ctrs.ptr,
+                                      n <= kListPassViews ? ts->brick_list.ptr : nullptr);
     }));
     return LR3D_OK;
 }
@@ -1568,7 +1572,8 @@ static int32_t integrate_impl(lr3d_tsdf *ts, const uint32_t *maps, std::vector<T
         return tsdf_launch_integrate(ctx->stream, ts->grid, ts->views.ptr, n, ts->vis.ptr, words,
                                      ts->coords.ptr, maps, pass_priors(ts, tv), ctx->ntab.ptr,
                                      integrate_fast_ok(ts, tv), ts->tsdf.ptr, ts->weight.ptr,
-                                     ts->counting ? ts->counter.ptr : nullptr, ts->bricks_per_block,
+                                     ts->counting ? ts->counter.ptr : nullptr, ts->brick_ctrs.ptr,
+                                     n <= kListPassViews ? ts->brick_list.ptr : nullptr, ts->bricks_per_block,
                                      ts->xcd_chunk);
     }));
     return finish(ctx);
@@ -1783,7 +1788,8 @@ extern "C" int32_t lr3d_tsdf_autotune(lr3d_tsdf *ts, lr3d_stereo *st, const int3
                 HIP_TRY(hipEventRecord(ev.a, ctx->stream));
                 HIP_TRY(tsdf_launch_integrate(ctx->stream, ts->grid, ts->views.ptr, n, ts->vis.ptr, words,
                                               ts->coords.ptr, maps, priors, ctx->ntab.ptr, fast, ts->tsdf.ptr,
-                                              ts->weight.ptr, nullptr, bricks, chunk));
+                                              ts->weight.ptr, nullptr, ts->brick_ctrs.ptr,
+                                              n <= kListPassViews ? ts->brick_list.ptr : nullptr, bricks, chunk));
                 HIP_TRY(hipEventRecord(ev.b, ctx->stream));
                 HIP_TRY(hipEventSynchronize(ev.b));
                 float ms = 0.0f;

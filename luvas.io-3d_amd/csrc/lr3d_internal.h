@@ -194,16 +194,24 @@ hipError_t tsdf_launch_depth_tiles(hipStream_t s, const TsdfView *views, int32_t
                                    int32_t max_th, const uint32_t *maps, const float *priors, float *tiles);
 // Culling bits: the reference's brick test, then a depth test against the view's depth tiles
 // (tiles == nullptr: the reference's test alone).
+// Culling of every (brick, view) pair into vis, then, when list is not NULL, the list of the bricks
+// with a kept view (list: n_bricks entries; ctrs[0]: its length).
 hipError_t tsdf_launch_visibility(hipStream_t s, const TsdfGridDev &g, const int4 *coords,
                                   const TsdfView *views, int32_t n_views, uint32_t *vis,
-                                  int32_t words, const float *tiles);
+                                  int32_t words, const float *tiles, uint32_t *ctrs, uint32_t *list);
+// Words of the pass counters (ctrs): the brick list's length.
+constexpr int32_t kTsdfCounters = 1;
+// Passes of at most this many views walk the brick list (tsdf_launch_integrate with a list);
+// larger ones launch a wavefront per pool brick.
+constexpr int32_t kListPassViews = 16;
 // ntab: the decoded normal of each packed octahedral code (tsdf_launch_normal_table); fast: the
 // pass lies in the ranges of the correctly rounded short forms (see integrate_kernel).
 hipError_t tsdf_launch_integrate(hipStream_t s, const TsdfGridDev &g, const TsdfView *views,
                                  int32_t n_views, const uint32_t *vis, int32_t words,
                                  const int4 *coords, const uint32_t *maps, const float *priors,
                                  const float4 *ntab, bool fast, float *tsdf, float *weight,
-                                 unsigned long long *counter, int32_t bricks_per_block, int32_t chunk);
+                                 unsigned long long *counter, const uint32_t *ctrs, const uint32_t *list,
+                                 int32_t bricks_per_block, int32_t chunk);
 // Largest chunk of consecutive blocks dealt to one XCD by tsdf_launch_integrate.
 constexpr int32_t kMaxXcdChunk = 4096;
 // The 4096-entry normal table of tsdf_launch_integrate.

@@ -17,6 +17,8 @@
 
 #include <math.h>
 
+#include <algorithm>
+
 namespace lr3d {
 
 namespace {
@@ -26,6 +28,9 @@ namespace {
 #endif
 
 constexpr float kMaxWeight = 96.0f;   // TV:7
+// Wavefronts of a list pass's grid (integrate_kernel, LIST): enough to fill 256 CUs at the pass's
+// 5 waves per SIMD, with a margin.
+constexpr uint32_t kIntegrateWaves = 8192;
 constexpr float kPriorWeight = 0.12f; // TV:8
 
 __device__ __forceinline__ float f16_bits_to_f32(uint32_t bits) {
@@ -151,8 +156,10 @@ __device__ bool depth_reachable(const TsdfGridDev &g, const TsdfView &V, int4 c,
 __global__ void __launch_bounds__(256) visibility_kernel(TsdfGridDev g, const int4 *__restrict__ coords,
                                                          const TsdfView *__restrict__ views,
                                                          int32_t n_views, uint32_t *__restrict__ vis,
-                                                         int32_t words, const float *__restrict__ tiles) {
+                                                         int32_t words, const float *__restrict__ tiles,
+                                                         uint32_t *__restrict__ ctrs) {
     const uint32_t gid = blockIdx.x * blockDim.x + threadIdx.x;
+    if (gid == 0u) ctrs[0] = 0u;  // the brick list's length, counted by brick_list_kernel
     const uint32_t pair = gid >> 5;
     const uint32_t brick = pair / (uint32_t)words;
     const int32_t word = (int32_t)(pair % (uint32_t)words);
@@ -315,11 +322,113 @@ __global__ void __launch_bounds__(256) normal_table_kernel(float4 *__restrict__ 
     tab[i] = make_float4(nx, ny, nz, 0.0f);
 }
 
-// TV:103-154, one wavefront per brick, all visible views of the pass in order.  Visible views are
-// taken kViewGroup at a time: the group's depth samples are all gathered before the first view's
-// updates, so the later views' gathers are in flight while the earlier ones are applied (a brick's
-// chain of views is not one gather latency per view); the updates are the same sequence as
-// applying the views one at a time.  Groups of 2 measured fastest (4 and 6 hold more registers).
+// Bricks a pass reads: the pool indices of the bricks with at least one kept view, written to
+// list[0 .. ctrs[0]) (order within each run of 64 pool bricks kept; runs in any order).  ctrs[0]
+// (the list length) is zeroed by the culling kernel before this one runs.
+__global__ void __launch_bounds__(256) brick_list_kernel(const uint32_t *__restrict__ vis, int32_t words,
+                                                         int32_t n_bricks, uint32_t *__restrict__ ctrs,
+                                                         uint32_t *__restrict__ list) {
+    const uint32_t brick = blockIdx.x * blockDim.x + threadIdx.x;
+    bool kept = false;
+    if (brick < (uint32_t)n_bricks) {
+        const uint32_t *__restrict__ w = vis + (size_t)brick * words;
+        for (int32_t k = 0; k < words; k++) kept = kept || w[k] != 0u;
+    }
+    const uint64_t mask = __ballot(kept);
+    if (mask == 0u) return;
+    const uint32_t lane = threadIdx.x & 63u;
+    uint32_t base = 0u;
+    if (lane == 0u) base = atomicAdd(ctrs, (uint32_t)__popcll(mask));
+    base = __shfl(base, 0, 64);
+    if (kept) list[base + (uint32_t)__popcll(mask & ((1ull << lane) - 1ull))] = brick;
+}
+
+// TV:103-154 for one brick of the pass: one wavefront, lane = (x, y), each lane walks its z column,
+// all visible views of the brick in order.  Visible views are taken kViewGroup at a time: the
+// group's depth samples are all gathered before the first view's updates, so the later views'
+// gathers are in flight while the earlier ones are applied (a brick's chain of views is not one
+// gather latency per view); the updates are the same sequence as applying the views one at a time.
+// Groups of 2 measured fastest (4 and 6 hold more registers).  Adds to the wavefront's counts:
+// voxel updates and voxels written (per lane), and views applied (wave-uniform).
+template <bool COUNT, bool PRIORS, bool FAST>
+__device__ __forceinline__ void integrate_brick(const TsdfGridDev &g, const TsdfView *__restrict__ views,
+                                                const uint32_t *__restrict__ vis, int32_t words,
+                                                const int4 *__restrict__ coords, const uint32_t *__restrict__ maps,
+                                                const float *__restrict__ priors, const float4 *__restrict__ ntab,
+                                                float *__restrict__ tsdf, float *__restrict__ weight, uint32_t brick,
+                                                uint32_t lane, uint32_t &updates, uint32_t &written,
+                                                uint32_t &views_applied) {
+    // the brick, its visibility words and the views are wave-uniform (scalar loads, and the prior
+    // branch below is decided per view, not per lane)
+    const int4 c = coords[brick];
+    const int32_t vx = (int32_t)(lane & 7u), vy = (int32_t)(lane >> 3);
+    const float wx = g.origin[0] + ((float)(c.x * 8 + vx) + 0.5f) * g.voxel;
+    const float wy = g.origin[1] + ((float)(c.y * 8 + vy) + 0.5f) * g.voxel;
+    float wz[8];
+#pragma unroll
+    for (int z = 0; z < 8; z++) wz[z] = g.origin[2] + ((float)(c.z * 8 + z) + 0.5f) * g.voxel;
+    const uint32_t *__restrict__ bvis = vis + (size_t)brick * words;
+    int32_t word = 0;
+    uint32_t bits = words > 0 ? __builtin_amdgcn_readfirstlane(bvis[0]) : 0u;
+    auto next_view = [&]() -> int32_t {
+        while (bits == 0u) {
+            if (++word >= words) return -1;
+            bits = __builtin_amdgcn_readfirstlane(bvis[word]);
+        }
+        const int32_t v = __builtin_amdgcn_readfirstlane(word * 32 + (int32_t)__builtin_ctz(bits));
+        bits &= bits - 1u;
+        return v;
+    };
+    int32_t v = next_view();
+    if (v < 0) return;  // no view reaches this brick: nothing read, nothing written
+    const float mu = g.mu;
+    const size_t base = (size_t)brick * 512u + lane;
+
+    float ts[8], ws[8];
+#pragma unroll
+    for (int z = 0; z < 8; z++) {
+        ts[z] = tsdf[base + 64u * z];
+        ws[z] = weight[base + 64u * z];
+    }
+    uint32_t changed = 0;  // bit z: voxel z of this lane was updated
+    while (v >= 0) {
+        int32_t vs[kViewGroup];
+        int32_t n = 1;
+        vs[0] = v;
+        for (; n < kViewGroup; n++) {
+            vs[n] = next_view();
+            if (vs[n] < 0) break;
+        }
+        views_applied += (uint32_t)n;
+        ViewGather grp[kViewGroup];
+#pragma unroll
+        for (int k = 0; k < kViewGroup; k++)
+            if (k < n) gather_view<PRIORS, FAST>(views[vs[k]], maps, priors, wx, wy, wz, grp[k]);
+#pragma unroll
+        for (int k = 0; k < kViewGroup; k++)
+            if (k < n)
+                apply_view<COUNT, FAST>(views[vs[k]], grp[k], mu, g.inv_mu, ntab, wx, wy, wz, ts, ws, changed,
+                                        updates);
+        v = n == kViewGroup ? next_view() : -1;
+    }
+#pragma unroll
+    for (int z = 0; z < 8; z++) {
+        if (changed & (1u << z)) {
+            tsdf[base + 64u * z] = ts[z];
+            weight[base + 64u * z] = ws[z];
+        }
+    }
+    if (COUNT) written += (uint32_t)__builtin_popcount(changed);
+}
+
+// One integration pass, one wavefront per pool brick (a brick without a kept view leaves at once).
+// Hardware block i runs on XCD i % 8; blocks are dealt to the XCDs in chunks of `chunk` consecutive
+// blocks, so neighbouring bricks run on one XCD and share the depth-map lines of their footprints in
+// its L2, while every XCD still takes work from the whole pool (chunk 1: round robin; the blocks past
+// the last whole round keep their index).  The hardware hands bricks to free wavefronts, which
+// balances passes where bricks see very different numbers of views.  The brick's work is written out
+// here rather than through integrate_brick: inlined into this kernel, the shared function compiles
+// to 108 VGPRs (4 waves per SIMD) instead of 84 (5), and the pass runs ~4 % slower.
 template <bool COUNT, bool PRIORS, bool FAST>
 __global__ void __launch_bounds__(256) integrate_kernel(TsdfGridDev g, const TsdfView *__restrict__ views,
                                                         int32_t n_views, const uint32_t *__restrict__ vis,
@@ -330,11 +439,7 @@ __global__ void __launch_bounds__(256) integrate_kernel(TsdfGridDev g, const Tsd
                                                         float *__restrict__ tsdf, float *__restrict__ weight,
                                                         unsigned long long *__restrict__ counter, uint32_t chunk) {
     // one brick per wavefront: the brick, its visibility words and the views are wave-uniform
-    // (scalar loads, and the prior branch below is decided per view, not per lane).  Hardware
-    // block i runs on XCD i % 8; blocks are dealt to the XCDs in chunks of `chunk` consecutive
-    // blocks, so neighbouring bricks run on one XCD and share the depth-map lines of their
-    // footprints in its L2, while every XCD still takes work from the whole pool (chunk 1: plain
-    // round robin).  The blocks past the last whole round keep their index.
+    // (scalar loads, and the prior branch below is decided per view, not per lane)
     uint32_t block = blockIdx.x;
     const uint32_t per_round = 8u * chunk;
     if (chunk > 1u && block < gridDim.x / per_round * per_round) {
@@ -418,6 +523,54 @@ __global__ void __launch_bounds__(256) integrate_kernel(TsdfGridDev g, const Tsd
             if (updates) atomicAdd(counter, (unsigned long long)updates);
             atomicAdd(counter + 1, 512ull * views_applied);
             atomicAdd(counter + 2, 1ull);
+            if (written) atomicAdd(counter + 3, (unsigned long long)written);
+        }
+    }
+}
+
+// The same pass over the list of bricks with a kept view (brick_list_kernel), for passes that reach
+// few bricks (one view per pass, as the reference integrates keyframes): a fixed grid walks the list
+// instead of launching a wavefront per pool brick.  Block b (XCD b % 8) serves queue q = b % 8,
+// which holds the list chunks q, q + 8, q + 16, ... of `chunk` entries; the queue's wavefronts take
+// its entries in turn (static striding: no atomics, which on this memory are resolved beyond the
+// L2).  Which wavefront updates a brick changes no value: each brick is owned by one wavefront,
+// which applies its views in order.
+template <bool COUNT, bool PRIORS, bool FAST>
+__global__ void __launch_bounds__(256) integrate_list_kernel(TsdfGridDev g, const TsdfView *__restrict__ views,
+                                                             int32_t n_views, const uint32_t *__restrict__ vis,
+                                                             int32_t words, const int4 *__restrict__ coords,
+                                                             const uint32_t *__restrict__ maps,
+                                                             const float *__restrict__ priors,
+                                                             const float4 *__restrict__ ntab,
+                                                             float *__restrict__ tsdf, float *__restrict__ weight,
+                                                             unsigned long long *__restrict__ counter,
+                                                             const uint32_t *__restrict__ ctrs,
+                                                             const uint32_t *__restrict__ list, uint32_t chunk) {
+    const uint32_t lane = threadIdx.x & 63u;
+    const uint32_t wave = threadIdx.x >> 6, waves_per_block = blockDim.x / 64u;
+    const uint32_t q = blockIdx.x % 8u;
+    const uint32_t count = __builtin_amdgcn_readfirstlane(ctrs[0]);
+    const uint32_t stride = (gridDim.x / 8u) * waves_per_block;  // wavefronts serving the queue
+    uint32_t updates = 0, written = 0;       // per lane
+    uint32_t views_applied = 0, bricks = 0;  // wave-uniform
+    for (uint32_t k = (blockIdx.x / 8u) * waves_per_block + wave;; k += stride) {
+        const uint32_t pos = ((k / chunk) * 8u + q) * chunk + k % chunk;  // increasing in k
+        if (pos >= count) break;
+        const uint32_t brick = __builtin_amdgcn_readfirstlane(list[pos]);
+        integrate_brick<COUNT, PRIORS, FAST>(g, views, vis, words, coords, maps, priors, ntab, tsdf, weight, brick,
+                                             lane, updates, written, views_applied);
+        bricks++;
+    }
+    if (COUNT) {
+        // wavefront sums, one atomic each, as integrate_kernel
+        for (int offset = 32; offset > 0; offset >>= 1) {
+            updates += __shfl_down(updates, offset, 64);
+            written += __shfl_down(written, offset, 64);
+        }
+        if (lane == 0) {
+            if (updates) atomicAdd(counter, (unsigned long long)updates);
+            if (views_applied) atomicAdd(counter + 1, 512ull * views_applied);
+            if (bricks) atomicAdd(counter + 2, (unsigned long long)bricks);
             if (written) atomicAdd(counter + 3, (unsigned long long)written);
         }
     }
@@ -642,12 +795,16 @@ hipError_t tsdf_launch_depth_tiles(hipStream_t s, const TsdfView *views, int32_t
 
 hipError_t tsdf_launch_visibility(hipStream_t s, const TsdfGridDev &g, const int4 *coords,
                                   const TsdfView *views, int32_t n_views, uint32_t *vis,
-                                  int32_t words, const float *tiles) {
+                                  int32_t words, const float *tiles, uint32_t *ctrs, uint32_t *list) {
     const uint64_t threads = (uint64_t)g.n_bricks * (uint64_t)words * 32u;
     if (threads == 0) return hipSuccess;
     if (threads > 0xffffff00ull) return hipErrorInvalidValue;
     dim3 block(256), grid((uint32_t)((threads + 255u) / 256u));
-    hipLaunchKernelGGL(visibility_kernel, grid, block, 0, s, g, coords, views, n_views, vis, words, tiles);
+    hipLaunchKernelGGL(visibility_kernel, grid, block, 0, s, g, coords, views, n_views, vis, words, tiles, ctrs);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess || !list) return e;
+    hipLaunchKernelGGL(brick_list_kernel, dim3(((uint32_t)g.n_bricks + 255u) / 256u), dim3(256), 0, s, vis, words,
+                       g.n_bricks, ctrs, list);
     return hipGetLastError();
 }
 
@@ -656,42 +813,58 @@ hipError_t tsdf_launch_normal_table(hipStream_t s, float4 *tab) {
     return hipGetLastError();
 }
 
-template <bool FAST>
+template <bool FAST, bool LIST>
 static void launch_integrate(hipStream_t s, dim3 grid, dim3 block, const TsdfGridDev &g, const TsdfView *views,
                              int32_t n_views, const uint32_t *vis, int32_t words, const int4 *coords,
                              const uint32_t *maps, const float *priors, const float4 *ntab, float *tsdf,
-                             float *weight, unsigned long long *counter, uint32_t chunk) {
+                             float *weight, unsigned long long *counter, const uint32_t *ctrs, const uint32_t *list,
+                             uint32_t chunk) {
     // priors != nullptr: some view of the pass has a prior (the caller passes nullptr otherwise)
-    if (counter && priors)
-        hipLaunchKernelGGL((integrate_kernel<true, true, FAST>), grid, block, 0, s, g, views, n_views, vis, words,
-                           coords, maps, priors, ntab, tsdf, weight, counter, chunk);
-    else if (counter)
-        hipLaunchKernelGGL((integrate_kernel<true, false, FAST>), grid, block, 0, s, g, views, n_views, vis, words,
-                           coords, maps, priors, ntab, tsdf, weight, counter, chunk);
-    else if (priors)
-        hipLaunchKernelGGL((integrate_kernel<false, true, FAST>), grid, block, 0, s, g, views, n_views, vis, words,
-                           coords, maps, priors, ntab, tsdf, weight, counter, chunk);
-    else
-        hipLaunchKernelGGL((integrate_kernel<false, false, FAST>), grid, block, 0, s, g, views, n_views, vis, words,
-                           coords, maps, priors, ntab, tsdf, weight, counter, chunk);
+    const bool c = counter != nullptr, p = priors != nullptr;
+    if (LIST) {
+        auto k = c ? (p ? integrate_list_kernel<true, true, FAST> : integrate_list_kernel<true, false, FAST>)
+                   : (p ? integrate_list_kernel<false, true, FAST> : integrate_list_kernel<false, false, FAST>);
+        hipLaunchKernelGGL(k, grid, block, 0, s, g, views, n_views, vis, words, coords, maps, priors, ntab, tsdf,
+                           weight, counter, ctrs, list, chunk);
+    } else {
+        auto k = c ? (p ? integrate_kernel<true, true, FAST> : integrate_kernel<true, false, FAST>)
+                   : (p ? integrate_kernel<false, true, FAST> : integrate_kernel<false, false, FAST>);
+        hipLaunchKernelGGL(k, grid, block, 0, s, g, views, n_views, vis, words, coords, maps, priors, ntab, tsdf,
+                           weight, counter, chunk);
+    }
 }
 
 hipError_t tsdf_launch_integrate(hipStream_t s, const TsdfGridDev &g, const TsdfView *views,
                                  int32_t n_views, const uint32_t *vis, int32_t words,
                                  const int4 *coords, const uint32_t *maps, const float *priors,
                                  const float4 *ntab, bool fast, float *tsdf, float *weight,
-                                 unsigned long long *counter, int32_t bricks_per_block, int32_t chunk) {
+                                 unsigned long long *counter, const uint32_t *ctrs, const uint32_t *list,
+                                 int32_t bricks_per_block, int32_t chunk) {
     if (bricks_per_block != 1 && bricks_per_block != 2 && bricks_per_block != 4) return hipErrorInvalidValue;
     if (chunk < 1 || chunk > kMaxXcdChunk) return hipErrorInvalidValue;
     if (fast && !ntab) return hipErrorInvalidValue;
+    if (list && !ctrs) return hipErrorInvalidValue;
+    if (words <= 0) return hipSuccess;
     const uint32_t bpb = (uint32_t)bricks_per_block;
     dim3 block(64u * bpb), grid(((uint32_t)g.n_bricks + bpb - 1u) / bpb);
-    if (fast)
-        launch_integrate<true>(s, grid, block, g, views, n_views, vis, words, coords, maps, priors, ntab, tsdf,
-                               weight, counter, (uint32_t)chunk);
-    else
-        launch_integrate<false>(s, grid, block, g, views, n_views, vis, words, coords, maps, priors, ntab, tsdf,
-                                weight, counter, (uint32_t)chunk);
+    if (list) {
+        // a fixed grid of at most kIntegrateWaves wavefronts, in whole rounds of 8 blocks so that
+        // every queue is served
+        const uint32_t waves = std::min<uint32_t>(kIntegrateWaves, (uint32_t)g.n_bricks);
+        grid = dim3(((waves + bpb - 1u) / bpb + 7u) / 8u * 8u);
+        if (fast)
+            launch_integrate<true, true>(s, grid, block, g, views, n_views, vis, words, coords, maps, priors, ntab,
+                                         tsdf, weight, counter, ctrs, list, (uint32_t)chunk);
+        else
+            launch_integrate<false, true>(s, grid, block, g, views, n_views, vis, words, coords, maps, priors, ntab,
+                                          tsdf, weight, counter, ctrs, list, (uint32_t)chunk);
+    } else if (fast) {
+        launch_integrate<true, false>(s, grid, block, g, views, n_views, vis, words, coords, maps, priors, ntab,
+                                      tsdf, weight, counter, ctrs, list, (uint32_t)chunk);
+    } else {
+        launch_integrate<false, false>(s, grid, block, g, views, n_views, vis, words, coords, maps, priors, ntab,
+                                       tsdf, weight, counter, ctrs, list, (uint32_t)chunk);
+    }
     return hipGetLastError();
 }
 

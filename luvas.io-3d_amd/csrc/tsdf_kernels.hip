@@ -343,93 +343,28 @@ __global__ void __launch_bounds__(256) brick_list_kernel(const uint32_t *__restr
     if (kept) list[base + (uint32_t)__popcll(mask & ((1ull << lane) - 1ull))] = brick;
 }
 
-// TV:103-154 for one brick of the pass: one wavefront, lane = (x, y), each lane walks its z column,
-// all visible views of the brick in order.  Visible views are taken kViewGroup at a time: the
-// group's depth samples are all gathered before the first view's updates, so the later views'
-// gathers are in flight while the earlier ones are applied (a brick's chain of views is not one
-// gather latency per view); the updates are the same sequence as applying the views one at a time.
-// Groups of 2 measured fastest (4 and 6 hold more registers).  Adds to the wavefront's counts:
-// voxel updates and voxels written (per lane), and views applied (wave-uniform).
-template <bool COUNT, bool PRIORS, bool FAST>
-__device__ __forceinline__ void integrate_brick(const TsdfGridDev &g, const TsdfView *__restrict__ views,
-                                                const uint32_t *__restrict__ vis, int32_t words,
-                                                const int4 *__restrict__ coords, const uint32_t *__restrict__ maps,
-                                                const float *__restrict__ priors, const float4 *__restrict__ ntab,
-                                                float *__restrict__ tsdf, float *__restrict__ weight, uint32_t brick,
-                                                uint32_t lane, uint32_t &updates, uint32_t &written,
-                                                uint32_t &views_applied) {
-    // the brick, its visibility words and the views are wave-uniform (scalar loads, and the prior
-    // branch below is decided per view, not per lane)
-    const int4 c = coords[brick];
-    const int32_t vx = (int32_t)(lane & 7u), vy = (int32_t)(lane >> 3);
-    const float wx = g.origin[0] + ((float)(c.x * 8 + vx) + 0.5f) * g.voxel;
-    const float wy = g.origin[1] + ((float)(c.y * 8 + vy) + 0.5f) * g.voxel;
-    float wz[8];
-#pragma unroll
-    for (int z = 0; z < 8; z++) wz[z] = g.origin[2] + ((float)(c.z * 8 + z) + 0.5f) * g.voxel;
-    const uint32_t *__restrict__ bvis = vis + (size_t)brick * words;
-    int32_t word = 0;
-    uint32_t bits = words > 0 ? __builtin_amdgcn_readfirstlane(bvis[0]) : 0u;
-    auto next_view = [&]() -> int32_t {
-        while (bits == 0u) {
-            if (++word >= words) return -1;
-            bits = __builtin_amdgcn_readfirstlane(bvis[word]);
-        }
-        const int32_t v = __builtin_amdgcn_readfirstlane(word * 32 + (int32_t)__builtin_ctz(bits));
-        bits &= bits - 1u;
-        return v;
-    };
-    int32_t v = next_view();
-    if (v < 0) return;  // no view reaches this brick: nothing read, nothing written
-    const float mu = g.mu;
-    const size_t base = (size_t)brick * 512u + lane;
-
-    float ts[8], ws[8];
-#pragma unroll
-    for (int z = 0; z < 8; z++) {
-        ts[z] = tsdf[base + 64u * z];
-        ws[z] = weight[base + 64u * z];
-    }
-    uint32_t changed = 0;  // bit z: voxel z of this lane was updated
-    while (v >= 0) {
-        int32_t vs[kViewGroup];
-        int32_t n = 1;
-        vs[0] = v;
-        for (; n < kViewGroup; n++) {
-            vs[n] = next_view();
-            if (vs[n] < 0) break;
-        }
-        views_applied += (uint32_t)n;
-        ViewGather grp[kViewGroup];
-#pragma unroll
-        for (int k = 0; k < kViewGroup; k++)
-            if (k < n) gather_view<PRIORS, FAST>(views[vs[k]], maps, priors, wx, wy, wz, grp[k]);
-#pragma unroll
-        for (int k = 0; k < kViewGroup; k++)
-            if (k < n)
-                apply_view<COUNT, FAST>(views[vs[k]], grp[k], mu, g.inv_mu, ntab, wx, wy, wz, ts, ws, changed,
-                                        updates);
-        v = n == kViewGroup ? next_view() : -1;
-    }
-#pragma unroll
-    for (int z = 0; z < 8; z++) {
-        if (changed & (1u << z)) {
-            tsdf[base + 64u * z] = ts[z];
-            weight[base + 64u * z] = ws[z];
-        }
-    }
-    if (COUNT) written += (uint32_t)__builtin_popcount(changed);
-}
-
-// One integration pass, one wavefront per pool brick (a brick without a kept view leaves at once).
-// Hardware block i runs on XCD i % 8; blocks are dealt to the XCDs in chunks of `chunk` consecutive
-// blocks, so neighbouring bricks run on one XCD and share the depth-map lines of their footprints in
-// its L2, while every XCD still takes work from the whole pool (chunk 1: round robin; the blocks past
-// the last whole round keep their index).  The hardware hands bricks to free wavefronts, which
-// balances passes where bricks see very different numbers of views.  The brick's work is written out
-// here rather than through integrate_brick: inlined into this kernel, the shared function compiles
-// to 108 VGPRs (4 waves per SIMD) instead of 84 (5), and the pass runs ~4 % slower.
-template <bool COUNT, bool PRIORS, bool FAST>
+// TV:103-154: one integration pass, one wavefront per brick at a time; lane = (x, y), each lane walks
+// its z column, all visible views of the brick in order.  Two schedules:
+//   LIST = false  one wavefront per pool brick (a brick without a kept view leaves at once).
+//                 Hardware block i runs on XCD i % 8; blocks are dealt to the XCDs in chunks of
+//                 `chunk` consecutive blocks, so neighbouring bricks run on one XCD and share the
+//                 depth-map lines of their footprints in its L2, while every XCD still takes work
+//                 from the whole pool (chunk 1: round robin; the blocks past the last whole round keep
+//                 their index).  The hardware hands bricks to free wavefronts, which balances passes
+//                 where bricks see very different numbers of views.
+//   LIST = true   a fixed grid walks the list of bricks with a kept view (brick_list_kernel), for
+//                 passes that reach few bricks (one view per pass, as the reference integrates
+//                 keyframes): no wavefront per empty pool brick.  Block b (XCD b % 8) serves queue
+//                 q = b % 8, which holds the list chunks q, q + 8, q + 16, ... of `chunk` entries;
+//                 the queue's wavefronts take its entries in turn (static striding: no atomics, which
+//                 on this memory are resolved beyond the L2).
+// Which wavefront updates a brick changes no value: each brick is owned by one wavefront, which
+// applies its views in order.  Visible views are taken kViewGroup at a time: the group's depth
+// samples are all gathered before the first view's updates, so the later views' gathers are in
+// flight while the earlier ones are applied (a brick's chain of views is not one gather latency per
+// view); the updates are the same sequence as applying the views one at a time.  Groups of 2
+// measured fastest (4 and 6 hold more registers).
+template <bool COUNT, bool PRIORS, bool FAST, bool LIST>
 __global__ void __launch_bounds__(256) integrate_kernel(TsdfGridDev g, const TsdfView *__restrict__ views,
                                                         int32_t n_views, const uint32_t *__restrict__ vis,
                                                         int32_t words, const int4 *__restrict__ coords,
@@ -437,132 +372,103 @@ __global__ void __launch_bounds__(256) integrate_kernel(TsdfGridDev g, const Tsd
                                                         const float *__restrict__ priors,
                                                         const float4 *__restrict__ ntab,
                                                         float *__restrict__ tsdf, float *__restrict__ weight,
-                                                        unsigned long long *__restrict__ counter, uint32_t chunk) {
-    // one brick per wavefront: the brick, its visibility words and the views are wave-uniform
-    // (scalar loads, and the prior branch below is decided per view, not per lane)
-    uint32_t block = blockIdx.x;
-    const uint32_t per_round = 8u * chunk;
-    if (chunk > 1u && block < gridDim.x / per_round * per_round) {
-        const uint32_t round = block / per_round, r = block % per_round;
-        block = round * per_round + (r % 8u) * chunk + r / 8u;
-    }
-    const uint32_t brick = __builtin_amdgcn_readfirstlane(block * (blockDim.x / 64u) + (threadIdx.x >> 6));
+                                                        unsigned long long *__restrict__ counter,
+                                                        const uint32_t *__restrict__ ctrs,
+                                                        const uint32_t *__restrict__ list, uint32_t chunk) {
     const uint32_t lane = threadIdx.x & 63u;
-    if (brick >= (uint32_t)g.n_bricks) return;
-    const int4 c = coords[brick];
-    const int32_t vx = (int32_t)(lane & 7u), vy = (int32_t)(lane >> 3);
-    const float wx = g.origin[0] + ((float)(c.x * 8 + vx) + 0.5f) * g.voxel;
-    const float wy = g.origin[1] + ((float)(c.y * 8 + vy) + 0.5f) * g.voxel;
-    float wz[8];
-#pragma unroll
-    for (int z = 0; z < 8; z++) wz[z] = g.origin[2] + ((float)(c.z * 8 + z) + 0.5f) * g.voxel;
-    const float mu = g.mu;
-    const size_t base = (size_t)brick * 512u + lane;
-    const uint32_t *__restrict__ bvis = vis + (size_t)brick * words;
-
-    // visible views in order (bit j of word k = view 32k + j)
-    int32_t word = 0;
-    uint32_t bits = words > 0 ? __builtin_amdgcn_readfirstlane(bvis[0]) : 0u;
-    auto next_view = [&]() -> int32_t {
-        while (bits == 0u) {
-            if (++word >= words) return -1;
-            bits = __builtin_amdgcn_readfirstlane(bvis[word]);
+    const uint32_t wave = threadIdx.x >> 6, waves_per_block = blockDim.x / 64u;
+    // LIST: the queue, the list length and the queue's wavefront count; else the block's bricks
+    const uint32_t q = blockIdx.x % 8u;
+    const uint32_t count = LIST ? __builtin_amdgcn_readfirstlane(ctrs[0]) : 0u;
+    const uint32_t stride = (gridDim.x / 8u) * waves_per_block;
+    uint32_t block = blockIdx.x;
+    if (!LIST) {
+        const uint32_t per_round = 8u * chunk;
+        if (chunk > 1u && block < gridDim.x / per_round * per_round) {
+            const uint32_t round = block / per_round, r = block % per_round;
+            block = round * per_round + (r % 8u) * chunk + r / 8u;
         }
-        const int32_t v = __builtin_amdgcn_readfirstlane(word * 32 + (int32_t)__builtin_ctz(bits));
-        bits &= bits - 1u;
-        return v;
-    };
-    int32_t v = next_view();
-    if (v < 0) return;  // no view reaches this brick: nothing read, nothing written
-    uint32_t views_applied = 0;  // visible views of the brick (wave-uniform)
-
-    float ts[8], ws[8];
-#pragma unroll
-    for (int z = 0; z < 8; z++) {
-        ts[z] = tsdf[base + 64u * z];
-        ws[z] = weight[base + 64u * z];
     }
-    uint32_t changed = 0;  // bit z: voxel z of this lane was updated
-    uint32_t updates = 0;
-    while (v >= 0) {
-        int32_t vs[kViewGroup];
-        int32_t n = 1;
-        vs[0] = v;
-        for (; n < kViewGroup; n++) {
-            vs[n] = next_view();
-            if (vs[n] < 0) break;
+    uint32_t updates = 0, written = 0;       // per lane
+    uint32_t views_applied = 0, bricks = 0;  // wave-uniform
+    for (uint32_t k = (blockIdx.x / 8u) * waves_per_block + wave, first = 1u;; k += stride, first = 0u) {
+        uint32_t brick;
+        if (LIST) {
+            const uint32_t pos = ((k / chunk) * 8u + q) * chunk + k % chunk;  // increasing in k
+            if (pos >= count) break;
+            brick = __builtin_amdgcn_readfirstlane(list[pos]);
+        } else {
+            brick = __builtin_amdgcn_readfirstlane(block * waves_per_block + wave);
+            if (!first || brick >= (uint32_t)g.n_bricks) break;
         }
-        views_applied += (uint32_t)n;
-        ViewGather grp[kViewGroup];
+        // the brick, its visibility words and the views are wave-uniform (scalar loads, and the
+        // prior branch below is decided per view, not per lane)
+        const int4 c = coords[brick];
+        const int32_t vx = (int32_t)(lane & 7u), vy = (int32_t)(lane >> 3);
+        const float wx = g.origin[0] + ((float)(c.x * 8 + vx) + 0.5f) * g.voxel;
+        const float wy = g.origin[1] + ((float)(c.y * 8 + vy) + 0.5f) * g.voxel;
+        float wz[8];
 #pragma unroll
-        for (int k = 0; k < kViewGroup; k++)
-            if (k < n) gather_view<PRIORS, FAST>(views[vs[k]], maps, priors, wx, wy, wz, grp[k]);
+        for (int z = 0; z < 8; z++) wz[z] = g.origin[2] + ((float)(c.z * 8 + z) + 0.5f) * g.voxel;
+        const float mu = g.mu;
+        const size_t base = (size_t)brick * 512u + lane;
+        const uint32_t *__restrict__ bvis = vis + (size_t)brick * words;
+
+        // visible views in order (bit j of word k = view 32k + j)
+        int32_t word = 0;
+        uint32_t bits = words > 0 ? __builtin_amdgcn_readfirstlane(bvis[0]) : 0u;
+        auto next_view = [&]() -> int32_t {
+            while (bits == 0u) {
+                if (++word >= words) return -1;
+                bits = __builtin_amdgcn_readfirstlane(bvis[word]);
+            }
+            const int32_t v = __builtin_amdgcn_readfirstlane(word * 32 + (int32_t)__builtin_ctz(bits));
+            bits &= bits - 1u;
+            return v;
+        };
+        int32_t v = next_view();
+        if (v < 0) continue;  // no view reaches this brick: nothing read, nothing written
+        bricks++;
+
+        float ts[8], ws[8];
 #pragma unroll
-        for (int k = 0; k < kViewGroup; k++)
-            if (k < n)
-                apply_view<COUNT, FAST>(views[vs[k]], grp[k], mu, g.inv_mu, ntab, wx, wy, wz, ts, ws, changed,
-                                        updates);
-        v = n == kViewGroup ? next_view() : -1;
-    }
-#pragma unroll
-    for (int z = 0; z < 8; z++) {
-        if (changed & (1u << z)) {
-            tsdf[base + 64u * z] = ts[z];
-            weight[base + 64u * z] = ws[z];
+        for (int z = 0; z < 8; z++) {
+            ts[z] = tsdf[base + 64u * z];
+            ws[z] = weight[base + 64u * z];
         }
+        uint32_t changed = 0;  // bit z: voxel z of this lane was updated
+        while (v >= 0) {
+            int32_t vs[kViewGroup];
+            int32_t n = 1;
+            vs[0] = v;
+            for (; n < kViewGroup; n++) {
+                vs[n] = next_view();
+                if (vs[n] < 0) break;
+            }
+            views_applied += (uint32_t)n;
+            ViewGather grp[kViewGroup];
+#pragma unroll
+            for (int j = 0; j < kViewGroup; j++)
+                if (j < n) gather_view<PRIORS, FAST>(views[vs[j]], maps, priors, wx, wy, wz, grp[j]);
+#pragma unroll
+            for (int j = 0; j < kViewGroup; j++)
+                if (j < n)
+                    apply_view<COUNT, FAST>(views[vs[j]], grp[j], mu, g.inv_mu, ntab, wx, wy, wz, ts, ws, changed,
+                                            updates);
+            v = n == kViewGroup ? next_view() : -1;
+        }
+#pragma unroll
+        for (int z = 0; z < 8; z++) {
+            if (changed & (1u << z)) {
+                tsdf[base + 64u * z] = ts[z];
+                weight[base + 64u * z] = ws[z];
+            }
+        }
+        if (COUNT) written += (uint32_t)__builtin_popcount(changed);
     }
     if (COUNT) {
         // wavefront sums, one atomic each: voxel updates, (voxel, view) pairs evaluated (512 per
-        // visible view of the brick), bricks read, voxels written
-        uint32_t written = (uint32_t)__builtin_popcount(changed);
-        for (int offset = 32; offset > 0; offset >>= 1) {
-            updates += __shfl_down(updates, offset, 64);
-            written += __shfl_down(written, offset, 64);
-        }
-        if (lane == 0) {
-            if (updates) atomicAdd(counter, (unsigned long long)updates);
-            atomicAdd(counter + 1, 512ull * views_applied);
-            atomicAdd(counter + 2, 1ull);
-            if (written) atomicAdd(counter + 3, (unsigned long long)written);
-        }
-    }
-}
-
-// The same pass over the list of bricks with a kept view (brick_list_kernel), for passes that reach
-// few bricks (one view per pass, as the reference integrates keyframes): a fixed grid walks the list
-// instead of launching a wavefront per pool brick.  Block b (XCD b % 8) serves queue q = b % 8,
-// which holds the list chunks q, q + 8, q + 16, ... of `chunk` entries; the queue's wavefronts take
-// its entries in turn (static striding: no atomics, which on this memory are resolved beyond the
-// L2).  Which wavefront updates a brick changes no value: each brick is owned by one wavefront,
-// which applies its views in order.
-template <bool COUNT, bool PRIORS, bool FAST>
-__global__ void __launch_bounds__(256) integrate_list_kernel(TsdfGridDev g, const TsdfView *__restrict__ views,
-                                                             int32_t n_views, const uint32_t *__restrict__ vis,
-                                                             int32_t words, const int4 *__restrict__ coords,
-                                                             const uint32_t *__restrict__ maps,
-                                                             const float *__restrict__ priors,
-                                                             const float4 *__restrict__ ntab,
-                                                             float *__restrict__ tsdf, float *__restrict__ weight,
-                                                             unsigned long long *__restrict__ counter,
-                                                             const uint32_t *__restrict__ ctrs,
-                                                             const uint32_t *__restrict__ list, uint32_t chunk) {
-    const uint32_t lane = threadIdx.x & 63u;
-    const uint32_t wave = threadIdx.x >> 6, waves_per_block = blockDim.x / 64u;
-    const uint32_t q = blockIdx.x % 8u;
-    const uint32_t count = __builtin_amdgcn_readfirstlane(ctrs[0]);
-    const uint32_t stride = (gridDim.x / 8u) * waves_per_block;  // wavefronts serving the queue
-    uint32_t updates = 0, written = 0;       // per lane
-    uint32_t views_applied = 0, bricks = 0;  // wave-uniform
-    for (uint32_t k = (blockIdx.x / 8u) * waves_per_block + wave;; k += stride) {
-        const uint32_t pos = ((k / chunk) * 8u + q) * chunk + k % chunk;  // increasing in k
-        if (pos >= count) break;
-        const uint32_t brick = __builtin_amdgcn_readfirstlane(list[pos]);
-        integrate_brick<COUNT, PRIORS, FAST>(g, views, vis, words, coords, maps, priors, ntab, tsdf, weight, brick,
-                                             lane, updates, written, views_applied);
-        bricks++;
-    }
-    if (COUNT) {
-        // wavefront sums, one atomic each, as integrate_kernel
+        // visible view of a brick), bricks read, voxels written
         for (int offset = 32; offset > 0; offset >>= 1) {
             updates += __shfl_down(updates, offset, 64);
             written += __shfl_down(written, offset, 64);
@@ -821,17 +727,10 @@ static void launch_integrate(hipStream_t s, dim3 grid, dim3 block, const TsdfGri
                              uint32_t chunk) {
     // priors != nullptr: some view of the pass has a prior (the caller passes nullptr otherwise)
     const bool c = counter != nullptr, p = priors != nullptr;
-    if (LIST) {
-        auto k = c ? (p ? integrate_list_kernel<true, true, FAST> : integrate_list_kernel<true, false, FAST>)
-                   : (p ? integrate_list_kernel<false, true, FAST> : integrate_list_kernel<false, false, FAST>);
-        hipLaunchKernelGGL(k, grid, block, 0, s, g, views, n_views, vis, words, coords, maps, priors, ntab, tsdf,
-                           weight, counter, ctrs, list, chunk);
-    } else {
-        auto k = c ? (p ? integrate_kernel<true, true, FAST> : integrate_kernel<true, false, FAST>)
-                   : (p ? integrate_kernel<false, true, FAST> : integrate_kernel<false, false, FAST>);
-        hipLaunchKernelGGL(k, grid, block, 0, s, g, views, n_views, vis, words, coords, maps, priors, ntab, tsdf,
-                           weight, counter, chunk);
-    }
+    auto k = c ? (p ? integrate_kernel<true, true, FAST, LIST> : integrate_kernel<true, false, FAST, LIST>)
+               : (p ? integrate_kernel<false, true, FAST, LIST> : integrate_kernel<false, false, FAST, LIST>);
+    hipLaunchKernelGGL(k, grid, block, 0, s, g, views, n_views, vis, words, coords, maps, priors, ntab, tsdf, weight,
+                       counter, ctrs, list, chunk);
 }
 
 hipError_t tsdf_launch_integrate(hipStream_t s, const TsdfGridDev &g, const TsdfView *views,

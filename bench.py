@@ -263,7 +263,9 @@ def main():
                      "voxel_upd_per_s": updates / t_avg,
                      "pass_counts": counts,
                      "pairs_per_update": round(counts["pairs"] / max(updates, 1), 3),
-                     "roofline": {"kernel": "tsdf_integrate", "bound": "hbm",
+                     # readings against the HBM peak; the pass itself is limited by VALU issue (one
+                     # projection and update per (voxel, view) pair; the counter profiles)
+                     "roofline": {"kernel": "tsdf_integrate", "bound": "hbm", "limiter": "valu",
                                   "achieved": round(bytes_alg / t_avg / 1e9, 1), "peak": HBM_PEAK_GBS,
                                   "unit": "GB/s", "frac": round(bytes_alg / t_avg / 1e9 / HBM_PEAK_GBS, 4),
                                   "bytes_per_launch": bytes_alg, "traffic": traffic,

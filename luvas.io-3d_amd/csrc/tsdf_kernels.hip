@@ -205,10 +205,10 @@ struct ViewGather {
 // host finds the pass inside their ranges (FAST; integrate_fast_ok): every view's camera
 // coordinates of the grid's voxel centres below 2^40 in magnitude, |fx|, |fy| below 2^40, and mu in
 // [2^-40, 2^40].  Then, with the same bits as the IEEE forms:
-//   - the projection divides k*cam by zc in [0.01, 2^40) with one shared reciprocal.  A quotient
-//     below 2^-70 in magnitude (where a residual could underflow and the last bit differ) is then
-//     added to the principal point: either |c| >= 2^-40 and both sums round to c, or both sums are
-//     below 0.5 and round to pixel 0;
+//   - the projection divides k*cam (below 2^80) by zc in [0.01, 2^40) with one shared reciprocal.
+//     Only a quotient below 2^-70 in magnitude can have a residual that underflows and a last bit
+//     that differs; it is then added to the principal point c: either |c| >= 2^-40 and both sums
+//     round to c, or both sums are below 0.5 and round to pixel 0;
 //   - sdf / mu: sdf = d - cam.z with d > 0 and cam.z > 0.01 is 0 or at least 2^-30 in magnitude
 //     (both are multiples of 2^-30 when d >= 2^-7, and |sdf| > 0.002 otherwise), so quotients below
 //     1 stay normal; quotients from 1 up, and the NaN an overflowing intermediate gives, all clamp to

@@ -2,8 +2,9 @@
 sources, a zero-view TSDF pass, volumes nothing can see, empty meshes and clouds, an empty voxel
 list for the adjoint), rejected calls (duplicate or unknown layers, a coarse start without a
 coarse level, a set as its own coarse level, a coarse level destroyed before its fine set), a
-context destroyed before its objects, and ragged and extreme image sizes (1x1, 17x13, 16384x16,
-16x16384: partial tiles and the widest rows) solved bit-exact to the oracle."""
+context destroyed before its objects, ragged and extreme image sizes (1x1, 17x13, 16384x16,
+16x16384: partial tiles and the widest rows) solved bit-exact to the oracle, and a TSDF pass outside
+the ranges of the pass's correctly rounded short forms (the IEEE fallback) equal to the oracle."""
 import numpy as np
 import pytest
 
@@ -175,3 +176,36 @@ def test_fusion_of_empty_maps_and_a_tiny_budget(ctx):
     assert 1 <= len(cloud["positions"]) <= 8 and cell > 0.5
     assert int(cloud["support"].astype(np.int64).sum()) == samples or cloud["support"].max() == 65535
     st.destroy()
+
+
+def test_tsdf_ieee_fallback_matches_oracle(ctx):
+    """A pass outside the ranges of the TSDF pass's correctly rounded short forms runs the IEEE
+    divisions and square roots instead (focal lengths of 2^41 px: only voxels on the optical axis
+    project into the map), and one inside them runs the short forms: both equal the oracle bit for
+    bit, applied one after the other to the same volume."""
+    import lr3d
+    from lr3d import Intrinsics, Pose
+    say("case: TSDF pass with fx = fy = 2^41 (IEEE forms) then fx = fy = 64 (short forms), 64^3 grid")
+    voxel = 2.0 ** -7                      # exact in FP32, so the axis voxels' centres are exactly 0
+    origin = (-31.5 * voxel, -31.5 * voxel, 1.0)
+    grid, layout = lr3d.dense_grid(origin, voxel, 64)
+    W = H = 64
+    packed = np.full(W * H, O.pack_sample(1.25, (0.0, 0.0, -1.0), 0.1), np.uint32)
+    vol = lr3d.HipTsdfVolume(ctx, grid, layout, 4.0)
+    vol.set_counting(True)
+    g = O.Grid(grid.origin, grid.voxel, grid.dims, layout[0], voxel * 4.0)
+    otsdf = np.ones(layout[2] * 512, np.float32)
+    oweight = np.zeros(layout[2] * 512, np.float32)
+    counts = []
+    for f in (2.0 ** 41, 64.0):
+        k = Intrinsics(f, f, 31.5, 31.5)
+        vol.integrate_host(packed, lr3d.IntegrationView(k, Pose(), 3.0, W, H))
+        iv = O.integration_view(W, H, f, f, 31.5, 31.5, np.eye(3), np.zeros(3), 3.0)
+        counts.append(O.tsdf_integrate(g, iv, packed, None, otsdf, oweight))
+        tsdf, weight = vol.read()
+        assert np.array_equal(tsdf.view(np.uint32), otsdf.view(np.uint32))
+        assert np.array_equal(weight.view(np.uint32), oweight.view(np.uint32))
+    say(f"tsdf fallback: {counts[0]} updates on the axis, then {counts[1]}; device counted {vol.update_count()}")
+    assert 0 < counts[0] <= 64 and counts[1] > 10000
+    assert vol.update_count() == sum(counts)
+    vol.destroy()

@@ -320,7 +320,7 @@ def main():
     per_view = None
     if not args.no_plain:
         counts_pv = None
-        pv_ms = pv_launches = 0
+        pv_ms = pv_launches = pv_cull_ms = 0
         for timed in (False, True):
             vol3 = lr3d.HipTsdfVolume(ctx, grid, layout, spec.truncation_voxels)
             vol3.bricks_per_block = vol.bricks_per_block
@@ -333,6 +333,7 @@ def main():
             ctx.synchronize()
             if timed:
                 pv_launches, pv_ms = ctx.kernel_stats("tsdf_integrate")
+                pv_cull_ms = ctx.kernel_stats("tsdf_tiles")[1] + ctx.kernel_stats("tsdf_visibility")[1]
             else:
                 counts_pv = vol3.pass_counts()
             ctx.set_profiling(False)
@@ -342,7 +343,9 @@ def main():
             touched = counts_pv["bricks_read"] * 512 * 8 + counts_pv["voxels_written"] * 8 + map_bytes
             pv_s = pv_ms / 1e3
             per_view = {"passes": pv_launches, "avg_launch_ms": round(pv_ms / pv_launches, 4),
-                        "total_ms": round(pv_ms, 3), "pass_counts": counts_pv,
+                        "total_ms": round(pv_ms, 3),
+                        # depth tiles, culling and the brick list before each pass's integration
+                        "culling_avg_ms": round(pv_cull_ms / pv_launches, 4), "pass_counts": counts_pv,
                         "bytes_touched": touched, "achieved_gbs": round(touched / pv_s / 1e9, 1),
                         "frac_touched": round(touched / pv_s / 1e9 / HBM_PEAK_GBS, 4),
                         "voxel_upd_per_s": counts_pv["updates"] / pv_s,

@@ -150,20 +150,23 @@ __device__ bool depth_reachable(const TsdfGridDev &g, const TsdfView &V, int4 c,
     return (double)dmax >= zmin - g.mu_d - (1e-4 + 1e-6 * zmin);
 }
 
-// TV:220-234, one lane per (brick, view); a half-wavefront holds one 32-view word of one brick
-// (bit j of the word = view 32*word + j visible).  With depth tiles the reference's test is
+// TV:220-234, one lane per (brick, view).  A group of 2^lane_shift consecutive lanes holds one
+// 32-view word of one brick (bit j of the word = view 32*word + j visible): 32 lanes when a pass
+// has more than 32 views, else the number of views rounded up to a power of two, so that a pass
+// of one view does not leave 31 lanes in 32 idle.  With depth tiles the reference's test is
 // followed by depth_reachable, which drops only (brick, view) pairs that update no voxel.
 __global__ void __launch_bounds__(256) visibility_kernel(TsdfGridDev g, const int4 *__restrict__ coords,
                                                          const TsdfView *__restrict__ views,
                                                          int32_t n_views, uint32_t *__restrict__ vis,
                                                          int32_t words, const float *__restrict__ tiles,
-                                                         uint32_t *__restrict__ ctrs) {
+                                                         uint32_t *__restrict__ ctrs, uint32_t lane_shift) {
     const uint32_t gid = blockIdx.x * blockDim.x + threadIdx.x;
     if (gid == 0u) ctrs[0] = 0u;  // the brick list's length, counted by brick_list_kernel
-    const uint32_t pair = gid >> 5;
+    const uint32_t group = 1u << lane_shift;
+    const uint32_t pair = gid >> lane_shift;
     const uint32_t brick = pair / (uint32_t)words;
     const int32_t word = (int32_t)(pair % (uint32_t)words);
-    const int32_t v = word * 32 + (int32_t)(threadIdx.x & 31u);
+    const int32_t v = word * 32 + (int32_t)(gid & (group - 1u));
     bool visible = false;
     if (brick < (uint32_t)g.n_bricks && v < n_views) {
         const int4 c = coords[brick];
@@ -187,8 +190,12 @@ __global__ void __launch_bounds__(256) visibility_kernel(TsdfGridDev g, const in
         if (visible && tiles) visible = depth_reachable(g, V, c, tiles);
     }
     const uint64_t bits = __ballot(visible);
-    if ((threadIdx.x & 31u) == 0 && brick < (uint32_t)g.n_bricks)
-        vis[(size_t)brick * words + word] = (threadIdx.x & 32u) ? (uint32_t)(bits >> 32) : (uint32_t)bits;
+    const uint32_t lane = threadIdx.x & 63u;
+    if ((lane & (group - 1u)) == 0u && brick < (uint32_t)g.n_bricks) {
+        // the group starts at this lane: its bits are the ballot's bits lane .. lane + group - 1
+        const uint32_t mask = group == 32u ? 0xffffffffu : (1u << group) - 1u;
+        vis[(size_t)brick * words + word] = (uint32_t)(bits >> lane) & mask;
+    }
 }
 
 // Depth samples of one view for a lane's 8 voxels: the pixel index (-1 where the voxel is skipped
@@ -702,11 +709,18 @@ hipError_t tsdf_launch_depth_tiles(hipStream_t s, const TsdfView *views, int32_t
 hipError_t tsdf_launch_visibility(hipStream_t s, const TsdfGridDev &g, const int4 *coords,
                                   const TsdfView *views, int32_t n_views, uint32_t *vis,
                                   int32_t words, const float *tiles, uint32_t *ctrs, uint32_t *list) {
-    const uint64_t threads = (uint64_t)g.n_bricks * (uint64_t)words * 32u;
+    // lanes per (brick, word): 32, or for a one-word pass the views rounded up to a power of two
+    uint32_t lane_shift = 5;
+    if (words == 1) {
+        lane_shift = 0;
+        while ((1 << lane_shift) < n_views) lane_shift++;
+    }
+    const uint64_t threads = (uint64_t)g.n_bricks * (uint64_t)words << lane_shift;
     if (threads == 0) return hipSuccess;
     if (threads > 0xffffff00ull) return hipErrorInvalidValue;
     dim3 block(256), grid((uint32_t)((threads + 255u) / 256u));
-    hipLaunchKernelGGL(visibility_kernel, grid, block, 0, s, g, coords, views, n_views, vis, words, tiles, ctrs);
+    hipLaunchKernelGGL(visibility_kernel, grid, block, 0, s, g, coords, views, n_views, vis, words, tiles, ctrs,
+                       lane_shift);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess || !list) return e;
     hipLaunchKernelGGL(brick_list_kernel, dim3(((uint32_t)g.n_bricks + 255u) / 256u), dim3(256), 0, s, vis, words,

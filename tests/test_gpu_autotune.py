@@ -95,25 +95,30 @@ def test_tsdf_autotune_and_every_block_size(ctx, work):
     for t, w in results[1:]:
         assert np.array_equal(t.view(np.uint32), results[0][0].view(np.uint32))
         assert np.array_equal(w.view(np.uint32), results[0][1].view(np.uint32))
-    # passes of more than 16 views launch a wavefront per pool brick (the other schedule): the 8 maps
-    # three times over in one pass, every chunk size and bricks per block, and the list schedule's
-    # result for the same 24 views applied 8 at a time
+    # the 8-view passes above launch a wavefront per pool brick; passes of one or two views walk the
+    # list of bricks with a kept view (the other schedule).  The 8 maps three times over in one pass
+    # with more shapes, and the same 24 views applied one or two per pass with several chunk sizes
     layers24, views24 = list(range(8)) * 3, views * 3
     pool = []
-    for bricks, chunk in ((1, 1), (2, 1), (4, 1), (1, 16), (1, 64), (4, 64), (1, 4096)):
+    for bricks, chunk in ((1, 1), (4, 1), (1, 64), (1, 4096)):
         vol = lr3d.HipTsdfVolume(ctx, grid, layout, spec.truncation_voxels)
         vol.bricks_per_block = bricks
         vol.xcd_chunk = chunk
         vol.integrate_batched(st, layers24, views24, map=lr3d.MAP_RAW)
         pool.append(vol.read())
         vol.destroy()
-    vol = lr3d.HipTsdfVolume(ctx, grid, layout, spec.truncation_voxels)
-    for k in range(3):
-        vol.integrate_batched(st, list(range(8)), views)
-    listed = vol.read()
-    vol.destroy()
-    say(f"tsdf: 24-view pool passes with {len(pool)} launch shapes, and three 8-view list passes")
-    for t, w in pool[1:] + [listed]:
+    listed = []
+    for per, bricks, chunk in ((1, 1, 1), (1, 2, 64), (2, 4, 16), (1, 1, 4096)):
+        vol = lr3d.HipTsdfVolume(ctx, grid, layout, spec.truncation_voxels)
+        vol.bricks_per_block = bricks
+        vol.xcd_chunk = chunk
+        for a in range(0, 24, per):
+            vol.integrate_batched(st, layers24[a:a + per], views24[a:a + per], map=lr3d.MAP_RAW)
+        listed.append(vol.read())
+        vol.destroy()
+    say(f"tsdf: 24 views in one pass with {len(pool)} launch shapes, and in one- and two-view list "
+        f"passes with {len(listed)}")
+    for t, w in pool[1:] + listed:
         assert np.array_equal(t.view(np.uint32), pool[0][0].view(np.uint32))
         assert np.array_equal(w.view(np.uint32), pool[0][1].view(np.uint32))
     vol = lr3d.HipTsdfVolume(ctx, grid, layout, spec.truncation_voxels)

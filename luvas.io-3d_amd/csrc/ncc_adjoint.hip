@@ -9,10 +9,9 @@
 //   pass 1  every view's cost exactly as the forward kernels compute it (FP32), and the k
 //           smallest by (cost, index);
 //   pass 2  for each selected view whose cost is differentiable (all taps inside, variance above
-//           the floor, cost strictly inside (0, 2)), the NCC sums;
-//   pass 3  dC/dH accumulated over the 36 taps and chained through
-//           H = Ks (R + t n^T / rho) Kr^-1, rho = d (n . ray).
-// Passes 2 and 3 run in FP64 from the FP32 plane, the u8 images and the FP64 relative poses: in
+//           the floor, cost strictly inside (0, 2)), one sweep over the 36 taps for the NCC sums
+//           and dC/dH, chained through H = Ks (R + t n^T / rho) Kr^-1, rho = d (n . ray).
+// Pass 2 runs in FP64 from the FP32 plane, the u8 images and the FP64 relative poses: in
 // FP32 the taps q = H (X, Y, 1) at pixel coordinates in the hundreds carry ~1e-4 px of rounding,
 // which moves single-pixel gradients by up to ~1 %.  The top-k selection, INVALID views and the
 // clamp contribute no gradient (the cost is piecewise constant there).
@@ -60,14 +59,20 @@ struct Mat3d {
     double m[9];
 };
 
-// H = Ks (R + t n^T / rho) Kr^-1 in FP64.
-__device__ __forceinline__ void homography64(const AdjGeom &g, int s, const double n[3], double rho,
-                                             Mat3d &H) {
+// Reciprocals of the reference intrinsics (Kr^-1 = [1/fx 0 -cx/fx; 0 1/fy -cy/fy; 0 0 1]).
+struct KrInv {
+    double ifx, ify, cxf, cyf;
+};
+
+// H = Ks (R + t n^T / rho) Kr^-1 in FP64, given 1 / rho.
+__device__ __forceinline__ void homography64(const AdjGeom &g, int s, const double n[3], double inv_rho,
+                                             const KrInv &ki, Mat3d &H) {
+    const double nr[3] = {n[0] * inv_rho, n[1] * inv_rho, n[2] * inv_rho};
     double M[9];
 #pragma unroll
     for (int i = 0; i < 3; i++)
 #pragma unroll
-        for (int j = 0; j < 3; j++) M[i * 3 + j] = g.R[s][i * 3 + j] + g.t[s][i] * n[j] / rho;
+        for (int j = 0; j < 3; j++) M[i * 3 + j] = g.R[s][i * 3 + j] + g.t[s][i] * nr[j];
     const double *k = g.k[s];
     double A[9];
 #pragma unroll
@@ -76,20 +81,13 @@ __device__ __forceinline__ void homography64(const AdjGeom &g, int s, const doub
         A[3 + j] = k[1] * M[3 + j] + k[3] * M[6 + j];
         A[6 + j] = M[6 + j];
     }
-    const double *kr = g.kr;
 #pragma unroll
     for (int i = 0; i < 3; i++) {
-        H.m[i * 3 + 0] = A[i * 3 + 0] / kr[0];
-        H.m[i * 3 + 1] = A[i * 3 + 1] / kr[1];
-        H.m[i * 3 + 2] = A[i * 3 + 2] - A[i * 3 + 0] * kr[2] / kr[0] - A[i * 3 + 1] * kr[3] / kr[1];
+        H.m[i * 3 + 0] = A[i * 3 + 0] * ki.ifx;
+        H.m[i * 3 + 1] = A[i * 3 + 1] * ki.ify;
+        H.m[i * 3 + 2] = A[i * 3 + 2] - A[i * 3 + 0] * ki.cxf - A[i * 3 + 1] * ki.cyf;
     }
 }
-
-// Tap of a source view in FP64: position (clamped into the image, so reads stay inside), the
-// bilinear sample and its texel-space gradient.
-struct Tap {
-    double u, v, rz, val, gu, gv;
-};
 
 // 1 / z to within an ulp or two of double precision: v_rcp_f64 and two Newton steps (the passes are
 // compared with the FP64 reference to 1e-4, far above that; the IEEE quotient's scaling and
@@ -100,44 +98,104 @@ __device__ __forceinline__ double rcp64(double z) {
     return fma(fma(-z, r, 1.0), r, r);
 }
 
-__device__ __forceinline__ Tap source_tap(const uint32_t *__restrict__ img, int32_t qrows, double wmax, double hmax,
-                                          const Mat3d &H, double X, double Y) {
+// Position of a tap of a source view in FP64, clamped into the image so that its texel read stays
+// inside: u, v, 1 / q_z, the bilinear fractions and the index of its 2 x 2 texel quad.
+struct TapPos {
+    double u, v, rz, fx, fy;
+    uint32_t texel;
+};
+
+__device__ __forceinline__ TapPos tap_position(int32_t qrows, double wmax, double hmax, const Mat3d &H, double X,
+                                               double Y) {
     const double qx = fma(H.m[0], X, fma(H.m[1], Y, H.m[2]));
     const double qy = fma(H.m[3], X, fma(H.m[4], Y, H.m[5]));
     const double qz = fma(H.m[6], X, fma(H.m[7], Y, H.m[8]));
-    Tap t;
+    TapPos t;
     t.rz = rcp64(qz);
     t.u = fmin(fmax(qx * t.rz, 0.0), wmax);
     t.v = fmin(fmax(qy * t.rz, 0.0), hmax);
     const double x0 = floor(t.u), y0 = floor(t.v);
-    const double fx = t.u - x0, fy = t.v - y0;
-    const uint32_t q = img[quad_index((uint32_t)x0, (uint32_t)y0, (uint32_t)qrows)];
-    const double a = (double)(q & 0xffu), b = (double)((q >> 8) & 0xffu);
-    const double c = (double)((q >> 16) & 0xffu), e = (double)(q >> 24);
+    t.fx = t.u - x0;
+    t.fy = t.v - y0;
+    t.texel = quad_index((uint32_t)x0, (uint32_t)y0, (uint32_t)qrows);
+    return t;
+}
+
+// Bilinear sample of a texel quad (u8 texels a b / c e) and its texel-space gradient.
+struct TapVal {
+    double val, gu, gv;
+};
+
+// A byte as a double, exactly: the bits of 2^52 + b less 2^52 (integer and FP64 additions in place
+// of conversions).
+__device__ __forceinline__ double byte_double(uint32_t b) {
+    return __hiloint2double(0x43300000, (int)b) - 4503599627370496.0;
+}
+
+__device__ __forceinline__ TapVal tap_sample(uint32_t q, double fx, double fy) {
+    const double a = byte_double(q & 0xffu), b = byte_double((q >> 8) & 0xffu);
+    const double c = byte_double((q >> 16) & 0xffu), e = byte_double(q >> 24);
     const double top = fma(fx, b - a, a), bot = fma(fx, e - c, c);
+    TapVal t;
     t.val = fma(fy, bot - top, top);
     t.gu = (b - a) * (1.0 - fy) + (e - c) * fy;
     t.gv = bot - top;
     return t;
 }
 
-// dC_view/dH (row-major) of a differentiable view (passes 2 and 3).
+// acc += k (a X, a Y, a, b X, b Y, b, c X, c Y, c)
+__device__ __forceinline__ void add_tap_terms(double acc[9], double k, double a, double b, double c,
+                                              double X, double Y) {
+    const double ka = k * a, kb = k * b, kc = k * c;
+    acc[0] = fma(ka, X, acc[0]);
+    acc[1] = fma(ka, Y, acc[1]);
+    acc[2] += ka;
+    acc[3] = fma(kb, X, acc[3]);
+    acc[4] = fma(kb, Y, acc[4]);
+    acc[5] += kb;
+    acc[6] = fma(kc, X, acc[6]);
+    acc[7] = fma(kc, Y, acc[7]);
+    acc[8] += kc;
+}
+
+// dC_view/dH (row-major) of a differentiable view, in one sweep over the 36 taps.
 //   dC/dval_i = (w_i / S) [B (val_i - mean_s) - (r_i - mean_r) A],  A = 1/den, B = cov var_r / den^3
 //   du/dH_0j = x_j / q_z, dv/dH_1j = x_j / q_z, d(u, v)/dH_2j = -(u, v) x_j / q_z
+// so dC/dH = sum_i dC/dval_i T_i with the tap term T_i = (g_u x, g_v x, -(g_u u + g_v v) x) / q_z
+// (g the texel-space image gradient, x = (X, Y, 1)).  A, B and mean_s need the whole window's
+// sums, so the sweep gathers P = sum w val T, Q = sum w T and R = sum w (r - mean_r) T beside
+// the NCC sums, and dC/dH = (B (P - mean_s Q) - A R) / S.
 __device__ __forceinline__ void view_grad_h(const PmSrc &s, const Mat3d &H, double px, double py,
                                             const WindowLds &win, int lane, const RefStats &rs,
                                             const uint32_t *__restrict__ quad, double G[9]) {
     const uint32_t *__restrict__ img = quad + s.qoff;
     const double wmax = (double)s.W - 1.0, hmax = (double)s.H - 1.0;
     double ss = 0.0, sss = 0.0, srs = 0.0;
+    double P[9], Q[9], R[9];
+#pragma unroll
+    for (int k = 0; k < 9; k++) P[k] = Q[k] = R[k] = 0.0;
+    const int32_t qrows = s.qrows;
+    // rows of taps; within a row the column offsets are constants
 #pragma unroll 1
-    for (int i = 0; i < kSamples; i++) {
-        const double X = px + (double)((i % 6) * 2 - 5), Y = py + (double)((i / 6) * 2 - 5);
-        const Tap t = source_tap(img, s.qrows, wmax, hmax, H, X, Y);
-        const double wv = (double)win.w[i][lane] * t.val;
-        ss += wv;
-        sss = fma(wv, t.val, sss);
-        srs = fma(wv, (double)win.v[i][lane], srs);
+    for (int row = 0; row < 6; row++) {
+        const double Y = py + (double)(row * 2 - 5);
+#pragma unroll
+        for (int col = 0; col < 6; col++) {
+            const int i = row * 6 + col;
+            const double X = px + (double)(col * 2 - 5);
+            const TapPos t = tap_position(qrows, wmax, hmax, H, X, Y);
+            const TapVal tv = tap_sample(img[t.texel], t.fx, t.fy);
+            const double w = (double)win.w[i][lane], r = (double)win.v[i][lane];
+            const double wv = w * tv.val;
+            ss += wv;
+            sss = fma(wv, tv.val, sss);
+            srs = fma(wv, r, srs);
+            const double a = tv.gu * t.rz, b = tv.gv * t.rz;
+            const double c = -(a * t.u + b * t.v);
+            add_tap_terms(P, wv, a, b, c, X, Y);
+            add_tap_terms(Q, w, a, b, c, X, Y);
+            add_tap_terms(R, w * (r - rs.mean), a, b, c, X, Y);
+        }
     }
     const double inv_sw = 1.0 / rs.sw;
     const double mean = ss * inv_sw;
@@ -147,25 +205,31 @@ __device__ __forceinline__ void view_grad_h(const PmSrc &s, const Mat3d &H, doub
     const double A = den > 0.0 ? 1.0 / den : 0.0;
     const double B = cov * rs.var * A * A * A;
 #pragma unroll
-    for (int k = 0; k < 9; k++) G[k] = 0.0;
-#pragma unroll 1
-    for (int i = 0; i < kSamples; i++) {
-        const double X = px + (double)((i % 6) * 2 - 5), Y = py + (double)((i / 6) * 2 - 5);
-        const Tap t = source_tap(img, s.qrows, wmax, hmax, H, X, Y);
-        const double dval = (double)win.w[i][lane] * inv_sw *
-                            (B * (t.val - mean) - ((double)win.v[i][lane] - rs.mean) * A);
-        const double gx = dval * t.gu * t.rz, gy = dval * t.gv * t.rz;
-        const double gz = -(gx * t.u + gy * t.v);
-        G[0] = fma(gx, X, G[0]);
-        G[1] = fma(gx, Y, G[1]);
-        G[2] += gx;
-        G[3] = fma(gy, X, G[3]);
-        G[4] = fma(gy, Y, G[4]);
-        G[5] += gy;
-        G[6] = fma(gz, X, G[6]);
-        G[7] = fma(gz, Y, G[7]);
-        G[8] += gz;
-    }
+    for (int k = 0; k < 9; k++) G[k] = inv_sw * (B * (P[k] - mean * Q[k]) - A * R[k]);
+}
+
+// Number of set bits of a wavefront mask in the lanes below this one.
+__device__ __forceinline__ uint32_t below(uint64_t mask) {
+    return __builtin_amdgcn_mbcnt_hi((uint32_t)(mask >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)mask, 0u));
+}
+
+// Sums of 16 values over the wavefront by recursive halving: at each step a lane keeps half of its
+// values and adds its partner's copies of them (partners 32, 16, 8 and 4 lanes apart), then the
+// last value is summed over the 4 lanes left (partners 2 and 1 apart).  Every lane returns the
+// total of value (lane >> 2) & 15.  17 exchanges instead of 16 shuffle trees of 6; the pattern is
+// fixed, so the sums are deterministic.  Every lane of the wavefront must call it.
+__device__ __forceinline__ double wave_sums16(const double v[16], int lane) {
+    double a8[8], a4[4], a2[2];
+    const bool b5 = lane & 32, b4 = lane & 16, b3 = lane & 8, b2 = lane & 4;
+#pragma unroll
+    for (int k = 0; k < 8; k++) a8[k] = (b5 ? v[8 + k] : v[k]) + __shfl_xor(b5 ? v[k] : v[8 + k], 32, 64);
+#pragma unroll
+    for (int k = 0; k < 4; k++) a4[k] = (b4 ? a8[4 + k] : a8[k]) + __shfl_xor(b4 ? a8[k] : a8[4 + k], 16, 64);
+#pragma unroll
+    for (int k = 0; k < 2; k++) a2[k] = (b3 ? a4[2 + k] : a4[k]) + __shfl_xor(b3 ? a4[k] : a4[2 + k], 8, 64);
+    double a1 = (b2 ? a2[1] : a2[0]) + __shfl_xor(b2 ? a2[0] : a2[1], 4, 64);
+    a1 += __shfl_xor(a1, 2, 64);
+    return a1 + __shfl_xor(a1, 1, 64);
 }
 
 // Wavefront sum (fixed shuffle tree, deterministic), result valid in lane 0.  Every lane of the
@@ -249,36 +313,70 @@ __global__ void __launch_bounds__(64) ncc_backward_kernel(PmPlan plan, PmPools p
         }
         cost = total / (float)d.topk;
     }
-    // FP64 plane: X = d ray, rho = n . X
-    const double n[3] = {(double)h.x, (double)h.y, (double)h.z};
-    const double dep = (double)h.w;
-    const double rayd[3] = {((double)x - g.kr[2]) / g.kr[0], ((double)y - g.kr[3]) / g.kr[1], 1.0};
-    const double Xp[3] = {rayd[0] * dep, rayd[1] * dep, dep};
-    const double nrd = n[0] * rayd[0] + n[1] * rayd[1] + n[2] * rayd[2];
-    const double rhod = nrd * dep, r2 = rhod * rhod;
     const double scale = (double)weight / (double)d.topk;
     double dd = 0.0, dn0 = 0.0, dn1 = 0.0, dn2 = 0.0;
     const uint32_t b = xcd_remap(blockIdx.x, gridDim.x);
     double *partial = pose_partials ? pose_partials + (size_t)b * kAdjPartial : nullptr;
     const bool lead = (threadIdx.x | threadIdx.y) == 0;
-    // passes 2 and 3 per source view; the loop is uniform over the wavefront for the reduction
-    for (int s = 0; s < d.n_src; s++) {
+    // Pass 2 runs over the wavefront's (pixel, view) items compacted across lanes: a pixel
+    // differentiates at most k of its views, so a loop over the views with one lane per pixel
+    // would leave most lanes idle.  Items are listed view-major (pixels ascending within a view):
+    // view s owns list entries [first[s], first[s + 1]).  INVALID (2), degenerate (exactly 1) or
+    // clamped (0, 2) views carry no gradient, the cost being constant there; a non-degenerate
+    // view whose FP32 cost is exactly 1 is skipped as well.
+    __shared__ uint8_t items[64 * kMaxSources];
+    uint64_t owns[kMaxSources];
+    uint32_t first[kMaxSources + 1];
+    first[0] = 0u;
+#pragma unroll
+    for (int s = 0; s < kMaxSources; s++) {
+        const float cs = costs[s];
+        const bool smooth = s < d.n_src && valid && (selected & (1u << s)) && cs > 0.0f && cs < 2.0f && cs != 1.0f;
+        owns[s] = __ballot(smooth);
+        if (smooth) items[first[s] + below(owns[s])] = (uint8_t)lane;
+        first[s + 1] = first[s] + (uint32_t)__popcll(owns[s]);
+    }
+    __syncthreads();
+    const uint32_t n_items = first[kMaxSources];
+    // divisions shared by every item of the block
+    const double inv_topk = 1.0 / (double)d.topk;
+    KrInv ki;
+    ki.ifx = 1.0 / g.kr[0];
+    ki.ify = 1.0 / g.kr[1];
+    ki.cxf = g.kr[2] * ki.ifx;
+    ki.cyf = g.kr[3] * ki.ify;
+    for (uint32_t base = 0; base < n_items; base += 64) {
+        // this lane's item, its view and pixel; the pixel's plane, weight and window statistics
+        // come from the pixel's lane (every lane takes part in the exchanges)
+        const uint32_t i = base + (uint32_t)lane;
+        const bool act = i < n_items;
+        const int32_t p = act ? (int32_t)items[i] : 0;
+        int s = 0;
+#pragma unroll
+        for (int q = 1; q < kMaxSources; q++) s += i >= first[q] ? 1 : 0;
+        const float hx = __shfl(h.x, p), hy = __shfl(h.y, p), hz = __shfl(h.z, p), hw = __shfl(h.w, p);
+        const float wp = __shfl(weight, p);
+        const int32_t xp = __shfl(x, p), yp = __shfl(y, p);
+        const RefStats rsp{__shfl(rs.sw, p), __shfl(rs.mean, p), __shfl(rs.var, p)};
+        double c[4] = {0.0, 0.0, 0.0, 0.0};
         double gpose[12];
 #pragma unroll
         for (int j = 0; j < 12; j++) gpose[j] = 0.0;
-        float cs = INFINITY;
-#pragma unroll
-        for (int q = 0; q < kMaxSources; q++)
-            if (q == s) cs = costs[q];
-        // INVALID (2), degenerate (exactly 1) or clamped (0, 2) views carry no gradient: the cost
-        // is constant there.  A non-degenerate view whose FP32 cost is exactly 1 is skipped as well.
-        const bool smooth = valid && (selected & (1u << s)) && cs > 0.0f && cs < 2.0f && cs != 1.0f;
-        if (smooth) {
+        if (act) {
+            // FP64 plane: X = d ray, rho = n . X
+            const double n[3] = {(double)hx, (double)hy, (double)hz};
+            const double dep = (double)hw;
+            const double rayd[3] = {((double)xp - g.kr[2]) * ki.ifx, ((double)yp - g.kr[3]) * ki.ify, 1.0};
+            const double Xp[3] = {rayd[0] * dep, rayd[1] * dep, dep};
+            const double nrd = n[0] * rayd[0] + n[1] * rayd[1] + n[2] * rayd[2];
+            const double rhod = nrd * dep;
+            const double inv_rho = 1.0 / rhod, inv_r2 = inv_rho * inv_rho;
+            const double scp = (double)wp * inv_topk;
             const PmSrc &src = d.src[s];
             Mat3d H;
-            homography64(g, s, n, rhod, H);
+            homography64(g, s, n, inv_rho, ki, H);
             double G[9];
-            view_grad_h(src, H, (double)x, (double)y, lds, lane, rs, pools.quad, G);
+            view_grad_h(src, H, (double)xp, (double)yp, lds, p, rsp, pools.quad, G);
             // G_M = Ks^T G Kr^-T
             const double *k = g.k[s];
             double T[9];
@@ -290,30 +388,59 @@ __global__ void __launch_bounds__(64) ncc_backward_kernel(PmPlan plan, PmPools p
             }
             double GM[9];
 #pragma unroll
-            for (int i = 0; i < 3; i++) {
-                GM[i * 3 + 0] = (T[i * 3 + 0] - T[i * 3 + 2] * g.kr[2]) / g.kr[0];
-                GM[i * 3 + 1] = (T[i * 3 + 1] - T[i * 3 + 2] * g.kr[3]) / g.kr[1];
-                GM[i * 3 + 2] = T[i * 3 + 2];
+            for (int a = 0; a < 3; a++) {
+                GM[a * 3 + 0] = (T[a * 3 + 0] - T[a * 3 + 2] * g.kr[2]) * ki.ifx;
+                GM[a * 3 + 1] = (T[a * 3 + 1] - T[a * 3 + 2] * g.kr[3]) * ki.ify;
+                GM[a * 3 + 2] = T[a * 3 + 2];
             }
             const double *t = g.t[s];
             double gmn[3];
 #pragma unroll
-            for (int i = 0; i < 3; i++) gmn[i] = GM[i * 3] * n[0] + GM[i * 3 + 1] * n[1] + GM[i * 3 + 2] * n[2];
+            for (int a = 0; a < 3; a++) gmn[a] = GM[a * 3] * n[0] + GM[a * 3 + 1] * n[1] + GM[a * 3 + 2] * n[2];
             const double tGn = t[0] * gmn[0] + t[1] * gmn[1] + t[2] * gmn[2];
-            dd += -tGn * nrd / r2;
-            dn0 += (t[0] * GM[0] + t[1] * GM[3] + t[2] * GM[6]) / rhod - tGn * Xp[0] / r2;
-            dn1 += (t[0] * GM[1] + t[1] * GM[4] + t[2] * GM[7]) / rhod - tGn * Xp[1] / r2;
-            dn2 += (t[0] * GM[2] + t[1] * GM[5] + t[2] * GM[8]) / rhod - tGn * Xp[2] / r2;
+            c[0] = -tGn * nrd * inv_r2;
+            c[1] = (t[0] * GM[0] + t[1] * GM[3] + t[2] * GM[6]) * inv_rho - tGn * Xp[0] * inv_r2;
+            c[2] = (t[0] * GM[1] + t[1] * GM[4] + t[2] * GM[7]) * inv_rho - tGn * Xp[1] * inv_r2;
+            c[3] = (t[0] * GM[2] + t[1] * GM[5] + t[2] * GM[8]) * inv_rho - tGn * Xp[2] * inv_r2;
 #pragma unroll
-            for (int j = 0; j < 9; j++) gpose[j] = scale * GM[j];
+            for (int j = 0; j < 9; j++) gpose[j] = scp * GM[j];
 #pragma unroll
-            for (int j = 0; j < 3; j++) gpose[9 + j] = scale * gmn[j] / rhod;
+            for (int j = 0; j < 3; j++) gpose[9 + j] = scp * gmn[j] * inv_rho;
         }
-        if (partial) {
+        // views with items in this round: s_lo .. s_hi (wave-uniform)
+        const uint32_t last = min(base + 64u, n_items) - 1u;
+        int s_lo = 0, s_hi = 0;
 #pragma unroll
-            for (int j = 0; j < 12; j++) {
-                const double sum = wave_sum(gpose[j]);
-                if (lead) partial[2 + s * 12 + j] = sum;
+        for (int q = 1; q < kMaxSources; q++) {
+            s_lo += base >= first[q] ? 1 : 0;
+            s_hi += last >= first[q] ? 1 : 0;
+        }
+        // each pixel adds its items' terms in view order, as a loop over its views would
+#pragma unroll
+        for (int q = 0; q < kMaxSources; q++) {
+            if (q < s_lo || q > s_hi) continue;
+            const uint32_t at = first[q] + below(owns[q]);
+            const bool mine = ((owns[q] >> lane) & 1u) && at >= base && at < base + 64u;
+            const int32_t from = mine ? (int32_t)(at - base) : 0;
+            const double t0 = __shfl(c[0], from), t1 = __shfl(c[1], from);
+            const double t2 = __shfl(c[2], from), t3 = __shfl(c[3], from);
+            if (mine) {
+                dd += t0;
+                dn0 += t1;
+                dn1 += t2;
+                dn2 += t3;
+            }
+        }
+        // pose gradients: per view, the 12 sums over the round's lanes (a fixed exchange pattern),
+        // added in round order to the block's partial (zeroed before the launch)
+        if (partial) {
+            for (int q = s_lo; q <= s_hi; q++) {
+                double v[16];
+#pragma unroll
+                for (int j = 0; j < 16; j++) v[j] = j < 12 && act && s == q ? gpose[j] : 0.0;
+                const double sum = wave_sums16(v, lane);
+                const int j = (lane >> 2) & 15;
+                if ((lane & 3) == 0 && j < 12) partial[2 + q * 12 + j] += sum;
             }
         }
     }

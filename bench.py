@@ -26,6 +26,8 @@ Prints one JSON line.
   accuracy       bad pixels (> 1 % relative depth error vs the analytic depth) on the filtered maps
                  that are fused (the gated map) and on the raw PatchMatch maps
   adjoint        the NCC adjoint of every view in one launch (outside the step)
+  downstream     dense-cloud fusion of the filtered maps, surface nets of the volume and small-
+                 component removal (outside the step)
 """
 import argparse
 import json
@@ -74,6 +76,7 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint legs")
+    ap.add_argument("--no-downstream", action="store_true", help="skip the fusion and meshing leg")
     ap.add_argument("--no-profile", action="store_true", help="time without kernel events")
     ap.add_argument("--no-autotune", action="store_true", help="keep the default launch shapes")
     ap.add_argument("--tiles", default=None,
@@ -355,6 +358,58 @@ def main():
                 f"{per_view['achieved_gbs']} GB/s on the bytes they touch")
         tsdf["per_view"] = per_view
 
+    # The stages after the step (outside it), on its outputs, with the reference's settings:
+    # dense-cloud fusion of every filtered map (fuseDepthMaps with 4 mm cells, the high-quality
+    # profile's 600000-cell budget, quarter-size grey thumbnails), surface nets of the step's volume
+    # (min weight 0.6) and the removal of components under max(200 faces, 1 %) (tsdf-meshing.ts:87-91).
+    # The calls are synchronous: wall time per call (fusion: thumbnails up, cloud back), the median
+    # of 3; the surface-net kernels also with HIP events, against the bytes they must move (8 B per
+    # pool voxel read, 24 B per vertex and 4 B per index written).
+    downstream = None
+    if not args.no_downstream:
+        thumbs = []
+        for im in work.images:
+            g = np.asarray(im)[::4, ::4]
+            thumbs.append(np.ascontiguousarray(np.stack([g, g, g], axis=2)))
+        fus_s = []
+        for _ in range(3):
+            ctx.synchronize()
+            t3 = time.perf_counter()
+            cloud, final_cell = lr3d.fuse_depth_maps(st, list(range(n)), thumbs, 0.004, 600_000)
+            fus_s.append(time.perf_counter() - t3)
+        fus = sorted(fus_s)[1]
+        samples = sum(int(np.count_nonzero(st.read_filtered(r))) for r in range(n))
+        mesh_s, comp_s = [], []
+        ctx.set_profiling(True)
+        for _ in range(3):
+            ctx.synchronize()
+            t3 = time.perf_counter()
+            mesh = vol.extract_surface(0.6)
+            ctx.synchronize()
+            t4 = time.perf_counter()
+            kept = mesh.remove_small_components(0.01, 200)
+            ctx.synchronize()
+            mesh_s.append(t4 - t3)
+            comp_s.append(time.perf_counter() - t4)
+            (verts, indices), (kverts, kindices) = mesh.size, kept.size
+            mesh.destroy()
+            kept.destroy()
+        mesh_kernel_ms = sum(ctx.kernel_stats(k)[1] for k in ("mesh_active", "mesh_vertices", "mesh_quads")) / 3
+        ctx.set_profiling(False)
+        mesh_bytes = 8 * layout[2] * 512 + 24 * verts + 4 * indices
+        downstream = {
+            "fusion": {"ms": round(fus * 1e3, 2), "samples": samples, "samples_per_s": round(samples / fus, 1),
+                       "points": int(len(cloud["support"])), "cell_m": 0.004, "cell_budget": 600_000,
+                       "final_cell_m": round(final_cell, 6)},
+            "surface_nets": {"ms": round(sorted(mesh_s)[1] * 1e3, 3), "kernel_ms": round(mesh_kernel_ms, 3),
+                             "pool_voxels": layout[2] * 512, "vertices": verts, "triangles": indices // 3,
+                             "bytes": mesh_bytes, "achieved_gbs": round(mesh_bytes / mesh_kernel_ms / 1e6, 1),
+                             "frac": round(mesh_bytes / mesh_kernel_ms / 1e6 / HBM_PEAK_GBS, 4)},
+            "small_components": {"ms": round(sorted(comp_s)[1] * 1e3, 3), "vertices_kept": kverts,
+                                 "triangles_kept": kindices // 3}}
+        log(f"downstream: fusion {downstream['fusion']['ms']} ms ({len(cloud['support'])} points), surface nets "
+            f"{downstream['surface_nets']['ms']} ms, small components {downstream['small_components']['ms']} ms")
+
     cpu = None
     if not args.no_cpu_baseline:
         sys.path.insert(0, os.path.join(ROOT, "oracle"))
@@ -415,6 +470,7 @@ def main():
         "tsdf": tsdf,
         "plain": plain,
         "adjoint": adjoint,
+        "downstream": downstream,
         "accuracy": {"gate": "filtered maps (the maps that are fused) <= 1 % bad pixels",
                      "gate_pass": bad / max(valid, 1) <= 0.01,
                      "filtered_bad_frac": round(bad / max(valid, 1), 5), "filtered_valid_px": valid,

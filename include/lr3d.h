@@ -266,9 +266,13 @@ int32_t lr3d_mesh_destroy(lr3d_mesh *m);
  * Decodes the layers' packed maps (map selector) with the stereo set's cameras, colours each
  * sample from its view's RGB thumbnail, and averages position, normal and colour per cell of
  * `cell` metres, growing the cell by 1.4 while more than cell_budget cells are occupied (after
- * each view).  Per-cell sums equal the reference's (float sums in sample order); points come out in
- * cell-key order (the reference's follow its hash table), and a coarsening merges cells in key
- * order (the reference: table order).  final_cell (may be NULL) receives the cell size used. */
+ * each view).  Like the reference's accumulator, it re-bins every cell by its mean position when
+ * its hash table (65536 slots at first, doubling) would pass 60 % load, at the same sample.
+ * Per-cell sums equal the reference's (float sums in sample order); points come out in cell-key
+ * order (the reference's follow its hash table), and cells that a re-binning merges are merged in
+ * key order (the reference: table order).  final_cell (may be NULL) receives the cell size used.
+ * The views are processed in chunks (LR3D_FUSION_CHUNK_VIEWS=k in the environment fixes them at
+ * k views; the result does not depend on it). */
 typedef struct {
     int32_t width, height;
     const uint8_t *rgb;  /* width*height*3 */

@@ -3,13 +3,15 @@
 // frontend/src/app/vision/fusion/point-accumulator.ts (PA:17-143).
 //
 // The accumulator sums position, normal and colour per integer cell in Float32 (each += rounds
-// to float) in the order the samples arrive (views in order, pixels in row-major order).  Here a
-// view's valid samples get their cell key in double (unpackDepthSample, fromCamera, floor(x/cell)
-// as the JavaScript computes them), a stable radix sort groups them by cell keeping pixel order,
-// and one thread per cell adds them in that order onto the cell's running sums, so every cell's
-// sums equal the reference's.  Cells are kept sorted by key rather than in hash-table order: the
-// output order differs from the reference's slot order, and coarsening (re-binning by mean
-// position, PA:98-131) merges cells in key order where the reference merges in slot order.
+// to float) in the order the samples arrive (views in order, pixels in row-major order).  Here
+// the valid samples of a chunk of views get their cell key in double (unpackDepthSample,
+// fromCamera, floor(x/cell) as the JavaScript computes them), a stable radix sort groups them by
+// cell keeping arrival order, and one thread per cell adds them in that order onto the cell's
+// running sums, so every cell's sums equal the reference's.  The host ends a chunk where the
+// reference re-bins its cells (a coarsening after a view, a table growth before a sample).  Cells
+// are kept sorted by key rather than in hash-table order: the output order differs from the
+// reference's slot order, and a re-binning (by mean position, PA:87-131) merges cells in key
+// order where the reference merges in slot order.
 #include "lr3d_internal.h"
 
 #include <math.h>
@@ -77,24 +79,73 @@ __device__ __forceinline__ bool sample_of(const FusionView &v, const uint32_t *_
     return true;
 }
 
-__global__ void valid_kernel(FusionView v, const uint32_t *__restrict__ maps, uint32_t *__restrict__ flag) {
+// valid samples of a view from pixel `from` on (earlier pixels are flagged empty)
+__global__ void valid_kernel(FusionView v, uint32_t from, const uint32_t *__restrict__ maps,
+                             uint32_t *__restrict__ flag) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= (uint32_t)(v.W * v.H)) return;
-    const uint32_t value = maps[v.map_off + i];
+    const uint32_t value = i >= from ? maps[v.map_off + i] : 0u;
     const double d = value ? (double)(float)half_to_double(value & 0xffffu) : 0.0;
     flag[i] = d > 0.0 ? 1u : 0u;
 }
 
-// PA:37-48: cell key floor(x / cell) of every valid sample, with its pixel index
-__global__ void key_kernel(FusionView v, const uint32_t *__restrict__ maps, const uint8_t *__restrict__ thumbs,
-                           const uint32_t *__restrict__ flag, const uint32_t *__restrict__ pos, double cell,
-                           uint64_t *__restrict__ keys, uint32_t *__restrict__ idx, uint32_t *__restrict__ overflow) {
+// PA:37-48: cell key floor(x / cell) of every valid sample of a view, with its sample number: the
+// view's first pixel is sample number `base` of the chunk (views in order, pixels row-major)
+__global__ void key_kernel(FusionView v, uint32_t base, const uint32_t *__restrict__ maps,
+                           const uint8_t *__restrict__ thumbs, const uint32_t *__restrict__ flag,
+                           const uint32_t *__restrict__ pos, double cell, uint64_t *__restrict__ keys,
+                           uint32_t *__restrict__ idx, uint32_t *__restrict__ overflow) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= (uint32_t)(v.W * v.H) || !flag[i]) return;
+    if (i >= (uint32_t)(v.W * v.H) || !flag[base + i]) return;
     double s[9];
     sample_of(v, maps, thumbs, i, s);
-    keys[pos[i]] = pack_key((int64_t)floor(s[0] / cell), (int64_t)floor(s[1] / cell), (int64_t)floor(s[2] / cell), overflow);
-    idx[pos[i]] = i;
+    const uint32_t at = pos[base + i];
+    keys[at] = pack_key((int64_t)floor(s[0] / cell), (int64_t)floor(s[1] / cell), (int64_t)floor(s[2] / cell), overflow);
+    idx[at] = base + i;
+}
+
+// The chunk view holding sample number p: the last c with vstart[c] <= p (vstart[0] = 0).
+__device__ __forceinline__ uint32_t view_of(const uint32_t *__restrict__ vstart, uint32_t nv, uint32_t p) {
+    uint32_t lo = 0, hi = nv - 1;
+    while (lo < hi) {
+        const uint32_t mid = (lo + hi + 1) >> 1;
+        if (vstart[mid] <= p) lo = mid; else hi = mid - 1;
+    }
+    return lo;
+}
+
+// New cells per chunk view of their first sample (integer atomics: the counts do not depend on the
+// order of the additions)
+__global__ void first_view_kernel(const uint32_t *__restrict__ idx, const uint32_t *__restrict__ start,
+                                  const uint32_t *__restrict__ is_new, uint32_t segments,
+                                  const uint32_t *__restrict__ vstart, uint32_t nv, uint32_t *__restrict__ hist) {
+    const uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= segments || !is_new[g]) return;
+    atomicAdd(&hist[view_of(vstart, nv, idx[start[g]])], 1u);
+}
+
+// Flags, over the pixels of chunk view c, the first samples of the new cells first seen there.
+__global__ void new_cell_flag_kernel(const uint32_t *__restrict__ idx, const uint32_t *__restrict__ start,
+                                     const uint32_t *__restrict__ is_new, uint32_t segments, uint32_t lo, uint32_t hi,
+                                     uint32_t *__restrict__ flag) {
+    const uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= segments || !is_new[g]) return;
+    const uint32_t p = idx[start[g]];
+    if (p >= lo && p < hi) flag[p - lo] = 1u;
+}
+
+// The pixel of the k-th flagged pixel (k from 0), given the exclusive scan of the flags.
+__global__ void select_kernel(const uint32_t *__restrict__ flag, const uint32_t *__restrict__ scan, uint32_t n,
+                              uint32_t k, uint32_t *__restrict__ out) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n && flag[i] && scan[i] == k) *out = i;
+}
+
+// Cells first seen at sample `limit` or later are not created in this pass.
+__global__ void limit_kernel(const uint32_t *__restrict__ idx, const uint32_t *__restrict__ start, uint32_t segments,
+                             uint32_t limit, uint32_t *__restrict__ is_new) {
+    const uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
+    if (g < segments && idx[start[g]] >= limit) is_new[g] = 0u;
 }
 
 __global__ void head_kernel(const uint64_t *__restrict__ keys, uint32_t n, uint32_t *__restrict__ head) {
@@ -125,26 +176,33 @@ __global__ void lookup_kernel(const uint64_t *__restrict__ keys, const uint32_t 
     is_new[g] = found ? 0u : 1u;
 }
 
-// PA:49-62: one thread per cell, the samples added in pixel order onto the running sums
-__global__ void reduce_kernel(FusionView v, const uint32_t *__restrict__ maps, const uint8_t *__restrict__ thumbs,
+// PA:49-62: one thread per cell, the samples added in arrival order (views in order, pixels
+// row-major: the stable sort kept it) onto the running sums; samples numbered `limit` or later
+// belong to views after the chunk's end and are left out.
+__global__ void reduce_kernel(const FusionView *__restrict__ views, const uint32_t *__restrict__ vstart, uint32_t nv,
+                              uint32_t limit, const uint32_t *__restrict__ maps, const uint8_t *__restrict__ thumbs,
                               const uint64_t *__restrict__ keys, const uint32_t *__restrict__ idx,
                               const uint32_t *__restrict__ start, uint32_t segments, uint32_t n,
-                              const int32_t *__restrict__ exist, const uint32_t *__restrict__ new_pos,
-                              float *__restrict__ state_sums, uint32_t *__restrict__ state_counts,
-                              uint64_t *__restrict__ new_keys, float *__restrict__ new_sums,
-                              uint32_t *__restrict__ new_counts) {
+                              const int32_t *__restrict__ exist, const uint32_t *__restrict__ is_new,
+                              const uint32_t *__restrict__ new_pos, float *__restrict__ state_sums,
+                              uint32_t *__restrict__ state_counts, uint64_t *__restrict__ new_keys,
+                              float *__restrict__ new_sums, uint32_t *__restrict__ new_counts) {
     const uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
     if (g >= segments) return;
     const uint32_t a = start[g], b = g + 1 < segments ? start[g + 1] : n;
     const int32_t e = exist[g];
+    if (e < 0 && !is_new[g]) return;  // a new cell first seen after the chunk's end
     float sums[9];
     uint32_t count = 0;
     for (int c = 0; c < 9; c++) sums[c] = e >= 0 ? state_sums[(size_t)e * 9 + c] : 0.0f;
     if (e >= 0) count = state_counts[e];
     for (uint32_t j = a; j < b; j++) {
+        const uint32_t p = idx[j];
+        if (p >= limit) break;
+        const uint32_t c = view_of(vstart, nv, p);
         double s[9];
-        sample_of(v, maps, thumbs, idx[j], s);
-        for (int c = 0; c < 9; c++) sums[c] = (float)((double)sums[c] + s[c]);
+        sample_of(views[c], maps, thumbs, p - vstart[c], s);
+        for (int k = 0; k < 9; k++) sums[k] = (float)((double)sums[k] + s[k]);
         count++;
     }
     float *dst = e >= 0 ? state_sums + (size_t)e * 9 : new_sums + (size_t)new_pos[g] * 9;
@@ -240,13 +298,29 @@ inline dim3 grid_for(uint64_t n) { return dim3((unsigned)((n + 255) / 256)); }
         return hipGetLastError();                                                           \
     } while (0)
 
-hipError_t fusion_launch_valid(hipStream_t s, const FusionView &v, const uint32_t *maps, uint32_t *flag) {
-    LAUNCH(valid_kernel, (uint64_t)v.W * v.H, v, maps, flag);
+hipError_t fusion_launch_valid(hipStream_t s, const FusionView &v, uint32_t from, const uint32_t *maps, uint32_t *flag) {
+    LAUNCH(valid_kernel, (uint64_t)v.W * v.H, v, from, maps, flag);
 }
-hipError_t fusion_launch_keys(hipStream_t s, const FusionView &v, const uint32_t *maps, const uint8_t *thumbs,
-                              const uint32_t *flag, const uint32_t *pos, double cell, uint64_t *keys, uint32_t *idx,
-                              uint32_t *overflow) {
-    LAUNCH(key_kernel, (uint64_t)v.W * v.H, v, maps, thumbs, flag, pos, cell, keys, idx, overflow);
+hipError_t fusion_launch_new_cell_flags(hipStream_t s, const uint32_t *idx, const uint32_t *start, const uint32_t *is_new,
+                                        uint32_t segments, uint32_t lo, uint32_t hi, uint32_t *flag) {
+    LAUNCH(new_cell_flag_kernel, segments, idx, start, is_new, segments, lo, hi, flag);
+}
+hipError_t fusion_launch_select(hipStream_t s, const uint32_t *flag, const uint32_t *scan, uint32_t n, uint32_t k,
+                                uint32_t *out) {
+    LAUNCH(select_kernel, n, flag, scan, n, k, out);
+}
+hipError_t fusion_launch_keys(hipStream_t s, const FusionView &v, uint32_t base, const uint32_t *maps,
+                              const uint8_t *thumbs, const uint32_t *flag, const uint32_t *pos, double cell,
+                              uint64_t *keys, uint32_t *idx, uint32_t *overflow) {
+    LAUNCH(key_kernel, (uint64_t)v.W * v.H, v, base, maps, thumbs, flag, pos, cell, keys, idx, overflow);
+}
+hipError_t fusion_launch_first_views(hipStream_t s, const uint32_t *idx, const uint32_t *start, const uint32_t *is_new,
+                                     uint32_t segments, const uint32_t *vstart, uint32_t nv, uint32_t *hist) {
+    LAUNCH(first_view_kernel, segments, idx, start, is_new, segments, vstart, nv, hist);
+}
+hipError_t fusion_launch_limit(hipStream_t s, const uint32_t *idx, const uint32_t *start, uint32_t segments,
+                               uint32_t limit, uint32_t *is_new) {
+    LAUNCH(limit_kernel, segments, idx, start, segments, limit, is_new);
 }
 hipError_t fusion_launch_heads(hipStream_t s, const uint64_t *keys, uint32_t n, uint32_t *head) {
     LAUNCH(head_kernel, n, keys, n, head);
@@ -258,12 +332,14 @@ hipError_t fusion_launch_lookup(hipStream_t s, const uint64_t *keys, const uint3
                                 const uint64_t *state_keys, uint32_t state_n, int32_t *exist, uint32_t *is_new) {
     LAUNCH(lookup_kernel, segments, keys, start, segments, state_keys, state_n, exist, is_new);
 }
-hipError_t fusion_launch_reduce(hipStream_t s, const FusionView &v, const uint32_t *maps, const uint8_t *thumbs,
-                                const uint64_t *keys, const uint32_t *idx, const uint32_t *start, uint32_t segments,
-                                uint32_t n, const int32_t *exist, const uint32_t *new_pos, float *state_sums,
-                                uint32_t *state_counts, uint64_t *new_keys, float *new_sums, uint32_t *new_counts) {
-    LAUNCH(reduce_kernel, segments, v, maps, thumbs, keys, idx, start, segments, n, exist, new_pos, state_sums,
-           state_counts, new_keys, new_sums, new_counts);
+hipError_t fusion_launch_reduce(hipStream_t s, const FusionView *views, const uint32_t *vstart, uint32_t nv,
+                                uint32_t limit, const uint32_t *maps, const uint8_t *thumbs, const uint64_t *keys,
+                                const uint32_t *idx, const uint32_t *start, uint32_t segments, uint32_t n,
+                                const int32_t *exist, const uint32_t *is_new, const uint32_t *new_pos,
+                                float *state_sums, uint32_t *state_counts, uint64_t *new_keys, float *new_sums,
+                                uint32_t *new_counts) {
+    LAUNCH(reduce_kernel, segments, views, vstart, nv, limit, maps, thumbs, keys, idx, start, segments, n, exist,
+           is_new, new_pos, state_sums, state_counts, new_keys, new_sums, new_counts);
 }
 hipError_t fusion_launch_gather(hipStream_t s, const uint32_t *perm, uint32_t n, const uint64_t *keys, const float *sums,
                                 const uint32_t *counts, uint64_t *out_keys, float *out_sums, uint32_t *out_counts) {

@@ -2088,38 +2088,157 @@ extern "C" int32_t lr3d_fuse_depth_maps(lr3d_stereo *st, const int32_t *layers, 
     Scanner scan;
     CellSet state;
     HIP_TRY(state.alloc(0));
-    for (int32_t l = 0; l < n; l++) {
-        const FusionView &v = fv[l];
-        const uint32_t P = (uint32_t)(v.W * v.H);
-        DevBuf<uint32_t> flag, pos;
-        HIP_TRY(flag.alloc(P));
-        HIP_TRY(pos.alloc(P));
-        HIP_TRY(fusion_launch_valid(s, v, maps, flag.ptr));
+    // The samples go in chunks: a chunk's valid samples are keyed, sorted (stably, so each cell keeps
+    // the arrival order: views in order, pixels row-major) and added in one pass.  The reference
+    // changes its cells at two kinds of points, and a chunk ends at the first of them:
+    //  - after a view, while the accumulator holds more than the budget's cells, it coarsens
+    //    (DF:36): the new cells per view of first appearance give the size after each view;
+    //  - before a sample, when its hash table is over 60 % full (PA:67), it doubles the table and
+    //    re-bins every cell by its mean position at the same cell size (PA:87-110), which moves a
+    //    cell whose float mean rounds outside it: the first sample of the cell that fills the
+    //    table ends the chunk, and the next chunk starts with the re-binning.
+    // Chunks grow from 4 views while nothing cuts them; LR3D_FUSION_CHUNK_VIEWS=k fixes them at k
+    // views.
+    DevBuf<FusionView> dviews;
+    DevBuf<uint32_t> vstart, hist, flag, pos, idx, sidx, head, seg, start, is_new, new_pos, gflag, gscan, gsel;
+    DevBuf<uint64_t> keys, skeys;
+    DevBuf<int32_t> exist;
+    HIP_TRY(dviews.alloc(n));
+    HIP_TRY(hipMemcpyAsync(dviews.ptr, fv.data(), sizeof(FusionView) * n, hipMemcpyHostToDevice, s));
+    HIP_TRY(vstart.alloc((size_t)n + 1));
+    HIP_TRY(hist.alloc(n));
+    HIP_TRY(gsel.alloc(1));
+    int32_t fixed = 0;
+    if (const char *e = getenv("LR3D_FUSION_CHUNK_VIEWS")) fixed = std::max(0, atoi(e));
+    int32_t cap = fixed > 0 ? fixed : 4;
+    int64_t capacity = 1 << 16;  // PA:17 the hash table's slots
+    // PA:87-110 rebuild: every cell re-binned by its mean position at `to`, cells that fall
+    // together merged (in key order; the reference merges in slot order)
+    auto rebin = [&](double to) -> int32_t {
+        const uint32_t S = state.n;
+        DevBuf<uint64_t> keys, skeys;
+        DevBuf<uint32_t> idx, sidx, head, seg, start;
+        HIP_TRY(keys.alloc(S));
+        HIP_TRY(skeys.alloc(S));
+        HIP_TRY(idx.alloc(S));
+        HIP_TRY(sidx.alloc(S));
+        HIP_TRY(head.alloc(S));
+        HIP_TRY(seg.alloc(S));
+        HIP_TRY(fusion_launch_rekey(s, state.sums.ptr, state.counts.ptr, S, to, keys.ptr, idx.ptr, overflow.ptr));
+        HIP_TRY(sort_pairs(s, sort_tmp, keys.ptr, skeys.ptr, idx.ptr, sidx.ptr, S));
+        HIP_TRY(fusion_launch_heads(s, skeys.ptr, S, head.ptr));
+        uint32_t G = 0;
+        HIP_TRY(scan.run(s, head.ptr, seg.ptr, S, &G));
+        HIP_TRY(start.alloc(G));
+        HIP_TRY(fusion_launch_starts(s, head.ptr, seg.ptr, S, start.ptr));
+        CellSet next;
+        HIP_TRY(next.alloc(G));
+        HIP_TRY(fusion_launch_merge(s, skeys.ptr, sidx.ptr, start.ptr, G, S, state.sums.ptr, state.counts.ptr,
+                                    next.keys.ptr, next.sums.ptr, next.counts.ptr));
+        std::swap(state.keys, next.keys);
+        std::swap(state.sums, next.sums);
+        std::swap(state.counts, next.counts);
+        state.n = G;
+        return LR3D_OK;
+    };
+    std::vector<uint32_t> hstart, hhist;
+    uint32_t from = 0;  // first pixel of view a still to add
+    for (int32_t a = 0; a < n;) {
+        // chunk [a, b): at most cap views and kFusionChunkPixels pixels, at least one view
+        int32_t b = a;
+        uint64_t P = 0;
+        hstart.assign(1, 0u);
+        while (b < n && b - a < cap) {
+            const uint64_t px = (uint64_t)fv[b].W * fv[b].H;
+            if (b > a && P + px > kFusionChunkPixels) break;
+            P += px;
+            hstart.push_back((uint32_t)P);
+            b++;
+        }
+        const uint32_t nv = (uint32_t)(b - a);
+        int32_t last = b - 1;     // the view holding the chunk's last sample
+        uint32_t limit = (uint32_t)P;  // samples numbered below it are added
+        bool view_end = true;     // the chunk ends at the end of view `last`
+        HIP_TRY(flag.reserve(P));
+        HIP_TRY(pos.reserve(P));
+        for (uint32_t c = 0; c < nv; c++)
+            HIP_TRY(fusion_launch_valid(s, fv[a + c], c == 0 ? from : 0u, maps, flag.ptr + hstart[c]));
         uint32_t M = 0;
-        HIP_TRY(scan.run(s, flag.ptr, pos.ptr, P, &M));
+        HIP_TRY(scan.run(s, flag.ptr, pos.ptr, (uint32_t)P, &M));
         if (M > 0) {
-            // DF:52-71 + PA:37-48: samples keyed by cell, grouped by a stable sort (pixel order kept)
-            DevBuf<uint64_t> keys, skeys;
-            DevBuf<uint32_t> idx, sidx, head, seg, start;
-            HIP_TRY(keys.alloc(M));
-            HIP_TRY(skeys.alloc(M));
-            HIP_TRY(idx.alloc(M));
-            HIP_TRY(sidx.alloc(M));
-            HIP_TRY(fusion_launch_keys(s, v, maps, dthumbs.ptr, flag.ptr, pos.ptr, cell, keys.ptr, idx.ptr, overflow.ptr));
+            // PA:67: before its next sample the table doubles if it is over 60 % full
+            if ((int64_t)state.n * 10 > capacity * 6) {
+                capacity *= 2;
+                int32_t rc = rebin(cell);
+                if (rc) return rc;
+            }
+            const uint64_t fill = (uint64_t)(capacity * 6 / 10) + 1;  // the cells that make it over 60 % full
+            // DF:52-71 + PA:37-48: samples keyed by cell, grouped by a stable sort
+            HIP_TRY(keys.reserve(M));
+            HIP_TRY(skeys.reserve(M));
+            HIP_TRY(idx.reserve(M));
+            HIP_TRY(sidx.reserve(M));
+            HIP_TRY(head.reserve(M));
+            HIP_TRY(seg.reserve(M));
+            for (uint32_t c = 0; c < nv; c++)
+                HIP_TRY(fusion_launch_keys(s, fv[a + c], hstart[c], maps, dthumbs.ptr, flag.ptr, pos.ptr, cell, keys.ptr,
+                                           idx.ptr, overflow.ptr));
             HIP_TRY(sort_pairs(s, sort_tmp, keys.ptr, skeys.ptr, idx.ptr, sidx.ptr, M));
-            HIP_TRY(head.alloc(M));
-            HIP_TRY(seg.alloc(M));
             HIP_TRY(fusion_launch_heads(s, skeys.ptr, M, head.ptr));
             uint32_t G = 0;
             HIP_TRY(scan.run(s, head.ptr, seg.ptr, M, &G));
-            HIP_TRY(start.alloc(G));
+            HIP_TRY(start.reserve(G));
+            HIP_TRY(exist.reserve(G));
+            HIP_TRY(is_new.reserve(G));
+            HIP_TRY(new_pos.reserve(G));
             HIP_TRY(fusion_launch_starts(s, head.ptr, seg.ptr, M, start.ptr));
-            DevBuf<int32_t> exist;
-            DevBuf<uint32_t> is_new, new_pos;
-            HIP_TRY(exist.alloc(G));
-            HIP_TRY(is_new.alloc(G));
-            HIP_TRY(new_pos.alloc(G));
             HIP_TRY(fusion_launch_lookup(s, skeys.ptr, start.ptr, G, state.keys.ptr, state.n, exist.ptr, is_new.ptr));
+            // new cells per view of first appearance
+            HIP_TRY(hipMemcpyAsync(vstart.ptr, hstart.data(), (size_t)(nv + 1) * 4, hipMemcpyHostToDevice, s));
+            HIP_TRY(hipMemsetAsync(hist.ptr, 0, (size_t)nv * 4, s));
+            HIP_TRY(fusion_launch_first_views(s, sidx.ptr, start.ptr, is_new.ptr, G, vstart.ptr, nv, hist.ptr));
+            hhist.resize(nv);
+            HIP_TRY(hipMemcpyAsync(hhist.data(), hist.ptr, (size_t)nv * 4, hipMemcpyDeviceToHost, s));
+            HIP_TRY(hipStreamSynchronize(s));
+            // the first view after which the budget is exceeded (DF:36), and the view of the new
+            // cell that fills the table (its rank among that view's new cells)
+            int32_t over = -1, fills = -1;
+            uint32_t rank = 0;
+            uint64_t size = state.n;
+            for (uint32_t c = 0; c < nv && over < 0; c++) {
+                if (fills < 0 && size + hhist[c] >= fill) {
+                    fills = a + (int32_t)c;
+                    rank = (uint32_t)(fill - size - 1);
+                }
+                size += hhist[c];
+                if ((int64_t)size > cell_budget) over = a + (int32_t)c;
+            }
+            if (fills >= 0 && (over < 0 || fills <= over)) {
+                // the sample that creates the filling cell is the chunk's last
+                const uint32_t lo = hstart[fills - a], hi = hstart[fills - a + 1];
+                HIP_TRY(gflag.reserve(hi - lo));
+                HIP_TRY(gscan.reserve(hi - lo));
+                HIP_TRY(hipMemsetAsync(gflag.ptr, 0, (size_t)(hi - lo) * 4, s));
+                HIP_TRY(fusion_launch_new_cell_flags(s, sidx.ptr, start.ptr, is_new.ptr, G, lo, hi, gflag.ptr));
+                uint32_t flagged = 0;
+                HIP_TRY(scan.run(s, gflag.ptr, gscan.ptr, hi - lo, &flagged));
+                HIP_TRY(fusion_launch_select(s, gflag.ptr, gscan.ptr, hi - lo, rank, gsel.ptr));
+                uint32_t at = 0, later[2] = {0, 0};
+                HIP_TRY(hipMemcpyAsync(&at, gsel.ptr, 4, hipMemcpyDeviceToHost, s));
+                HIP_TRY(hipStreamSynchronize(s));
+                limit = lo + at + 1;
+                last = fills;
+                // valid samples of that view after it: none means the cut is at the view's end
+                HIP_TRY(hipMemcpyAsync(&later[0], pos.ptr + limit - 1, 4, hipMemcpyDeviceToHost, s));
+                if (hi < P) HIP_TRY(hipMemcpyAsync(&later[1], pos.ptr + hi, 4, hipMemcpyDeviceToHost, s));
+                HIP_TRY(hipStreamSynchronize(s));
+                const uint32_t upto = hi < P ? later[1] : M;  // valid samples before the view's end
+                view_end = upto == later[0] + 1;              // the cut sample is the view's last
+            } else if (over >= 0) {
+                last = over;
+                limit = hstart[over - a + 1];
+            }
+            if (limit < P) HIP_TRY(fusion_launch_limit(s, sidx.ptr, start.ptr, G, limit, is_new.ptr));
             uint32_t Nn = 0;
             HIP_TRY(scan.run(s, is_new.ptr, new_pos.ptr, G, &Nn));
             // merged set: the old cells, then the new ones, re-sorted by key below
@@ -2131,9 +2250,10 @@ extern "C" int32_t lr3d_fuse_depth_maps(lr3d_stereo *st, const int32_t *layers, 
                 HIP_TRY(hipMemcpyAsync(cat.counts.ptr, state.counts.ptr, (size_t)state.n * 4, hipMemcpyDeviceToDevice, s));
             }
             // PA:49-62: running sums of existing cells updated in place, new cells appended
-            HIP_TRY(fusion_launch_reduce(s, v, maps, dthumbs.ptr, skeys.ptr, sidx.ptr, start.ptr, G, M, exist.ptr,
-                                         new_pos.ptr, cat.sums.ptr, cat.counts.ptr, cat.keys.ptr + state.n,
-                                         cat.sums.ptr + (size_t)state.n * 9, cat.counts.ptr + state.n));
+            HIP_TRY(fusion_launch_reduce(s, dviews.ptr + a, vstart.ptr, nv, limit, maps, dthumbs.ptr, skeys.ptr, sidx.ptr,
+                                         start.ptr, G, M, exist.ptr, is_new.ptr, new_pos.ptr, cat.sums.ptr,
+                                         cat.counts.ptr, cat.keys.ptr + state.n, cat.sums.ptr + (size_t)state.n * 9,
+                                         cat.counts.ptr + state.n));
             const uint32_t T = state.n + Nn;
             DevBuf<uint64_t> tk;
             DevBuf<uint32_t> perm0, perm;
@@ -2153,34 +2273,23 @@ extern "C" int32_t lr3d_fuse_depth_maps(lr3d_stereo *st, const int32_t *layers, 
             std::swap(state.counts, next.counts);
             state.n = T;
         }
-        // DF:36-37 / PA:98-131: coarsen by 1.4 while the cloud is over budget
-        while ((int64_t)state.n > cell_budget) {
-            cell = cell * 1.4;
-            const uint32_t S = state.n;
-            DevBuf<uint64_t> keys, skeys;
-            DevBuf<uint32_t> idx, sidx, head, seg, start;
-            HIP_TRY(keys.alloc(S));
-            HIP_TRY(skeys.alloc(S));
-            HIP_TRY(idx.alloc(S));
-            HIP_TRY(sidx.alloc(S));
-            HIP_TRY(head.alloc(S));
-            HIP_TRY(seg.alloc(S));
-            HIP_TRY(fusion_launch_rekey(s, state.sums.ptr, state.counts.ptr, S, cell, keys.ptr, idx.ptr, overflow.ptr));
-            HIP_TRY(sort_pairs(s, sort_tmp, keys.ptr, skeys.ptr, idx.ptr, sidx.ptr, S));
-            HIP_TRY(fusion_launch_heads(s, skeys.ptr, S, head.ptr));
-            uint32_t G = 0;
-            HIP_TRY(scan.run(s, head.ptr, seg.ptr, S, &G));
-            HIP_TRY(start.alloc(G));
-            HIP_TRY(fusion_launch_starts(s, head.ptr, seg.ptr, S, start.ptr));
-            CellSet next;
-            HIP_TRY(next.alloc(G));
-            HIP_TRY(fusion_launch_merge(s, skeys.ptr, sidx.ptr, start.ptr, G, S, state.sums.ptr, state.counts.ptr,
-                                        next.keys.ptr, next.sums.ptr, next.counts.ptr));
-            std::swap(state.keys, next.keys);
-            std::swap(state.sums, next.sums);
-            std::swap(state.counts, next.counts);
-            state.n = G;
-            if (G == S && cell > 1e300) break;  // cannot merge further
+        if (view_end) {
+            // DF:36-37 / PA:98-131: coarsen by 1.4 while the cloud is over budget
+            while ((int64_t)state.n > cell_budget) {
+                const uint32_t before = state.n;
+                cell = cell * 1.4;
+                int32_t rc = rebin(cell);
+                if (rc) return rc;
+                if (state.n == before && cell > 1e300) break;  // cannot merge further
+            }
+            from = 0;
+            if (fixed == 0) cap = last == b - 1 ? std::min(cap * 2, kFusionMaxChunkViews) : std::max(1, last - a + 1);
+            a = last + 1;
+        } else {
+            // the next chunk resumes inside view `last`, after the cut sample
+            from = limit - hstart[last - a];
+            if (fixed == 0) cap = std::max(1, last - a + 1);
+            a = last;
         }
     }
     uint32_t ovf = 0;
@@ -2191,9 +2300,8 @@ extern "C" int32_t lr3d_fuse_depth_maps(lr3d_stereo *st, const int32_t *layers, 
     // PA:108-143 cloud(minSupport), in key order
     std::unique_ptr<lr3d_cloud> c(new lr3d_cloud());
     c->ctx = ctx;
-    DevBuf<uint32_t> flag, pos;
-    HIP_TRY(flag.alloc(std::max<uint32_t>(state.n, 1)));
-    HIP_TRY(pos.alloc(std::max<uint32_t>(state.n, 1)));
+    HIP_TRY(flag.reserve(std::max<uint32_t>(state.n, 1)));
+    HIP_TRY(pos.reserve(std::max<uint32_t>(state.n, 1)));
     HIP_TRY(fusion_launch_support(s, state.counts.ptr, state.n, (uint32_t)std::max(min_support, 0), flag.ptr));
     uint32_t total = 0;
     HIP_TRY(scan.run(s, flag.ptr, pos.ptr, state.n, &total));

@@ -263,6 +263,11 @@ hipError_t mesh_launch_compact(hipStream_t s, const uint32_t *indices, uint32_t 
                                const float *nrm, uint32_t *out_idx, float *out_pos, float *out_nrm);
 
 // One view of a dense-cloud fusion pass (DF:44-74): camera in double (JavaScript numbers).
+// Dense-cloud fusion takes the views in chunks: at most this many map pixels per chunk (one flag and
+// one sample number each), and at most this many views.
+constexpr uint64_t kFusionChunkPixels = (uint64_t)1 << 26;
+constexpr int32_t kFusionMaxChunkViews = 1024;
+
 struct FusionView {
     int32_t W, H;
     double k[4];
@@ -272,18 +277,28 @@ struct FusionView {
     uint64_t thumb_off;    // byte offset of the thumbnail (RGB)
 };
 
-hipError_t fusion_launch_valid(hipStream_t s, const FusionView &v, const uint32_t *maps, uint32_t *flag);
-hipError_t fusion_launch_keys(hipStream_t s, const FusionView &v, const uint32_t *maps, const uint8_t *thumbs,
-                              const uint32_t *flag, const uint32_t *pos, double cell, uint64_t *keys, uint32_t *idx,
-                              uint32_t *overflow);
+hipError_t fusion_launch_valid(hipStream_t s, const FusionView &v, uint32_t from, const uint32_t *maps, uint32_t *flag);
+hipError_t fusion_launch_new_cell_flags(hipStream_t s, const uint32_t *idx, const uint32_t *start, const uint32_t *is_new,
+                                        uint32_t segments, uint32_t lo, uint32_t hi, uint32_t *flag);
+hipError_t fusion_launch_select(hipStream_t s, const uint32_t *flag, const uint32_t *scan, uint32_t n, uint32_t k,
+                                uint32_t *out);
+hipError_t fusion_launch_keys(hipStream_t s, const FusionView &v, uint32_t base, const uint32_t *maps,
+                              const uint8_t *thumbs, const uint32_t *flag, const uint32_t *pos, double cell,
+                              uint64_t *keys, uint32_t *idx, uint32_t *overflow);
+hipError_t fusion_launch_first_views(hipStream_t s, const uint32_t *idx, const uint32_t *start, const uint32_t *is_new,
+                                     uint32_t segments, const uint32_t *vstart, uint32_t nv, uint32_t *hist);
+hipError_t fusion_launch_limit(hipStream_t s, const uint32_t *idx, const uint32_t *start, uint32_t segments,
+                               uint32_t limit, uint32_t *is_new);
 hipError_t fusion_launch_heads(hipStream_t s, const uint64_t *keys, uint32_t n, uint32_t *head);
 hipError_t fusion_launch_starts(hipStream_t s, const uint32_t *head, const uint32_t *seg, uint32_t n, uint32_t *start);
 hipError_t fusion_launch_lookup(hipStream_t s, const uint64_t *keys, const uint32_t *start, uint32_t segments,
                                 const uint64_t *state_keys, uint32_t state_n, int32_t *exist, uint32_t *is_new);
-hipError_t fusion_launch_reduce(hipStream_t s, const FusionView &v, const uint32_t *maps, const uint8_t *thumbs,
-                                const uint64_t *keys, const uint32_t *idx, const uint32_t *start, uint32_t segments,
-                                uint32_t n, const int32_t *exist, const uint32_t *new_pos, float *state_sums,
-                                uint32_t *state_counts, uint64_t *new_keys, float *new_sums, uint32_t *new_counts);
+hipError_t fusion_launch_reduce(hipStream_t s, const FusionView *views, const uint32_t *vstart, uint32_t nv,
+                                uint32_t limit, const uint32_t *maps, const uint8_t *thumbs, const uint64_t *keys,
+                                const uint32_t *idx, const uint32_t *start, uint32_t segments, uint32_t n,
+                                const int32_t *exist, const uint32_t *is_new, const uint32_t *new_pos,
+                                float *state_sums, uint32_t *state_counts, uint64_t *new_keys, float *new_sums,
+                                uint32_t *new_counts);
 hipError_t fusion_launch_gather(hipStream_t s, const uint32_t *perm, uint32_t n, const uint64_t *keys, const float *sums,
                                 const uint32_t *counts, uint64_t *out_keys, float *out_sums, uint32_t *out_counts);
 hipError_t fusion_launch_rekey(hipStream_t s, const float *sums, const uint32_t *counts, uint32_t n, double cell,

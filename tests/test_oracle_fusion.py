@@ -1,7 +1,8 @@
 """Pins of the oracle's dense-cloud fusion (oracle/fusion_oracle.c).
 
 The reference's own known answers for the accumulator (fusion.spec.ts:85-106): samples average per
-cell, cells merge when coarsened, and the table grows past its initial capacity; and closed forms
+cell, cells merge when coarsened, and the table grows past its initial capacity; a closed form for
+what the growth's rebuild does (it re-bins every cell by its Float32 mean); and closed forms
 for the whole fusion: a fronto-parallel constant-depth map seen by an identity camera lands on one
 plane of cells with the depth, the camera-facing normal and the thumbnail colour."""
 import ctypes as C
@@ -95,3 +96,30 @@ def test_cell_budget_coarsens_the_grid():
     small, cell = O.fuse_depth_maps([view], [packed], [thumb], 0.002, 200)
     assert len(full["support"]) > 200 >= len(small["support"])
     assert cell > 0.002 and small["support"].sum() == W * H
+
+
+def test_table_growth_rebins_a_cell_whose_float_mean_leaves_it():
+    """PA:67,87-110: the table doubles (65536 -> 131072 slots) before the first sample after more than
+    60 % of its slots are used (39322 cells), and the rebuild re-bins every cell by its mean position
+    at the same cell size.  A sample at x = 1000.99999 lies in cell 1000 (1 m cells), but its Float32
+    sum rounds to 1001 (float spacing there is 2^-14), so the growth moves its cell to 1001; a later
+    sample at x = 1000.5 then opens a new cell 1000 instead of joining it."""
+    assert np.float32(1000.99999) == np.float32(1001.0)  # the closed form the case rests on
+    lib = _acc()
+    a = lib.orc_acc_create(1.0)
+    lib.orc_acc_add(a, (C.c_double * 9)(1000.99999, 0.5, 0.5, 0, 0, 1, 10, 20, 30))
+    for i in range(1, 39322):  # 39321 more cells: 39322 in use, more than 60 % of 65536 slots
+        lib.orc_acc_add(a, (C.c_double * 9)(5.5, 2.0 * i + 0.5, 0.5, 0, 0, 1, 0, 0, 0))
+    assert lib.orc_acc_size(a) == 39322
+    lib.orc_acc_add(a, (C.c_double * 9)(1000.5, 0.5, 0.5, 0, 0, 1, 40, 50, 60))  # growth, then cell 1000
+    assert lib.orc_acc_size(a) == 39323
+    pos = np.zeros((39323, 3), np.float32)
+    col = np.zeros((39323, 3), np.uint8)
+    sup = np.zeros(39323, np.uint16)
+    n = lib.orc_acc_cloud(a, 1, pos.ctypes.data, col.ctypes.data, sup.ctypes.data)
+    assert n == 39323
+    moved = np.flatnonzero(pos[:n, 0] == np.float32(1001.0))
+    assert len(moved) == 1 and sup[moved[0]] == 1 and (col[moved[0]] == [10, 20, 30]).all()
+    fresh = np.flatnonzero(pos[:n, 0] == np.float32(1000.5))
+    assert len(fresh) == 1 and sup[fresh[0]] == 1 and (col[fresh[0]] == [40, 50, 60]).all()
+    lib.orc_acc_destroy(a)

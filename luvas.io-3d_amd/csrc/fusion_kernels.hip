@@ -114,14 +114,21 @@ __device__ __forceinline__ uint32_t view_of(const uint32_t *__restrict__ vstart,
     return lo;
 }
 
-// New cells per chunk view of their first sample (integer atomics: the counts do not depend on the
-// order of the additions)
-__global__ void first_view_kernel(const uint32_t *__restrict__ idx, const uint32_t *__restrict__ start,
-                                  const uint32_t *__restrict__ is_new, uint32_t segments,
-                                  const uint32_t *__restrict__ vstart, uint32_t nv, uint32_t *__restrict__ hist) {
+// New cells per chunk view of their first sample: a histogram per block in LDS, added to the
+// global one bin by bin (integer atomics: the counts do not depend on the order of the additions).
+__global__ void __launch_bounds__(256) first_view_kernel(const uint32_t *__restrict__ idx,
+                                                         const uint32_t *__restrict__ start,
+                                                         const uint32_t *__restrict__ is_new, uint32_t segments,
+                                                         const uint32_t *__restrict__ vstart, uint32_t nv,
+                                                         uint32_t *__restrict__ hist) {
+    __shared__ uint32_t local[kFusionMaxChunkViews];
+    for (uint32_t c = threadIdx.x; c < nv; c += blockDim.x) local[c] = 0u;
+    __syncthreads();
     const uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
-    if (g >= segments || !is_new[g]) return;
-    atomicAdd(&hist[view_of(vstart, nv, idx[start[g]])], 1u);
+    if (g < segments && is_new[g]) atomicAdd(&local[view_of(vstart, nv, idx[start[g]])], 1u);
+    __syncthreads();
+    for (uint32_t c = threadIdx.x; c < nv; c += blockDim.x)
+        if (local[c]) atomicAdd(&hist[c], local[c]);
 }
 
 // Flags, over the pixels of chunk view c, the first samples of the new cells first seen there.

@@ -78,10 +78,29 @@ extern "C" const char *lr3d_last_error(void) { return g_error.c_str(); }
 
 // ---- device buffers ----------------------------------------------------------------------------------
 
+// Owns its allocation: moves hand it over, copies are not allowed (a copy's destructor would free
+// memory the original still uses).
 template <typename T>
 struct DevBuf {
     T *ptr = nullptr;
     size_t count = 0;
+    DevBuf() = default;
+    DevBuf(const DevBuf &) = delete;
+    DevBuf &operator=(const DevBuf &) = delete;
+    DevBuf(DevBuf &&o) noexcept : ptr(o.ptr), count(o.count) {
+        o.ptr = nullptr;
+        o.count = 0;
+    }
+    DevBuf &operator=(DevBuf &&o) noexcept {
+        if (this != &o) {
+            release();
+            ptr = o.ptr;
+            count = o.count;
+            o.ptr = nullptr;
+            o.count = 0;
+        }
+        return *this;
+    }
     hipError_t alloc(size_t n) {
         release();
         if (n == 0) return hipSuccess;
@@ -2034,6 +2053,14 @@ struct CellSet {
         n = count;
         return e;
     }
+    // Grow-only (contents are not kept).
+    hipError_t reserve(uint32_t count) {
+        hipError_t e = keys.reserve(std::max<uint32_t>(count, 1));
+        if (e == hipSuccess) e = sums.reserve((size_t)std::max<uint32_t>(count, 1) * 9);
+        if (e == hipSuccess) e = counts.reserve(std::max<uint32_t>(count, 1));
+        n = count;
+        return e;
+    }
 };
 
 extern "C" int32_t lr3d_fuse_depth_maps(lr3d_stereo *st, const int32_t *layers, int32_t map, int32_t n,
@@ -2101,15 +2128,17 @@ extern "C" int32_t lr3d_fuse_depth_maps(lr3d_stereo *st, const int32_t *layers, 
     // views.
     DevBuf<FusionView> dviews;
     DevBuf<uint32_t> vstart, hist, flag, pos, idx, sidx, head, seg, start, is_new, new_pos, gflag, gscan, gsel;
-    DevBuf<uint64_t> keys, skeys;
+    DevBuf<uint64_t> keys, skeys, tk;
+    DevBuf<uint32_t> perm0, perm;
     DevBuf<int32_t> exist;
+    CellSet cat, next;
     HIP_TRY(dviews.alloc(n));
     HIP_TRY(hipMemcpyAsync(dviews.ptr, fv.data(), sizeof(FusionView) * n, hipMemcpyHostToDevice, s));
     HIP_TRY(vstart.alloc((size_t)n + 1));
     HIP_TRY(hist.alloc(n));
     HIP_TRY(gsel.alloc(1));
     int32_t fixed = 0;
-    if (const char *e = getenv("LR3D_FUSION_CHUNK_VIEWS")) fixed = std::max(0, atoi(e));
+    if (const char *e = getenv("LR3D_FUSION_CHUNK_VIEWS")) fixed = std::min(std::max(0, atoi(e)), kFusionMaxChunkViews);
     int32_t cap = fixed > 0 ? fixed : 4;
     int64_t capacity = 1 << 16;  // PA:17 the hash table's slots
     // PA:87-110 rebuild: every cell re-binned by its mean position at `to`, cells that fall
@@ -2242,8 +2271,7 @@ extern "C" int32_t lr3d_fuse_depth_maps(lr3d_stereo *st, const int32_t *layers, 
             uint32_t Nn = 0;
             HIP_TRY(scan.run(s, is_new.ptr, new_pos.ptr, G, &Nn));
             // merged set: the old cells, then the new ones, re-sorted by key below
-            CellSet cat;
-            HIP_TRY(cat.alloc(state.n + Nn));
+            HIP_TRY(cat.reserve(state.n + Nn));
             if (state.n) {
                 HIP_TRY(hipMemcpyAsync(cat.keys.ptr, state.keys.ptr, (size_t)state.n * 8, hipMemcpyDeviceToDevice, s));
                 HIP_TRY(hipMemcpyAsync(cat.sums.ptr, state.sums.ptr, (size_t)state.n * 36, hipMemcpyDeviceToDevice, s));
@@ -2255,17 +2283,14 @@ extern "C" int32_t lr3d_fuse_depth_maps(lr3d_stereo *st, const int32_t *layers, 
                                          cat.counts.ptr, cat.keys.ptr + state.n, cat.sums.ptr + (size_t)state.n * 9,
                                          cat.counts.ptr + state.n));
             const uint32_t T = state.n + Nn;
-            DevBuf<uint64_t> tk;
-            DevBuf<uint32_t> perm0, perm;
-            HIP_TRY(tk.alloc(T));
-            HIP_TRY(perm0.alloc(T));
-            HIP_TRY(perm.alloc(T));
+            HIP_TRY(tk.reserve(T));
+            HIP_TRY(perm0.reserve(T));
+            HIP_TRY(perm.reserve(T));
             (void)hipGetLastError();
             hipLaunchKernelGGL(iota_u32_kernel, dim3((T + 255) / 256), dim3(256), 0, s, perm0.ptr, T);
             HIP_TRY(hipGetLastError());
             HIP_TRY(sort_pairs(s, sort_tmp, cat.keys.ptr, tk.ptr, perm0.ptr, perm.ptr, T));
-            CellSet next;
-            HIP_TRY(next.alloc(T));
+            HIP_TRY(next.reserve(T));
             HIP_TRY(fusion_launch_gather(s, perm.ptr, T, cat.keys.ptr, cat.sums.ptr, cat.counts.ptr, next.keys.ptr,
                                          next.sums.ptr, next.counts.ptr));
             std::swap(state.keys, next.keys);

@@ -2251,10 +2251,14 @@ extern "C" int32_t lr3d_fuse_depth_maps(lr3d_stereo *st, const int32_t *layers, 
                 HIP_TRY(fusion_launch_new_cell_flags(s, sidx.ptr, start.ptr, is_new.ptr, G, lo, hi, gflag.ptr));
                 uint32_t flagged = 0;
                 HIP_TRY(scan.run(s, gflag.ptr, gscan.ptr, hi - lo, &flagged));
+                HIP_TRY(hipMemsetAsync(gsel.ptr, 0xff, 4, s));
                 HIP_TRY(fusion_launch_select(s, gflag.ptr, gscan.ptr, hi - lo, rank, gsel.ptr));
                 uint32_t at = 0, later[2] = {0, 0};
                 HIP_TRY(hipMemcpyAsync(&at, gsel.ptr, 4, hipMemcpyDeviceToHost, s));
                 HIP_TRY(hipStreamSynchronize(s));
+                if (rank >= flagged || at >= hi - lo)
+                    return fail(LR3D_ERR_DEVICE, "lr3d_fuse_depth_maps: new cell %u of view %d not found (%u flagged)",
+                                rank, fills, flagged);
                 limit = lo + at + 1;
                 last = fills;
                 // valid samples of that view after it: none means the cut is at the view's end

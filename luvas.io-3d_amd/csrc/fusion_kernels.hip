@@ -21,6 +21,10 @@ namespace lr3d {
 namespace {
 
 constexpr int32_t kKeyBias = 1 << 20;
+// Cells with more samples than this are added by a wavefront each (big_reduce_kernel).
+constexpr uint32_t kBigCell = 64;
+// Blocks of big_reduce_kernel (each strides over the list of big cells).
+constexpr uint32_t kBigReduceBlocks = 2048;
 
 __device__ __forceinline__ uint64_t pack_key(int64_t ix, int64_t iy, int64_t iz, uint32_t *overflow) {
     if (ix < -kKeyBias || ix >= kKeyBias || iy < -kKeyBias || iy >= kKeyBias || iz < -kKeyBias || iz >= kKeyBias) {
@@ -193,12 +197,17 @@ __global__ void reduce_kernel(const FusionView *__restrict__ views, const uint32
                               const int32_t *__restrict__ exist, const uint32_t *__restrict__ is_new,
                               const uint32_t *__restrict__ new_pos, float *__restrict__ state_sums,
                               uint32_t *__restrict__ state_counts, uint64_t *__restrict__ new_keys,
-                              float *__restrict__ new_sums, uint32_t *__restrict__ new_counts) {
+                              float *__restrict__ new_sums, uint32_t *__restrict__ new_counts,
+                              uint32_t *__restrict__ big, uint32_t *__restrict__ n_big) {
     const uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
     if (g >= segments) return;
     const uint32_t a = start[g], b = g + 1 < segments ? start[g + 1] : n;
     const int32_t e = exist[g];
     if (e < 0 && !is_new[g]) return;  // a new cell first seen after the chunk's end
+    if (b - a > kBigCell) {           // a wavefront takes it (big_reduce_kernel)
+        big[atomicAdd(n_big, 1u)] = g;
+        return;
+    }
     float sums[9];
     uint32_t count = 0;
     for (int c = 0; c < 9; c++) sums[c] = e >= 0 ? state_sums[(size_t)e * 9 + c] : 0.0f;
@@ -219,6 +228,58 @@ __global__ void reduce_kernel(const FusionView *__restrict__ views, const uint32
     } else {
         new_counts[new_pos[g]] = count;
         new_keys[new_pos[g]] = keys[a];
+    }
+}
+
+// The cells with more than kBigCell samples, one wavefront (one 64-thread block) each, cells taken
+// by striding over the list reduce_kernel made: 64 samples are decoded at once (one per lane, into
+// LDS), then lanes 0-8 each add one of the 9 components of those samples in order, so every sum
+// takes the same sequence of Float32 additions as a thread walking the cell would.
+__global__ void __launch_bounds__(64) big_reduce_kernel(
+    const FusionView *__restrict__ views, const uint32_t *__restrict__ vstart, uint32_t nv, uint32_t limit,
+    const uint32_t *__restrict__ maps, const uint8_t *__restrict__ thumbs, const uint64_t *__restrict__ keys,
+    const uint32_t *__restrict__ idx, const uint32_t *__restrict__ start, uint32_t segments, uint32_t n,
+    const int32_t *__restrict__ exist, const uint32_t *__restrict__ new_pos, float *__restrict__ state_sums,
+    uint32_t *__restrict__ state_counts, uint64_t *__restrict__ new_keys, float *__restrict__ new_sums,
+    uint32_t *__restrict__ new_counts, const uint32_t *__restrict__ big, const uint32_t *__restrict__ n_big) {
+    __shared__ double vals[64][9];
+    const uint32_t lane = threadIdx.x;
+    const uint32_t cells = *n_big;
+    for (uint32_t i = blockIdx.x; i < cells; i += gridDim.x) {
+        const uint32_t g = big[i];
+        const uint32_t a = start[g], b = g + 1 < segments ? start[g + 1] : n;
+        const int32_t e = exist[g];
+        float sum = 0.0f;
+        uint32_t count = e >= 0 ? state_counts[e] : 0u;
+        if (lane < 9 && e >= 0) sum = state_sums[(size_t)e * 9 + lane];
+        for (uint32_t j0 = a; j0 < b; j0 += 64) {
+            const uint32_t j = j0 + lane;
+            const uint32_t p = j < b ? idx[j] : limit;
+            const bool take = p < limit;  // samples below `limit` are a prefix of the cell's
+            if (take) {
+                const uint32_t c = view_of(vstart, nv, p);
+                double sv[9];
+                sample_of(views[c], maps, thumbs, p - vstart[c], sv);
+                for (int k = 0; k < 9; k++) vals[lane][k] = sv[k];
+            }
+            const uint32_t m = (uint32_t)__popcll(__ballot(take));
+            __syncthreads();
+            if (lane < 9)
+                for (uint32_t t = 0; t < m; t++) sum = (float)((double)sum + vals[t][lane]);
+            count += m;
+            __syncthreads();
+            if (m < 64) break;
+        }
+        float *dst = e >= 0 ? state_sums + (size_t)e * 9 : new_sums + (size_t)new_pos[g] * 9;
+        if (lane < 9) dst[lane] = sum;
+        if (lane == 0) {
+            if (e >= 0) {
+                state_counts[e] = count;
+            } else {
+                new_counts[new_pos[g]] = count;
+                new_keys[new_pos[g]] = keys[a];
+            }
+        }
     }
 }
 
@@ -344,9 +405,18 @@ hipError_t fusion_launch_reduce(hipStream_t s, const FusionView *views, const ui
                                 const uint32_t *idx, const uint32_t *start, uint32_t segments, uint32_t n,
                                 const int32_t *exist, const uint32_t *is_new, const uint32_t *new_pos,
                                 float *state_sums, uint32_t *state_counts, uint64_t *new_keys, float *new_sums,
-                                uint32_t *new_counts) {
-    LAUNCH(reduce_kernel, segments, views, vstart, nv, limit, maps, thumbs, keys, idx, start, segments, n, exist,
-           is_new, new_pos, state_sums, state_counts, new_keys, new_sums, new_counts);
+                                uint32_t *new_counts, uint32_t *big, uint32_t *n_big) {
+    (void)hipGetLastError();
+    if (segments == 0) return hipSuccess;
+    hipLaunchKernelGGL(reduce_kernel, grid_for(segments), dim3(256), 0, s, views, vstart, nv, limit, maps, thumbs, keys,
+                       idx, start, segments, n, exist, is_new, new_pos, state_sums, state_counts, new_keys, new_sums,
+                       new_counts, big, n_big);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(big_reduce_kernel, dim3(kBigReduceBlocks), dim3(64), 0, s, views, vstart, nv, limit, maps, thumbs,
+                       keys, idx, start, segments, n, exist, new_pos, state_sums, state_counts, new_keys, new_sums,
+                       new_counts, big, n_big);
+    return hipGetLastError();
 }
 hipError_t fusion_launch_gather(hipStream_t s, const uint32_t *perm, uint32_t n, const uint64_t *keys, const float *sums,
                                 const uint32_t *counts, uint64_t *out_keys, float *out_sums, uint32_t *out_counts) {

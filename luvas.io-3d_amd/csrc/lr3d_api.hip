@@ -2129,10 +2129,11 @@ extern "C" int32_t lr3d_fuse_depth_maps(lr3d_stereo *st, const int32_t *layers, 
     DevBuf<FusionView> dviews;
     DevBuf<uint32_t> vstart, hist, flag, pos, idx, sidx, head, seg, start, is_new, new_pos, gflag, gscan, gsel;
     DevBuf<uint64_t> keys, skeys, tk;
-    DevBuf<uint32_t> perm0, perm;
+    DevBuf<uint32_t> perm0, perm, big, n_big;
     DevBuf<int32_t> exist;
     CellSet cat, next;
     HIP_TRY(dviews.alloc(n));
+    HIP_TRY(n_big.alloc(1));
     HIP_TRY(hipMemcpyAsync(dviews.ptr, fv.data(), sizeof(FusionView) * n, hipMemcpyHostToDevice, s));
     HIP_TRY(vstart.alloc((size_t)n + 1));
     HIP_TRY(hist.alloc(n));
@@ -2282,10 +2283,12 @@ extern "C" int32_t lr3d_fuse_depth_maps(lr3d_stereo *st, const int32_t *layers, 
                 HIP_TRY(hipMemcpyAsync(cat.counts.ptr, state.counts.ptr, (size_t)state.n * 4, hipMemcpyDeviceToDevice, s));
             }
             // PA:49-62: running sums of existing cells updated in place, new cells appended
+            HIP_TRY(big.reserve(G));
+            HIP_TRY(hipMemsetAsync(n_big.ptr, 0, 4, s));
             HIP_TRY(fusion_launch_reduce(s, dviews.ptr + a, vstart.ptr, nv, limit, maps, dthumbs.ptr, skeys.ptr, sidx.ptr,
                                          start.ptr, G, M, exist.ptr, is_new.ptr, new_pos.ptr, cat.sums.ptr,
                                          cat.counts.ptr, cat.keys.ptr + state.n, cat.sums.ptr + (size_t)state.n * 9,
-                                         cat.counts.ptr + state.n));
+                                         cat.counts.ptr + state.n, big.ptr, n_big.ptr));
             const uint32_t T = state.n + Nn;
             HIP_TRY(tk.reserve(T));
             HIP_TRY(perm0.reserve(T));

@@ -298,7 +298,7 @@ hipError_t fusion_launch_reduce(hipStream_t s, const FusionView *views, const ui
                                 const uint32_t *idx, const uint32_t *start, uint32_t segments, uint32_t n,
                                 const int32_t *exist, const uint32_t *is_new, const uint32_t *new_pos,
                                 float *state_sums, uint32_t *state_counts, uint64_t *new_keys, float *new_sums,
-                                uint32_t *new_counts);
+                                uint32_t *new_counts, uint32_t *big, uint32_t *n_big);
 hipError_t fusion_launch_gather(hipStream_t s, const uint32_t *perm, uint32_t n, const uint64_t *keys, const float *sums,
                                 const uint32_t *counts, uint64_t *out_keys, float *out_sums, uint32_t *out_counts);
 hipError_t fusion_launch_rekey(hipStream_t s, const float *sums, const uint32_t *counts, uint32_t n, double cell,

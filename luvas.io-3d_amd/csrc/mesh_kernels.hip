@@ -240,10 +240,25 @@ __global__ void jump_kernel(uint32_t *__restrict__ parent, uint32_t n) {
     parent[i] = p;
 }
 
+// Faces per component root.  Most faces of a mesh belong to one component, so one counter would take
+// nearly every addition: the lanes of a wavefront whose root is that of its first face add their
+// count once (integer additions: the totals do not depend on the grouping).
 __global__ void component_faces_kernel(const uint32_t *__restrict__ indices, uint32_t n_faces,
                                        const uint32_t *__restrict__ parent, uint32_t *__restrict__ faces) {
     const uint32_t f = blockIdx.x * blockDim.x + threadIdx.x;
-    if (f < n_faces) atomicAdd(&faces[parent[indices[f * 3]]], 1u);
+    const bool active = f < n_faces;
+    const uint32_t root = active ? parent[indices[f * 3]] : 0u;
+    const uint64_t live = __ballot(active);
+    if (live == 0) return;  // the whole wavefront
+    const int first = __ffsll((unsigned long long)live) - 1;
+    const uint32_t common = __shfl(root, first, 64);
+    const bool same = active && root == common;
+    const uint64_t group = __ballot(same);
+    if (same) {
+        if ((int)(threadIdx.x & 63u) == first) atomicAdd(&faces[common], (uint32_t)__popcll(group));
+    } else if (active) {
+        atomicAdd(&faces[root], 1u);
+    }
 }
 
 __global__ void keep_kernel(const uint32_t *__restrict__ indices, uint32_t n_faces, const uint32_t *__restrict__ parent,

@@ -20,7 +20,9 @@ Prints one JSON line.
   tsdf           voxel updates per step and per second of the integration pass, and the adjoint
   roofline       the dominant kernel (pm_propagate) against the FP32 vector peak, timed with HIP
                  events on the library's stream over the timed region
-  cpu_baseline   the C oracle (same arithmetic, OpenMP) on a bounded sample of the same workload
+  cpu_baseline   the C oracle (same arithmetic, OpenMP) on a bounded sample of the same workload;
+                 under "downstream" the oracle's fusion, surface nets and small-component removal
+                 (one thread) on the same inputs as the GPU's, with whether the results are equal
   plain          the per-view path (one solve / filter / integrate call per view, the same adjoint)
                  for the batched / plain ratio
   accuracy       bad pixels (> 1 % relative depth error vs the analytic depth) on the filtered maps
@@ -392,6 +394,7 @@ def main():
             mesh_s.append(t4 - t3)
             comp_s.append(time.perf_counter() - t4)
             (verts, indices), (kverts, kindices) = mesh.size, kept.size
+            gpu_mesh, gpu_kept = mesh.read(), kept.read()
             mesh.destroy()
             kept.destroy()
         mesh_kernel_ms = sum(ctx.kernel_stats(k)[1] for k in ("mesh_active", "mesh_vertices", "mesh_quads")) / 3
@@ -444,6 +447,44 @@ def main():
         cpu = {"value": round(sample / cpu_s, 3), "unit": "Mpx*hyp/s", "cores": threads, "kind": "port",
                "sample": f"{what} (init + {cpu_iters} red-black iteration(s) each, same sources and top-k) = "
                          f"{sample:.2f} Mpx*hyp in {cpu_s:.1f}s on {cpu_model()}"}
+        if downstream is not None:
+            # the stages after the step on the CPU (the oracle, one thread) beside the GPU, same inputs:
+            # surface nets and small-component removal of the step's volume, and fusion of the first
+            # 10 filtered maps without coarsening (the GPU fuses the same maps again for the comparison)
+            tsdf_h, weight_h = vol.read()
+            t3 = time.perf_counter()
+            ref_mesh = O.extract_surface(grid.origin, grid.voxel, grid.dims, layout[1], tsdf_h, weight_h, 0.6)
+            mesh_cpu = time.perf_counter() - t3
+            del tsdf_h, weight_h
+            t3 = time.perf_counter()
+            ref_kept = O.remove_small_components(ref_mesh, 0.01, 200)
+            comp_cpu = time.perf_counter() - t3
+            same_mesh = all(np.array_equal(np.asarray(gpu_mesh[k]).reshape(-1), np.asarray(ref_mesh[k]).reshape(-1))
+                            for k in ("positions", "normals", "indices"))
+            same_kept = all(np.array_equal(np.asarray(gpu_kept[k]).reshape(-1), np.asarray(ref_kept[k]).reshape(-1))
+                            for k in ("positions", "normals", "indices"))
+            k10 = min(10, n)
+            maps10 = [st.read_filtered(r) for r in range(k10)]
+            t3 = time.perf_counter()
+            ref_cloud, _ = O.fuse_depth_maps(work.views[:k10], maps10, thumbs[:k10], 0.004, 1 << 40)
+            fus_cpu = time.perf_counter() - t3
+            gpu_cloud, _ = lr3d.fuse_depth_maps(st, list(range(k10)), thumbs[:k10], 0.004, 1 << 40)
+
+            def rows(c):
+                r = np.concatenate([c["positions"].view(np.uint32), c["normals"].view(np.uint32),
+                                    c["colors"].astype(np.uint32), c["support"][:, None].astype(np.uint32)], axis=1)
+                return r[np.lexsort(r.T[::-1])]
+            same_cloud = bool(np.array_equal(rows(gpu_cloud), rows(ref_cloud)))
+            samples10 = int(ref_cloud["support"].astype(np.int64).sum())
+            cpu["downstream"] = {
+                "cores": 1,
+                "surface_nets": {"ms": round(mesh_cpu * 1e3, 1), "voxels_per_s": round(layout[2] * 512 / mesh_cpu, 1),
+                                 "same_as_gpu": bool(same_mesh)},
+                "small_components": {"ms": round(comp_cpu * 1e3, 1), "same_as_gpu": bool(same_kept)},
+                "fusion": {"views": k10, "samples": samples10, "ms": round(fus_cpu * 1e3, 1),
+                           "samples_per_s": round(samples10 / fus_cpu, 1), "same_as_gpu": same_cloud}}
+            log(f"cpu downstream: surface nets {mesh_cpu:.1f}s (same as GPU: {same_mesh}), small components "
+                f"{comp_cpu:.1f}s ({same_kept}), fusion of {k10} maps {fus_cpu:.1f}s ({same_cloud})")
 
     out = {
         "metric": "PatchMatch Mpx*hyp/s (batched PatchMatch + filter + TSDF integration + TSDF adjoint step)",

@@ -365,8 +365,9 @@ def main():
     # profile's 600000-cell budget, quarter-size grey thumbnails), surface nets of the step's volume
     # (min weight 0.6) and the removal of components under max(200 faces, 1 %) (tsdf-meshing.ts:87-91).
     # The calls are synchronous: wall time per call (fusion: thumbnails up, cloud back), the median
-    # of 3; the surface-net kernels also with HIP events, against the bytes they must move (8 B per
-    # pool voxel read, 24 B per vertex and 4 B per index written).
+    # of 3; the surface-net kernels also with HIP events, against the bytes they must move at least
+    # (every pool voxel's weight read, 4 B, to find the observed bricks; 24 B per vertex and 4 B per
+    # index written).
     downstream = None
     if not args.no_downstream:
         thumbs = []
@@ -397,9 +398,10 @@ def main():
             gpu_mesh, gpu_kept = mesh.read(), kept.read()
             mesh.destroy()
             kept.destroy()
-        mesh_kernel_ms = sum(ctx.kernel_stats(k)[1] for k in ("mesh_active", "mesh_vertices", "mesh_quads")) / 3
+        mesh_kernel_ms = sum(ctx.kernel_stats(k)[1]
+                             for k in ("mesh_bricks", "mesh_active", "mesh_vertices", "mesh_quads")) / 3
         ctx.set_profiling(False)
-        mesh_bytes = 8 * layout[2] * 512 + 24 * verts + 4 * indices
+        mesh_bytes = 4 * layout[2] * 512 + 24 * verts + 4 * indices
         downstream = {
             "fusion": {"ms": round(fus * 1e3, 2), "samples": samples, "samples_per_s": round(samples / fus, 1),
                        "points": int(len(cloud["support"])), "cell_m": 0.004, "cell_budget": 600_000,

@@ -1926,29 +1926,43 @@ extern "C" int32_t lr3d_tsdf_extract_surface(lr3d_tsdf *ts, double min_weight, l
     std::unique_ptr<lr3d_mesh> m(new lr3d_mesh());
     m->ctx = ctx;
     Scanner scan;
-    DevBuf<uint32_t> flag, vidx, qcount, qoff;
+    DevBuf<uint32_t> flag, vidx, qcount, qoff, live, lpos, bricks;
     DevBuf<int32_t> cell_vertex;
     DevBuf<double> pos_d;
+    // the cell kernels run over the bricks holding an observed voxel; the others' cells are empty
+    const uint32_t n_bricks = (uint32_t)ts->grid.n_bricks;
+    HIP_TRY(live.alloc(std::max<uint32_t>(n_bricks, 1)));
+    HIP_TRY(lpos.alloc(std::max<uint32_t>(n_bricks, 1)));
+    HIP_TRY(timed(ctx, "mesh_bricks", [&] { return mesh_launch_live_bricks(s, v, n_bricks, live.ptr); }));
+    uint32_t L = 0;
+    HIP_TRY(scan.run(s, live.ptr, lpos.ptr, n_bricks, &L));
+    HIP_TRY(bricks.alloc(std::max<uint32_t>(L, 1)));
+    HIP_TRY(mesh_launch_brick_list(s, live.ptr, lpos.ptr, n_bricks, bricks.ptr));
+    v.bricks = bricks.ptr;
+    const uint32_t threads = L * 512u;
     HIP_TRY(flag.alloc(cells));
     HIP_TRY(vidx.alloc(cells));
     HIP_TRY(cell_vertex.alloc(cells));
+    HIP_TRY(hipMemsetAsync(flag.ptr, 0, (size_t)cells * 4, s));
+    HIP_TRY(hipMemsetAsync(cell_vertex.ptr, 0xff, (size_t)cells * 4, s));  // -1: no vertex
     // SN:87-115: active cells, numbered in pool order
-    HIP_TRY(timed(ctx, "mesh_active", [&] { return mesh_launch_active(s, v, cells, flag.ptr); }));
+    HIP_TRY(timed(ctx, "mesh_active", [&] { return mesh_launch_active(s, v, threads, flag.ptr); }));
     uint32_t V = 0;
     HIP_TRY(scan.run(s, flag.ptr, vidx.ptr, cells, &V));
     m->n_vertices = V;
     HIP_TRY(pos_d.alloc((size_t)std::max<uint32_t>(V, 1) * 3));
     HIP_TRY(m->positions.alloc((size_t)std::max<uint32_t>(V, 1) * 3));
     HIP_TRY(timed(ctx, "mesh_vertices", [&] {
-        return mesh_launch_vertices(s, v, cells, vidx.ptr, cell_vertex.ptr, pos_d.ptr, m->positions.ptr);
+        return mesh_launch_vertices(s, v, threads, vidx.ptr, cell_vertex.ptr, pos_d.ptr, m->positions.ptr);
     }));
     flag.release();
     vidx.release();
     // SN:117-156: quads counted per voxel, written at their sequential offsets
     HIP_TRY(qcount.alloc(cells));
     HIP_TRY(qoff.alloc(cells));
+    HIP_TRY(hipMemsetAsync(qcount.ptr, 0, (size_t)cells * 4, s));
     HIP_TRY(timed(ctx, "mesh_quads", [&] {
-        return mesh_launch_quads(s, v, cells, cell_vertex.ptr, pos_d.ptr, nullptr, qcount.ptr, nullptr);
+        return mesh_launch_quads(s, v, threads, cell_vertex.ptr, pos_d.ptr, nullptr, qcount.ptr, nullptr);
     }));
     uint32_t Q = 0;
     HIP_TRY(scan.run(s, qcount.ptr, qoff.ptr, cells, &Q));
@@ -1956,7 +1970,7 @@ extern "C" int32_t lr3d_tsdf_extract_surface(lr3d_tsdf *ts, double min_weight, l
     m->n_indices = Q * 6;
     HIP_TRY(m->indices.alloc((size_t)std::max<uint32_t>(m->n_indices, 1)));
     HIP_TRY(timed(ctx, "mesh_quads", [&] {
-        return mesh_launch_quads(s, v, cells, cell_vertex.ptr, pos_d.ptr, qoff.ptr, nullptr, m->indices.ptr);
+        return mesh_launch_quads(s, v, threads, cell_vertex.ptr, pos_d.ptr, qoff.ptr, nullptr, m->indices.ptr);
     }));
     int32_t rc = finish(ctx);
     if (rc) return rc;

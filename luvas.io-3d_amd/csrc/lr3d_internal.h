@@ -238,8 +238,12 @@ struct MeshVolume {
     const int32_t *lookup;      // grid index -> pool brick or -1
     const int4 *coords;         // pool brick -> grid coordinates
     const float *tsdf, *weight;
+    const uint32_t *bricks = nullptr;  // the pool bricks the cell kernels run over (nullptr: all)
 };
 
+hipError_t mesh_launch_live_bricks(hipStream_t s, const MeshVolume &v, uint32_t n_bricks, uint32_t *live);
+hipError_t mesh_launch_brick_list(hipStream_t s, const uint32_t *live, const uint32_t *pos, uint32_t n_bricks,
+                                  uint32_t *list);
 hipError_t mesh_launch_active(hipStream_t s, const MeshVolume &v, uint32_t cells, uint32_t *flag);
 hipError_t mesh_launch_vertices(hipStream_t s, const MeshVolume &v, uint32_t cells, const uint32_t *vidx,
                                 int32_t *cell_vertex, double *pos_d, float *pos_f);

@@ -56,9 +56,32 @@ __device__ __forceinline__ bool sample_cell(const MeshVolume &v, int32_t x, int3
     return inside > 0 && inside < 8;
 }
 
+// Pool voxel of thread t: the t-th voxel of the listed bricks, or of the pool without a list.
+__device__ __forceinline__ uint32_t pool_voxel(const MeshVolume &v, uint32_t t) {
+    return v.bricks ? v.bricks[t >> 9] * 512u + (t & 511u) : t;
+}
+
+// Bricks holding a voxel observed at the minimum weight: every cell and quad of the others is
+// empty (a cell needs its own corner observed, a quad its own voxel), so the cell kernels skip them.
+__global__ void __launch_bounds__(256) live_brick_kernel(MeshVolume v, uint32_t n_bricks, uint32_t *__restrict__ live) {
+    const uint32_t b = blockIdx.x;
+    if (b >= n_bricks) return;  // the whole block
+    const size_t base = (size_t)b * 512u + threadIdx.x;
+    const bool seen = (double)v.weight[base] >= v.min_weight || (double)v.weight[base + 256] >= v.min_weight;
+    const int any = __syncthreads_or(seen ? 1 : 0);
+    if (threadIdx.x == 0) live[b] = any ? 1u : 0u;
+}
+
+__global__ void brick_list_kernel(const uint32_t *__restrict__ live, const uint32_t *__restrict__ pos, uint32_t n,
+                                  uint32_t *__restrict__ list) {
+    const uint32_t b = blockIdx.x * blockDim.x + threadIdx.x;
+    if (b < n && live[b]) list[pos[b]] = b;
+}
+
 __global__ void __launch_bounds__(256) active_kernel(MeshVolume v, uint32_t cells, uint32_t *__restrict__ flag) {
-    const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
-    if (s >= cells) return;
+    const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= cells) return;
+    const uint32_t s = pool_voxel(v, t);
     int32_t x, y, z;
     cell_of(v, s, x, y, z);
     float values[8];
@@ -69,8 +92,9 @@ __global__ void __launch_bounds__(256) active_kernel(MeshVolume v, uint32_t cell
 __global__ void __launch_bounds__(256) vertex_kernel(MeshVolume v, uint32_t cells, const uint32_t *__restrict__ vidx,
                                                      int32_t *__restrict__ cell_vertex, double *__restrict__ pos_d,
                                                      float *__restrict__ pos_f) {
-    const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
-    if (s >= cells) return;
+    const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= cells) return;
+    const uint32_t s = pool_voxel(v, t);
     int32_t x, y, z;
     cell_of(v, s, x, y, z);
     float values[8];
@@ -153,16 +177,18 @@ __device__ __forceinline__ uint32_t quads_of(const MeshVolume &v, uint32_t s, co
 
 __global__ void __launch_bounds__(256) quad_count_kernel(MeshVolume v, uint32_t cells, const int32_t *__restrict__ cell_vertex,
                                                          uint32_t *__restrict__ qcount) {
-    const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
-    if (s >= cells) return;
+    const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= cells) return;
+    const uint32_t s = pool_voxel(v, t);
     qcount[s] = quads_of(v, s, cell_vertex, nullptr, nullptr);
 }
 
 __global__ void __launch_bounds__(256) quad_emit_kernel(MeshVolume v, uint32_t cells, const int32_t *__restrict__ cell_vertex,
                                                         const double *__restrict__ pos, const uint32_t *__restrict__ qoff,
                                                         uint32_t *__restrict__ indices) {
-    const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
-    if (s >= cells) return;
+    const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= cells) return;
+    const uint32_t s = pool_voxel(v, t);
     (void)quads_of(v, s, cell_vertex, pos, indices + (size_t)qoff[s] * 6);
 }
 
@@ -309,6 +335,21 @@ inline dim3 grid_for(uint64_t n) { return dim3((unsigned)((n + 255) / 256)); }
 
 }  // namespace
 
+hipError_t mesh_launch_live_bricks(hipStream_t s, const MeshVolume &v, uint32_t n_bricks, uint32_t *live) {
+    (void)hipGetLastError();  // report this launch's status, not an earlier call's
+    if (n_bricks == 0) return hipSuccess;
+    hipLaunchKernelGGL(live_brick_kernel, dim3(n_bricks), dim3(256), 0, s, v, n_bricks, live);
+    return hipGetLastError();
+}
+
+hipError_t mesh_launch_brick_list(hipStream_t s, const uint32_t *live, const uint32_t *pos, uint32_t n_bricks,
+                                  uint32_t *list) {
+    (void)hipGetLastError();  // report this launch's status, not an earlier call's
+    if (n_bricks == 0) return hipSuccess;
+    hipLaunchKernelGGL(brick_list_kernel, grid_for(n_bricks), dim3(256), 0, s, live, pos, n_bricks, list);
+    return hipGetLastError();
+}
+
 hipError_t mesh_launch_active(hipStream_t s, const MeshVolume &v, uint32_t cells, uint32_t *flag) {
     (void)hipGetLastError();  // report this launch's status, not an earlier call's
     if (cells == 0) return hipSuccess;
@@ -319,6 +360,7 @@ hipError_t mesh_launch_active(hipStream_t s, const MeshVolume &v, uint32_t cells
 hipError_t mesh_launch_vertices(hipStream_t s, const MeshVolume &v, uint32_t cells, const uint32_t *vidx,
                                 int32_t *cell_vertex, double *pos_d, float *pos_f) {
     (void)hipGetLastError();  // report this launch's status, not an earlier call's
+    if (cells == 0) return hipSuccess;
     hipLaunchKernelGGL(vertex_kernel, grid_for(cells), dim3(256), 0, s, v, cells, vidx, cell_vertex, pos_d, pos_f);
     return hipGetLastError();
 }
@@ -326,6 +368,7 @@ hipError_t mesh_launch_vertices(hipStream_t s, const MeshVolume &v, uint32_t cel
 hipError_t mesh_launch_quads(hipStream_t s, const MeshVolume &v, uint32_t cells, const int32_t *cell_vertex,
                              const double *pos, const uint32_t *qoff, uint32_t *qcount, uint32_t *indices) {
     (void)hipGetLastError();  // report this launch's status, not an earlier call's
+    if (cells == 0) return hipSuccess;
     if (!indices)
         hipLaunchKernelGGL(quad_count_kernel, grid_for(cells), dim3(256), 0, s, v, cells, cell_vertex, qcount);
     else

@@ -407,7 +407,7 @@ __global__ void __launch_bounds__(64) ncc_backward_kernel(PmPlan plan, PmPools p
 #pragma unroll
             for (int j = 0; j < 3; j++) gpose[9 + j] = scp * gmn[j] * inv_rho;
         }
-        // views with items in this round: s_lo .. s_hi (wave-uniform)
+        // views with items in this batch of 64: s_lo .. s_hi (wave-uniform)
         const uint32_t last = min(base + 64u, n_items) - 1u;
         int s_lo = 0, s_hi = 0;
 #pragma unroll
@@ -431,8 +431,8 @@ __global__ void __launch_bounds__(64) ncc_backward_kernel(PmPlan plan, PmPools p
                 dn2 += t3;
             }
         }
-        // pose gradients: per view, the 12 sums over the round's lanes (a fixed exchange pattern),
-        // added in round order to the block's partial (zeroed before the launch)
+        // pose gradients: per view, the 12 sums over the batch's lanes (a fixed exchange pattern),
+        // added in batch order to the block's partial (zeroed before the launch)
         if (partial) {
             for (int q = s_lo; q <= s_hi; q++) {
                 double v[16];

@@ -83,31 +83,6 @@ __device__ __forceinline__ bool sample_of(const FusionView &v, const uint32_t *_
     return true;
 }
 
-// valid samples of a view from pixel `from` on (earlier pixels are flagged empty)
-__global__ void valid_kernel(FusionView v, uint32_t from, const uint32_t *__restrict__ maps,
-                             uint32_t *__restrict__ flag) {
-    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= (uint32_t)(v.W * v.H)) return;
-    const uint32_t value = i >= from ? maps[v.map_off + i] : 0u;
-    const double d = value ? (double)(float)half_to_double(value & 0xffffu) : 0.0;
-    flag[i] = d > 0.0 ? 1u : 0u;
-}
-
-// PA:37-48: cell key floor(x / cell) of every valid sample of a view, with its sample number: the
-// view's first pixel is sample number `base` of the chunk (views in order, pixels row-major)
-__global__ void key_kernel(FusionView v, uint32_t base, const uint32_t *__restrict__ maps,
-                           const uint8_t *__restrict__ thumbs, const uint32_t *__restrict__ flag,
-                           const uint32_t *__restrict__ pos, double cell, uint64_t *__restrict__ keys,
-                           uint32_t *__restrict__ idx, uint32_t *__restrict__ overflow) {
-    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= (uint32_t)(v.W * v.H) || !flag[base + i]) return;
-    double s[9];
-    sample_of(v, maps, thumbs, i, s);
-    const uint32_t at = pos[base + i];
-    keys[at] = pack_key((int64_t)floor(s[0] / cell), (int64_t)floor(s[1] / cell), (int64_t)floor(s[2] / cell), overflow);
-    idx[at] = base + i;
-}
-
 // The chunk view holding sample number p: the last c with vstart[c] <= p (vstart[0] = 0).
 __device__ __forceinline__ uint32_t view_of(const uint32_t *__restrict__ vstart, uint32_t nv, uint32_t p) {
     uint32_t lo = 0, hi = nv - 1;
@@ -116,6 +91,35 @@ __device__ __forceinline__ uint32_t view_of(const uint32_t *__restrict__ vstart,
         if (vstart[mid] <= p) lo = mid; else hi = mid - 1;
     }
     return lo;
+}
+
+// Over every pixel of a chunk of views at once (pixel p of the chunk is pixel p - vstart[c] of chunk
+// view c): the valid samples (the chunk's first view from pixel `from` on), then (PA:37-48) the cell
+// key floor(x / cell) of every valid sample with its sample number p.
+__global__ void chunk_valid_kernel(const FusionView *__restrict__ views, const uint32_t *__restrict__ vstart,
+                                   uint32_t nv, uint32_t from, const uint32_t *__restrict__ maps,
+                                   uint32_t *__restrict__ flag) {
+    const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= vstart[nv]) return;
+    const uint32_t c = view_of(vstart, nv, p);
+    const uint32_t i = p - vstart[c];
+    const uint32_t value = (c > 0 || i >= from) ? maps[views[c].map_off + i] : 0u;
+    const double d = value ? (double)(float)half_to_double(value & 0xffffu) : 0.0;
+    flag[p] = d > 0.0 ? 1u : 0u;
+}
+
+__global__ void chunk_key_kernel(const FusionView *__restrict__ views, const uint32_t *__restrict__ vstart, uint32_t nv,
+                                 const uint32_t *__restrict__ maps, const uint8_t *__restrict__ thumbs,
+                                 const uint32_t *__restrict__ flag, const uint32_t *__restrict__ pos, double cell,
+                                 uint64_t *__restrict__ keys, uint32_t *__restrict__ idx, uint32_t *__restrict__ overflow) {
+    const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= vstart[nv] || !flag[p]) return;
+    const uint32_t c = view_of(vstart, nv, p);
+    double s[9];
+    sample_of(views[c], maps, thumbs, p - vstart[c], s);
+    const uint32_t at = pos[p];
+    keys[at] = pack_key((int64_t)floor(s[0] / cell), (int64_t)floor(s[1] / cell), (int64_t)floor(s[2] / cell), overflow);
+    idx[at] = p;
 }
 
 // New cells per chunk view of their first sample: a histogram per block in LDS, added to the
@@ -366,9 +370,6 @@ inline dim3 grid_for(uint64_t n) { return dim3((unsigned)((n + 255) / 256)); }
         return hipGetLastError();                                                           \
     } while (0)
 
-hipError_t fusion_launch_valid(hipStream_t s, const FusionView &v, uint32_t from, const uint32_t *maps, uint32_t *flag) {
-    LAUNCH(valid_kernel, (uint64_t)v.W * v.H, v, from, maps, flag);
-}
 hipError_t fusion_launch_new_cell_flags(hipStream_t s, const uint32_t *idx, const uint32_t *start, const uint32_t *is_new,
                                         uint32_t segments, uint32_t lo, uint32_t hi, uint32_t *flag) {
     LAUNCH(new_cell_flag_kernel, segments, idx, start, is_new, segments, lo, hi, flag);
@@ -377,10 +378,15 @@ hipError_t fusion_launch_select(hipStream_t s, const uint32_t *flag, const uint3
                                 uint32_t *out) {
     LAUNCH(select_kernel, n, flag, scan, n, k, out);
 }
-hipError_t fusion_launch_keys(hipStream_t s, const FusionView &v, uint32_t base, const uint32_t *maps,
-                              const uint8_t *thumbs, const uint32_t *flag, const uint32_t *pos, double cell,
-                              uint64_t *keys, uint32_t *idx, uint32_t *overflow) {
-    LAUNCH(key_kernel, (uint64_t)v.W * v.H, v, base, maps, thumbs, flag, pos, cell, keys, idx, overflow);
+hipError_t fusion_launch_chunk_valid(hipStream_t s, const FusionView *views, const uint32_t *vstart, uint32_t nv,
+                                     uint32_t pixels, uint32_t from, const uint32_t *maps, uint32_t *flag) {
+    LAUNCH(chunk_valid_kernel, pixels, views, vstart, nv, from, maps, flag);
+}
+hipError_t fusion_launch_chunk_keys(hipStream_t s, const FusionView *views, const uint32_t *vstart, uint32_t nv,
+                                    uint32_t pixels, const uint32_t *maps, const uint8_t *thumbs, const uint32_t *flag,
+                                    const uint32_t *pos, double cell, uint64_t *keys, uint32_t *idx,
+                                    uint32_t *overflow) {
+    LAUNCH(chunk_key_kernel, pixels, views, vstart, nv, maps, thumbs, flag, pos, cell, keys, idx, overflow);
 }
 hipError_t fusion_launch_first_views(hipStream_t s, const uint32_t *idx, const uint32_t *start, const uint32_t *is_new,
                                      uint32_t segments, const uint32_t *vstart, uint32_t nv, uint32_t *hist) {

@@ -2205,8 +2205,8 @@ extern "C" int32_t lr3d_fuse_depth_maps(lr3d_stereo *st, const int32_t *layers, 
         bool view_end = true;     // the chunk ends at the end of view `last`
         HIP_TRY(flag.reserve(P));
         HIP_TRY(pos.reserve(P));
-        for (uint32_t c = 0; c < nv; c++)
-            HIP_TRY(fusion_launch_valid(s, fv[a + c], c == 0 ? from : 0u, maps, flag.ptr + hstart[c]));
+        HIP_TRY(hipMemcpyAsync(vstart.ptr, hstart.data(), (size_t)(nv + 1) * 4, hipMemcpyHostToDevice, s));
+        HIP_TRY(fusion_launch_chunk_valid(s, dviews.ptr + a, vstart.ptr, nv, (uint32_t)P, from, maps, flag.ptr));
         uint32_t M = 0;
         HIP_TRY(scan.run(s, flag.ptr, pos.ptr, (uint32_t)P, &M));
         if (M > 0) {
@@ -2224,9 +2224,8 @@ extern "C" int32_t lr3d_fuse_depth_maps(lr3d_stereo *st, const int32_t *layers, 
             HIP_TRY(sidx.reserve(M));
             HIP_TRY(head.reserve(M));
             HIP_TRY(seg.reserve(M));
-            for (uint32_t c = 0; c < nv; c++)
-                HIP_TRY(fusion_launch_keys(s, fv[a + c], hstart[c], maps, dthumbs.ptr, flag.ptr, pos.ptr, cell, keys.ptr,
-                                           idx.ptr, overflow.ptr));
+            HIP_TRY(fusion_launch_chunk_keys(s, dviews.ptr + a, vstart.ptr, nv, (uint32_t)P, maps, dthumbs.ptr, flag.ptr,
+                                             pos.ptr, cell, keys.ptr, idx.ptr, overflow.ptr));
             HIP_TRY(sort_pairs(s, sort_tmp, keys.ptr, skeys.ptr, idx.ptr, sidx.ptr, M));
             HIP_TRY(fusion_launch_heads(s, skeys.ptr, M, head.ptr));
             uint32_t G = 0;
@@ -2238,7 +2237,6 @@ extern "C" int32_t lr3d_fuse_depth_maps(lr3d_stereo *st, const int32_t *layers, 
             HIP_TRY(fusion_launch_starts(s, head.ptr, seg.ptr, M, start.ptr));
             HIP_TRY(fusion_launch_lookup(s, skeys.ptr, start.ptr, G, state.keys.ptr, state.n, exist.ptr, is_new.ptr));
             // new cells per view of first appearance
-            HIP_TRY(hipMemcpyAsync(vstart.ptr, hstart.data(), (size_t)(nv + 1) * 4, hipMemcpyHostToDevice, s));
             HIP_TRY(hipMemsetAsync(hist.ptr, 0, (size_t)nv * 4, s));
             HIP_TRY(fusion_launch_first_views(s, sidx.ptr, start.ptr, is_new.ptr, G, vstart.ptr, nv, hist.ptr));
             hhist.resize(nv);

@@ -281,14 +281,16 @@ struct FusionView {
     uint64_t thumb_off;    // byte offset of the thumbnail (RGB)
 };
 
-hipError_t fusion_launch_valid(hipStream_t s, const FusionView &v, uint32_t from, const uint32_t *maps, uint32_t *flag);
 hipError_t fusion_launch_new_cell_flags(hipStream_t s, const uint32_t *idx, const uint32_t *start, const uint32_t *is_new,
                                         uint32_t segments, uint32_t lo, uint32_t hi, uint32_t *flag);
 hipError_t fusion_launch_select(hipStream_t s, const uint32_t *flag, const uint32_t *scan, uint32_t n, uint32_t k,
                                 uint32_t *out);
-hipError_t fusion_launch_keys(hipStream_t s, const FusionView &v, uint32_t base, const uint32_t *maps,
-                              const uint8_t *thumbs, const uint32_t *flag, const uint32_t *pos, double cell,
-                              uint64_t *keys, uint32_t *idx, uint32_t *overflow);
+hipError_t fusion_launch_chunk_valid(hipStream_t s, const FusionView *views, const uint32_t *vstart, uint32_t nv,
+                                     uint32_t pixels, uint32_t from, const uint32_t *maps, uint32_t *flag);
+hipError_t fusion_launch_chunk_keys(hipStream_t s, const FusionView *views, const uint32_t *vstart, uint32_t nv,
+                                    uint32_t pixels, const uint32_t *maps, const uint8_t *thumbs, const uint32_t *flag,
+                                    const uint32_t *pos, double cell, uint64_t *keys, uint32_t *idx,
+                                    uint32_t *overflow);
 hipError_t fusion_launch_first_views(hipStream_t s, const uint32_t *idx, const uint32_t *start, const uint32_t *is_new,
                                      uint32_t segments, const uint32_t *vstart, uint32_t nv, uint32_t *hist);
 hipError_t fusion_launch_limit(hipStream_t s, const uint32_t *idx, const uint32_t *start, uint32_t segments,

@@ -2067,12 +2067,11 @@ struct CellSet {
         n = count;
         return e;
     }
-    // Grow-only (contents are not kept).
+    // Room for `count` cells, grow-only (contents are not kept; n is left to the caller).
     hipError_t reserve(uint32_t count) {
         hipError_t e = keys.reserve(std::max<uint32_t>(count, 1));
         if (e == hipSuccess) e = sums.reserve((size_t)std::max<uint32_t>(count, 1) * 9);
         if (e == hipSuccess) e = counts.reserve(std::max<uint32_t>(count, 1));
-        n = count;
         return e;
     }
 };
@@ -2158,25 +2157,33 @@ extern "C" int32_t lr3d_fuse_depth_maps(lr3d_stereo *st, const int32_t *layers, 
     int64_t capacity = 1 << 16;  // PA:17 the hash table's slots
     // PA:87-110 rebuild: every cell re-binned by its mean position at `to`, cells that fall
     // together merged (in key order; the reference merges in slot order)
+    // (its buffers, like the chunks' below, grow with headroom and are kept: every allocation and
+    // free of device memory costs more than the work of a re-binning)
+    auto grown = [](size_t n) { return n + n / 2 + 1024; };
+    DevBuf<uint64_t> rk, rsk;
+    DevBuf<uint32_t> ri, rsi, rh, rs, rst;
+    CellSet rnext;
     auto rebin = [&](double to) -> int32_t {
         const uint32_t S = state.n;
-        DevBuf<uint64_t> keys, skeys;
-        DevBuf<uint32_t> idx, sidx, head, seg, start;
-        HIP_TRY(keys.alloc(S));
-        HIP_TRY(skeys.alloc(S));
-        HIP_TRY(idx.alloc(S));
-        HIP_TRY(sidx.alloc(S));
-        HIP_TRY(head.alloc(S));
-        HIP_TRY(seg.alloc(S));
+        if (S > rk.count) {
+            HIP_TRY(rk.alloc(grown(S)));
+            HIP_TRY(rsk.alloc(grown(S)));
+            HIP_TRY(ri.alloc(grown(S)));
+            HIP_TRY(rsi.alloc(grown(S)));
+            HIP_TRY(rh.alloc(grown(S)));
+            HIP_TRY(rs.alloc(grown(S)));
+            HIP_TRY(rst.alloc(grown(S)));
+        }
+        DevBuf<uint64_t> &keys = rk, &skeys = rsk;
+        DevBuf<uint32_t> &idx = ri, &sidx = rsi, &head = rh, &seg = rs, &start = rst;
         HIP_TRY(fusion_launch_rekey(s, state.sums.ptr, state.counts.ptr, S, to, keys.ptr, idx.ptr, overflow.ptr));
         HIP_TRY(sort_pairs(s, sort_tmp, keys.ptr, skeys.ptr, idx.ptr, sidx.ptr, S));
         HIP_TRY(fusion_launch_heads(s, skeys.ptr, S, head.ptr));
         uint32_t G = 0;
         HIP_TRY(scan.run(s, head.ptr, seg.ptr, S, &G));
-        HIP_TRY(start.alloc(G));
         HIP_TRY(fusion_launch_starts(s, head.ptr, seg.ptr, S, start.ptr));
-        CellSet next;
-        HIP_TRY(next.alloc(G));
+        CellSet &next = rnext;
+        HIP_TRY(next.reserve(G));
         HIP_TRY(fusion_launch_merge(s, skeys.ptr, sidx.ptr, start.ptr, G, S, state.sums.ptr, state.counts.ptr,
                                     next.keys.ptr, next.sums.ptr, next.counts.ptr));
         std::swap(state.keys, next.keys);
@@ -2185,6 +2192,23 @@ extern "C" int32_t lr3d_fuse_depth_maps(lr3d_stereo *st, const int32_t *layers, 
         state.n = G;
         return LR3D_OK;
     };
+    // the per-sample buffers sized once for the largest chunk this call can form
+    uint64_t total_px = 0, max_px = 0;
+    for (int32_t i = 0; i < n; i++) {
+        total_px += (uint64_t)fv[i].W * fv[i].H;
+        max_px = std::max<uint64_t>(max_px, (uint64_t)fv[i].W * fv[i].H);
+    }
+    const size_t most = (size_t)std::min<uint64_t>(total_px, kFusionChunkPixels + max_px);
+    HIP_TRY(flag.reserve(most));
+    HIP_TRY(pos.reserve(most));
+    HIP_TRY(keys.reserve(most));
+    HIP_TRY(skeys.reserve(most));
+    HIP_TRY(idx.reserve(most));
+    HIP_TRY(sidx.reserve(most));
+    HIP_TRY(head.reserve(most));
+    HIP_TRY(seg.reserve(most));
+    HIP_TRY(gflag.reserve(max_px));
+    HIP_TRY(gscan.reserve(max_px));
     std::vector<uint32_t> hstart, hhist;
     uint32_t from = 0;  // first pixel of view a still to add
     for (int32_t a = 0; a < n;) {
@@ -2230,10 +2254,13 @@ extern "C" int32_t lr3d_fuse_depth_maps(lr3d_stereo *st, const int32_t *layers, 
             HIP_TRY(fusion_launch_heads(s, skeys.ptr, M, head.ptr));
             uint32_t G = 0;
             HIP_TRY(scan.run(s, head.ptr, seg.ptr, M, &G));
-            HIP_TRY(start.reserve(G));
-            HIP_TRY(exist.reserve(G));
-            HIP_TRY(is_new.reserve(G));
-            HIP_TRY(new_pos.reserve(G));
+            if (G > start.count) {  // per-cell buffers: grown with headroom
+                HIP_TRY(start.alloc(grown(G)));
+                HIP_TRY(exist.alloc(grown(G)));
+                HIP_TRY(is_new.alloc(grown(G)));
+                HIP_TRY(new_pos.alloc(grown(G)));
+                HIP_TRY(big.alloc(grown(G)));
+            }
             HIP_TRY(fusion_launch_starts(s, head.ptr, seg.ptr, M, start.ptr));
             HIP_TRY(fusion_launch_lookup(s, skeys.ptr, start.ptr, G, state.keys.ptr, state.n, exist.ptr, is_new.ptr));
             // new cells per view of first appearance
@@ -2288,28 +2315,27 @@ extern "C" int32_t lr3d_fuse_depth_maps(lr3d_stereo *st, const int32_t *layers, 
             uint32_t Nn = 0;
             HIP_TRY(scan.run(s, is_new.ptr, new_pos.ptr, G, &Nn));
             // merged set: the old cells, then the new ones, re-sorted by key below
-            HIP_TRY(cat.reserve(state.n + Nn));
+            HIP_TRY(cat.reserve(state.n + Nn > cat.keys.count ? grown(state.n + Nn) : state.n + Nn));
             if (state.n) {
                 HIP_TRY(hipMemcpyAsync(cat.keys.ptr, state.keys.ptr, (size_t)state.n * 8, hipMemcpyDeviceToDevice, s));
                 HIP_TRY(hipMemcpyAsync(cat.sums.ptr, state.sums.ptr, (size_t)state.n * 36, hipMemcpyDeviceToDevice, s));
                 HIP_TRY(hipMemcpyAsync(cat.counts.ptr, state.counts.ptr, (size_t)state.n * 4, hipMemcpyDeviceToDevice, s));
             }
             // PA:49-62: running sums of existing cells updated in place, new cells appended
-            HIP_TRY(big.reserve(G));
             HIP_TRY(hipMemsetAsync(n_big.ptr, 0, 4, s));
             HIP_TRY(fusion_launch_reduce(s, dviews.ptr + a, vstart.ptr, nv, limit, maps, dthumbs.ptr, skeys.ptr, sidx.ptr,
                                          start.ptr, G, M, exist.ptr, is_new.ptr, new_pos.ptr, cat.sums.ptr,
                                          cat.counts.ptr, cat.keys.ptr + state.n, cat.sums.ptr + (size_t)state.n * 9,
                                          cat.counts.ptr + state.n, big.ptr, n_big.ptr));
             const uint32_t T = state.n + Nn;
-            HIP_TRY(tk.reserve(T));
-            HIP_TRY(perm0.reserve(T));
-            HIP_TRY(perm.reserve(T));
+            HIP_TRY(tk.reserve(T > tk.count ? grown(T) : T));
+            HIP_TRY(perm0.reserve(T > perm0.count ? grown(T) : T));
+            HIP_TRY(perm.reserve(T > perm.count ? grown(T) : T));
             (void)hipGetLastError();
             hipLaunchKernelGGL(iota_u32_kernel, dim3((T + 255) / 256), dim3(256), 0, s, perm0.ptr, T);
             HIP_TRY(hipGetLastError());
             HIP_TRY(sort_pairs(s, sort_tmp, cat.keys.ptr, tk.ptr, perm0.ptr, perm.ptr, T));
-            HIP_TRY(next.reserve(T));
+            HIP_TRY(next.reserve(T > next.keys.count ? grown(T) : T));
             HIP_TRY(fusion_launch_gather(s, perm.ptr, T, cat.keys.ptr, cat.sums.ptr, cat.counts.ptr, next.keys.ptr,
                                          next.sums.ptr, next.counts.ptr));
             std::swap(state.keys, next.keys);
